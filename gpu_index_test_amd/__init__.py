@@ -1,0 +1,536 @@
+"""MI355X-native strobealign seeding path (syncmers -> randstrobes -> index
+lookup -> NAMs) on hand-written gfx950 kernels.
+
+This module is the host-side mirror of the reference's C++ interface for the
+path -- ``IndexParameters``, ``StrobemerIndex``, ``randstrobes_query``,
+``find_nams``, ``find_nams_rescue`` (step2/indexparameters.hpp:72-95,
+step2/index.hpp:37-183, step2/randstrobes.hpp:107, step2/nam.hpp:11-20) --
+over the C ABI of ``libgis.so`` (include/gis.h). Whole batches go through
+``Mapper``. Every call runs on the GPU; without the built library (or a GPU)
+calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _native as N
+from ._native import (FLAG_NO_RANDSTROBES, FLAG_RESCUED, GIS_DEFAULT, GIS_NOT_FOUND, HIT, NAM,
+                      QUERY_RANDSTROBE, REF_RANDSTROBE, SYNCMER, GisError)
+
+__all__ = [
+    "IndexParameters", "References", "Context", "StrobemerIndex", "Mapper", "MapResult",
+    "randstrobes_query", "find_nams", "find_nams_rescue", "synth_genome", "synth_reads",
+    "pack_reads", "GisError", "NAM", "QUERY_RANDSTROBE", "REF_RANDSTROBE", "SYNCMER", "HIT",
+    "GIS_NOT_FOUND", "FLAG_RESCUED", "FLAG_NO_RANDSTROBES",
+]
+
+
+def _check(code, ctx=None):
+    if code != 0:
+        msg = N.lib().gis_last_error(ctx).decode() if ctx else ""
+        raise GisError(code, msg)
+
+
+class _Syncmer:
+    def __init__(self, k, s, t):
+        self.k, self.s, self.t_syncmer = k, s, t
+
+
+class _Randstrobe:
+    def __init__(self, l, u, q, max_dist, w_min, w_max):
+        self.l, self.u, self.q, self.max_dist, self.w_min, self.w_max = l, u, q, max_dist, w_min, w_max
+
+
+class IndexParameters:
+    """IndexParameters (step2/indexparameters.hpp:72-95)."""
+    DEFAULT = GIS_DEFAULT
+
+    def __init__(self, canonical_read_length, k, s, l, u, q, max_dist):
+        p = N.Params()
+        _check(N.lib().gis_params_make(canonical_read_length, k, s, l, u, q, max_dist, C.byref(p)))
+        self._set(p)
+
+    def _set(self, p):
+        self.c = p
+        self.canonical_read_length = p.canonical_read_length
+        self.syncmer = _Syncmer(p.k, p.s, p.t_syncmer)
+        self.randstrobe = _Randstrobe(p.l, p.u, p.q, p.max_dist, p.w_min, p.w_max)
+
+    @classmethod
+    def from_read_length(cls, read_length, k=GIS_DEFAULT, s=GIS_DEFAULT, l=GIS_DEFAULT,
+                         u=GIS_DEFAULT, c=GIS_DEFAULT, max_seed_len=GIS_DEFAULT):
+        p = N.Params()
+        _check(N.lib().gis_params_from_read_length(read_length, k, s, l, u, c, max_seed_len,
+                                                   C.byref(p)))
+        obj = cls.__new__(cls)
+        obj._set(p)
+        return obj
+
+    @classmethod
+    def _from_c(cls, p):
+        obj = cls.__new__(cls)
+        q = N.Params()
+        C.memmove(C.byref(q), C.byref(p), C.sizeof(N.Params))
+        obj._set(q)
+        return obj
+
+    def as_tuple(self):
+        p = self.c
+        return (p.canonical_read_length, p.k, p.s, p.t_syncmer, p.l, p.u, p.q, p.max_dist,
+                p.w_min, p.w_max)
+
+    def __eq__(self, other):
+        return isinstance(other, IndexParameters) and self.as_tuple() == other.as_tuple()
+
+    def filename_extension(self):
+        """".r<len>.sti" when nothing was overridden (step2/indexparameters.cpp:116-125)"""
+        if self == IndexParameters.from_read_length(self.canonical_read_length):
+            return f".r{self.canonical_read_length}.sti"
+        return ".sti"
+
+    def __repr__(self):
+        p = self.c
+        return (f"IndexParameters(r={p.canonical_read_length}, k={p.k}, s={p.s}, "
+                f"t_syncmer={p.t_syncmer}, l={p.l}, u={p.u}, q={p.q}, max_dist={p.max_dist}, "
+                f"w_min={p.w_min}, w_max={p.w_max})")
+
+
+class References:
+    """Reference sequences (step2/refs.hpp:53-91): names, sequences, lengths."""
+
+    def __init__(self, sequences, names=None):
+        self.sequences = [s.encode() if isinstance(s, str) else bytes(s) for s in sequences]
+        self.names = list(names) if names is not None else [f"contig{i}" for i in range(len(sequences))]
+        if len(self.names) != len(self.sequences):
+            raise ValueError("lengths do not match")
+        self.lengths = [len(s) for s in self.sequences]
+
+    def total_length(self):
+        return sum(self.lengths)
+
+    def size(self):
+        return len(self.sequences)
+
+    @classmethod
+    def from_fasta(cls, path):
+        """FASTA reader: names cut at the first space, sequences upper-cased
+        with c & ~32 (step2/refs.cpp:32-79)."""
+        names, seqs, cur, name = [], [], [], None
+        with open(path, "rb") as fh:
+            first = fh.read(1)
+            if first != b">":
+                raise GisError(10, "FASTA file must begin with '>' character")
+            fh.seek(0)
+            for line in fh:
+                line = line.rstrip(b"\r\n")
+                if line.startswith(b">"):
+                    if cur:
+                        seqs.append(bytes(x & ~32 & 0xFF for x in b"".join(cur)))
+                        names.append(name)
+                    name = line[1:].split(b" ")[0].decode()
+                    cur = []
+                else:
+                    cur.append(line)
+            if cur:
+                seqs.append(bytes(x & ~32 & 0xFF for x in b"".join(cur)))
+                names.append(name)
+        return cls(seqs, names)
+
+
+class Context:
+    """A device context (one HIP stream, serialised calls)."""
+
+    def __init__(self, device=0):
+        self.L = N.lib()
+        h = C.c_void_p()
+        code = self.L.gis_ctx_create(device, C.byref(h))
+        if code != 0:
+            raise GisError(code, f"cannot create a context on device {device}")
+        self.h = h
+
+    def check(self, code):
+        _check(code, self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gis_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class StrobemerIndex:
+    """StrobemerIndex (step2/index.hpp:37-183), resident in HBM."""
+
+    def __init__(self, references, parameters, bits=-1, ctx=None):
+        self.ctx = ctx or default_context()
+        self.references = references
+        self.parameters = parameters
+        self._bits_arg = bits
+        self._host = None
+
+    # --- construction / persistence ---
+    def populate(self, f=0.0002, n_threads=None):
+        refs = self.references
+        seqs = [s if isinstance(s, bytes) else bytes(s) for s in refs.sequences]
+        bufs = [C.create_string_buffer(s, len(s)) for s in seqs]
+        arr = (C.c_void_p * max(1, len(bufs)))(*[C.cast(b, C.c_void_p) for b in bufs])
+        lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+        n_threads = n_threads or min(16, os.cpu_count() or 1)
+        self.ctx.check(self.ctx.L.gis_index_build(self.ctx.h, arr, N.ptr(lens), len(seqs),
+                                                  C.byref(self.parameters.c), self._bits_arg,
+                                                  C.c_float(f), n_threads))
+        self._host = None
+
+    def populate_arrays(self, genome, contig_lens, f=0.0002, n_threads=None):
+        """populate() from one concatenated uint8 genome array (no copies)."""
+        contig_lens = np.ascontiguousarray(contig_lens, dtype=np.uint64)
+        starts = np.concatenate([[0], np.cumsum(contig_lens)[:-1]]).astype(np.uint64)
+        base = genome.ctypes.data
+        arr = (C.c_void_p * len(contig_lens))(*[base + int(s) for s in starts])
+        n_threads = n_threads or min(16, os.cpu_count() or 1)
+        self.ctx.check(self.ctx.L.gis_index_build(self.ctx.h, arr, N.ptr(contig_lens),
+                                                  len(contig_lens), C.byref(self.parameters.c),
+                                                  self._bits_arg, C.c_float(f), n_threads))
+        self._host = None
+
+    def upload(self, randstrobes, starts, bits, filter_cutoff):
+        randstrobes = np.ascontiguousarray(randstrobes, dtype=REF_RANDSTROBE)
+        starts = np.ascontiguousarray(starts, dtype=np.uint64)
+        self.ctx.check(self.ctx.L.gis_index_upload(self.ctx.h, N.ptr(randstrobes), len(randstrobes),
+                                                   N.ptr(starts), bits, filter_cutoff,
+                                                   C.byref(self.parameters.c)))
+        self._host = None
+
+    def read(self, filename):
+        self.ctx.check(self.ctx.L.gis_index_read_sti(self.ctx.h, os.fsencode(filename),
+                                                     C.byref(self.parameters.c)))
+        self._host = None
+
+    def write(self, filename):
+        self.ctx.check(self.ctx.L.gis_index_write_sti(self.ctx.h, os.fsencode(filename)))
+
+    # --- properties ---
+    def info(self):
+        i = N.IndexInfo()
+        self.ctx.check(self.ctx.L.gis_index_info_get(self.ctx.h, C.byref(i)))
+        return i
+
+    @property
+    def bits(self):
+        return self.info().bits
+
+    @property
+    def filter_cutoff(self):
+        return self.info().filter_cutoff
+
+    def size(self):
+        return self.info().n_randstrobes
+
+    def k(self):
+        return self.parameters.syncmer.k
+
+    def _download(self):
+        if self._host is None:
+            i = self.info()
+            rs = np.empty(i.n_randstrobes, dtype=REF_RANDSTROBE)
+            st = np.empty((1 << i.bits) + 1, dtype=np.uint64)
+            self.ctx.check(self.ctx.L.gis_index_download(self.ctx.h, N.ptr(rs), N.ptr(st)))
+            self._host = (rs, st)
+        return self._host
+
+    @property
+    def randstrobes(self):
+        return self._download()[0]
+
+    @property
+    def randstrobe_start_indices(self):
+        return self._download()[1]
+
+    # --- lookups (on the GPU) ---
+    def find_many(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.empty(len(keys), dtype=np.uint64)
+        self.ctx.check(self.ctx.L.gis_lookup_batch(self.ctx.h, N.ptr(keys), len(keys), N.ptr(out)))
+        return out
+
+    def find(self, key):
+        return int(self.find_many(np.array([key], dtype=np.uint64))[0])
+
+    def end(self):
+        return GIS_NOT_FOUND
+
+    # host-side accessors over the downloaded table
+    def get_hash(self, position):
+        rs = self.randstrobes
+        return int(rs["hash"][position]) if position < len(rs) else GIS_NOT_FOUND
+
+    def is_filtered(self, position):
+        return self.get_hash(position) == self.get_hash(position + self.filter_cutoff)
+
+    def get_strobe1_position(self, position):
+        return int(self.randstrobes["position"][position])
+
+    def strobe2_offset(self, position):
+        return int(self.randstrobes["packed"][position]) & 0xFF
+
+    def reference_index(self, position):
+        return int(self.randstrobes["packed"][position]) >> 8
+
+
+def pack_reads(reads):
+    """Concatenate reads (str/bytes) -> (uint8 seqs, u64 offsets, u32 lens)."""
+    bs = [r.encode() if isinstance(r, str) else bytes(r) for r in reads]
+    lens = np.array([len(b) for b in bs], dtype=np.uint32)
+    offsets = np.zeros(len(bs), dtype=np.uint64)
+    if len(bs) > 1:
+        offsets[1:] = np.cumsum(lens[:-1], dtype=np.uint64)
+    seqs = np.frombuffer(b"".join(bs) or b"\0", dtype=np.uint8).copy()
+    return seqs, offsets, lens
+
+
+def _seq_args(seqs, offsets, lens):
+    seqs = np.ascontiguousarray(seqs, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    return seqs, offsets, lens
+
+
+def syncmers_batch(index, seqs, offsets, lens):
+    """Canonical syncmers of every read -> (offsets[n+1], SYNCMER array)."""
+    ctx = index.ctx
+    seqs, offsets, lens = _seq_args(seqs, offsets, lens)
+    n = len(lens)
+    cap = int(np.maximum(lens.astype(np.int64) - index.parameters.syncmer.k + 1, 0).sum())
+    out = np.empty(max(cap, 1), dtype=SYNCMER)
+    oo = np.empty(n + 1, dtype=np.uint64)
+    nout = C.c_uint64()
+    ctx.check(ctx.L.gis_syncmers_batch(ctx.h, N.ptr(seqs), N.ptr(offsets), N.ptr(lens), n,
+                                       N.ptr(oo), N.ptr(out), cap, C.byref(nout)))
+    return oo, out[: nout.value]
+
+
+def randstrobes_batch(index, seqs, offsets, lens):
+    """randstrobes_query of every read -> (offsets[n+1], QUERY_RANDSTROBE array)."""
+    ctx = index.ctx
+    seqs, offsets, lens = _seq_args(seqs, offsets, lens)
+    n = len(lens)
+    cap = int(2 * np.maximum(lens.astype(np.int64) - index.parameters.syncmer.k + 1, 0).sum())
+    out = np.empty(max(cap, 1), dtype=QUERY_RANDSTROBE)
+    oo = np.empty(n + 1, dtype=np.uint64)
+    nout = C.c_uint64()
+    ctx.check(ctx.L.gis_randstrobes_batch(ctx.h, N.ptr(seqs), N.ptr(offsets), N.ptr(lens), n,
+                                          N.ptr(oo), N.ptr(out), cap, C.byref(nout)))
+    return oo, out[: nout.value]
+
+
+def randstrobes_query(seq, index):
+    """randstrobes_query(seq, parameters) (step2/randstrobes.hpp:107) for one
+    read, on the GPU; the parameters are those of ``index``."""
+    s, o, l = pack_reads([seq])
+    return randstrobes_batch(index, s, o, l)[1]
+
+
+def _map_options(rescue_level=2, hard_cutoff=1000, mode=N.MODE_PLAIN):
+    o = N.MapOptions()
+    o.rescue_level, o.hard_cutoff, o.mode = rescue_level, hard_cutoff, mode
+    return o
+
+
+class MapResult:
+    def __init__(self, nam_offsets, nams, frac, flags):
+        self.nam_offsets, self.nams, self.nonrepetitive_fraction, self.flags = nam_offsets, nams, frac, flags
+
+    def read_nams(self, r):
+        return self.nams[int(self.nam_offsets[r]): int(self.nam_offsets[r + 1])]
+
+
+def _nam_out(n_reads, cap):
+    offs = np.empty(n_reads + 1, dtype=np.uint64)
+    nams = np.empty(max(cap, 1), dtype=NAM)
+    frac = np.empty(max(n_reads, 1), dtype=np.float32)
+    flags = np.empty(max(n_reads, 1), dtype=np.uint8)
+    o = N.NamOut()
+    o.nam_offsets, o.nams, o.nam_capacity = offs.ctypes.data, nams.ctypes.data, cap
+    o.nonrepetitive_fraction, o.flags = frac.ctypes.data, flags.ctypes.data
+    return o, offs, nams, frac, flags
+
+
+def _call_with_nam_out(ctx, n_reads, fn, cap=None):
+    cap = cap if cap is not None else max(16, 8 * n_reads)
+    while True:
+        o, offs, nams, frac, flags = _nam_out(n_reads, cap)
+        code = fn(o)
+        if code == 5 and o.n_nams > cap:   # GIS_ERR_CAPACITY: retry with the reported size
+            cap = int(o.n_nams)
+            continue
+        ctx.check(code)
+        return MapResult(offs, nams[: o.n_nams], frac[:n_reads], flags[:n_reads])
+
+
+def find_nams_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000):
+    """Whole path for a batch of reads (plain per-stage launches)."""
+    ctx = index.ctx
+    seqs, offsets, lens = _seq_args(seqs, offsets, lens)
+    n = len(lens)
+    opt = _map_options(rescue_level, hard_cutoff)
+    return _call_with_nam_out(ctx, n, lambda o: ctx.L.gis_find_nams_batch(
+        ctx.h, N.ptr(seqs), N.ptr(offsets), N.ptr(lens), n, C.byref(opt), C.byref(o)))
+
+
+def hits_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000):
+    ctx = index.ctx
+    seqs, offsets, lens = _seq_args(seqs, offsets, lens)
+    n = len(lens)
+    opt = _map_options(rescue_level, hard_cutoff)
+    cap = max(64, 64 * n)
+    while True:
+        oo = np.empty(n + 1, dtype=np.uint64)
+        out = np.empty(cap, dtype=HIT)
+        nout = C.c_uint64()
+        code = ctx.L.gis_hits_batch(ctx.h, N.ptr(seqs), N.ptr(offsets), N.ptr(lens), n,
+                                    C.byref(opt), N.ptr(oo), N.ptr(out), cap, C.byref(nout))
+        if code == 5 and nout.value > cap:
+            cap = int(nout.value)
+            continue
+        ctx.check(code)
+        return oo, out[: nout.value]
+
+
+def _qrs_call(index, query_randstrobes_list, rescue, rescue_cutoff, hard_cutoff):
+    ctx = index.ctx
+    arrs = [np.ascontiguousarray(q, dtype=QUERY_RANDSTROBE) for q in query_randstrobes_list]
+    qoff = np.zeros(len(arrs) + 1, dtype=np.uint64)
+    qoff[1:] = np.cumsum([len(a) for a in arrs])
+    allq = np.concatenate(arrs) if arrs else np.empty(0, dtype=QUERY_RANDSTROBE)
+    allq = np.ascontiguousarray(allq)
+    n = len(arrs)
+    return _call_with_nam_out(ctx, n, lambda o: ctx.L.gis_find_nams_qrs(
+        ctx.h, N.ptr(allq), N.ptr(qoff), n, rescue, rescue_cutoff, hard_cutoff, C.byref(o)))
+
+
+def find_nams(query_randstrobes, index):
+    """find_nams (step2/nam.hpp:11-14): (nonrepetitive fraction, NAMs in
+    canonical order)."""
+    r = _qrs_call(index, [query_randstrobes], 0, 0, 1000)
+    return float(r.nonrepetitive_fraction[0]), r.read_nams(0)
+
+
+def find_nams_rescue(query_randstrobes, index, rescue_cutoff, hard_cutoff=1000):
+    """find_nams_rescue (step2/nam.hpp:16-20): NAMs in canonical order."""
+    return _qrs_call(index, [query_randstrobes], 1, rescue_cutoff, hard_cutoff).read_nams(0)
+
+
+class Mapper:
+    """Device-resident batches: upload reads, run the path, download NAMs."""
+
+    def __init__(self, index, max_reads, max_bases):
+        self.index, self.ctx = index, index.ctx
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.L.gis_batch_create(self.ctx.h, max_reads, max_bases, C.byref(h)))
+        self.h = h
+
+    def upload(self, seqs, offsets, lens):
+        seqs, offsets, lens = _seq_args(seqs, offsets, lens)
+        self._keep = (seqs, offsets, lens)
+        self.n = len(lens)
+        self.ctx.check(self.ctx.L.gis_batch_upload(self.h, N.ptr(seqs), N.ptr(offsets),
+                                                   N.ptr(lens), self.n))
+
+    def run(self, rescue_level=2, hard_cutoff=1000, mode=N.MODE_PLAIN):
+        opt = _map_options(rescue_level, hard_cutoff, mode)
+        self.ctx.check(self.ctx.L.gis_batch_run(self.h, C.byref(opt)))
+
+    def nam_count(self):
+        v = C.c_uint64()
+        self.ctx.check(self.ctx.L.gis_batch_nam_count(self.h, C.byref(v)))
+        return v.value
+
+    def download(self):
+        cap = self.nam_count()
+        return _call_with_nam_out(self.ctx, self.n,
+                                  lambda o: self.ctx.L.gis_batch_download(self.h, C.byref(o)), cap)
+
+    def stage_times(self):
+        ms = (C.c_double * 16)()
+        n = C.c_int()
+        names = C.create_string_buffer(512)
+        self.ctx.check(self.ctx.L.gis_batch_stage_times(self.h, ms, 16, C.byref(n), names, 512))
+        keys = names.value.decode().split(",") if n.value else []
+        return dict(zip(keys, list(ms)[: n.value]))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.gis_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------- synthetic inputs ----------------
+
+def synth_genome(total_length, n_contigs=1, gc=0.5, repeat_fraction=0.0, n_families=0,
+                 family_len=(1000, 10000), copies=(2, 50), divergence=(0.01, 0.05),
+                 tandem_fraction=0.0, tandem_unit=(2, 50), contig_len=(0, 0), seed=1):
+    """Deterministic genome -> (uint8 array of ACGT, u64 contig lengths)."""
+    s = N.GenomeSpec()
+    s.total_length, s.n_contigs, s.gc = total_length, n_contigs, gc
+    s.repeat_fraction, s.n_families = repeat_fraction, n_families
+    s.family_min_len, s.family_max_len = family_len
+    s.copies_min, s.copies_max = copies
+    s.div_min, s.div_max = divergence
+    s.tandem_fraction = tandem_fraction
+    s.tandem_unit_min, s.tandem_unit_max = tandem_unit
+    s.contig_min_len, s.contig_max_len = contig_len
+    s.seed = seed
+    seq = np.empty(total_length, dtype=np.uint8)
+    lens = np.empty(n_contigs, dtype=np.uint64)
+    _check(N.lib().gis_synth_genome(C.byref(s), N.ptr(seq), N.ptr(lens)))
+    return seq, lens
+
+
+def synth_reads(genome, contig_lens, n_reads, lengths=(150,), weights=(1.0,), paired=False,
+                sub_rate=0.001, indel_rate=0.0001, rc_fraction=0.5, insert=(400.0, 50.0), seed=2):
+    """Deterministic reads -> (seqs, offsets, lens, truth dict)."""
+    s = N.ReadSpec()
+    s.n_reads, s.paired = n_reads, 1 if paired else 0
+    for i, (L, w) in enumerate(zip(lengths, weights)):
+        s.lengths[i], s.weights[i] = L, w
+    s.n_lengths = len(lengths)
+    s.sub_rate, s.indel_rate, s.rc_fraction = sub_rate, indel_rate, rc_fraction
+    s.insert_mean, s.insert_sd = insert
+    s.seed = seed
+    n_out = n_reads * (2 if paired else 1)
+    cap = n_out * max(lengths)
+    seqs = np.empty(max(cap, 1), dtype=np.uint8)
+    offsets = np.empty(n_out, dtype=np.uint64)
+    lens = np.empty(n_out, dtype=np.uint32)
+    tc = np.empty(n_out, dtype=np.uint32)
+    tp = np.empty(n_out, dtype=np.uint32)
+    tr = np.empty(n_out, dtype=np.uint8)
+    contig_lens = np.ascontiguousarray(contig_lens, dtype=np.uint64)
+    genome = np.ascontiguousarray(genome, dtype=np.uint8)
+    _check(N.lib().gis_synth_reads(C.byref(s), N.ptr(genome), N.ptr(contig_lens), len(contig_lens),
+                                   N.ptr(seqs), N.ptr(offsets), N.ptr(lens), N.ptr(tc), N.ptr(tp),
+                                   N.ptr(tr)))
+    used = int(offsets[-1] + lens[-1]) if n_out else 0
+    return seqs[:max(used, 1)], offsets, lens, {"contig": tc, "pos": tp, "rc": tr}

@@ -1,0 +1,265 @@
+/*
+ * gis — MI355X-native strobealign seeding path behind a C ABI.
+ *
+ * syncmers -> randstrobes -> index lookup -> NAMs, computed by hand-written
+ * gfx950 kernels. Every entry point returns a gis_status (0 = GIS_OK) and
+ * never throws; gis_last_error() gives the message of the last failure on a
+ * context. Buffers are caller-owned unless stated. All citations are
+ * file:line in the reference repository (yanlifeng/GPU_index_test).
+ *
+ * Threading: a gis_ctx serialises its calls with an internal mutex; use one
+ * context per host thread for concurrency.
+ */
+#ifndef GIS_H
+#define GIS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GIS_ABI_VERSION 1
+#define GIS_DEFAULT ((int32_t)0x80000000) /* IndexParameters::DEFAULT, step2/indexparameters.hpp:78 */
+#define GIS_NOT_FOUND UINT64_MAX           /* StrobemerIndex::end(), step2/index.hpp:149-151 */
+#define GIS_MAX_READ_LEN 65535
+
+typedef enum {
+    GIS_OK = 0,
+    GIS_ERR_BAD_PARAMETER = 1, /* BadParameter, step2/exceptions.hpp:7-10 */
+    GIS_ERR_INVALID_INDEX = 2, /* InvalidIndexFile, step2/exceptions.hpp:17-20 */
+    GIS_ERR_HIP = 3,
+    GIS_ERR_NOMEM = 4,
+    GIS_ERR_CAPACITY = 5,      /* caller buffer too small; the required size is reported */
+    GIS_ERR_STATE = 6,         /* e.g. no index loaded */
+    GIS_ERR_IO = 7,
+    GIS_ERR_BUDGET = 8,        /* request larger than the serving HBM budget */
+    GIS_ERR_CANCELLED = 9,
+    GIS_ERR_INVALID_FASTA = 10 /* InvalidFasta, step2/exceptions.hpp:12-15 */
+} gis_status;
+
+/* ---------------- plain data types (layouts match the reference) ---------------- */
+
+/* IndexParameters, step2/indexparameters.hpp:11-95 (flattened) */
+typedef struct {
+    int32_t canonical_read_length;
+    int32_t k, s, t_syncmer;
+    int32_t l, u;
+    uint64_t q;
+    int32_t max_dist;
+    uint32_t w_min, w_max;
+    int32_t reserved_;
+} gis_params;
+
+/* Syncmer, step2/randstrobes.hpp:125-131 */
+typedef struct { uint64_t hash; uint64_t position; } gis_syncmer;
+
+/* QueryRandstrobe, step2/randstrobes.hpp:96-101 (24 bytes) */
+typedef struct {
+    uint64_t hash;
+    uint32_t start;
+    uint32_t end;
+    uint8_t is_reverse;
+    uint8_t pad_[7];
+} gis_query_randstrobe;
+
+/* RefRandstrobe, step2/randstrobes.hpp:65-94: packed = ref_index << 8 | strobe2 offset */
+typedef struct { uint64_t hash; uint32_t position; uint32_t packed; } gis_ref_randstrobe;
+
+/* Nam, step2/my_struct.hpp:25-69 (44 bytes) */
+typedef struct {
+    int32_t nam_id;
+    int32_t query_start;
+    int32_t query_end;
+    int32_t query_prev_hit_startpos;
+    int32_t ref_start;
+    int32_t ref_end;
+    int32_t ref_prev_hit_startpos;
+    int32_t n_hits;
+    int32_t ref_id;
+    float score;
+    uint8_t is_rc;
+    uint8_t pad_[3];
+} gis_nam;
+
+/* Hit, step2/my_struct.hpp:71-86, plus the reference index and strand it is
+ * grouped under (hits_per_ref[is_revcomp][ref_id], step2/nam.cu:565) */
+typedef struct {
+    int32_t query_start, query_end, ref_start, ref_end;
+    int32_t ref_id;
+    int32_t is_rc;
+} gis_hit;
+
+typedef struct {
+    uint64_t n_randstrobes;
+    int32_t bits;
+    uint32_t filter_cutoff;
+    gis_params params;
+    uint64_t distinct_strobemers; /* IndexCreationStatistics, step2/index.hpp:22-35 */
+    uint64_t tot_occur_once, tot_high_ab, tot_mid_ab, index_cutoff;
+    uint64_t device_bytes;
+} gis_index_info;
+
+/* Per-read result flags */
+enum { GIS_FLAG_RESCUED = 1, GIS_FLAG_NO_RANDSTROBES = 2 };
+
+/* Output of the NAM entry points. The caller provides arrays; nam_offsets has
+ * n_reads + 1 entries and the NAMs of read r are nams[nam_offsets[r] ..
+ * nam_offsets[r+1]) in canonical order (is_rc, ref_id, query_start,
+ * ref_start) with nam_id = rank. If nam_capacity is too small the call
+ * returns GIS_ERR_CAPACITY and sets n_nams to the required count. */
+typedef struct {
+    uint64_t *nam_offsets;
+    gis_nam *nams;
+    uint64_t nam_capacity;
+    float *nonrepetitive_fraction; /* optional, per read */
+    uint8_t *flags;                /* optional, per read */
+    uint64_t n_nams;               /* out */
+} gis_nam_out;
+
+/* Options of the mapping step. rescue_level is -R (step2/cmdline.hpp:56);
+ * rescue_cutoff = R < 100 ? R * filter_cutoff : 100 (step2/main.cu:4204);
+ * hard_cutoff is the count limit of find_nams_rescue (step2/nam.cu:964). */
+typedef struct {
+    int32_t rescue_level;  /* default 2 */
+    uint32_t hard_cutoff;  /* default 1000 */
+    int32_t mode;          /* GIS_MODE_* */
+    int32_t reserved_;
+} gis_map_options;
+
+enum { GIS_MODE_PLAIN = 0, GIS_MODE_FUSED = 1 };
+
+typedef struct gis_ctx gis_ctx;
+
+/* ---------------- parameters ---------------- */
+
+/* IndexParameters::from_read_length, step2/indexparameters.hpp:87-89,
+ * step2/indexparameters.cpp:51-83. Pass GIS_DEFAULT for defaults. */
+int gis_params_from_read_length(int read_length, int k, int s, int l, int u, int c,
+                                int max_seed_len, gis_params *out);
+/* IndexParameters(r, k, s, l, u, q, max_dist), step2/indexparameters.hpp:80-85 */
+int gis_params_make(int canonical_read_length, int k, int s, int l, int u, int q,
+                    int max_dist, gis_params *out);
+void gis_map_options_default(gis_map_options *opt);
+
+/* ---------------- context ---------------- */
+int gis_ctx_create(int device, gis_ctx **out);
+void gis_ctx_destroy(gis_ctx *ctx);
+const char *gis_last_error(const gis_ctx *ctx);
+int gis_device_count(int *count);
+
+/* ---------------- index (StrobemerIndex, step2/index.hpp:37-183) ---------------- */
+
+/* StrobemerIndex::populate(f, n_threads), step2/index.hpp:54 (body
+ * step2/index.cu:143-311). bits = -1 picks bits from the total length
+ * (step2/index.cu:137-141). The index is left resident in HBM. */
+int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint64_t *lens,
+                    uint32_t n_refs, const gis_params *params, int bits, float f,
+                    int n_threads);
+/* Upload a prebuilt table (e.g. from another builder). The table must be
+ * sorted by (hash, position) and starts must have 2^bits + 1 entries. */
+int gis_index_upload(gis_ctx *ctx, const gis_ref_randstrobe *randstrobes, uint64_t n,
+                     const uint64_t *starts, int bits, uint32_t filter_cutoff,
+                     const gis_params *params);
+/* StrobemerIndex::read / write, .sti v2 format, step2/index.cu:73-134. When
+ * expected is non-NULL the parameters in the file must equal it. */
+int gis_index_read_sti(gis_ctx *ctx, const char *path, const gis_params *expected);
+int gis_index_write_sti(gis_ctx *ctx, const char *path);
+int gis_index_info_get(gis_ctx *ctx, gis_index_info *out);
+/* Copy the resident table back (for checks). Either pointer may be NULL. */
+int gis_index_download(gis_ctx *ctx, gis_ref_randstrobe *randstrobes, uint64_t *starts);
+
+/* ---------------- stage entry points (host buffers) ---------------- */
+/* Reads are concatenated in seqs; read r is seqs[offsets[r] .. offsets[r] + lens[r]). */
+
+/* canonical syncmers (SyncmerIterator, step2/randstrobes.cu:18-79) */
+int gis_syncmers_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
+                       const uint32_t *lens, uint32_t n_reads, uint64_t *out_offsets,
+                       gis_syncmer *out, uint64_t capacity, uint64_t *n_out);
+/* randstrobes_query, step2/randstrobes.hpp:107 (body step2/randstrobes.cu:195-241) */
+int gis_randstrobes_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
+                          const uint32_t *lens, uint32_t n_reads, uint64_t *out_offsets,
+                          gis_query_randstrobe *out, uint64_t capacity, uint64_t *n_out);
+/* StrobemerIndex::find over a list of hashes, step2/index.hpp:57-81 (the
+ * microbenchmark kernel gpu_find, gpu_test.cu:186-215). positions[i] is
+ * GIS_NOT_FOUND when absent. */
+int gis_lookup_batch(gis_ctx *ctx, const uint64_t *hashes, uint64_t n, uint64_t *positions);
+/* Hits of each read before NAM merging (hits_per_ref of find_nams,
+ * step2/nam.cu:565-708, or of the rescue path when the read is rescued),
+ * grouped by (is_rc, ref_id) and ordered by (query_start, ref_start). */
+int gis_hits_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
+                   const uint32_t *lens, uint32_t n_reads, const gis_map_options *opt,
+                   uint64_t *out_offsets, gis_hit *out, uint64_t capacity, uint64_t *n_out);
+/* find_nams (+ the caller's rescue decision and find_nams_rescue) for every
+ * read: step2/nam.hpp:11-20, step2/main.cu:3209-3216. */
+int gis_find_nams_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
+                        const uint32_t *lens, uint32_t n_reads, const gis_map_options *opt,
+                        gis_nam_out *out);
+/* Per-query forms over given QueryRandstrobes (one read's worth or many):
+ * query_offsets[n_queries + 1] partitions qrs into reads. */
+int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
+                      const uint64_t *query_offsets, uint32_t n_queries, int rescue,
+                      uint32_t rescue_cutoff, uint32_t hard_cutoff, gis_nam_out *out);
+
+/* ---------------- device-resident batches (inputs already in HBM) ---------------- */
+typedef struct gis_batch gis_batch;
+/* Capacity for up to max_reads reads with max_bases bases in total. */
+int gis_batch_create(gis_ctx *ctx, uint32_t max_reads, uint64_t max_bases, gis_batch **out);
+void gis_batch_destroy(gis_batch *b);
+int gis_batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets,
+                     const uint32_t *lens, uint32_t n_reads);
+/* Run the whole path on the uploaded reads; stays on the device. */
+int gis_batch_run(gis_batch *b, const gis_map_options *opt);
+/* Copy results out (canonical NAM order). */
+int gis_batch_download(gis_batch *b, gis_nam_out *out);
+/* Per-stage kernel times (ms) of the last run, measured with HIP events on the
+ * batch's stream. names is filled with a comma-separated list. */
+int gis_batch_stage_times(gis_batch *b, double *ms, int cap, int *n, char *names, int names_cap);
+/* Algorithmic bytes of the lookup kernel in the last run (see DESIGN). */
+int gis_batch_lookup_bytes(gis_batch *b, uint64_t *bytes, uint64_t *n_lookups);
+/* total NAMs of the last run (available after gis_batch_run, before download) */
+int gis_batch_nam_count(gis_batch *b, uint64_t *n_nams);
+
+/* ---------------- synthetic inputs (deterministic) ---------------- */
+typedef struct {
+    uint64_t total_length;
+    uint32_t n_contigs;
+    double gc;
+    double repeat_fraction;
+    uint32_t n_families;
+    uint32_t family_min_len, family_max_len;
+    uint32_t copies_min, copies_max;
+    double div_min, div_max;
+    double tandem_fraction;
+    uint32_t tandem_unit_min, tandem_unit_max;
+    uint32_t contig_min_len, contig_max_len; /* 0: equal split */
+    uint64_t seed;
+} gis_genome_spec;
+
+typedef struct {
+    uint64_t n_reads;           /* reads (single) or pairs (paired) */
+    int32_t paired;
+    uint32_t lengths[8];        /* length mixture */
+    double weights[8];
+    uint32_t n_lengths;
+    double sub_rate, indel_rate;
+    double rc_fraction;         /* single-end: fraction reverse-complemented */
+    double insert_mean, insert_sd;
+    uint64_t seed;
+} gis_read_spec;
+
+/* Genome: concatenated contigs in seq (capacity total_length), contig lengths
+ * in contig_lens[n_contigs]. */
+int gis_synth_genome(const gis_genome_spec *spec, char *seq, uint64_t *contig_lens);
+/* Reads sampled from the genome. Outputs for n_out = n_reads (x2 if paired)
+ * reads: seqs (capacity n_out * max length), offsets, lens, and the truth
+ * (contig, position, is_rc) of each read. Mates of a pair are adjacent. */
+int gis_synth_reads(const gis_read_spec *spec, const char *genome, const uint64_t *contig_lens,
+                    uint32_t n_contigs, char *seqs, uint64_t *offsets, uint32_t *lens,
+                    uint32_t *truth_contig, uint32_t *truth_pos, uint8_t *truth_rc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

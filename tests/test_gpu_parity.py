@@ -1,0 +1,246 @@
+"""Parity of the HIP path with the CPU oracle on the same seeded inputs.
+
+Every comparison is bit-exact (integer/byte work; the one float field, the
+NAM score, is an int->float conversion and the nonrepetitive fraction a
+float division, both compared bit for bit)."""
+import numpy as np
+import pytest
+
+import gpu_index_test_amd as G
+from oracle import oracle_lib as O
+from tests.helpers import length_reads, mutate, n_reads, rand_seq, revcomp, tie_heavy_reads
+
+pytestmark = pytest.mark.gpu
+
+
+def _genome():
+    # 600 kbp, 3 contigs, repeat families (filtered seeds, rescue) and tandem
+    # repeats (tied syncmer windows)
+    g, lens = G.synth_genome(600_000, n_contigs=3, gc=0.41, repeat_fraction=0.08, n_families=20,
+                             family_len=(300, 3000), copies=(2, 40), divergence=(0.0, 0.02),
+                             tandem_fraction=0.03, tandem_unit=(1, 12), seed=11)
+    return g, lens
+
+
+@pytest.fixture(scope="module")
+def setup(ctx):
+    g, lens = _genome()
+    p = G.IndexParameters.from_read_length(150)
+    idx = G.StrobemerIndex(None, p, ctx=ctx)
+    idx.populate_arrays(g, lens, f=0.0002, n_threads=8)
+    op = O.params_from_read_length(150)
+    oidx = O.Index.build_arrays(g, lens, op, n_threads=8)
+    return g, lens, p, idx, op, oidx
+
+
+def _read_set(g, lens, n=400, seed=5):
+    rng = np.random.default_rng(seed)
+    s, o, l, _ = G.synth_reads(g, lens, n, lengths=(50, 100, 150, 250, 300),
+                               weights=(0.1, 0.2, 0.4, 0.2, 0.1), sub_rate=0.005,
+                               indel_rate=0.0005, seed=seed)
+    reads = [s[int(o[i]): int(o[i]) + int(l[i])].tobytes() for i in range(n)]
+    reads += tie_heavy_reads(rng) + n_reads(rng) + length_reads(rng)
+    # exact copies of tandem-repeat and repeat-family regions of the genome
+    gb = g.tobytes()
+    for st in rng.integers(0, len(gb) - 400, size=60):
+        reads.append(gb[st: st + 150])
+        reads.append(revcomp(gb[st + 7: st + 157]))
+    return reads
+
+
+def test_index_build_matches_oracle(setup):
+    g, lens, p, idx, op, oidx = setup
+    info = idx.info()
+    assert info.bits == oidx.bits
+    assert info.filter_cutoff == oidx.filter_cutoff
+    assert info.n_randstrobes == oidx.n
+    assert idx.randstrobes.tobytes() == oidx.randstrobes.tobytes()
+    assert np.array_equal(idx.randstrobe_start_indices, oidx.starts)
+    assert info.distinct_strobemers == oidx.distinct_strobemers
+
+
+def test_syncmers_parity(setup):
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 200)
+    s, o, l = G.pack_reads(reads)
+    offs, sy = G.syncmers_batch(idx, s, o, l)
+    for r, seq in enumerate(reads):
+        exp = O.syncmers(seq, op)
+        got = sy[int(offs[r]): int(offs[r + 1])]
+        assert got.tobytes() == exp.tobytes(), (r, len(seq), seq[:60])
+
+
+def test_randstrobes_parity(setup):
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 300)
+    s, o, l = G.pack_reads(reads)
+    offs, rs = G.randstrobes_batch(idx, s, o, l)
+    for r, seq in enumerate(reads):
+        exp = O.randstrobes_query(seq, op)
+        got = rs[int(offs[r]): int(offs[r + 1])]
+        assert len(got) == len(exp), (r, len(seq))
+        for f in ("hash", "start", "end", "is_reverse"):
+            assert np.array_equal(got[f], exp[f]), (r, f, len(seq))
+
+
+def test_lookup_parity(setup):
+    g, lens, p, idx, op, oidx = setup
+    rng = np.random.default_rng(3)
+    tab = oidx.randstrobes["hash"]
+    keys = np.concatenate([
+        tab[rng.integers(0, len(tab), 5000)],                 # present
+        tab[:3], tab[-3:],                                    # first hash (never findable) and last
+        rng.integers(0, 2 ** 63, 5000, dtype=np.uint64) * 2 + 1,  # mostly absent
+        tab[rng.integers(0, len(tab), 500)] + 1,              # near misses
+        np.array([0, 2 ** 64 - 1], dtype=np.uint64),
+    ]).astype(np.uint64)
+    got = idx.find_many(keys)
+    exp = np.array([oidx.find(int(k)) for k in keys], dtype=np.uint64)
+    assert np.array_equal(got, exp)
+
+
+def _expected_hits(oidx, qrs, rescue, rescue_cutoff, hard=1000):
+    """Hits of one read from the oracle's find/is_filtered/get_count and its
+    table: the add_to_hits_per_ref walk (step2/nam.cu:45-62)."""
+    rs = oidx.randstrobes
+    k = oidx.params.k
+
+    def walk(pos, qs, qe):
+        out = []
+        key = int(rs["hash"][pos])
+        md = 2 ** 31 - 1
+        x = pos
+        while x < len(rs) and int(rs["hash"][x]) == key:
+            rspan = (int(rs["packed"][x]) & 0xFF) + k
+            d = abs((qe - qs) - rspan)
+            if d <= md:
+                out.append((int(rs["packed"][x]) >> 8, qs, qe, int(rs["position"][x]),
+                            int(rs["position"][x]) + rspan))
+                md = d
+            x += 1
+        return out
+
+    hits = {0: [], 1: []}
+    if not rescue:
+        for q in qrs:
+            pos = oidx.find(int(q["hash"]))
+            if pos == O.NOT_FOUND or oidx.is_filtered(pos):
+                continue
+            hits[int(q["is_reverse"])] += walk(pos, int(q["start"]), int(q["end"]))
+    else:
+        for strand in (0, 1):
+            items = []
+            for q in qrs[qrs["is_reverse"] == strand]:
+                pos = oidx.find(int(q["hash"]))
+                if pos != O.NOT_FOUND:
+                    items.append((oidx.get_count(pos), int(q["start"]), int(q["end"]), pos))
+            items.sort()
+            taken = []
+            for c, qs, qe, pos in items:
+                if (c > rescue_cutoff and len(taken) >= 5) or c > hard:
+                    break
+                taken.append((qs, qe, pos))
+            for qs, qe, pos in sorted(taken):
+                hits[strand] += walk(pos, qs, qe)
+    out = []
+    for strand in (0, 1):
+        # grouped by ref, (query_start, ref_start) order within a ref
+        for ref, qs, qe, rs0, re0 in sorted(hits[strand], key=lambda h: (h[0], h[1], h[3])):
+            out.append((qs, qe, rs0, re0, ref, strand))
+    return out
+
+
+def test_hits_parity(setup):
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 150, seed=9)
+    s, o, l = G.pack_reads(reads)
+    offs, hits = G.hits_batch(idx, s, o, l)
+    rcut = 2 * oidx.filter_cutoff
+    for r, seq in enumerate(reads):
+        _, fr, fl = O.map_read(oidx, seq, rcut)
+        qrs = O.randstrobes_query(seq, op)
+        exp = _expected_hits(oidx, qrs, bool(fl & 1), rcut)
+        got = hits[int(offs[r]): int(offs[r + 1])]
+        got_t = [tuple(int(x) for x in h) for h in got]
+        assert got_t == exp, (r, len(got_t), len(exp))
+
+
+def _compare_nams(got, exp, frac_g, frac_e, flags_g, flags_e):
+    assert np.array_equal(got[0], exp[0]), "nam offsets differ"
+    assert got[1].tobytes() == exp[1].tobytes(), "NAM records differ"
+    assert frac_g.tobytes() == frac_e.tobytes(), "nonrepetitive fractions differ"
+    assert np.array_equal(flags_g & 1, flags_e & 1), "rescue decisions differ"
+
+
+def test_find_nams_parity(setup):
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 1500, seed=21)
+    s, o, l = G.pack_reads(reads)
+    res = G.find_nams_batch(idx, s, o, l)
+    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
+    if not np.array_equal(res.nam_offsets, eo) or res.nams.tobytes() != en.tobytes():
+        for r in range(len(reads)):
+            a = res.read_nams(r)
+            b = en[int(eo[r]): int(eo[r + 1])]
+            assert a.tobytes() == b.tobytes(), (r, len(reads[r]), a, b)
+    _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
+                  res.flags, efl)
+    assert (efl & 1).sum() > 0, "the read set must exercise the rescue path"
+
+
+def test_find_nams_qrs_parity(setup):
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 120, seed=33)
+    rcut = 2 * oidx.filter_cutoff
+    for seq in reads:
+        qrs = O.randstrobes_query(seq, op)
+        fr, nams = G.find_nams(qrs, idx)
+        efr, enams, _ = O.find_nams(oidx, qrs, canonical=True)
+        assert np.float32(fr).tobytes() == np.float32(efr).tobytes()
+        assert nams.tobytes() == enams.tobytes()
+        rn = G.find_nams_rescue(qrs, idx, rcut)
+        ern = O.find_nams_rescue(oidx, qrs, rcut, 1000, canonical=True)
+        assert rn.tobytes() == ern.tobytes()
+
+
+def test_sti_roundtrip(setup, tmp_path):
+    g, lens, p, idx, op, oidx = setup
+    path = tmp_path / "ref.r150.sti"
+    idx.write(path)
+    raw = path.read_bytes()
+    assert raw[:4] == b"STI\x01"
+    assert np.frombuffer(raw[4:8], "<u4")[0] == 2
+    rsv = int(np.frombuffer(raw[8:16], "<u8")[0])
+    o = 16 + rsv
+    cutoff, bits, r, k, s_, l_, u_, q, md = np.frombuffer(raw[o: o + 36], "<i4")
+    assert (cutoff, bits) == (oidx.filter_cutoff, oidx.bits)
+    assert (r, k, s_, l_, u_, q, md) == (150, 20, 16, 1, 7, 255, 80)
+    o += 36
+    n = int(np.frombuffer(raw[o: o + 8], "<u8")[0])
+    o += 8
+    assert raw[o: o + 16 * n] == oidx.randstrobes.tobytes()
+    o += 16 * n
+    m = int(np.frombuffer(raw[o: o + 8], "<u8")[0])
+    assert m == (1 << bits) + 1
+    assert raw[o + 8:] == oidx.starts.tobytes()
+    idx2 = G.StrobemerIndex(None, p, ctx=idx.ctx)
+    idx2.read(path)
+    assert idx2.randstrobes.tobytes() == oidx.randstrobes.tobytes()
+    # and back to the original index for the other tests
+    idx.read(path)
+
+
+def test_c1_scale_parity(ctx):
+    """5 Mbp uniform genome, 20 000 single-end 150 bp reads, 50% reverse
+    complemented, 0.1% substitutions, 0.01% indels."""
+    g, lens = G.synth_genome(5_000_000, n_contigs=1, gc=0.5, seed=1)
+    p = G.IndexParameters.from_read_length(150)
+    idx = G.StrobemerIndex(None, p, ctx=ctx)
+    idx.populate_arrays(g, lens)
+    oidx = O.Index.build_arrays(g, lens, O.params_from_read_length(150), n_threads=8)
+    assert idx.randstrobes.tobytes() == oidx.randstrobes.tobytes()
+    s, o, l, _ = G.synth_reads(g, lens, 20_000, sub_rate=0.001, indel_rate=0.0001, seed=2)
+    res = G.find_nams_batch(idx, s, o, l)
+    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
+    _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
+                  res.flags, efl)
