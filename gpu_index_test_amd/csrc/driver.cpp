@@ -90,7 +90,7 @@ struct gis_batch {
     // device buffers
     DevBuf seqs, read_off, read_len, rs_off, sy_off, long_off, long_scratch;
     DevBuf rs_cnt, rs_nfwd, rs_hash, rs_start, rs_end, sy_cnt, sy_hash, sy_pos;
-    DevBuf pos, hcnt, rcnt, hit_total, rhit_total, nam_cnt, frac, flags;
+    DevBuf pos, lk_bytes, hcnt, rcnt, hit_total, rhit_total, nam_cnt, frac, flags;
     DevBuf hit_off, rhit_off, nam_off, hits, rhits, keys, rkeys, aux, raux;
     DevBuf nam_ws, nam_fin, rnam_ws, rnam_fin, nams_out, scan_tmp, totals;
     // results of the last run
@@ -172,6 +172,7 @@ DevBatch make_devbatch(gis_batch *b) {
     d.sy_pos = static_cast<uint64_t *>(b->sy_pos.p);
     d.pos = static_cast<uint64_t *>(b->pos.p);
     d.hcnt = static_cast<uint32_t *>(b->hcnt.p);
+    d.lk_bytes = static_cast<uint32_t *>(b->lk_bytes.p);
     d.rcnt = static_cast<uint32_t *>(b->rcnt.p);
     d.hit_total = static_cast<uint32_t *>(b->hit_total.p);
     d.frac = static_cast<float *>(b->frac.p);
@@ -208,6 +209,7 @@ void ensure_read_arrays(gis_batch *b, uint32_t n) {
     ensure<uint32_t>(b->rs_nfwd, n);
     ensure<uint32_t>(b->sy_cnt, n);
     ensure<uint32_t>(b->hit_total, n);
+    ensure<uint32_t>(b->lk_bytes, n);
     ensure<uint32_t>(b->rhit_total, n);
     ensure<uint32_t>(b->nam_cnt, n);
     ensure<float>(b->frac, n);
@@ -415,7 +417,7 @@ void free_batch(gis_batch *b) {
     if (!b) return;
     DevBuf *all[] = {&b->seqs, &b->read_off, &b->read_len, &b->rs_off, &b->sy_off, &b->long_off,
                      &b->long_scratch, &b->rs_cnt, &b->rs_nfwd, &b->rs_hash, &b->rs_start,
-                     &b->rs_end, &b->sy_cnt, &b->sy_hash, &b->sy_pos, &b->pos, &b->hcnt, &b->rcnt,
+                     &b->rs_end, &b->sy_cnt, &b->sy_hash, &b->sy_pos, &b->pos, &b->lk_bytes, &b->hcnt, &b->rcnt,
                      &b->hit_total, &b->rhit_total, &b->nam_cnt, &b->frac, &b->flags, &b->hit_off,
                      &b->rhit_off, &b->nam_off, &b->hits, &b->rhits, &b->keys, &b->rkeys, &b->aux,
                      &b->raux, &b->nam_ws, &b->nam_fin, &b->rnam_ws, &b->rnam_fin, &b->nams_out,
@@ -946,7 +948,22 @@ extern "C" int gis_batch_stage_times(gis_batch *b, double *ms, int cap, int *n, 
 
 extern "C" int gis_batch_lookup_bytes(gis_batch *b, uint64_t *bytes, uint64_t *n_lookups) {
     if (!b) return GIS_ERR_BAD_PARAMETER;
-    (void)bytes;
-    (void)n_lookups;
-    return fail(b->ctx, GIS_ERR_STATE, "not implemented");
+    gis_ctx *ctx = b->ctx;
+    API_BEGIN(ctx)
+    if (!b->ran) return fail(ctx, GIS_ERR_STATE, "batch has not been run");
+    const uint32_t n = b->n_reads;
+    std::vector<uint32_t> by(n), cnt(n);
+    if (n) {
+        HIP_CHECK(hipMemcpy(by.data(), b->lk_bytes.p, 4ull * n, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(cnt.data(), b->rs_cnt.p, 4ull * n, hipMemcpyDeviceToHost));
+    }
+    uint64_t tb = 0, tq = 0;
+    for (uint32_t r = 0; r < n; r++) {
+        tb += by[r];
+        tq += cnt[r];
+    }
+    if (bytes) *bytes = tb;
+    if (n_lookups) *n_lookups = tq;
+    return GIS_OK;
+    API_END(ctx)
 }

@@ -51,6 +51,7 @@ struct DevBatch {
     uint64_t *sy_pos;
     // lookup results per randstrobe slot
     uint64_t *pos;            // GIS_NOT_FOUND if absent
+    uint32_t *lk_bytes;       // [n] algorithmic bytes the lookup kernel moved for the read
     uint32_t *hcnt;           // hits this seed contributes (rescued reads: run length)
     uint32_t *rcnt;           // rescued reads: hits this seed contributes
     // per read

@@ -325,25 +325,43 @@ __device__ __forceinline__ uint64_t rec_hash(const DevIndex &ix, uint64_t x) {
 
 // first index of the bucket range holding key, or GIS_NOT_FOUND
 // (StrobemerIndex::find, step2/index.hpp:57-81: linear scan of a small bucket,
-// lower_bound otherwise; both return the first equal entry)
-__device__ __forceinline__ uint64_t dev_find(const DevIndex &ix, uint64_t key) {
+// lower_bound otherwise; both return the first equal entry). *probes counts
+// the table records read.
+__device__ __forceinline__ uint64_t dev_find(const DevIndex &ix, uint64_t key, uint32_t *probes) {
     const uint32_t bk = (uint32_t)(key >> (64 - ix.bits));
     const uint64_t s = ix.starts[bk], e = ix.starts[bk + 1];
-    if (s == e) return GIS_NOT_FOUND;
-    if (e - s < 8) {
+    uint32_t pr = 0;
+    uint64_t res = GIS_NOT_FOUND;
+    if (s == e) {
+    } else if (e - s < 8) {
         for (uint64_t x = s; x < e; x++) {
             const uint64_t h = rec_hash(ix, x);
-            if (h >= key) return h == key ? x : GIS_NOT_FOUND;
+            pr++;
+            if (h >= key) {
+                res = h == key ? x : GIS_NOT_FOUND;
+                break;
+            }
         }
-        return GIS_NOT_FOUND;
+    } else {
+        uint64_t lo = s, cnt = e - s;
+        while (cnt > 0) {
+            const uint64_t step = cnt >> 1, it = lo + step;
+            pr++;
+            if (rec_hash(ix, it) < key) { lo = it + 1; cnt -= step + 1; }
+            else cnt = step;
+        }
+        if (lo < e) {
+            pr++;
+            if (rec_hash(ix, lo) == key) res = lo;
+        }
     }
-    uint64_t lo = s, cnt = e - s;
-    while (cnt > 0) {
-        const uint64_t step = cnt >> 1, it = lo + step;
-        if (rec_hash(ix, it) < key) { lo = it + 1; cnt -= step + 1; }
-        else cnt = step;
-    }
-    return (lo < e && rec_hash(ix, lo) == key) ? lo : GIS_NOT_FOUND;
+    *probes = pr;
+    return res;
+}
+
+__device__ __forceinline__ uint64_t dev_find(const DevIndex &ix, uint64_t key) {
+    uint32_t pr;
+    return dev_find(ix, key, &pr);
 }
 
 // is_filtered: hash[pos] == hash[pos + filter_cutoff], past-the-end reads as ~0
@@ -381,12 +399,14 @@ __device__ __forceinline__ uint32_t dev_get_count(const DevIndex &ix, uint64_t p
 // keep entry e iff |query span - ref span| <= the running minimum. Counts the
 // kept hits, and writes them when out != nullptr.
 __device__ __forceinline__ uint32_t walk_run(const DevIndex &ix, uint64_t pos, uint64_t key,
-                                             int qs, int qe, uint32_t strand_bit, DevHit *out) {
+                                             int qs, int qe, uint32_t strand_bit, DevHit *out,
+                                             uint32_t *records = nullptr) {
     const int qspan = qe - qs;
     int min_diff = 0x7fffffff;
-    uint32_t c = 0;
+    uint32_t c = 0, rec = 0;
     for (uint64_t x = pos; x < ix.n; x++) {
         const uint4 v = ix.rs[x];
+        rec++;
         const uint64_t h = (uint64_t)v.x | ((uint64_t)v.y << 32);
         if (h != key) break;
         const int rspan = (int)(v.w & 0xffu) + ix.k;
@@ -406,6 +426,7 @@ __device__ __forceinline__ uint32_t walk_run(const DevIndex &ix, uint64_t pos, u
             min_diff = d;
         }
     }
+    if (records) *records = rec;
     return c;
 }
 
@@ -421,19 +442,24 @@ __global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, DevIndex ix, in
     const uint32_t n = b.rs_cnt[r];
     const uint64_t base = b.rs_off[r];
     const uint32_t half = b.rs_nfwd[r];
-    uint32_t found = 0, good = 0, hits = 0;
+    uint32_t found = 0, good = 0, hits = 0, bytes = 0;
     for (uint32_t i0 = 0; i0 < n; i0 += 64) {
         const uint32_t i = i0 + lane;
-        uint32_t f = 0, g = 0, hc = 0;
+        uint32_t f = 0, g = 0, hc = 0, by = 0;
         if (i < n) {
             const uint64_t key = b.rs_hash[base + i];
-            const uint64_t pos = dev_find(ix, key);
+            uint32_t probes = 0, records = 0;
+            const uint64_t pos = dev_find(ix, key, &probes);
+            // query in (16) + result out (12) + bucket bounds (16) + search records
+            by = 44 + 16 * probes;
             if (pos != GIS_NOT_FOUND) {
                 f = 1;
+                by += pos + ix.filter_cutoff < ix.n ? 8 : 0;
                 if (!dev_filtered(ix, pos, key)) {
                     g = 1;
                     hc = walk_run(ix, pos, key, (int)b.rs_start[base + i], (int)b.rs_end[base + i],
-                                  i >= half ? 0x80000000u : 0u, nullptr);
+                                  i >= half ? 0x80000000u : 0u, nullptr, &records);
+                    by += 16 * records;
                 }
             }
             b.pos[base + i] = pos;
@@ -442,8 +468,10 @@ __global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, DevIndex ix, in
         found += wave_sum(f);
         good += wave_sum(g);
         hits += wave_sum(hc);
+        bytes += wave_sum(by);
     }
     if (lane == 0) {
+        b.lk_bytes[r] = bytes;
         const float frac = found > 0 ? (float)good / (float)found : 1.0f;
         const bool rescue = rescue_mode == 2 || (rescue_mode == 0 && ((double)frac < 0.7 || hits == 0));
         b.frac[r] = frac;
