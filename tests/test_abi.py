@@ -1,0 +1,87 @@
+"""The C-ABI library loads and exports every entry point include/gis.h
+declares; host-only entry points (parameters, synthetic inputs) behave
+without a GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import gpu_index_test_amd as G
+from gpu_index_test_amd import _native as N
+from oracle import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(gis_\w+)\s*\(", txt, re.M))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 25
+    L = C.CDLL(N.LIB_PATH)
+    for name in sorted(names):
+        assert hasattr(L, name), name
+    # and the Python binding declares a prototype for each
+    assert names == set(N.PROTOTYPES), names ^ set(N.PROTOTYPES)
+
+
+def test_struct_layouts():
+    assert C.sizeof(N.Params) == 48
+    assert C.sizeof(N.NamOut) == 48
+    assert N.NAM.itemsize == 44 and N.QUERY_RANDSTROBE.itemsize == 24
+    assert N.REF_RANDSTROBE.itemsize == 16
+
+
+@pytest.mark.parametrize("r", [1, 50, 90, 91, 110, 111, 135, 136, 150, 175, 176, 250, 275, 276,
+                               300, 375, 376, 1000])
+def test_params_match_oracle(r):
+    a = G.IndexParameters.from_read_length(r).as_tuple()
+    b = O.params_from_read_length(r).as_tuple()
+    assert a == b
+
+
+def test_params_errors_and_overrides():
+    a = G.IndexParameters.from_read_length(150, k=24, s=18, c=4, max_seed_len=100).as_tuple()
+    assert a == O.params_from_read_length(150, k=24, s=18, c=4, max_seed_len=100).as_tuple()
+    for kw in ({"k": 7}, {"k": 33}, {"s": 17}, {"s": 21}, {"max_seed_len": 300}, {"l": 5, "u": 1}):
+        with pytest.raises(G.GisError) as e:
+            G.IndexParameters.from_read_length(150, **kw)
+        assert e.value.code == 1  # GIS_ERR_BAD_PARAMETER
+    p = G.IndexParameters.from_read_length(150)
+    assert p.filename_extension() == ".r150.sti"
+    assert G.IndexParameters.from_read_length(150, k=22, s=18).filename_extension() == ".sti"
+
+
+def test_synth_is_deterministic():
+    g1, l1 = G.synth_genome(200_000, n_contigs=3, gc=0.41, repeat_fraction=0.05, n_families=5,
+                            family_len=(500, 2000), tandem_fraction=0.02, seed=3)
+    g2, l2 = G.synth_genome(200_000, n_contigs=3, gc=0.41, repeat_fraction=0.05, n_families=5,
+                            family_len=(500, 2000), tandem_fraction=0.02, seed=3)
+    assert g1.tobytes() == g2.tobytes() and np.array_equal(l1, l2)
+    assert set(np.unique(g1)) <= set(b"ACGT")
+    gc = np.isin(g1, np.frombuffer(b"GC", np.uint8)).mean()
+    assert 0.38 < gc < 0.44
+    s1 = G.synth_reads(g1, l1, 500, lengths=(50, 150, 300), weights=(1, 2, 1), paired=True, seed=4)
+    s2 = G.synth_reads(g1, l1, 500, lengths=(50, 150, 300), weights=(1, 2, 1), paired=True, seed=4)
+    assert s1[0].tobytes() == s2[0].tobytes()
+    assert len(s1[2]) == 1000 and set(np.unique(s1[2])) <= {50, 150, 300}
+    # mates share a length
+    assert np.array_equal(s1[2][0::2], s1[2][1::2])
+
+
+def test_error_free_synthetic_read_is_a_genome_substring():
+    g, l = G.synth_genome(100_000, seed=9)
+    s, o, ln, t = G.synth_reads(g, l, 50, sub_rate=0.0, indel_rate=0.0, rc_fraction=0.0, seed=1)
+    gb = g.tobytes()
+    for i in range(50):
+        read = s[int(o[i]): int(o[i]) + int(ln[i])].tobytes()
+        assert gb[int(t["pos"][i]): int(t["pos"][i]) + int(ln[i])] == read
