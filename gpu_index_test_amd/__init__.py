@@ -15,14 +15,15 @@ import os
 import numpy as np
 
 from . import _native as N
-from ._native import (FLAG_NO_RANDSTROBES, FLAG_RESCUED, GIS_DEFAULT, GIS_NOT_FOUND, HIT, NAM,
-                      QUERY_RANDSTROBE, REF_RANDSTROBE, SYNCMER, GisError)
+from ._native import (FLAG_NO_RANDSTROBES, FLAG_RESCUED, GIS_DEFAULT, GIS_NOT_FOUND, HIT,
+                      MODE_FUSED, MODE_PLAIN, NAM, QUERY_RANDSTROBE, REF_RANDSTROBE, SYNCMER,
+                      GisError)
 
 __all__ = [
     "IndexParameters", "References", "Context", "StrobemerIndex", "Mapper", "MapResult",
     "randstrobes_query", "find_nams", "find_nams_rescue", "synth_genome", "synth_reads",
     "pack_reads", "GisError", "NAM", "QUERY_RANDSTROBE", "REF_RANDSTROBE", "SYNCMER", "HIT",
-    "GIS_NOT_FOUND", "FLAG_RESCUED", "FLAG_NO_RANDSTROBES",
+    "GIS_NOT_FOUND", "FLAG_RESCUED", "FLAG_NO_RANDSTROBES", "MODE_PLAIN", "MODE_FUSED",
 ]
 
 
@@ -383,12 +384,15 @@ def _call_with_nam_out(ctx, n_reads, fn, cap=None):
         return MapResult(offs, nams[: o.n_nams], frac[:n_reads], flags[:n_reads])
 
 
-def find_nams_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000):
-    """Whole path for a batch of reads (plain per-stage launches)."""
+def find_nams_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000,
+                    mode=N.MODE_PLAIN):
+    """Whole path for a batch of reads: per-stage launches (MODE_PLAIN) or
+    the fused kernel with the per-stage kernels taking its spilled reads
+    (MODE_FUSED)."""
     ctx = index.ctx
     seqs, offsets, lens = _seq_args(seqs, offsets, lens)
     n = len(lens)
-    opt = _map_options(rescue_level, hard_cutoff)
+    opt = _map_options(rescue_level, hard_cutoff, mode)
     return _call_with_nam_out(ctx, n, lambda o: ctx.L.gis_find_nams_batch(
         ctx.h, N.ptr(seqs), N.ptr(offsets), N.ptr(lens), n, C.byref(opt), C.byref(o)))
 

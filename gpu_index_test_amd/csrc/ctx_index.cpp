@@ -1,0 +1,195 @@
+// C ABI: contexts and the resident strobemer index (StrobemerIndex,
+// step2/index.hpp:37-183).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "driver_internal.h"
+#include "index_host.h"
+
+using namespace gis;
+
+namespace {
+
+void upload_index(gis_ctx *ctx, const HostIndex &h) {
+    DevIndexStore &ix = ctx->ix;
+    ix.loaded = false;
+    // drop the previous table first so two tables never coexist in HBM
+    release(ix.rs);
+    release(ix.starts);
+    ensure<gis_ref_randstrobe>(ix.rs, h.rs.size());
+    ensure<uint64_t>(ix.starts, h.starts.size());
+    if (!h.rs.empty())
+        HIP_CHECK(hipMemcpy(ix.rs.p, h.rs.data(), h.rs.size() * sizeof(gis_ref_randstrobe),
+                            hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(ix.starts.p, h.starts.data(), h.starts.size() * 8, hipMemcpyHostToDevice));
+    ix.n = h.rs.size();
+    ix.bits = h.bits;
+    ix.filter_cutoff = h.filter_cutoff;
+    ix.params = h.params;
+    ix.distinct = h.distinct_strobemers;
+    ix.once = h.tot_occur_once;
+    ix.high = h.tot_high_ab;
+    ix.mid = h.tot_mid_ab;
+    ix.index_cutoff = h.index_cutoff;
+    ix.loaded = true;
+}
+
+void download_index(gis_ctx *ctx, HostIndex &h) {
+    const DevIndexStore &ix = ctx->ix;
+    h.rs.resize(ix.n);
+    h.starts.resize((1ULL << ix.bits) + 1);
+    if (ix.n)
+        HIP_CHECK(hipMemcpy(h.rs.data(), ix.rs.p, ix.n * sizeof(gis_ref_randstrobe), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(h.starts.data(), ix.starts.p, h.starts.size() * 8, hipMemcpyDeviceToHost));
+    h.bits = ix.bits;
+    h.filter_cutoff = ix.filter_cutoff;
+    h.params = ix.params;
+}
+
+}  // namespace
+
+extern "C" int gis_device_count(int *count) {
+    if (!count) return GIS_ERR_BAD_PARAMETER;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return GIS_OK;
+}
+
+extern "C" int gis_ctx_create(int device, gis_ctx **out) {
+    if (!out) return GIS_ERR_BAD_PARAMETER;
+    *out = nullptr;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || c <= 0) return GIS_ERR_HIP;
+    if (device < 0 || device >= c) return GIS_ERR_BAD_PARAMETER;
+    if (hipSetDevice(device) != hipSuccess) return GIS_ERR_HIP;
+    gis_ctx *ctx = new gis_ctx();
+    ctx->device = device;
+    // default-priority stream
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return GIS_ERR_HIP;
+    }
+    *out = ctx;
+    return GIS_OK;
+}
+
+extern "C" void gis_ctx_destroy(gis_ctx *ctx) {
+    if (!ctx) return;
+    {
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        batch_free(ctx->scratch);
+        ctx->scratch = nullptr;
+        release(ctx->ix.rs);
+        release(ctx->ix.starts);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+extern "C" const char *gis_last_error(const gis_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint64_t *lens,
+                               uint32_t n_refs, const gis_params *params, int bits, float f,
+                               int n_threads) {
+    API_BEGIN(ctx)
+    if (!params || (n_refs && (!seqs || !lens))) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null argument");
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < n_refs; j++) total += lens[j];
+    if (total == 0) return fail(ctx, GIS_ERR_INVALID_FASTA, "No reference sequences found");
+    HostIndex h;
+    std::string err;
+    const int rc = build_host_index(seqs, lens, n_refs, *params, bits, f, n_threads, h, err);
+    if (rc) return fail(ctx, rc, err);
+    upload_index(ctx, h);
+    return GIS_OK;
+    API_END(ctx)
+}
+
+extern "C" int gis_index_upload(gis_ctx *ctx, const gis_ref_randstrobe *randstrobes, uint64_t n,
+                                const uint64_t *starts, int bits, uint32_t filter_cutoff,
+                                const gis_params *params) {
+    API_BEGIN(ctx)
+    if (!params || !starts || (n && !randstrobes)) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null argument");
+    if (bits < 8 || bits > 31) return fail(ctx, GIS_ERR_BAD_PARAMETER, "Bits must be between 8 and 31");
+    // the lookups and the hit order rely on the table's (hash, position) order
+    for (uint64_t i = 1; i < n; i++) {
+        const auto &a = randstrobes[i - 1], &c = randstrobes[i];
+        if (a.hash > c.hash || (a.hash == c.hash && a.position > c.position))
+            return fail(ctx, GIS_ERR_INVALID_INDEX, "randstrobes not sorted by (hash, position)");
+    }
+    const uint64_t nb = (1ULL << bits) + 1;
+    for (uint64_t i = 0; i < nb; i++)
+        if (starts[i] > n || (i && starts[i] < starts[i - 1]))
+            return fail(ctx, GIS_ERR_INVALID_INDEX, "bucket starts out of range");
+    HostIndex h;
+    h.rs.assign(randstrobes, randstrobes + n);
+    h.starts.assign(starts, starts + nb);
+    h.bits = bits;
+    h.filter_cutoff = filter_cutoff;
+    h.params = *params;
+    upload_index(ctx, h);
+    return GIS_OK;
+    API_END(ctx)
+}
+
+extern "C" int gis_index_read_sti(gis_ctx *ctx, const char *path, const gis_params *expected) {
+    API_BEGIN(ctx)
+    if (!path) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null path");
+    HostIndex h;
+    std::string err;
+    const int rc = read_sti(path, expected, h, err);
+    if (rc) return fail(ctx, rc, err);
+    upload_index(ctx, h);
+    return GIS_OK;
+    API_END(ctx)
+}
+
+extern "C" int gis_index_write_sti(gis_ctx *ctx, const char *path) {
+    API_BEGIN(ctx)
+    if (!path) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null path");
+    if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
+    HostIndex h;
+    download_index(ctx, h);
+    std::string err;
+    const int rc = write_sti(path, h, err);
+    if (rc) return fail(ctx, rc, err);
+    return GIS_OK;
+    API_END(ctx)
+}
+
+extern "C" int gis_index_info_get(gis_ctx *ctx, gis_index_info *out) {
+    API_BEGIN(ctx)
+    if (!out) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null output");
+    if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
+    const DevIndexStore &ix = ctx->ix;
+    out->n_randstrobes = ix.n;
+    out->bits = ix.bits;
+    out->filter_cutoff = ix.filter_cutoff;
+    out->params = ix.params;
+    out->distinct_strobemers = ix.distinct;
+    out->tot_occur_once = ix.once;
+    out->tot_high_ab = ix.high;
+    out->tot_mid_ab = ix.mid;
+    out->index_cutoff = ix.index_cutoff;
+    out->device_bytes = ix.rs.bytes + ix.starts.bytes;
+    return GIS_OK;
+    API_END(ctx)
+}
+
+extern "C" int gis_index_download(gis_ctx *ctx, gis_ref_randstrobe *randstrobes, uint64_t *starts) {
+    API_BEGIN(ctx)
+    if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
+    const DevIndexStore &ix = ctx->ix;
+    if (randstrobes && ix.n)
+        HIP_CHECK(hipMemcpy(randstrobes, ix.rs.p, ix.n * sizeof(gis_ref_randstrobe), hipMemcpyDeviceToHost));
+    if (starts)
+        HIP_CHECK(hipMemcpy(starts, ix.starts.p, ((1ULL << ix.bits) + 1) * 8, hipMemcpyDeviceToHost));
+    return GIS_OK;
+    API_END(ctx)
+}

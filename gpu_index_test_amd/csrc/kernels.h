@@ -30,17 +30,46 @@ struct DevHit {
     uint16_t query_end;
 };
 
-// Per-batch device arrays of the per-stage pipeline.
+// NAM under construction (the fields merge_hits_into_nams updates)
+struct NamWork {
+    int32_t ref_start, ref_end, ref_prev;
+    uint16_t query_start, query_end, query_prev, n_hits;
+};
+
+// Device control block of one batch run: bump allocators with capacity checks
+// and the spill list of the fused kernel. Reset by k_reset_ctl before a run.
+struct DevCtl {
+    unsigned long long hit_cursor;   // units of the normal-path hit pool in use
+    unsigned long long rhit_cursor;  // units of the rescue hit pool in use
+    unsigned long long nam_cursor;   // NAMs of the fused kernel's pool in use
+    unsigned int n_spill;            // reads handed from the fused kernel to the per-stage kernels
+    unsigned int overflow;           // OVF_* bits
+};
+
+enum : unsigned { OVF_HITS = 1, OVF_RHITS = 2, OVF_NAMPOOL = 4, OVF_OUT = 8 };
+
+// the public GIS_FLAG_SPILLED bit (include/gis.h)
+enum : uint8_t { FLAG_SPILL = GIS_FLAG_SPILLED };
+
+// The reads a per-read kernel visits: all of [0, n) (ids == nullptr) or the
+// first *n_dev entries of ids.
+struct ReadSet {
+    const uint32_t *ids;
+    const unsigned int *n_dev;
+    uint32_t n;
+};
+
+// Per-batch device arrays.
 struct DevBatch {
     // reads
     const char *seqs;
     const uint64_t *read_off;
     const uint32_t *read_len;
     uint32_t n_reads;
-    // randstrobes: per-read regions of capacity from the read length
-    const uint64_t *rs_off;   // [n+1] capacity offsets
-    uint32_t *rs_cnt;         // [n]
-    uint32_t *rs_nfwd;        // [n] forward-strand randstrobes (they come first)
+    // randstrobes: per-read regions, capacity from the read length
+    const uint64_t *rs_off;
+    uint32_t *rs_cnt;
+    uint32_t *rs_nfwd;        // forward-strand randstrobes (they come first)
     uint64_t *rs_hash;
     uint32_t *rs_start;
     uint32_t *rs_end;
@@ -51,61 +80,63 @@ struct DevBatch {
     uint64_t *sy_pos;
     // lookup results per randstrobe slot
     uint64_t *pos;            // GIS_NOT_FOUND if absent
-    uint32_t *lk_bytes;       // [n] algorithmic bytes the lookup kernel moved for the read
     uint32_t *hcnt;           // hits this seed contributes (rescued reads: run length)
     uint32_t *rcnt;           // rescued reads: hits this seed contributes
+    uint32_t *lk_bytes;       // [n] algorithmic bytes the lookup kernel moved for the read
     // per read
-    uint32_t *hit_total;      // normal path hits
+    uint32_t *hit_total;
+    uint32_t *rhit_total;
     float *frac;
     uint8_t *flags;
-    uint32_t *rhit_total;     // rescue hits
-    // hits
-    const uint64_t *hit_off;  // [n+1]
+    uint32_t *nam_cnt;
+    uint64_t *hit_off;        // bump-allocated region of the read's hits (units)
+    uint64_t *rhit_off;
+    uint64_t *nam_src;        // fused kernel: offset of the read's NAMs in nam_pool
+    // hit pools: a read with h hits takes h + 1 units; keys and aux take 2 per unit
     DevHit *hits;
-    const uint64_t *rhit_off; // [n+1]
     DevHit *rhits;
-    // merge scratch (indexed like the hit arrays)
-    uint64_t *keys;           // 2 per hit
-    uint32_t *aux;            // 2 per hit (group starts / open list)
-    gis_nam *nam_ws;          // working NAMs
-    gis_nam *nam_fin;         // finished NAMs in canonical order
-    gis_nam *rnam_ws;
-    gis_nam *rnam_fin;
-    uint64_t *rkeys;
-    uint32_t *raux;
-    uint32_t *nam_cnt;        // per read
-    // output
-    const uint64_t *nam_off;  // [n+1]
+    uint64_t hit_cap, rhit_cap;
+    uint64_t *keys, *rkeys;
+    uint32_t *aux, *raux;
+    NamWork *nam_ws, *rnam_ws;
+    gis_nam *nam_fin, *rnam_fin;
+    // fused kernel NAM pool
+    gis_nam *nam_pool;
+    uint64_t nam_pool_cap;
+    // dense output
+    uint64_t *nam_off;        // [n+1]
     gis_nam *nams_out;
-    // long-read scratch for the seeding stage
-    const uint64_t *long_off; // [n+1] per-read scratch offsets in bases (0 for short reads)
+    uint64_t out_cap;
+    // long-read scratch for the seeding stage (byte offsets)
+    const uint64_t *long_off;
     uint8_t *long_scratch;
+    DevCtl *ctl;
+    uint32_t *spill_ids;
 };
 
 constexpr int LMAX_LDS = 320;  // reads up to this length seed from LDS
 constexpr int SEED_WPB = 4;    // waves per block of the seeding kernel
 constexpr int WPB = 4;         // waves per block of the per-read kernels
-// bytes of long-read scratch per base: codes(1) + state(1) + s-mer hash(8)
-// + syncmer hash(8) + syncmer pos(4) + N mask (1/8 per base, rounded up)
-constexpr int LONG_BYTES_PER_BASE = 24;
+constexpr int MERGE_LDS = 128; // hits a read may have for the LDS merge
 
 // launch wrappers (kernels.hip)
-void launch_seed(const DevBatch &b, const SeedParams &sp, int want_syncmers, int long_reads,
-                 hipStream_t st);
+void launch_reset_ctl(DevCtl *ctl, hipStream_t st);
+void launch_seed(const DevBatch &b, const ReadSet &set, const SeedParams &sp, int want_syncmers,
+                 int long_reads, hipStream_t st);
 // rescue_mode: 0 = the caller's decision (nonrepetitive fraction < 0.7 or no
 // hits), 1 = never rescue, 2 = always rescue
-void launch_lookup(const DevBatch &b, const DevIndex &ix, int rescue_mode, hipStream_t st);
-void launch_hits(const DevBatch &b, const DevIndex &ix, hipStream_t st);
-void launch_merge(const DevBatch &b, int rescue, int k, hipStream_t st);
-void launch_rescue_count(const DevBatch &b, const DevIndex &ix, uint32_t rescue_cutoff,
-                         uint32_t hard_cutoff, hipStream_t st);
-void launch_rescue_hits(const DevBatch &b, const DevIndex &ix, uint32_t rescue_cutoff,
-                        uint32_t hard_cutoff, hipStream_t st);
-void launch_compact(const DevBatch &b, hipStream_t st);
+void launch_lookup(const DevBatch &b, const ReadSet &set, const DevIndex &ix, int rescue_mode,
+                   hipStream_t st);
+void launch_hits(const DevBatch &b, const ReadSet &set, const DevIndex &ix, hipStream_t st);
+void launch_rescue_count(const DevBatch &b, const ReadSet &set, const DevIndex &ix,
+                         uint32_t rescue_cutoff, uint32_t hard_cutoff, hipStream_t st);
+void launch_rescue_hits(const DevBatch &b, const ReadSet &set, const DevIndex &ix, hipStream_t st);
+void launch_merge(const DevBatch &b, const ReadSet &set, int rescue, hipStream_t st);
+void launch_fused(const DevBatch &b, const DevIndex &ix, const SeedParams &sp,
+                  uint32_t rescue_cutoff, uint32_t hard_cutoff, hipStream_t st);
+void launch_compact(const DevBatch &b, int fused, hipStream_t st);
 void launch_find_flat(const DevIndex &ix, const uint64_t *hashes, uint64_t n, uint64_t *positions,
                       hipStream_t st);
-void launch_rs_capacity(const uint32_t *lens, uint32_t n, const SeedParams &sp, uint32_t *cap_rs,
-                        uint32_t *cap_sy, hipStream_t st);
 // exclusive scan of u32 counts into u64 offsets[n+1] (offsets[n] = total);
 // tmp must hold scan_tmp_words(n) u64.
 size_t scan_tmp_words(uint64_t n);

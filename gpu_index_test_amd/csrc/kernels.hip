@@ -1,10 +1,18 @@
 // gfx950 kernels of the strobealign seeding path.
 //
 // One wavefront (64 lanes) owns one read in every per-read kernel. Lanes
-// work on positions, windows, seeds or hits of that read in parallel; the
-// wave synchronises through LDS (seeding) or through its own global scratch
-// regions (hits, merge). Reads are independent, so there is no inter-wave
-// communication at all.
+// work on positions, windows, seeds or hits of that read in parallel and the
+// wave synchronises through LDS or through its own scratch regions. Reads are
+// independent: waves only meet in three atomic bump allocators and the spill
+// list, whose order never changes a read's result.
+//
+// Two forms share the device functions below:
+//  - per-stage: k_seed -> k_lookup -> k_hits -> k_rescue_count ->
+//    k_rescue_hits -> k_merge (normal, rescue), one launch per stage;
+//  - fused: k_fused runs the whole path of a read in one wave out of LDS and
+//    hands reads that do not fit its LDS budget (long reads, many seeds,
+//    many hits) to the per-stage kernels through the spill list.
+// Both end in a scan of the per-read NAM counts and k_compact.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -21,8 +29,8 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// wave-local ordering of global-memory scratch written by other lanes of the
-// same wave (all lanes of a workgroup share one CU's L1)
+// ordering of scratch memory (LDS or global) written by other lanes of the
+// same wave; a workgroup's waves share one CU's L1
 __device__ __forceinline__ void wave_sync_mem() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -42,16 +50,8 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 
 __device__ __forceinline__ uint32_t wave_max(uint32_t x) {
     for (int d = 32; d >= 1; d >>= 1) {
-        uint32_t y = __shfl_xor(x, d, 64);
+        const uint32_t y = __shfl_xor(x, d, 64);
         x = x > y ? x : y;
-    }
-    return x;
-}
-
-__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
-    for (int d = 32; d >= 1; d >>= 1) {
-        uint32_t y = __shfl_xor(x, d, 64);
-        x = x < y ? x : y;
     }
     return x;
 }
@@ -60,11 +60,15 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
     const uint32_t lane = lane_id();
     uint32_t x = v;
     for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
+        const uint32_t y = __shfl_up(x, d, 64);
         if (lane >= (uint32_t)d) x += y;
     }
     return x - v;
 }
+
+// wave-uniform iteration over a ReadSet
+__device__ __forceinline__ uint32_t set_count(const ReadSet &s) { return s.ids ? *s.n_dev : s.n; }
+__device__ __forceinline__ uint32_t set_at(const ReadSet &s, uint32_t w) { return s.ids ? s.ids[w] : w; }
 
 // ---------------------------------------------------------------------------
 // seeding: canonical syncmers and query randstrobes of one read
@@ -90,18 +94,17 @@ __device__ __forceinline__ bool any_n(const uint64_t *nm, int a, int b) {
     return (m1 | m2) != 0;
 }
 
-// The whole wave calls this for the same read. Outputs: optional syncmer
-// list (out_sy_*), randstrobes (forward then reverse complement, as
-// randstrobes_query orders them, step2/randstrobes.cu:195-241).
-__device__ __forceinline__ void seed_one(const char *__restrict__ seq, int L, const SeedParams &P,
-                                         const SeedBufs &B, uint64_t *out_sy_hash,
-                                         uint64_t *out_sy_pos, uint64_t *rs_hash,
-                                         uint32_t *rs_start, uint32_t *rs_end,
-                                         uint32_t *n_sync_out, uint32_t *n_rs_out) {
+// Canonical syncmers of one read (SyncmerIterator, step2/randstrobes.cu:18-79),
+// the whole wave working on the same read. The first `cap` syncmers go to
+// B.sy_hash/B.sy_pos (and out_sy_* when given); returns the total count.
+__device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, int L,
+                                                  const SeedParams &P, const SeedBufs &B,
+                                                  uint32_t cap, uint64_t *out_sy_hash,
+                                                  uint64_t *out_sy_pos) {
     const int lane = (int)lane_id();
     const int k = P.k, s = P.s, w = P.w;
 
-    // 1. base codes and the N mask (step2/randstrobes.cu:21-22, 70-76)
+    // base codes and the N mask (step2/randstrobes.cu:21-22, 70-76)
     for (int b0 = 0; b0 < L; b0 += 64) {
         const int i = b0 + lane;
         int c = 4;
@@ -114,7 +117,7 @@ __device__ __forceinline__ void seed_one(const char *__restrict__ seq, int L, co
     }
     wave_sync();
 
-    // 2. canonical s-mer hashes, one s-mer per lane (step2/randstrobes.cu:23-32)
+    // canonical s-mer hashes, one s-mer per lane (step2/randstrobes.cu:23-32)
     const int ns = L - s + 1;
     for (int j0 = 0; j0 < ns; j0 += 64) {
         const int j = j0 + lane;
@@ -134,7 +137,7 @@ __device__ __forceinline__ void seed_one(const char *__restrict__ seq, int L, co
     }
     wave_sync();
 
-    // 3a. window states that need no history
+    // window states that need no history
     const int nk = L - k + 1;
     for (int p0 = 0; p0 < nk; p0 += 64) {
         const int p = p0 + lane;
@@ -151,8 +154,8 @@ __device__ __forceinline__ void seed_one(const char *__restrict__ seq, int L, co
     }
     wave_sync();
 
-    // 3b. resolve tied windows from the nearest anchor and compact syncmers;
-    // a window is a syncmer when its minimum sits at offset t-1
+    // resolve tied windows from the nearest anchor and compact syncmers; a
+    // window is a syncmer when its minimum sits at offset t-1
     // (step2/randstrobes.cu:64-69)
     uint32_t nsync = 0;
     for (int p0 = 0; p0 < nk; p0 += 64) {
@@ -183,8 +186,10 @@ __device__ __forceinline__ void seed_one(const char *__restrict__ seq, int L, co
             }
             f &= P.kmask;
             const uint64_t hk = xxh64_u64(f < r ? f : r);
-            B.sy_hash[idx] = hk;
-            B.sy_pos[idx] = (uint32_t)p;
+            if (idx < cap) {
+                B.sy_hash[idx] = hk;
+                B.sy_pos[idx] = (uint32_t)p;
+            }
             if (out_sy_hash) {
                 out_sy_hash[idx] = hk;
                 out_sy_pos[idx] = (uint64_t)p;
@@ -193,52 +198,80 @@ __device__ __forceinline__ void seed_one(const char *__restrict__ seq, int L, co
         nsync += (uint32_t)__popcll(bal);
     }
     wave_sync();
+    return nsync;
+}
 
-    // 4. randstrobes (RandstrobeIterator::get, step2/randstrobes.cu:136-159);
-    // the reverse-complement pass walks the forward syncmers backwards with
-    // mirrored positions L - pos - k (step2/randstrobes.cu:221-239)
-    const uint32_t n = nsync;
+// Forward and reverse-complement randstrobe number i of a read with n
+// syncmers (RandstrobeIterator::get, step2/randstrobes.cu:136-159); the
+// reverse-complement pass walks the forward syncmers backwards with mirrored
+// positions L - pos - k (step2/randstrobes.cu:221-239).
+struct RsPair {
+    uint64_t fh, rh;
+    uint32_t fs, fe, rs, re;
+};
+
+__device__ __forceinline__ RsPair randstrobe_pair(const SeedBufs &B, uint32_t n, uint32_t i, int L,
+                                                  const SeedParams &P) {
+    RsPair o;
+    const uint32_t k = (uint32_t)P.k;
+    const uint32_t wend = (i + P.w_max < n - 1) ? i + P.w_max : n - 1;
+    {
+        const uint64_t h1 = B.sy_hash[i];
+        const uint32_t p1 = B.sy_pos[i];
+        const uint64_t maxp = (uint64_t)p1 + P.max_dist_u;
+        uint32_t best = i;
+        uint64_t minv = UINT64_MAX;
+        for (uint32_t j = i + P.w_min; j <= wend && (uint64_t)B.sy_pos[j] <= maxp; j++) {
+            const uint64_t c = (uint64_t)__popcll((h1 ^ B.sy_hash[j]) & P.q);
+            if (c < minv) { minv = c; best = j; }
+        }
+        o.fh = h1 + B.sy_hash[best];
+        o.fs = p1;
+        o.fe = B.sy_pos[best] + k;
+    }
+    {
+        const uint32_t a = n - 1 - i;
+        const uint64_t h1 = B.sy_hash[a];
+        const uint32_t p1 = (uint32_t)L - B.sy_pos[a] - k;
+        const uint64_t maxp = (uint64_t)p1 + P.max_dist_u;
+        uint32_t best = a;
+        uint64_t minv = UINT64_MAX;
+        for (uint32_t jj = i + P.w_min; jj <= wend; jj++) {
+            const uint32_t bidx = n - 1 - jj;
+            const uint32_t pb = (uint32_t)L - B.sy_pos[bidx] - k;
+            if ((uint64_t)pb > maxp) break;
+            const uint64_t c = (uint64_t)__popcll((h1 ^ B.sy_hash[bidx]) & P.q);
+            if (c < minv) { minv = c; best = bidx; }
+        }
+        o.rh = h1 + B.sy_hash[best];
+        o.rs = p1;
+        o.re = (uint32_t)L - B.sy_pos[best];
+    }
+    return o;
+}
+
+// randstrobes_query order: forward (strobe1 ascending) then reverse complement
+__device__ __forceinline__ uint32_t seed_one(const char *__restrict__ seq, int L, const SeedParams &P,
+                                             const SeedBufs &B, uint64_t *out_sy_hash,
+                                             uint64_t *out_sy_pos, uint64_t *rs_hash,
+                                             uint32_t *rs_start, uint32_t *rs_end,
+                                             uint32_t *n_sync_out) {
+    const uint32_t n = seed_syncmers(seq, L, P, B, 0xffffffffu, out_sy_hash, out_sy_pos);
     const uint32_t n_fwd = ((uint32_t)L >= P.w_max && n > P.w_min) ? n - P.w_min : 0;
     for (uint32_t i0 = 0; i0 < n_fwd; i0 += 64) {
-        const uint32_t i = i0 + (uint32_t)lane;
+        const uint32_t i = i0 + lane_id();
         if (i < n_fwd) {
-            const uint32_t wend = (i + P.w_max < n - 1) ? i + P.w_max : n - 1;
-            {
-                const uint64_t h1 = B.sy_hash[i];
-                const uint32_t p1 = B.sy_pos[i];
-                const uint64_t maxp = (uint64_t)p1 + P.max_dist_u;
-                uint32_t best = i;
-                uint64_t minv = UINT64_MAX;
-                for (uint32_t j = i + P.w_min; j <= wend && (uint64_t)B.sy_pos[j] <= maxp; j++) {
-                    const uint64_t c = (uint64_t)__popcll((h1 ^ B.sy_hash[j]) & P.q);
-                    if (c < minv) { minv = c; best = j; }
-                }
-                rs_hash[i] = h1 + B.sy_hash[best];
-                rs_start[i] = p1;
-                rs_end[i] = B.sy_pos[best] + (uint32_t)k;
-            }
-            {
-                const uint32_t a = n - 1 - i;
-                const uint64_t h1 = B.sy_hash[a];
-                const uint32_t p1 = (uint32_t)L - B.sy_pos[a] - (uint32_t)k;
-                const uint64_t maxp = (uint64_t)p1 + P.max_dist_u;
-                uint32_t best = a;
-                uint64_t minv = UINT64_MAX;
-                for (uint32_t jj = i + P.w_min; jj <= wend; jj++) {
-                    const uint32_t bidx = n - 1 - jj;
-                    const uint32_t pb = (uint32_t)L - B.sy_pos[bidx] - (uint32_t)k;
-                    if ((uint64_t)pb > maxp) break;
-                    const uint64_t c = (uint64_t)__popcll((h1 ^ B.sy_hash[bidx]) & P.q);
-                    if (c < minv) { minv = c; best = bidx; }
-                }
-                rs_hash[n_fwd + i] = h1 + B.sy_hash[best];
-                rs_start[n_fwd + i] = p1;
-                rs_end[n_fwd + i] = (uint32_t)L - B.sy_pos[best];
-            }
+            const RsPair o = randstrobe_pair(B, n, i, L, P);
+            rs_hash[i] = o.fh;
+            rs_start[i] = o.fs;
+            rs_end[i] = o.fe;
+            rs_hash[n_fwd + i] = o.rh;
+            rs_start[n_fwd + i] = o.rs;
+            rs_end[n_fwd + i] = o.re;
         }
     }
     *n_sync_out = n;
-    *n_rs_out = 2 * n_fwd;
+    return 2 * n_fwd;
 }
 
 struct alignas(8) SeedLds {
@@ -250,70 +283,50 @@ struct alignas(8) SeedLds {
     uint8_t st[LMAX_LDS];
 };
 
-__device__ __forceinline__ void store_seed_out(const DevBatch &b, uint32_t r, int want_sy,
-                                               uint32_t nsync, uint32_t nrs) {
-    if (lane_id() == 0) {
-        b.rs_cnt[r] = nrs;
-        b.rs_nfwd[r] = nrs / 2;
-        if (want_sy) b.sy_cnt[r] = nsync;
-    }
-}
-
-__global__ __launch_bounds__(64 * SEED_WPB) void k_seed(DevBatch b, SeedParams P, int want_sy) {
-    __shared__ SeedLds lds[SEED_WPB];
-    const int wid = (int)(threadIdx.x >> 6);
-    const uint32_t r = blockIdx.x * SEED_WPB + (uint32_t)wid;
-    if (r >= b.n_reads) return;
-    const int L = (int)b.read_len[r];
-    if (L > LMAX_LDS) return;  // long reads go through k_seed_long
-    SeedLds &S = lds[wid];
-    SeedBufs B{S.codes, S.st, S.sh, S.sy_hash, S.sy_pos, S.nmask};
-    uint64_t *osh = want_sy ? b.sy_hash + b.sy_off[r] : nullptr;
-    uint64_t *osp = want_sy ? b.sy_pos + b.sy_off[r] : nullptr;
-    const uint64_t ro = b.rs_off[r];
-    uint32_t nsync = 0, nrs = 0;
-    seed_one(b.seqs + b.read_off[r], L, P, B, osh, osp, b.rs_hash + ro, b.rs_start + ro,
-             b.rs_end + ro, &nsync, &nrs);
-    store_seed_out(b, r, want_sy, nsync, nrs);
-}
-
-// long reads: the same wave algorithm with its buffers in global scratch
 __host__ __device__ inline uint64_t align8(uint64_t x) { return (x + 7) & ~7ULL; }
 
-__global__ __launch_bounds__(64 * SEED_WPB) void k_seed_long(DevBatch b, SeedParams P, int want_sy) {
-    const int wid = (int)(threadIdx.x >> 6);
-    const uint32_t r = blockIdx.x * SEED_WPB + (uint32_t)wid;
-    if (r >= b.n_reads) return;
-    const int L = (int)b.read_len[r];
-    if (L <= LMAX_LDS) return;
-    uint8_t *base = b.long_scratch + b.long_off[r];
-    const uint64_t Lu = (uint64_t)L;
+// global scratch layout of a long read (long_bytes() on the host matches it)
+__device__ __forceinline__ SeedBufs long_bufs(uint8_t *base, uint64_t L) {
     SeedBufs B;
     B.sh = (uint64_t *)base;
-    B.sy_hash = (uint64_t *)(base + 8 * Lu);
-    B.nmask = (uint64_t *)(base + 16 * Lu);
-    B.sy_pos = (uint32_t *)(base + 16 * Lu + 8 * (Lu / 64 + 2));
-    B.codes = base + 16 * Lu + 8 * (Lu / 64 + 2) + align8(4 * Lu);
-    B.st = B.codes + align8(Lu);
-    uint64_t *osh = want_sy ? b.sy_hash + b.sy_off[r] : nullptr;
-    uint64_t *osp = want_sy ? b.sy_pos + b.sy_off[r] : nullptr;
-    const uint64_t ro = b.rs_off[r];
-    uint32_t nsync = 0, nrs = 0;
-    seed_one(b.seqs + b.read_off[r], L, P, B, osh, osp, b.rs_hash + ro, b.rs_start + ro,
-             b.rs_end + ro, &nsync, &nrs);
-    store_seed_out(b, r, want_sy, nsync, nrs);
+    B.sy_hash = (uint64_t *)(base + 8 * L);
+    B.nmask = (uint64_t *)(base + 16 * L);
+    B.sy_pos = (uint32_t *)(base + 16 * L + 8 * (L / 64 + 2));
+    B.codes = base + 16 * L + 8 * (L / 64 + 2) + align8(4 * L);
+    B.st = B.codes + align8(L);
+    return B;
 }
 
-// per-read capacities: syncmers <= L-k+1, randstrobes <= 2 (L-k+1-w_min)
-__global__ void k_capacity(const uint32_t *lens, uint32_t n, SeedParams P, uint32_t *cap_rs,
-                           uint32_t *cap_sy) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int L = (int)lens[r];
-    const int nk = L - P.k + 1;
-    const int nf = nk - (int)P.w_min;
-    cap_sy[r] = nk > 0 ? (uint32_t)nk : 0u;
-    cap_rs[r] = (nf > 0 && (uint32_t)L >= P.w_max) ? 2u * (uint32_t)nf : 0u;
+template <bool LONG>
+__global__ __launch_bounds__(64 * SEED_WPB) void k_seed(DevBatch b, ReadSet set, SeedParams P,
+                                                        int want_sy) {
+    __shared__ SeedLds lds[LONG ? 1 : SEED_WPB];
+    const int wid = (int)(threadIdx.x >> 6);
+    const uint32_t cnt = set_count(set);
+    for (uint32_t w = blockIdx.x * SEED_WPB + (uint32_t)wid; w < cnt; w += gridDim.x * SEED_WPB) {
+        const uint32_t r = set_at(set, w);
+        const int L = (int)b.read_len[r];
+        if ((L > LMAX_LDS) != LONG) continue;
+        SeedBufs B;
+        if (LONG) {
+            B = long_bufs(b.long_scratch + b.long_off[r], (uint64_t)L);
+        } else {
+            SeedLds &S = lds[wid];
+            B = SeedBufs{S.codes, S.st, S.sh, S.sy_hash, S.sy_pos, S.nmask};
+        }
+        uint64_t *osh = want_sy ? b.sy_hash + b.sy_off[r] : nullptr;
+        uint64_t *osp = want_sy ? b.sy_pos + b.sy_off[r] : nullptr;
+        const uint64_t ro = b.rs_off[r];
+        uint32_t nsync = 0;
+        const uint32_t nrs = seed_one(b.seqs + b.read_off[r], L, P, B, osh, osp, b.rs_hash + ro,
+                                      b.rs_start + ro, b.rs_end + ro, &nsync);
+        if (lane_id() == 0) {
+            b.rs_cnt[r] = nrs;
+            b.rs_nfwd[r] = nrs / 2;
+            if (want_sy) b.sy_cnt[r] = nsync;
+        }
+        wave_sync_mem();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -431,162 +444,194 @@ __device__ __forceinline__ uint32_t walk_run(const DevIndex &ix, uint64_t pos, u
 }
 
 // ---------------------------------------------------------------------------
-// lookup: find + filter + hit count per randstrobe; nonrepetitive fraction
-// and the rescue decision per read (find_nams, step2/nam.cu:553-732, and the
-// caller's test, step2/main.cu:3209-3216)
+// per-stage: lookup + filter + hit count per randstrobe; nonrepetitive
+// fraction and the rescue decision per read (find_nams, step2/nam.cu:553-732,
+// and the caller's test, step2/main.cu:3209-3216); the read's hit region is
+// taken from the hit pool.
 
-__global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, DevIndex ix, int rescue_mode) {
-    const uint32_t r = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (r >= b.n_reads) return;
+__global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, ReadSet set, DevIndex ix,
+                                                     int rescue_mode, int keep_spill) {
     const uint32_t lane = lane_id();
-    const uint32_t n = b.rs_cnt[r];
-    const uint64_t base = b.rs_off[r];
-    const uint32_t half = b.rs_nfwd[r];
-    uint32_t found = 0, good = 0, hits = 0, bytes = 0;
-    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        uint32_t f = 0, g = 0, hc = 0, by = 0;
-        if (i < n) {
-            const uint64_t key = b.rs_hash[base + i];
-            uint32_t probes = 0, records = 0;
-            const uint64_t pos = dev_find(ix, key, &probes);
-            // query in (16) + result out (12) + bucket bounds (16) + search records
-            by = 44 + 16 * probes;
-            if (pos != GIS_NOT_FOUND) {
-                f = 1;
-                by += pos + ix.filter_cutoff < ix.n ? 8 : 0;
-                if (!dev_filtered(ix, pos, key)) {
-                    g = 1;
-                    hc = walk_run(ix, pos, key, (int)b.rs_start[base + i], (int)b.rs_end[base + i],
-                                  i >= half ? 0x80000000u : 0u, nullptr, &records);
-                    by += 16 * records;
+    const uint32_t cnt = set_count(set);
+    for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
+        const uint32_t r = set_at(set, w);
+        const uint32_t n = b.rs_cnt[r];
+        const uint64_t base = b.rs_off[r];
+        const uint32_t half = b.rs_nfwd[r];
+        uint32_t found = 0, good = 0, hits = 0, bytes = 0;
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            uint32_t f = 0, g = 0, hc = 0, by = 0;
+            if (i < n) {
+                const uint64_t key = b.rs_hash[base + i];
+                uint32_t probes = 0, records = 0;
+                const uint64_t pos = dev_find(ix, key, &probes);
+                // query in (16) + result out (12) + bucket bounds (16) + search records
+                by = 44 + 16 * probes;
+                if (pos != GIS_NOT_FOUND) {
+                    f = 1;
+                    by += pos + ix.filter_cutoff < ix.n ? 8 : 0;
+                    if (!dev_filtered(ix, pos, key)) {
+                        g = 1;
+                        hc = walk_run(ix, pos, key, (int)b.rs_start[base + i], (int)b.rs_end[base + i],
+                                      i >= half ? 0x80000000u : 0u, nullptr, &records);
+                        by += 16 * records;
+                    }
+                }
+                b.pos[base + i] = pos;
+                b.hcnt[base + i] = hc;
+            }
+            found += wave_sum(f);
+            good += wave_sum(g);
+            hits += wave_sum(hc);
+            bytes += wave_sum(by);
+        }
+        if (lane == 0) {
+            const float frac = found > 0 ? (float)good / (float)found : 1.0f;
+            const bool rescue =
+                rescue_mode == 2 || (rescue_mode == 0 && ((double)frac < 0.7 || hits == 0));
+            b.lk_bytes[r] = bytes;
+            b.frac[r] = frac;
+            const uint8_t keep = keep_spill ? (uint8_t)(b.flags[r] & FLAG_SPILL) : (uint8_t)0;
+            b.flags[r] = (uint8_t)(keep | (rescue ? GIS_FLAG_RESCUED : 0) |
+                                   (n == 0 ? GIS_FLAG_NO_RANDSTROBES : 0));
+            uint32_t keep_hits = rescue ? 0u : hits;  // rescued reads discard these hits
+            uint64_t off = 0;
+            if (keep_hits) {
+                off = atomicAdd(&b.ctl->hit_cursor, (unsigned long long)keep_hits + 1);
+                if (off + keep_hits + 1 > b.hit_cap) {
+                    atomicOr(&b.ctl->overflow, OVF_HITS);
+                    keep_hits = 0;
                 }
             }
-            b.pos[base + i] = pos;
-            b.hcnt[base + i] = hc;
+            b.hit_total[r] = keep_hits;
+            b.hit_off[r] = off;
+            b.rhit_total[r] = 0;
+            b.rhit_off[r] = 0;
         }
-        found += wave_sum(f);
-        good += wave_sum(g);
-        hits += wave_sum(hc);
-        bytes += wave_sum(by);
-    }
-    if (lane == 0) {
-        b.lk_bytes[r] = bytes;
-        const float frac = found > 0 ? (float)good / (float)found : 1.0f;
-        const bool rescue = rescue_mode == 2 || (rescue_mode == 0 && ((double)frac < 0.7 || hits == 0));
-        b.frac[r] = frac;
-        b.flags[r] = (uint8_t)((rescue ? GIS_FLAG_RESCUED : 0) | (n == 0 ? GIS_FLAG_NO_RANDSTROBES : 0));
-        b.hit_total[r] = rescue ? 0u : hits;  // rescued reads discard these hits
     }
 }
 
-// write the hits of non-rescued reads at hit_off (same walk as k_lookup)
-__global__ __launch_bounds__(64 * WPB) void k_hits(DevBatch b, DevIndex ix) {
-    const uint32_t r = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (r >= b.n_reads) return;
-    if (b.flags[r] & GIS_FLAG_RESCUED) return;
+// write the hits of non-rescued reads into their region (same walk as k_lookup)
+__global__ __launch_bounds__(64 * WPB) void k_hits(DevBatch b, ReadSet set, DevIndex ix) {
     const uint32_t lane = lane_id();
-    const uint32_t n = b.rs_cnt[r];
-    const uint64_t base = b.rs_off[r];
-    const uint32_t half = b.rs_nfwd[r];
-    DevHit *out = b.hits + b.hit_off[r];
-    uint32_t carry = 0;
-    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        const uint32_t hc = i < n ? b.hcnt[base + i] : 0u;
-        const uint32_t ex = wave_excl_scan(hc);
-        if (hc > 0) {
-            const uint64_t key = b.rs_hash[base + i];
-            walk_run(ix, b.pos[base + i], key, (int)b.rs_start[base + i], (int)b.rs_end[base + i],
-                     i >= half ? 0x80000000u : 0u, out + carry + ex);
+    const uint32_t cnt = set_count(set);
+    for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
+        const uint32_t r = set_at(set, w);
+        if (b.hit_total[r] == 0) continue;
+        const uint32_t n = b.rs_cnt[r];
+        const uint64_t base = b.rs_off[r];
+        const uint32_t half = b.rs_nfwd[r];
+        DevHit *out = b.hits + b.hit_off[r];
+        uint32_t carry = 0;
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            const uint32_t hc = i < n ? b.hcnt[base + i] : 0u;
+            const uint32_t ex = wave_excl_scan(hc);
+            if (hc > 0)
+                walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)b.rs_start[base + i],
+                         (int)b.rs_end[base + i], i >= half ? 0x80000000u : 0u, out + carry + ex);
+            carry += wave_sum(hc);
         }
-        carry += wave_sum(hc);
     }
 }
-
 
 // ---------------------------------------------------------------------------
 // rescue (find_nams_rescue, step2/nam.cu:927-984)
+//
+// A seed is taken when count <= hard_cutoff and (count <= rescue_cutoff or
+// fewer than 5 seeds of its strand precede it in (count, query_start,
+// query_end) order): exactly the prefix the reference's sorted loop keeps
+// (step2/nam.cu:957-972). Taken seeds walk their runs in query_start order,
+// which is the seeds' own order within a strand.
 
-// Per rescued read: the run length (get_count) of every found seed goes to
-// hcnt. A seed is taken when count <= hard_cutoff and (count <= rescue_cutoff
-// or fewer than 5 seeds of its strand precede it in (count, query_start,
-// query_end) order) -- exactly the prefix the reference's sorted loop keeps
-// (step2/nam.cu:957-972). Taken seeds later walk their runs in query_start
-// order, which is the seeds' own order within a strand. rcnt gets the number
-// of hits each seed adds.
-__global__ __launch_bounds__(64 * WPB) void k_rescue_count(DevBatch b, DevIndex ix,
-                                                         uint32_t rescue_cutoff,
-                                                         uint32_t hard_cutoff) {
-    const uint32_t r = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (r >= b.n_reads) return;
-    const uint32_t lane = lane_id();
-    if (!(b.flags[r] & GIS_FLAG_RESCUED)) {
-        if (lane == 0) b.rhit_total[r] = 0;
-        return;
-    }
-    const uint32_t n = b.rs_cnt[r];
-    const uint64_t base = b.rs_off[r];
-    const uint32_t half = b.rs_nfwd[r];
-    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        if (i < n) {
-            const uint64_t pos = b.pos[base + i];
-            b.hcnt[base + i] = pos != GIS_NOT_FOUND ? dev_get_count(ix, pos) : 0u;
-        }
-    }
-    wave_sync_mem();
-    uint32_t total = 0;
-    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        uint32_t rh = 0;
-        if (i < n) {
-            const uint32_t c = b.hcnt[base + i];
-            bool taken = false;
-            if (c > 0 && c <= hard_cutoff) {
-                taken = c <= rescue_cutoff;
-                if (!taken) {
-                    const uint32_t lo = i < half ? 0 : half, hi = i < half ? half : n;
-                    const uint32_t qs = b.rs_start[base + i], qe = b.rs_end[base + i];
-                    uint32_t rank = 0;
-                    for (uint32_t j = lo; j < hi && rank < 5; j++) {
-                        const uint32_t cj = b.hcnt[base + j];
-                        if (cj == 0) continue;
-                        const uint32_t qsj = b.rs_start[base + j], qej = b.rs_end[base + j];
-                        const bool less = cj < c || (cj == c && (qsj < qs || (qsj == qs && qej < qe)));
-                        rank += less ? 1u : 0u;
-                    }
-                    taken = rank < 5;
-                }
-            }
-            if (taken)
-                rh = walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)b.rs_start[base + i],
-                              (int)b.rs_end[base + i], 0u, nullptr);
-            b.rcnt[base + i] = rh;
-        }
-        total += wave_sum(rh);
-    }
-    if (lane == 0) b.rhit_total[r] = total;
+__device__ __forceinline__ bool rh_less(uint32_t ca, uint32_t qa, uint32_t ea, uint32_t cb,
+                                        uint32_t qb, uint32_t eb) {
+    return ca < cb || (ca == cb && (qa < qb || (qa == qb && ea < eb)));
 }
 
-__global__ __launch_bounds__(64 * WPB) void k_rescue_hits(DevBatch b, DevIndex ix) {
-    const uint32_t r = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (r >= b.n_reads) return;
-    if (!(b.flags[r] & GIS_FLAG_RESCUED)) return;
+__global__ __launch_bounds__(64 * WPB) void k_rescue_count(DevBatch b, ReadSet set, DevIndex ix,
+                                                           uint32_t rescue_cutoff,
+                                                           uint32_t hard_cutoff) {
     const uint32_t lane = lane_id();
-    const uint32_t n = b.rs_cnt[r];
-    const uint64_t base = b.rs_off[r];
-    const uint32_t half = b.rs_nfwd[r];
-    DevHit *out = b.rhits + b.rhit_off[r];
-    uint32_t carry = 0;
-    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        const uint32_t rh = i < n ? b.rcnt[base + i] : 0u;
-        const uint32_t ex = wave_excl_scan(rh);
-        if (rh > 0)
-            walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)b.rs_start[base + i],
-                     (int)b.rs_end[base + i], i >= half ? 0x80000000u : 0u, out + carry + ex);
-        carry += wave_sum(rh);
+    const uint32_t cnt = set_count(set);
+    for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
+        const uint32_t r = set_at(set, w);
+        if (!(b.flags[r] & GIS_FLAG_RESCUED)) continue;
+        const uint32_t n = b.rs_cnt[r];
+        const uint64_t base = b.rs_off[r];
+        const uint32_t half = b.rs_nfwd[r];
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            if (i < n) {
+                const uint64_t pos = b.pos[base + i];
+                b.hcnt[base + i] = pos != GIS_NOT_FOUND ? dev_get_count(ix, pos) : 0u;
+            }
+        }
+        wave_sync_mem();
+        uint32_t total = 0;
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            uint32_t rh = 0;
+            if (i < n) {
+                const uint32_t c = b.hcnt[base + i];
+                bool taken = false;
+                if (c > 0 && c <= hard_cutoff) {
+                    taken = c <= rescue_cutoff;
+                    if (!taken) {
+                        const uint32_t lo = i < half ? 0 : half, hi = i < half ? half : n;
+                        const uint32_t qs = b.rs_start[base + i], qe = b.rs_end[base + i];
+                        uint32_t rank = 0;
+                        for (uint32_t j = lo; j < hi && rank < 5; j++) {
+                            const uint32_t cj = b.hcnt[base + j];
+                            if (cj == 0) continue;
+                            rank += rh_less(cj, b.rs_start[base + j], b.rs_end[base + j], c, qs, qe) ? 1u : 0u;
+                        }
+                        taken = rank < 5;
+                    }
+                }
+                if (taken)
+                    rh = walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)b.rs_start[base + i],
+                                  (int)b.rs_end[base + i], 0u, nullptr);
+                b.rcnt[base + i] = rh;
+            }
+            total += wave_sum(rh);
+        }
+        if (lane == 0) {
+            uint64_t off = 0;
+            if (total) {
+                off = atomicAdd(&b.ctl->rhit_cursor, (unsigned long long)total + 1);
+                if (off + total + 1 > b.rhit_cap) {
+                    atomicOr(&b.ctl->overflow, OVF_RHITS);
+                    total = 0;
+                }
+            }
+            b.rhit_total[r] = total;
+            b.rhit_off[r] = off;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * WPB) void k_rescue_hits(DevBatch b, ReadSet set, DevIndex ix) {
+    const uint32_t lane = lane_id();
+    const uint32_t cnt = set_count(set);
+    for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
+        const uint32_t r = set_at(set, w);
+        if (b.rhit_total[r] == 0) continue;
+        const uint32_t n = b.rs_cnt[r];
+        const uint64_t base = b.rs_off[r];
+        const uint32_t half = b.rs_nfwd[r];
+        DevHit *out = b.rhits + b.rhit_off[r];
+        uint32_t carry = 0;
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            const uint32_t rh = i < n ? b.rcnt[base + i] : 0u;
+            const uint32_t ex = wave_excl_scan(rh);
+            if (rh > 0)
+                walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)b.rs_start[base + i],
+                         (int)b.rs_end[base + i], i >= half ? 0x80000000u : 0u, out + carry + ex);
+            carry += wave_sum(rh);
+        }
     }
 }
 
@@ -602,6 +647,10 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_hits(DevBatch b, DevIndex i
 // the open list; when they are written out does not change any field.
 // Creation order inside a group is (query_start, ref_start) order, so
 // writing groups in key order gives the canonical NAM order directly.
+//
+// Scratch of a read with nh hits (LDS or global): keys[2 nh] (sort keys,
+// padded to a power of two), aux[2 nh + 2] (group starts, then the groups'
+// open lists), ws[nh] (NAMs under construction).
 
 __device__ __forceinline__ void bitonic_regs(uint64_t &key) {
     const uint32_t lane = lane_id();
@@ -616,7 +665,7 @@ __device__ __forceinline__ void bitonic_regs(uint64_t &key) {
     }
 }
 
-__device__ void bitonic_global(uint64_t *keys, uint32_t P) {
+__device__ __forceinline__ void bitonic_mem(uint64_t *keys, uint32_t P) {
     const uint32_t lane = lane_id();
     for (uint32_t size = 2; size <= P; size <<= 1) {
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
@@ -635,25 +684,11 @@ __device__ void bitonic_global(uint64_t *keys, uint32_t P) {
     }
 }
 
-__global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, int rescue, int k) {
-    const uint32_t r = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (r >= b.n_reads) return;
-    const bool rescued = (b.flags[r] & GIS_FLAG_RESCUED) != 0;
-    if (rescued != (rescue != 0)) return;
+// Group and merge; returns the number of groups. After it, aux[0..G] hold the
+// group starts and aux[nh + 1 + gs] the NAM count of the group starting at gs.
+__device__ __forceinline__ uint32_t merge_build(uint32_t nh, const DevHit *hits, uint64_t *keys,
+                                                uint32_t *aux, NamWork *ws) {
     const uint32_t lane = lane_id();
-    const uint64_t hb = rescue ? b.rhit_off[r] : b.hit_off[r];
-    const uint32_t nh = (uint32_t)((rescue ? b.rhit_off[r + 1] : b.hit_off[r + 1]) - hb);
-    const DevHit *hits = (rescue ? b.rhits : b.hits) + hb;
-    uint64_t *keys = (rescue ? b.rkeys : b.keys) + 2 * hb;
-    uint32_t *aux = (rescue ? b.raux : b.aux) + 2 * hb + r;
-    gis_nam *ws = (rescue ? b.rnam_ws : b.nam_ws) + hb;
-    gis_nam *fin = (rescue ? b.rnam_fin : b.nam_fin) + hb;
-    if (nh == 0) {
-        if (lane == 0) b.nam_cnt[r] = 0;
-        return;
-    }
-    // keys = (is_rc << 31 | ref_id) << 32 | hit index; sort only when the
-    // hits are not already grouped in key order
     uint32_t unsorted = 0;
     for (uint32_t i0 = 0; i0 < nh; i0 += 64) {
         const uint32_t i = i0 + lane;
@@ -664,9 +699,9 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, int rescue, int 
         }
     }
     unsorted = wave_max(unsorted);
+    wave_sync_mem();
     if (unsorted) {
         if (nh <= 64) {
-            wave_sync_mem();
             uint64_t key = lane < nh ? keys[lane] : UINT64_MAX;
             bitonic_regs(key);
             if (lane < nh) keys[lane] = key;
@@ -675,11 +710,10 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, int rescue, int 
             while (P < nh) P <<= 1;
             for (uint32_t i = nh + lane; i < P; i += 64) keys[i] = UINT64_MAX;
             wave_sync_mem();
-            bitonic_global(keys, P);
+            bitonic_mem(keys, P);
         }
+        wave_sync_mem();
     }
-    wave_sync_mem();
-    // group starts -> aux[0..G]; open lists live at aux[nh + 1 + group start]
     uint32_t G = 0;
     for (uint32_t i0 = 0; i0 < nh; i0 += 64) {
         const uint32_t i = i0 + lane;
@@ -697,7 +731,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, int rescue, int 
         if (g < G) {
             const uint32_t gs = aux[g], ge = aux[g + 1];
             uint32_t *open = open_base + gs;
-            gis_nam *gw = ws + gs;
+            NamWork *gw = ws + gs;
             uint32_t n_open = 0, c = 0;
             int cur_qs = -1;
             for (uint32_t x = gs; x < ge; x++) {
@@ -707,27 +741,27 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, int rescue, int 
                     uint32_t w2 = 0;
                     for (uint32_t oi = 0; oi < n_open; oi++) {
                         const uint32_t o = open[oi];
-                        if (gw[o].query_end >= qs) open[w2++] = o;
+                        if ((int)gw[o].query_end >= qs) open[w2++] = o;
                     }
                     n_open = w2;
                     cur_qs = qs;
                 }
                 bool added = false;
                 for (uint32_t oi = 0; oi < n_open; oi++) {
-                    gis_nam &o = gw[open[oi]];
-                    if (o.query_prev_hit_startpos < qs && qs <= o.query_end &&
-                        o.ref_prev_hit_startpos < rs && rs <= o.ref_end) {
-                        if (qe > o.query_end && re > o.ref_end) {
-                            o.query_end = qe;
+                    NamWork &o = gw[open[oi]];
+                    if ((int)o.query_prev < qs && qs <= (int)o.query_end && o.ref_prev < rs &&
+                        rs <= o.ref_end) {
+                        if (qe > (int)o.query_end && re > o.ref_end) {
+                            o.query_end = (uint16_t)qe;
                             o.ref_end = re;
-                            o.query_prev_hit_startpos = qs;
-                            o.ref_prev_hit_startpos = rs;
+                            o.query_prev = (uint16_t)qs;
+                            o.ref_prev = rs;
                             o.n_hits++;
                             added = true;
                             break;
-                        } else if (qe <= o.query_end && re <= o.ref_end) {
-                            o.query_prev_hit_startpos = qs;
-                            o.ref_prev_hit_startpos = rs;
+                        } else if (qe <= (int)o.query_end && re <= o.ref_end) {
+                            o.query_prev = (uint16_t)qs;
+                            o.ref_prev = rs;
                             o.n_hits++;
                             added = true;
                             break;
@@ -735,23 +769,40 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, int rescue, int 
                     }
                 }
                 if (!added) {
-                    gis_nam &o = gw[c];
-                    o.query_start = qs;
-                    o.query_end = qe;
-                    o.query_prev_hit_startpos = qs;
+                    NamWork &o = gw[c];
+                    o.query_start = (uint16_t)qs;
+                    o.query_end = (uint16_t)qe;
+                    o.query_prev = (uint16_t)qs;
                     o.ref_start = rs;
                     o.ref_end = re;
-                    o.ref_prev_hit_startpos = rs;
+                    o.ref_prev = rs;
                     o.n_hits = 1;
                     open[n_open++] = c;
                     c++;
                 }
             }
-            open[0] = c;  // the group's NAM count (open list no longer needed)
+            open[0] = c;  // the group's NAM count (its open list is no longer needed)
         }
     }
     wave_sync_mem();
-    // ranks of groups -> final NAMs with nam_id = rank, score filled in
+    return G;
+}
+
+// total NAMs of the groups (wave-uniform)
+__device__ __forceinline__ uint32_t merge_count(uint32_t nh, uint32_t G, const uint32_t *aux) {
+    uint32_t total = 0;
+    for (uint32_t g0 = 0; g0 < G; g0 += 64) {
+        const uint32_t g = g0 + lane_id();
+        total += wave_sum(g < G ? aux[nh + 1 + aux[g]] : 0u);
+    }
+    return total;
+}
+
+// final NAMs in canonical order with nam_id = rank and the score filled in
+__device__ __forceinline__ void merge_emit(uint32_t nh, uint32_t G, const uint64_t *keys,
+                                           const uint32_t *aux, const NamWork *ws, gis_nam *out) {
+    const uint32_t lane = lane_id();
+    const uint32_t *open_base = aux + nh + 1;
     uint32_t carry = 0;
     for (uint32_t g0 = 0; g0 < G; g0 += 64) {
         const uint32_t g = g0 + lane;
@@ -764,32 +815,241 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, int rescue, int 
         if (g < G) {
             const uint32_t rsd = (uint32_t)(keys[gs] >> 32);
             for (uint32_t m = 0; m < c; m++) {
-                gis_nam o = ws[gs + m];
-                o.nam_id = (int32_t)(rank + m);
-                o.ref_id = (int32_t)(rsd & 0x7fffffffu);
-                o.is_rc = (uint8_t)(rsd >> 31);
-                o.pad_[0] = o.pad_[1] = o.pad_[2] = 0;
-                o.score = nam_score(o.query_start, o.query_end, o.ref_start, o.ref_end, o.n_hits);
-                fin[rank + m] = o;
+                const NamWork &o = ws[gs + m];
+                gis_nam x;
+                x.nam_id = (int32_t)(rank + m);
+                x.query_start = o.query_start;
+                x.query_end = o.query_end;
+                x.query_prev_hit_startpos = o.query_prev;
+                x.ref_start = o.ref_start;
+                x.ref_end = o.ref_end;
+                x.ref_prev_hit_startpos = o.ref_prev;
+                x.n_hits = o.n_hits;
+                x.ref_id = (int32_t)(rsd & 0x7fffffffu);
+                x.score = nam_score(o.query_start, o.query_end, o.ref_start, o.ref_end, o.n_hits);
+                x.is_rc = (uint8_t)(rsd >> 31);
+                x.pad_[0] = x.pad_[1] = x.pad_[2] = 0;
+                out[rank + m] = x;
             }
         }
         carry += wave_sum(c);
     }
-    if (lane == 0) b.nam_cnt[r] = carry;
 }
 
-// dense output: NAMs of read r to nams_out[nam_off[r] ..]
-__global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b) {
-    const uint32_t r = blockIdx.x * WPB + (threadIdx.x >> 6);
-    if (r >= b.n_reads) return;
+struct alignas(8) MergeLds {
+    DevHit hits[MERGE_LDS];
+    uint64_t keys[MERGE_LDS];
+    uint32_t aux[2 * MERGE_LDS + 2];
+    NamWork ws[MERGE_LDS];
+};
+
+__global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int rescue) {
+    __shared__ MergeLds lds[WPB];
     const uint32_t lane = lane_id();
-    const bool rescued = (b.flags[r] & GIS_FLAG_RESCUED) != 0;
-    const uint32_t c = b.nam_cnt[r];
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(
-        rescued ? b.rnam_fin + b.rhit_off[r] : b.nam_fin + b.hit_off[r]);
-    uint32_t *dst = reinterpret_cast<uint32_t *>(b.nams_out + b.nam_off[r]);
-    const uint32_t words = c * (uint32_t)(sizeof(gis_nam) / 4);
-    for (uint32_t i = lane; i < words; i += 64) dst[i] = src[i];
+    const int wid = (int)(threadIdx.x >> 6);
+    const uint32_t cnt = set_count(set);
+    for (uint32_t w = blockIdx.x * WPB + (uint32_t)wid; w < cnt; w += gridDim.x * WPB) {
+        const uint32_t r = set_at(set, w);
+        const bool rescued = (b.flags[r] & GIS_FLAG_RESCUED) != 0;
+        if (rescued != (rescue != 0)) continue;
+        const uint64_t hb = rescue ? b.rhit_off[r] : b.hit_off[r];
+        const uint32_t nh = rescue ? b.rhit_total[r] : b.hit_total[r];
+        if (nh == 0) {
+            if (lane == 0) b.nam_cnt[r] = 0;
+            continue;
+        }
+        const DevHit *ghits = (rescue ? b.rhits : b.hits) + hb;
+        gis_nam *fin = (rescue ? b.rnam_fin : b.nam_fin) + hb;
+        uint32_t total;
+        if (nh <= (uint32_t)MERGE_LDS) {
+            MergeLds &S = lds[wid];
+            for (uint32_t i = lane; i < nh; i += 64) S.hits[i] = ghits[i];
+            wave_sync_mem();
+            const uint32_t G = merge_build(nh, S.hits, S.keys, S.aux, S.ws);
+            total = merge_count(nh, G, S.aux);
+            merge_emit(nh, G, S.keys, S.aux, S.ws, fin);
+        } else {
+            uint64_t *keys = (rescue ? b.rkeys : b.keys) + 2 * hb;
+            uint32_t *aux = (rescue ? b.raux : b.aux) + 2 * hb;
+            NamWork *ws = (rescue ? b.rnam_ws : b.nam_ws) + hb;
+            const uint32_t G = merge_build(nh, ghits, keys, aux, ws);
+            total = merge_count(nh, G, aux);
+            merge_emit(nh, G, keys, aux, ws, fin);
+        }
+        if (lane == 0) b.nam_cnt[r] = total;
+        wave_sync_mem();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused: the whole path of one read in one wave. Each lane holds randstrobe
+// pair i (forward i, reverse complement i) in registers, so a read fits when
+// it has at most 64 + w_min syncmers; hits and the merge live in LDS. Reads
+// beyond these limits are appended to the spill list for the per-stage
+// kernels.
+
+constexpr int F_LMAX = 320;
+constexpr int F_SYCAP = 96;
+constexpr int F_WPB = 4;
+
+struct alignas(8) FusedSeedLds {
+    uint64_t sh[F_LMAX];
+    uint64_t nmask[F_LMAX / 64 + 2];
+    uint64_t sy_hash[F_SYCAP];
+    uint32_t sy_pos[F_SYCAP];
+    uint8_t codes[F_LMAX];
+    uint8_t st[F_LMAX];
+};
+
+union alignas(8) FusedLds {
+    FusedSeedLds seed;
+    MergeLds merge;
+};
+
+__global__ __launch_bounds__(64 * F_WPB) void k_fused(DevBatch b, DevIndex ix, SeedParams P,
+                                                      uint32_t rescue_cutoff, uint32_t hard_cutoff) {
+    __shared__ FusedLds lds[F_WPB];
+    const uint32_t lane = lane_id();
+    const int wid = (int)(threadIdx.x >> 6);
+    FusedLds &S = lds[wid];
+    for (uint32_t r = blockIdx.x * F_WPB + (uint32_t)wid; r < b.n_reads; r += gridDim.x * F_WPB) {
+        const int L = (int)b.read_len[r];
+        bool spill = L > F_LMAX;
+        uint32_t nh = 0, n_fwd = 0;
+        bool rescue = false;
+        float frac = 1.0f;
+        if (!spill) {
+            SeedBufs B{S.seed.codes, S.seed.st, S.seed.sh, S.seed.sy_hash, S.seed.sy_pos, S.seed.nmask};
+            const uint32_t n = seed_syncmers(b.seqs + b.read_off[r], L, P, B, F_SYCAP, nullptr, nullptr);
+            n_fwd = ((uint32_t)L >= P.w_max && n > P.w_min) ? n - P.w_min : 0;
+            spill = n > (uint32_t)F_SYCAP || n_fwd > 64;
+            RsPair rp{};
+            const bool act = !spill && lane < n_fwd;
+            if (act) rp = randstrobe_pair(B, n, lane, L, P);
+            wave_sync();  // the syncmer lists are dead from here; LDS becomes MergeLds
+            if (!spill) {
+                // both randstrobes of the lane through find / filter / run walk
+                uint64_t pf = GIS_NOT_FOUND, pr = GIS_NOT_FOUND;
+                uint32_t hf = 0, hr = 0;
+                bool gf = false, gr = false;
+                if (act) {
+                    pf = dev_find(ix, rp.fh);
+                    pr = dev_find(ix, rp.rh);
+                    if (pf != GIS_NOT_FOUND && !dev_filtered(ix, pf, rp.fh)) {
+                        gf = true;
+                        hf = walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, nullptr);
+                    }
+                    if (pr != GIS_NOT_FOUND && !dev_filtered(ix, pr, rp.rh)) {
+                        gr = true;
+                        hr = walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, nullptr);
+                    }
+                }
+                const uint32_t found = wave_sum((pf != GIS_NOT_FOUND ? 1u : 0u) + (pr != GIS_NOT_FOUND ? 1u : 0u));
+                const uint32_t good = wave_sum((gf ? 1u : 0u) + (gr ? 1u : 0u));
+                const uint32_t hits_norm = wave_sum(hf + hr);
+                frac = found > 0 ? (float)good / (float)found : 1.0f;
+                rescue = (double)frac < 0.7 || hits_norm == 0;
+                if (rescue) {
+                    // run lengths and the taken prefix of each strand
+                    const uint32_t cf = act && pf != GIS_NOT_FOUND ? dev_get_count(ix, pf) : 0u;
+                    const uint32_t cr = act && pr != GIS_NOT_FOUND ? dev_get_count(ix, pr) : 0u;
+                    uint32_t rankf = 0, rankr = 0;
+                    for (uint32_t j = 0; j < n_fwd; j++) {
+                        const uint32_t cjf = __shfl(cf, (int)j, 64), sjf = __shfl(rp.fs, (int)j, 64),
+                                       ejf = __shfl(rp.fe, (int)j, 64);
+                        const uint32_t cjr = __shfl(cr, (int)j, 64), sjr = __shfl(rp.rs, (int)j, 64),
+                                       ejr = __shfl(rp.re, (int)j, 64);
+                        rankf += (cjf > 0 && rh_less(cjf, sjf, ejf, cf, rp.fs, rp.fe)) ? 1u : 0u;
+                        rankr += (cjr > 0 && rh_less(cjr, sjr, ejr, cr, rp.rs, rp.re)) ? 1u : 0u;
+                    }
+                    const bool tf = cf > 0 && cf <= hard_cutoff && (cf <= rescue_cutoff || rankf < 5);
+                    const bool tr = cr > 0 && cr <= hard_cutoff && (cr <= rescue_cutoff || rankr < 5);
+                    hf = tf ? walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, nullptr) : 0u;
+                    hr = tr ? walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, nullptr) : 0u;
+                }
+                // hits into LDS: forward seeds in lane order, then reverse ones
+                const uint32_t tot_f = wave_sum(hf);
+                nh = tot_f + wave_sum(hr);
+                spill = nh > (uint32_t)MERGE_LDS;
+                if (!spill) {
+                    const uint32_t of = wave_excl_scan(hf), orr = tot_f + wave_excl_scan(hr);
+                    if (hf) walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, S.merge.hits + of);
+                    if (hr) walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, S.merge.hits + orr);
+                    wave_sync_mem();
+                }
+            }
+        }
+        if (spill) {
+            if (lane == 0) {
+                b.flags[r] = FLAG_SPILL;
+                b.nam_cnt[r] = 0;
+                const unsigned int at = atomicAdd(&b.ctl->n_spill, 1u);
+                b.spill_ids[at] = r;
+            }
+            wave_sync_mem();
+            continue;
+        }
+        uint32_t total = 0, G = 0;
+        uint64_t base = 0;
+        if (nh > 0) {
+            G = merge_build(nh, S.merge.hits, S.merge.keys, S.merge.aux, S.merge.ws);
+            total = merge_count(nh, G, S.merge.aux);
+        }
+        if (total > 0) {
+            unsigned long long bl = 0;
+            if (lane == 0) bl = atomicAdd(&b.ctl->nam_cursor, (unsigned long long)total);
+            base = __shfl(bl, 0, 64);
+            if (base + total > b.nam_pool_cap) {
+                if (lane == 0) atomicOr(&b.ctl->overflow, OVF_NAMPOOL);
+                total = 0;
+            } else {
+                merge_emit(nh, G, S.merge.keys, S.merge.aux, S.merge.ws, b.nam_pool + base);
+            }
+        }
+        if (lane == 0) {
+            b.frac[r] = frac;
+            b.flags[r] = (uint8_t)((rescue ? GIS_FLAG_RESCUED : 0) | (n_fwd == 0 ? GIS_FLAG_NO_RANDSTROBES : 0));
+            b.nam_cnt[r] = total;
+            b.nam_src[r] = base;
+            b.hit_total[r] = 0;
+            b.rhit_total[r] = 0;
+        }
+        wave_sync_mem();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dense output: NAMs of read r to nams_out[nam_off[r] ..]
+
+__global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
+    const uint32_t lane = lane_id();
+    if (b.nam_off[b.n_reads] > b.out_cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&b.ctl->overflow, OVF_OUT);
+        return;
+    }
+    for (uint32_t r = blockIdx.x * WPB + (threadIdx.x >> 6); r < b.n_reads; r += gridDim.x * WPB) {
+        const uint32_t c = b.nam_cnt[r];
+        if (c == 0) continue;
+        const uint8_t fl = b.flags[r];
+        const gis_nam *srcn;
+        if (fused && !(fl & FLAG_SPILL)) srcn = b.nam_pool + b.nam_src[r];
+        else if (fl & GIS_FLAG_RESCUED) srcn = b.rnam_fin + b.rhit_off[r];
+        else srcn = b.nam_fin + b.hit_off[r];
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(srcn);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(b.nams_out + b.nam_off[r]);
+        const uint32_t words = c * (uint32_t)(sizeof(gis_nam) / 4);
+        for (uint32_t i = lane; i < words; i += 64) dst[i] = src[i];
+    }
+}
+
+__global__ void k_reset_ctl(DevCtl *ctl) {
+    if (threadIdx.x == 0) {
+        ctl->hit_cursor = 0;
+        ctl->rhit_cursor = 0;
+        ctl->nam_cursor = 0;
+        ctl->n_spill = 0;
+        ctl->overflow = 0;
+    }
 }
 
 // plain lookup over a list of hashes (the microbenchmark gpu_find,
@@ -806,11 +1066,10 @@ __global__ void k_find_flat(DevIndex ix, const uint64_t *hashes, uint64_t n, uin
 constexpr int SCAN_BLOCK = 256, SCAN_ITEMS = 4, SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
 
 __device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t *lds_w, uint64_t *total) {
-    // v per thread; 4 waves
     const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
     uint64_t x = v;
     for (int d = 1; d < 64; d <<= 1) {
-        uint64_t y = __shfl_up(x, d, 64);
+        const uint64_t y = __shfl_up(x, d, 64);
         if (lane >= (uint32_t)d) x += y;
     }
     if (lane == 63) lds_w[wid] = x;
@@ -875,59 +1134,72 @@ static inline unsigned grid_for(uint64_t items, unsigned per_block) {
     return (unsigned)((items + per_block - 1) / per_block);
 }
 
-void launch_seed(const DevBatch &b, const SeedParams &sp, int want_syncmers, int long_reads,
-                 hipStream_t st) {
+// blocks for a per-read kernel over a set: one wave per read for a full set,
+// a bounded grid-stride grid for a spill list (its size is on the device)
+static inline unsigned grid_for_set(const DevBatch &b, const ReadSet &set, unsigned wpb) {
+    const unsigned g = grid_for(b.n_reads, wpb);
+    if (!set.ids) return g;
+    return g < 512u ? g : 512u;
+}
+
+void launch_reset_ctl(DevCtl *ctl, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(64), 0, st, ctl);
+}
+
+void launch_seed(const DevBatch &b, const ReadSet &set, const SeedParams &sp, int want_syncmers,
+                 int long_reads, hipStream_t st) {
     if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_seed, dim3(grid_for(b.n_reads, SEED_WPB)), dim3(64 * SEED_WPB), 0, st, b, sp,
-                       want_syncmers);
+    const unsigned g = grid_for_set(b, set, SEED_WPB);
+    hipLaunchKernelGGL(k_seed<false>, dim3(g), dim3(64 * SEED_WPB), 0, st, b, set, sp, want_syncmers);
     if (long_reads)
-        hipLaunchKernelGGL(k_seed_long, dim3(grid_for(b.n_reads, SEED_WPB)), dim3(64 * SEED_WPB), 0,
-                           st, b, sp, want_syncmers);
+        hipLaunchKernelGGL(k_seed<true>, dim3(g), dim3(64 * SEED_WPB), 0, st, b, set, sp, want_syncmers);
 }
 
-void launch_lookup(const DevBatch &b, const DevIndex &ix, int rescue_mode, hipStream_t st) {
+void launch_lookup(const DevBatch &b, const ReadSet &set, const DevIndex &ix, int rescue_mode,
+                   hipStream_t st) {
     if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_lookup, dim3(grid_for(b.n_reads, WPB)), dim3(64 * WPB), 0, st, b, ix,
-                       rescue_mode);
+    hipLaunchKernelGGL(k_lookup, dim3(grid_for_set(b, set, WPB)), dim3(64 * WPB), 0, st, b, set, ix,
+                       rescue_mode, set.ids ? 1 : 0);
 }
 
-void launch_hits(const DevBatch &b, const DevIndex &ix, hipStream_t st) {
+void launch_hits(const DevBatch &b, const ReadSet &set, const DevIndex &ix, hipStream_t st) {
     if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_hits, dim3(grid_for(b.n_reads, WPB)), dim3(64 * WPB), 0, st, b, ix);
+    hipLaunchKernelGGL(k_hits, dim3(grid_for_set(b, set, WPB)), dim3(64 * WPB), 0, st, b, set, ix);
 }
 
-void launch_merge(const DevBatch &b, int rescue, int k, hipStream_t st) {
+void launch_rescue_count(const DevBatch &b, const ReadSet &set, const DevIndex &ix,
+                         uint32_t rescue_cutoff, uint32_t hard_cutoff, hipStream_t st) {
     if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_merge, dim3(grid_for(b.n_reads, WPB)), dim3(64 * WPB), 0, st, b, rescue, k);
+    hipLaunchKernelGGL(k_rescue_count, dim3(grid_for_set(b, set, WPB)), dim3(64 * WPB), 0, st, b, set,
+                       ix, rescue_cutoff, hard_cutoff);
 }
 
-void launch_rescue_count(const DevBatch &b, const DevIndex &ix, uint32_t rescue_cutoff,
-                         uint32_t hard_cutoff, hipStream_t st) {
+void launch_rescue_hits(const DevBatch &b, const ReadSet &set, const DevIndex &ix, hipStream_t st) {
     if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_rescue_count, dim3(grid_for(b.n_reads, WPB)), dim3(64 * WPB), 0, st, b, ix,
+    hipLaunchKernelGGL(k_rescue_hits, dim3(grid_for_set(b, set, WPB)), dim3(64 * WPB), 0, st, b, set, ix);
+}
+
+void launch_merge(const DevBatch &b, const ReadSet &set, int rescue, hipStream_t st) {
+    if (b.n_reads == 0) return;
+    hipLaunchKernelGGL(k_merge, dim3(grid_for_set(b, set, WPB)), dim3(64 * WPB), 0, st, b, set, rescue);
+}
+
+void launch_fused(const DevBatch &b, const DevIndex &ix, const SeedParams &sp,
+                  uint32_t rescue_cutoff, uint32_t hard_cutoff, hipStream_t st) {
+    if (b.n_reads == 0) return;
+    hipLaunchKernelGGL(k_fused, dim3(grid_for(b.n_reads, F_WPB)), dim3(64 * F_WPB), 0, st, b, ix, sp,
                        rescue_cutoff, hard_cutoff);
 }
 
-void launch_rescue_hits(const DevBatch &b, const DevIndex &ix, uint32_t, uint32_t, hipStream_t st) {
+void launch_compact(const DevBatch &b, int fused, hipStream_t st) {
     if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_rescue_hits, dim3(grid_for(b.n_reads, WPB)), dim3(64 * WPB), 0, st, b, ix);
-}
-
-void launch_compact(const DevBatch &b, hipStream_t st) {
-    if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_compact, dim3(grid_for(b.n_reads, WPB)), dim3(64 * WPB), 0, st, b);
+    hipLaunchKernelGGL(k_compact, dim3(grid_for(b.n_reads, WPB)), dim3(64 * WPB), 0, st, b, fused);
 }
 
 void launch_find_flat(const DevIndex &ix, const uint64_t *hashes, uint64_t n, uint64_t *positions,
                       hipStream_t st) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_find_flat, dim3(grid_for(n, 256)), dim3(256), 0, st, ix, hashes, n, positions);
-}
-
-void launch_rs_capacity(const uint32_t *lens, uint32_t n, const SeedParams &sp, uint32_t *cap_rs,
-                        uint32_t *cap_sy, hipStream_t st) {
-    if (n == 0) return;
-    hipLaunchKernelGGL(k_capacity, dim3(grid_for(n, 256)), dim3(256), 0, st, lens, n, sp, cap_rs, cap_sy);
 }
 
 size_t scan_tmp_words(uint64_t n) { return (size_t)grid_for(n ? n : 1, SCAN_TILE) + 1; }

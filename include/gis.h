@@ -101,8 +101,10 @@ typedef struct {
     uint64_t device_bytes;
 } gis_index_info;
 
-/* Per-read result flags */
-enum { GIS_FLAG_RESCUED = 1, GIS_FLAG_NO_RANDSTROBES = 2 };
+/* Per-read result flags. GIS_FLAG_SPILLED: in GIS_MODE_FUSED the read did not
+ * fit the fused kernel's LDS budget and went through the per-stage kernels
+ * (the NAMs are the same either way). */
+enum { GIS_FLAG_RESCUED = 1, GIS_FLAG_NO_RANDSTROBES = 2, GIS_FLAG_SPILLED = 0x80 };
 
 /* Output of the NAM entry points. The caller provides arrays; nam_offsets has
  * n_reads + 1 entries and the NAMs of read r are nams[nam_offsets[r] ..

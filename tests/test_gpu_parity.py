@@ -172,11 +172,12 @@ def _compare_nams(got, exp, frac_g, frac_e, flags_g, flags_e):
     assert np.array_equal(flags_g & 1, flags_e & 1), "rescue decisions differ"
 
 
-def test_find_nams_parity(setup):
+@pytest.mark.parametrize("mode", [G.MODE_PLAIN, G.MODE_FUSED])
+def test_find_nams_parity(setup, mode):
     g, lens, p, idx, op, oidx = setup
     reads = _read_set(g, lens, 1500, seed=21)
     s, o, l = G.pack_reads(reads)
-    res = G.find_nams_batch(idx, s, o, l)
+    res = G.find_nams_batch(idx, s, o, l, mode=mode)
     eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
     if not np.array_equal(res.nam_offsets, eo) or res.nams.tobytes() != en.tobytes():
         for r in range(len(reads)):
@@ -244,3 +245,44 @@ def test_c1_scale_parity(ctx):
     eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
     _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
                   res.flags, efl)
+
+
+@pytest.mark.parametrize("mode", [G.MODE_PLAIN, G.MODE_FUSED])
+def test_pool_overflow_reruns(setup, mode, monkeypatch):
+    """Tiny initial hit/NAM pools: every pool overflows, grows to what the
+    device allocators asked for, and the batch runs again."""
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 300, seed=44)
+    s, o, l = G.pack_reads(reads)
+    monkeypatch.setenv("GIS_TEST_SMALL_POOLS", "1")
+    ctx2 = G.Context(0)
+    idx2 = G.StrobemerIndex(None, p, ctx=ctx2)
+    idx2.populate_arrays(g, lens)
+    res = G.find_nams_batch(idx2, s, o, l, mode=mode)
+    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
+    _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
+                  res.flags, efl)
+    ctx2.close()
+
+
+def test_fused_spills_and_mapper(setup):
+    """The fused form hands long and hit-heavy reads to the per-stage kernels;
+    a device-resident batch (upload / run / download) gives the same NAMs in
+    both forms."""
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 400, seed=55)
+    s, o, l = G.pack_reads(reads)
+    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
+    m = G.Mapper(idx, len(l), len(s))
+    m.upload(s, o, l)
+    for mode in (G.MODE_FUSED, G.MODE_PLAIN, G.MODE_FUSED):
+        m.run(mode=mode)
+        res = m.download()
+        _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
+                      res.flags, efl)
+        if mode == G.MODE_FUSED:
+            spilled = (res.flags & 0x80) != 0
+            assert spilled.any(), "long reads must take the per-stage path"
+            assert (~spilled).sum() > 0.8 * len(l)
+            assert spilled[np.asarray(l) > 320].all()
+    m.close()
