@@ -77,23 +77,32 @@ def main():
         m.upload(seqs[base:end], offs[lo:hi] - base, lens[lo:hi])
         mappers.append(m)
 
-    # set-up: one untimed run per batch so every device buffer has been
-    # allocated and touched before timing
-    for m in mappers:
-        m.run()
-    for i in range(args.warmup):
-        mappers[i % nb].run()
+    def timed(mode):
+        # set-up: one untimed run per batch so every device buffer has been
+        # allocated and touched before timing
+        for m in mappers:
+            m.run(mode=mode)
+        for i in range(args.warmup):
+            mappers[i % nb].run(mode=mode)
+        step_ms, reads_done = [], 0
+        t_start = time.perf_counter()
+        for i in range(args.steps):
+            m = mappers[i % nb]
+            ts = time.perf_counter()
+            m.run(mode=mode)  # ends with a stream synchronisation
+            step_ms.append((time.perf_counter() - ts) * 1e3)
+            reads_done += m.n
+        elapsed = time.perf_counter() - t_start
+        return reads_done / elapsed, elapsed, step_ms
 
-    step_ms, reads_done = [], 0
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        m = mappers[i % nb]
-        ts = time.perf_counter()
-        m.run()  # ends with a stream synchronisation
-        step_ms.append((time.perf_counter() - ts) * 1e3)
-        reads_done += m.n
-    elapsed = time.perf_counter() - t_start
-    reads_per_s = reads_done / elapsed
+    # the fused form (one wave-per-read kernel, spills to the per-stage
+    # kernels), reported beside the plain per-stage drop-in
+    fused_rps, fused_elapsed, fused_ms = timed(G.MODE_FUSED)
+    fused_stage = mappers[(args.steps - 1) % nb].stage_times()
+    log(f"fused form: {fused_rps / 1e6:.1f} M reads/s; stage ms " +
+        ", ".join(f"{k}={v:.3f}" for k, v in fused_stage.items()))
+    # the plain per-stage drop-in: the configuration's form and the value
+    reads_per_s, elapsed, step_ms = timed(G.MODE_PLAIN)
     # per-kernel times of the timed runs (HIP events on the batch stream; each
     # batch keeps the times of its last run)
     lookup_ms = [mappers[i % nb].stage_times().get("lookup", float("nan"))
@@ -171,6 +180,10 @@ def main():
             "avg_launch_ms": float(np.mean(lookup_ms)),
         },
         "cpu_baseline": cpu,
+        "fused_form": {"value": fused_rps, "unit": "reads/s",
+                       "ms_per_step": fused_elapsed * 1e3 / args.steps,
+                       "p50_batch_ms": float(np.percentile(fused_ms, 50)),
+                       "p99_batch_ms": float(np.percentile(fused_ms, 99))},
     }
     print(json.dumps(out), flush=True)
 

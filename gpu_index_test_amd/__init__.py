@@ -176,14 +176,26 @@ def default_context():
 
 
 class StrobemerIndex:
-    """StrobemerIndex (step2/index.hpp:37-183), resident in HBM."""
+    """StrobemerIndex (step2/index.hpp:37-183), resident in HBM.
+
+    A context holds one resident index; each StrobemerIndex gets a context of
+    its own unless one is passed, and loading another index on a shared
+    context invalidates this one (its calls then raise)."""
 
     def __init__(self, references, parameters, bits=-1, ctx=None):
-        self.ctx = ctx or default_context()
+        self.ctx = ctx or Context(0)
         self.references = references
         self.parameters = parameters
         self._bits_arg = bits
         self._host = None
+
+    def _claim(self):
+        self.ctx._owner = id(self)
+        self._host = None
+
+    def _check_owner(self):
+        if getattr(self.ctx, "_owner", None) != id(self):
+            raise GisError(6, "another index was loaded on this context after this one")
 
     # --- construction / persistence ---
     def populate(self, f=0.0002, n_threads=None):
@@ -196,7 +208,7 @@ class StrobemerIndex:
         self.ctx.check(self.ctx.L.gis_index_build(self.ctx.h, arr, N.ptr(lens), len(seqs),
                                                   C.byref(self.parameters.c), self._bits_arg,
                                                   C.c_float(f), n_threads))
-        self._host = None
+        self._claim()
 
     def populate_arrays(self, genome, contig_lens, f=0.0002, n_threads=None):
         """populate() from one concatenated uint8 genome array (no copies)."""
@@ -208,7 +220,7 @@ class StrobemerIndex:
         self.ctx.check(self.ctx.L.gis_index_build(self.ctx.h, arr, N.ptr(contig_lens),
                                                   len(contig_lens), C.byref(self.parameters.c),
                                                   self._bits_arg, C.c_float(f), n_threads))
-        self._host = None
+        self._claim()
 
     def upload(self, randstrobes, starts, bits, filter_cutoff):
         randstrobes = np.ascontiguousarray(randstrobes, dtype=REF_RANDSTROBE)
@@ -216,18 +228,20 @@ class StrobemerIndex:
         self.ctx.check(self.ctx.L.gis_index_upload(self.ctx.h, N.ptr(randstrobes), len(randstrobes),
                                                    N.ptr(starts), bits, filter_cutoff,
                                                    C.byref(self.parameters.c)))
-        self._host = None
+        self._claim()
 
     def read(self, filename):
         self.ctx.check(self.ctx.L.gis_index_read_sti(self.ctx.h, os.fsencode(filename),
                                                      C.byref(self.parameters.c)))
-        self._host = None
+        self._claim()
 
     def write(self, filename):
+        self._check_owner()
         self.ctx.check(self.ctx.L.gis_index_write_sti(self.ctx.h, os.fsencode(filename)))
 
     # --- properties ---
     def info(self):
+        self._check_owner()
         i = N.IndexInfo()
         self.ctx.check(self.ctx.L.gis_index_info_get(self.ctx.h, C.byref(i)))
         return i
@@ -265,6 +279,7 @@ class StrobemerIndex:
 
     # --- lookups (on the GPU) ---
     def find_many(self, keys):
+        self._check_owner()
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
         out = np.empty(len(keys), dtype=np.uint64)
         self.ctx.check(self.ctx.L.gis_lookup_batch(self.ctx.h, N.ptr(keys), len(keys), N.ptr(out)))
@@ -314,6 +329,7 @@ def _seq_args(seqs, offsets, lens):
 
 def syncmers_batch(index, seqs, offsets, lens):
     """Canonical syncmers of every read -> (offsets[n+1], SYNCMER array)."""
+    index._check_owner()
     ctx = index.ctx
     seqs, offsets, lens = _seq_args(seqs, offsets, lens)
     n = len(lens)
@@ -328,6 +344,7 @@ def syncmers_batch(index, seqs, offsets, lens):
 
 def randstrobes_batch(index, seqs, offsets, lens):
     """randstrobes_query of every read -> (offsets[n+1], QUERY_RANDSTROBE array)."""
+    index._check_owner()
     ctx = index.ctx
     seqs, offsets, lens = _seq_args(seqs, offsets, lens)
     n = len(lens)
@@ -389,6 +406,7 @@ def find_nams_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000
     """Whole path for a batch of reads: per-stage launches (MODE_PLAIN) or
     the fused kernel with the per-stage kernels taking its spilled reads
     (MODE_FUSED)."""
+    index._check_owner()
     ctx = index.ctx
     seqs, offsets, lens = _seq_args(seqs, offsets, lens)
     n = len(lens)
@@ -398,6 +416,7 @@ def find_nams_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000
 
 
 def hits_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000):
+    index._check_owner()
     ctx = index.ctx
     seqs, offsets, lens = _seq_args(seqs, offsets, lens)
     n = len(lens)
@@ -417,6 +436,7 @@ def hits_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000):
 
 
 def _qrs_call(index, query_randstrobes_list, rescue, rescue_cutoff, hard_cutoff):
+    index._check_owner()
     ctx = index.ctx
     arrs = [np.ascontiguousarray(q, dtype=QUERY_RANDSTROBE) for q in query_randstrobes_list]
     qoff = np.zeros(len(arrs) + 1, dtype=np.uint64)
@@ -444,6 +464,7 @@ class Mapper:
     """Device-resident batches: upload reads, run the path, download NAMs."""
 
     def __init__(self, index, max_reads, max_bases):
+        index._check_owner()
         self.index, self.ctx = index, index.ctx
         h = C.c_void_p()
         self.ctx.check(self.ctx.L.gis_batch_create(self.ctx.h, max_reads, max_bases, C.byref(h)))
@@ -457,6 +478,7 @@ class Mapper:
                                                    N.ptr(lens), self.n))
 
     def run(self, rescue_level=2, hard_cutoff=1000, mode=N.MODE_PLAIN):
+        self.index._check_owner()
         opt = _map_options(rescue_level, hard_cutoff, mode)
         self.ctx.check(self.ctx.L.gis_batch_run(self.h, C.byref(opt)))
 

@@ -26,7 +26,7 @@ def _genome():
 def setup(ctx):
     g, lens = _genome()
     p = G.IndexParameters.from_read_length(150)
-    idx = G.StrobemerIndex(None, p, ctx=ctx)
+    idx = G.StrobemerIndex(None, p)  # a context of its own
     idx.populate_arrays(g, lens, f=0.0002, n_threads=8)
     op = O.params_from_read_length(150)
     oidx = O.Index.build_arrays(g, lens, op, n_threads=8)
@@ -236,15 +236,27 @@ def test_c1_scale_parity(ctx):
     complemented, 0.1% substitutions, 0.01% indels."""
     g, lens = G.synth_genome(5_000_000, n_contigs=1, gc=0.5, seed=1)
     p = G.IndexParameters.from_read_length(150)
-    idx = G.StrobemerIndex(None, p, ctx=ctx)
+    idx = G.StrobemerIndex(None, p)
     idx.populate_arrays(g, lens)
     oidx = O.Index.build_arrays(g, lens, O.params_from_read_length(150), n_threads=8)
     assert idx.randstrobes.tobytes() == oidx.randstrobes.tobytes()
     s, o, l, _ = G.synth_reads(g, lens, 20_000, sub_rate=0.001, indel_rate=0.0001, seed=2)
-    res = G.find_nams_batch(idx, s, o, l)
-    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
-    _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
-                  res.flags, efl)
+    for mode in (G.MODE_PLAIN, G.MODE_FUSED):
+        res = G.find_nams_batch(idx, s, o, l, mode=mode)
+        eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
+        _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
+                      res.flags, efl)
+    idx.ctx.close()
+
+
+def test_shared_context_detects_replaced_index(setup):
+    g, lens, p, idx, op, oidx = setup
+    other = G.StrobemerIndex(None, p, ctx=idx.ctx)
+    other.upload(oidx.randstrobes, oidx.starts, oidx.bits, oidx.filter_cutoff)
+    with pytest.raises(G.GisError):
+        idx.find(1)
+    idx.upload(oidx.randstrobes, oidx.starts, oidx.bits, oidx.filter_cutoff)
+    assert idx.randstrobes.tobytes() == oidx.randstrobes.tobytes()
 
 
 @pytest.mark.parametrize("mode", [G.MODE_PLAIN, G.MODE_FUSED])
