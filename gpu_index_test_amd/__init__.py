@@ -364,9 +364,10 @@ def randstrobes_query(seq, index):
     return randstrobes_batch(index, s, o, l)[1]
 
 
-def _map_options(rescue_level=2, hard_cutoff=1000, mode=N.MODE_PLAIN):
+def _map_options(rescue_level=2, hard_cutoff=1000, mode=N.MODE_PLAIN, graph=True):
     o = N.MapOptions()
     o.rescue_level, o.hard_cutoff, o.mode = rescue_level, hard_cutoff, mode
+    o.flags = 0 if graph else N.RUN_NO_GRAPH
     return o
 
 
@@ -402,7 +403,7 @@ def _call_with_nam_out(ctx, n_reads, fn, cap=None):
 
 
 def find_nams_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000,
-                    mode=N.MODE_PLAIN):
+                    mode=N.MODE_PLAIN, graph=True):
     """Whole path for a batch of reads: per-stage launches (MODE_PLAIN) or
     the fused kernel with the per-stage kernels taking its spilled reads
     (MODE_FUSED)."""
@@ -410,7 +411,7 @@ def find_nams_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000
     ctx = index.ctx
     seqs, offsets, lens = _seq_args(seqs, offsets, lens)
     n = len(lens)
-    opt = _map_options(rescue_level, hard_cutoff, mode)
+    opt = _map_options(rescue_level, hard_cutoff, mode, graph)
     return _call_with_nam_out(ctx, n, lambda o: ctx.L.gis_find_nams_batch(
         ctx.h, N.ptr(seqs), N.ptr(offsets), N.ptr(lens), n, C.byref(opt), C.byref(o)))
 
@@ -477,9 +478,9 @@ class Mapper:
         self.ctx.check(self.ctx.L.gis_batch_upload(self.h, N.ptr(seqs), N.ptr(offsets),
                                                    N.ptr(lens), self.n))
 
-    def run(self, rescue_level=2, hard_cutoff=1000, mode=N.MODE_PLAIN):
+    def run(self, rescue_level=2, hard_cutoff=1000, mode=N.MODE_PLAIN, graph=True):
         self.index._check_owner()
-        opt = _map_options(rescue_level, hard_cutoff, mode)
+        opt = _map_options(rescue_level, hard_cutoff, mode, graph)
         self.ctx.check(self.ctx.L.gis_batch_run(self.h, C.byref(opt)))
 
     def nam_count(self):

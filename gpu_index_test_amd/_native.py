@@ -18,6 +18,7 @@ FLAG_RESCUED = 1
 FLAG_NO_RANDSTROBES = 2
 MODE_PLAIN = 0
 MODE_FUSED = 1
+RUN_NO_GRAPH = 1
 
 STATUS = {
     0: "GIS_OK", 1: "GIS_ERR_BAD_PARAMETER", 2: "GIS_ERR_INVALID_INDEX", 3: "GIS_ERR_HIP",
@@ -63,7 +64,7 @@ class NamOut(C.Structure):
 
 class MapOptions(C.Structure):
     _fields_ = [("rescue_level", C.c_int32), ("hard_cutoff", C.c_uint32), ("mode", C.c_int32),
-                ("reserved_", C.c_int32)]
+                ("flags", C.c_int32)]
 
 
 class GenomeSpec(C.Structure):

@@ -2,11 +2,11 @@
 //
 // A batch holds reads resident in HBM and every buffer of the path. A run is
 // a fixed sequence of launches on the context's stream with no host round
-// trip: variable-size intermediates (hits, rescue hits, fused NAMs) come from
-// device bump allocators with capacity checks, and the single synchronisation
-// at the end reads the control block. When a pool overflowed, the pools grow
-// to the size the allocator asked for and the batch runs again; results never
-// depend on it.
+// trip: variable-size intermediates (hits, rescue hits) are placed by device
+// scans of per-read counts and checked against their capacity on the device,
+// and the single synchronisation at the end reads the totals and the
+// overflow bits. When an array was too small it grows to the total the scan
+// reported and the batch runs again; results never depend on it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,10 +29,10 @@ struct gis_batch {
     DevBuf seqs, read_off, read_len, rs_off, sy_off, long_off, long_scratch;
     DevBuf rs_cnt, rs_nfwd, rs_hash, rs_start, rs_end, sy_cnt, sy_hash, sy_pos;
     DevBuf pos, hcnt, rcnt, lk_bytes;
-    DevBuf hit_total, rhit_total, frac, flags, nam_cnt, hit_off, rhit_off, nam_src;
+    DevBuf hit_total, rhit_total, frac, flags, nam_cnt, hit_off, rhit_off;
     DevBuf hits, rhits, keys, rkeys, aux, raux, nam_ws, rnam_ws, nam_fin, rnam_fin;
-    DevBuf nam_pool, nam_off, nams_out, scan_tmp, ctl, spill_ids;
-    uint64_t hit_cap = 0, rhit_cap = 0, pool_cap = 0, out_cap = 0;
+    DevBuf nam_slots, nam_off, nams_out, scan_tmp, ctl, spill_ids;
+    uint64_t hit_cap = 0, rhit_cap = 0, out_cap = 0;
     uint64_t tot_rs_cap = 0;
     // results of the last run
     uint64_t tot_nams = 0;
@@ -41,10 +41,15 @@ struct gis_batch {
     bool ran = false;
     // pinned landing area for the end-of-run readback
     DevCtl *h_ctl = nullptr;
-    uint64_t *h_total = nullptr;
+    uint64_t *h_total = nullptr;  // [3]: hits, rescue hits, NAMs
     hipEvent_t ev[16] = {};
     int n_ev = 0;
     std::vector<std::string> ev_names;
+    // captured launch sequence of the last configuration
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    DevBatch graph_key{};
+    int graph_cfg[6] = {-1, -1, -1, -1, -1, -1};
 };
 
 namespace gis {
@@ -61,15 +66,17 @@ void batch_free(gis_batch *b) {
                      &b->long_scratch, &b->rs_cnt, &b->rs_nfwd, &b->rs_hash, &b->rs_start,
                      &b->rs_end, &b->sy_cnt, &b->sy_hash, &b->sy_pos, &b->pos, &b->hcnt, &b->rcnt,
                      &b->lk_bytes, &b->hit_total, &b->rhit_total, &b->frac, &b->flags, &b->nam_cnt,
-                     &b->hit_off, &b->rhit_off, &b->nam_src, &b->hits, &b->rhits, &b->keys,
+                     &b->hit_off, &b->rhit_off, &b->hits, &b->rhits, &b->keys,
                      &b->rkeys, &b->aux, &b->raux, &b->nam_ws, &b->rnam_ws, &b->nam_fin,
-                     &b->rnam_fin, &b->nam_pool, &b->nam_off, &b->nams_out, &b->scan_tmp, &b->ctl,
+                     &b->rnam_fin, &b->nam_slots, &b->nam_off, &b->nams_out, &b->scan_tmp, &b->ctl,
                      &b->spill_ids};
     for (DevBuf *x : all) release(*x);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     if (b->h_ctl) (void)hipHostFree(b->h_ctl);
     if (b->h_total) (void)hipHostFree(b->h_total);
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     delete b;
 }
 
@@ -111,7 +118,6 @@ DevBatch make_devbatch(gis_batch *b) {
     d.nam_cnt = static_cast<uint32_t *>(b->nam_cnt.p);
     d.hit_off = static_cast<uint64_t *>(b->hit_off.p);
     d.rhit_off = static_cast<uint64_t *>(b->rhit_off.p);
-    d.nam_src = static_cast<uint64_t *>(b->nam_src.p);
     d.hits = static_cast<DevHit *>(b->hits.p);
     d.rhits = static_cast<DevHit *>(b->rhits.p);
     d.hit_cap = b->hit_cap;
@@ -124,8 +130,7 @@ DevBatch make_devbatch(gis_batch *b) {
     d.rnam_ws = static_cast<NamWork *>(b->rnam_ws.p);
     d.nam_fin = static_cast<gis_nam *>(b->nam_fin.p);
     d.rnam_fin = static_cast<gis_nam *>(b->rnam_fin.p);
-    d.nam_pool = static_cast<gis_nam *>(b->nam_pool.p);
-    d.nam_pool_cap = b->pool_cap;
+    d.nam_slots = static_cast<gis_nam *>(b->nam_slots.p);
     d.nam_off = static_cast<uint64_t *>(b->nam_off.p);
     d.nams_out = static_cast<gis_nam *>(b->nams_out.p);
     d.out_cap = b->out_cap;
@@ -151,15 +156,15 @@ void ensure_read_arrays(gis_batch *b, uint32_t n) {
     ensure<uint32_t>(b->nam_cnt, n);
     ensure<float>(b->frac, n);
     ensure<uint8_t>(b->flags, n);
-    ensure<uint64_t>(b->hit_off, n);
-    ensure<uint64_t>(b->rhit_off, n);
-    ensure<uint64_t>(b->nam_src, n);
+    ensure<uint64_t>(b->hit_off, n + 1);
+    ensure<uint64_t>(b->rhit_off, n + 1);
+    ensure<gis_nam>(b->nam_slots, (size_t)n * F_NAMCAP);
     ensure<uint64_t>(b->nam_off, n + 1);
     ensure<uint64_t>(b->scan_tmp, scan_tmp_words(n) + 8);
     ensure<DevCtl>(b->ctl, 1);
     ensure<uint32_t>(b->spill_ids, n);
     if (!b->h_ctl) HIP_CHECK(hipHostMalloc((void **)&b->h_ctl, sizeof(DevCtl)));
-    if (!b->h_total) HIP_CHECK(hipHostMalloc((void **)&b->h_total, sizeof(uint64_t)));
+    if (!b->h_total) HIP_CHECK(hipHostMalloc((void **)&b->h_total, 3 * sizeof(uint64_t)));
 }
 
 void ensure_slot_arrays(gis_batch *b, uint64_t slots) {
@@ -171,23 +176,22 @@ void ensure_slot_arrays(gis_batch *b, uint64_t slots) {
     ensure<uint32_t>(b->rcnt, slots);
 }
 
-// hit pools and NAM buffers; a read with h hits takes h + 1 pool units
-void ensure_pools(gis_batch *b, uint64_t hit_cap, uint64_t rhit_cap, uint64_t pool_cap, uint64_t out_cap) {
+// hit arrays, merge scratch and the NAM output (capacities in hits / NAMs)
+void ensure_pools(gis_batch *b, uint64_t hit_cap, uint64_t rhit_cap, uint64_t out_cap) {
     b->hit_cap = std::max(b->hit_cap, hit_cap);
     b->rhit_cap = std::max(b->rhit_cap, rhit_cap);
-    b->pool_cap = std::max(b->pool_cap, pool_cap);
     b->out_cap = std::max(b->out_cap, out_cap);
+    const uint64_t n = b->n_reads;
     ensure<DevHit>(b->hits, b->hit_cap);
     ensure<uint64_t>(b->keys, 2 * b->hit_cap);
-    ensure<uint32_t>(b->aux, 2 * b->hit_cap);
+    ensure<uint32_t>(b->aux, 2 * b->hit_cap + n + 1);
     ensure<NamWork>(b->nam_ws, b->hit_cap);
     ensure<gis_nam>(b->nam_fin, b->hit_cap);
     ensure<DevHit>(b->rhits, b->rhit_cap);
     ensure<uint64_t>(b->rkeys, 2 * b->rhit_cap);
-    ensure<uint32_t>(b->raux, 2 * b->rhit_cap);
+    ensure<uint32_t>(b->raux, 2 * b->rhit_cap + n + 1);
     ensure<NamWork>(b->rnam_ws, b->rhit_cap);
     ensure<gis_nam>(b->rnam_fin, b->rhit_cap);
-    ensure<gis_nam>(b->nam_pool, b->pool_cap);
     ensure<gis_nam>(b->nams_out, b->out_cap);
 }
 
@@ -242,9 +246,9 @@ int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const 
     // GIS_TEST_SMALL_POOLS starts them tiny so tests exercise the overflow path.
     const uint64_t bases = span;
     if (const char *e = std::getenv("GIS_TEST_SMALL_POOLS"); e && *e == '1')
-        ensure_pools(b, 64, 64, 16, 16);
+        ensure_pools(b, 64, 64, 16);
     else
-        ensure_pools(b, 48ull * n + bases / 8 + 4096, 8ull * n + 4096, 16ull * n + 1024, 16ull * n + 1024);
+        ensure_pools(b, 48ull * n + bases / 8 + 4096, 8ull * n + 4096, 16ull * n + 1024);
     ensure<char>(b->seqs, span);
     hipStream_t st = ctx->stream;
     if (span) HIP_CHECK(hipMemcpyAsync(b->seqs.p, seqs, span, hipMemcpyHostToDevice, st));
@@ -267,64 +271,107 @@ uint32_t rescue_cutoff_of(const gis_ctx *ctx, const gis_map_options &o) {
 
 enum { STOP_SEED = 0, STOP_HITS = 1, STOP_NONE = 2 };
 
-// One pass of the launch sequence. Returns the overflow bits.
-unsigned run_once(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_t hard, int stop) {
+// The launch sequence of one run, enqueued on the context's stream.
+void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_t hard, int stop) {
     gis_ctx *ctx = b->ctx;
     hipStream_t st = ctx->stream;
     const DevIndex ix = dev_index(ctx);
     const SeedParams sp = make_seed_params(ctx->ix.params);
     const DevBatch d = make_devbatch(b);
+    const uint32_t n = b->n_reads;
+    uint64_t *tmp = static_cast<uint64_t *>(b->scan_tmp.p);
     b->n_ev = 0;
     b->ev_names.clear();
     mark(b, "start");
     launch_reset_ctl(d.ctl, st);
-    ReadSet set{nullptr, nullptr, b->n_reads};
+    ReadSet set{nullptr, nullptr, n};
     const bool fused = mode == GIS_MODE_FUSED && stop == STOP_NONE;
     if (fused) {
         launch_fused(d, ix, sp, rcut, hard, st);
         mark(b, "fused");
-        set = ReadSet{d.spill_ids, &d.ctl->n_spill, b->n_reads};
+        set = ReadSet{d.spill_ids, &d.ctl->n_spill, n};
     }
     launch_seed(d, set, sp, b->want_sy ? 1 : 0, b->any_long ? 1 : 0, st);
     mark(b, "seed");
     if (stop != STOP_SEED) {
         launch_lookup(d, set, ix, rescue_mode, st);
         mark(b, "lookup");
+        launch_scan(d.hit_total, d.hit_off, n, tmp, st);
         launch_hits(d, set, ix, st);
         mark(b, "hits");
         launch_rescue_count(d, set, ix, rcut, hard, st);
+        launch_scan(d.rhit_total, d.rhit_off, n, tmp, st);
         launch_rescue_hits(d, set, ix, st);
         mark(b, "rescue");
+        HIP_CHECK(hipMemcpyAsync(b->h_total, d.hit_off + n, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->h_total + 1, d.rhit_off + n, 8, hipMemcpyDeviceToHost, st));
     }
     if (stop == STOP_NONE) {
         launch_merge(d, set, 0, st);
         mark(b, "merge");
         launch_merge(d, set, 1, st);
         mark(b, "rescue_merge");
-        launch_scan(d.nam_cnt, d.nam_off, b->n_reads, static_cast<uint64_t *>(b->scan_tmp.p), st);
+        launch_scan(d.nam_cnt, d.nam_off, n, tmp, st);
         launch_compact(d, fused ? 1 : 0, st);
         mark(b, "output");
-        HIP_CHECK(hipMemcpyAsync(b->h_total, d.nam_off + b->n_reads, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->h_total + 2, d.nam_off + n, 8, hipMemcpyDeviceToHost, st));
     }
     HIP_CHECK(hipMemcpyAsync(b->h_ctl, d.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, st));
+}
+
+// One run; with use_graph the sequence is captured once per configuration
+// (buffers, mode, cutoffs) and replayed. Returns the overflow bits.
+unsigned run_once(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_t hard, int stop,
+                  bool use_graph) {
+    gis_ctx *ctx = b->ctx;
+    hipStream_t st = ctx->stream;
+    if (use_graph) {
+        const DevBatch key = make_devbatch(b);
+        const int cfg[6] = {mode, rescue_mode, (int)rcut, (int)hard, stop,
+                            (b->want_sy ? 1 : 0) | (b->any_long ? 2 : 0)};
+        const bool same = b->graph_exec && std::memcmp(&key, &b->graph_key, sizeof key) == 0 &&
+                          std::memcmp(cfg, b->graph_cfg, sizeof cfg) == 0;
+        if (!same) {
+            if (b->graph_exec) HIP_CHECK(hipGraphExecDestroy(b->graph_exec));
+            if (b->graph) HIP_CHECK(hipGraphDestroy(b->graph));
+            b->graph_exec = nullptr;
+            b->graph = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_run(b, mode, rescue_mode, rcut, hard, stop);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIP_CHECK(hipStreamEndCapture(st, &b->graph));
+            HIP_CHECK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+            b->graph_key = key;
+            std::memcpy(b->graph_cfg, cfg, sizeof cfg);
+        }
+        HIP_CHECK(hipGraphLaunch(b->graph_exec, st));
+    } else {
+        enqueue_run(b, mode, rescue_mode, rcut, hard, stop);
+    }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(st));
+    const bool fused = mode == GIS_MODE_FUSED && stop == STOP_NONE;
     b->n_spill = fused ? b->h_ctl->n_spill : 0;
-    b->tot_nams = stop == STOP_NONE ? *b->h_total : 0;
+    b->tot_nams = stop == STOP_NONE ? b->h_total[2] : 0;
     return b->h_ctl->overflow;
 }
 
 void batch_run(gis_batch *b, const gis_map_options &o, int rescue_mode, int stop) {
     const uint32_t rcut = rescue_cutoff_of(b->ctx, o);
     const int mode = o.mode;
-    for (int attempt = 0; attempt < 8; attempt++) {
-        const unsigned ovf = run_once(b, mode, rescue_mode, rcut, o.hard_cutoff, stop);
+    const bool use_graph = stop == STOP_NONE && (o.flags & GIS_RUN_NO_GRAPH) == 0;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        const unsigned ovf = run_once(b, mode, rescue_mode, rcut, o.hard_cutoff, stop, use_graph);
         if (!ovf) break;
-        const DevCtl &c = *b->h_ctl;
-        ensure_pools(b, (ovf & OVF_HITS) ? std::max(2 * b->hit_cap, (uint64_t)c.hit_cursor + 4096) : 0,
-                     (ovf & OVF_RHITS) ? std::max(2 * b->rhit_cap, (uint64_t)c.rhit_cursor + 4096) : 0,
-                     (ovf & OVF_NAMPOOL) ? std::max(2 * b->pool_cap, (uint64_t)c.nam_cursor + 1024) : 0,
-                     (ovf & OVF_OUT) ? std::max(2 * b->out_cap, b->tot_nams + 1024) : 0);
+        ensure_pools(b, (ovf & OVF_HITS) ? b->h_total[0] + b->h_total[0] / 8 + 4096 : 0,
+                     (ovf & OVF_RHITS) ? b->h_total[1] + b->h_total[1] / 8 + 4096 : 0,
+                     (ovf & OVF_OUT) ? b->tot_nams + b->tot_nams / 8 + 1024 : 0);
     }
     b->last_mode = mode;
     b->ran = stop == STOP_NONE;
@@ -466,35 +513,31 @@ static int copy_hits_out(gis_ctx *ctx, gis_batch *b, uint64_t *out_offsets, gis_
                          uint64_t capacity, uint64_t *n_out) {
     const uint32_t n = b->n_reads;
     std::vector<uint8_t> flags(n);
-    std::vector<uint64_t> ho(n), rho(n);
-    std::vector<uint32_t> ht(n), rht(n);
-    if (n) {
-        HIP_CHECK(hipMemcpy(flags.data(), b->flags.p, n, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(ho.data(), b->hit_off.p, 8ull * n, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(rho.data(), b->rhit_off.p, 8ull * n, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(ht.data(), b->hit_total.p, 4ull * n, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(rht.data(), b->rhit_total.p, 4ull * n, hipMemcpyDeviceToHost));
-    }
+    std::vector<uint64_t> ho(n + 1), rho(n + 1);
+    if (n) HIP_CHECK(hipMemcpy(flags.data(), b->flags.p, n, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(ho.data(), b->hit_off.p, 8ull * (n + 1), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(rho.data(), b->rhit_off.p, 8ull * (n + 1), hipMemcpyDeviceToHost));
+    std::vector<DevHit> h(ho[n]), rh(rho[n]);
+    if (ho[n]) HIP_CHECK(hipMemcpy(h.data(), b->hits.p, sizeof(DevHit) * ho[n], hipMemcpyDeviceToHost));
+    if (rho[n]) HIP_CHECK(hipMemcpy(rh.data(), b->rhits.p, sizeof(DevHit) * rho[n], hipMemcpyDeviceToHost));
     uint64_t total = 0;
-    for (uint32_t r = 0; r < n; r++) total += (flags[r] & GIS_FLAG_RESCUED) ? rht[r] : ht[r];
+    for (uint32_t r = 0; r < n; r++)
+        total += (flags[r] & GIS_FLAG_RESCUED) ? rho[r + 1] - rho[r] : ho[r + 1] - ho[r];
     if (n_out) *n_out = total;
     if (total > capacity || (total && !out)) return fail(ctx, GIS_ERR_CAPACITY, "hit capacity too small");
     uint64_t o = 0;
-    std::vector<DevHit> buf;
+    std::vector<uint64_t> idx;
     for (uint32_t r = 0; r < n; r++) {
         if (out_offsets) out_offsets[r] = o;
         const bool resc = (flags[r] & GIS_FLAG_RESCUED) != 0;
-        const uint32_t cnt = resc ? rht[r] : ht[r];
-        if (cnt == 0) continue;
-        buf.resize(cnt);
-        const DevHit *src = static_cast<const DevHit *>(resc ? b->rhits.p : b->hits.p) + (resc ? rho[r] : ho[r]);
-        HIP_CHECK(hipMemcpy(buf.data(), src, sizeof(DevHit) * cnt, hipMemcpyDeviceToHost));
+        const std::vector<DevHit> &src = resc ? rh : h;
+        const uint64_t lo = resc ? rho[r] : ho[r], hi = resc ? rho[r + 1] : ho[r + 1];
         // group by (strand, ref) keeping the emission order inside a group
-        std::vector<uint64_t> idx(cnt);
-        for (uint32_t i = 0; i < cnt; i++) idx[i] = ((uint64_t)buf[i].ref_strand << 32) | i;
+        idx.clear();
+        for (uint64_t i = lo; i < hi; i++) idx.push_back(((uint64_t)src[i].ref_strand << 32) | (i - lo));
         std::sort(idx.begin(), idx.end());
         for (uint64_t key : idx) {
-            const DevHit &x = buf[(uint32_t)key];
+            const DevHit &x = src[lo + (uint32_t)key];
             gis_hit g;
             g.query_start = x.query_start;
             g.query_end = x.query_end;
@@ -581,7 +624,7 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
     b->h_rs_off = off;
     ensure_read_arrays(b, n);
     ensure_slot_arrays(b, total);
-    ensure_pools(b, 64ull * n + 4096, 16ull * n + 4096, 16ull * n + 1024, 16ull * n + 1024);
+    ensure_pools(b, 64ull * n + 4096, 16ull * n + 4096, 16ull * n + 1024);
     b->tot_rs_cap = total;
     hipStream_t st = ctx->stream;
     if (total) {
@@ -598,28 +641,32 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
     // the stages after seeding, with the rescue decision forced either way
     const DevIndex ix = dev_index(ctx);
     const ReadSet set{nullptr, nullptr, n};
-    for (int attempt = 0; attempt < 8; attempt++) {
+    uint64_t *tmp = static_cast<uint64_t *>(b->scan_tmp.p);
+    for (int attempt = 0; attempt < 4; attempt++) {
         const DevBatch d = make_devbatch(b);
         launch_reset_ctl(d.ctl, st);
         launch_lookup(d, set, ix, rescue ? 2 : 1, st);
+        launch_scan(d.hit_total, d.hit_off, n, tmp, st);
         launch_hits(d, set, ix, st);
         launch_rescue_count(d, set, ix, rescue_cutoff, hard_cutoff, st);
+        launch_scan(d.rhit_total, d.rhit_off, n, tmp, st);
         launch_rescue_hits(d, set, ix, st);
         launch_merge(d, set, 0, st);
         launch_merge(d, set, 1, st);
-        launch_scan(d.nam_cnt, d.nam_off, n, static_cast<uint64_t *>(b->scan_tmp.p), st);
+        launch_scan(d.nam_cnt, d.nam_off, n, tmp, st);
         launch_compact(d, 0, st);
-        HIP_CHECK(hipMemcpyAsync(b->h_total, d.nam_off + n, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->h_total, d.hit_off + n, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->h_total + 1, d.rhit_off + n, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->h_total + 2, d.nam_off + n, 8, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipMemcpyAsync(b->h_ctl, d.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(st));
-        b->tot_nams = *b->h_total;
+        b->tot_nams = b->h_total[2];
         const unsigned ovf = b->h_ctl->overflow;
         if (!ovf) break;
-        const DevCtl &c = *b->h_ctl;
-        ensure_pools(b, (ovf & OVF_HITS) ? std::max(2 * b->hit_cap, (uint64_t)c.hit_cursor + 4096) : 0,
-                     (ovf & OVF_RHITS) ? std::max(2 * b->rhit_cap, (uint64_t)c.rhit_cursor + 4096) : 0, 0,
-                     (ovf & OVF_OUT) ? std::max(2 * b->out_cap, b->tot_nams + 1024) : 0);
+        ensure_pools(b, (ovf & OVF_HITS) ? b->h_total[0] + 4096 : 0,
+                     (ovf & OVF_RHITS) ? b->h_total[1] + 4096 : 0,
+                     (ovf & OVF_OUT) ? b->tot_nams + 1024 : 0);
     }
     b->ran = true;
     return batch_download(b, out);

@@ -36,17 +36,14 @@ struct NamWork {
     uint16_t query_start, query_end, query_prev, n_hits;
 };
 
-// Device control block of one batch run: bump allocators with capacity checks
-// and the spill list of the fused kernel. Reset by k_reset_ctl before a run.
+// Device control block of one batch run: the spill list length of the fused
+// kernel and the overflow bits of the capacity checks. Reset by k_reset_ctl.
 struct DevCtl {
-    unsigned long long hit_cursor;   // units of the normal-path hit pool in use
-    unsigned long long rhit_cursor;  // units of the rescue hit pool in use
-    unsigned long long nam_cursor;   // NAMs of the fused kernel's pool in use
-    unsigned int n_spill;            // reads handed from the fused kernel to the per-stage kernels
-    unsigned int overflow;           // OVF_* bits
+    unsigned int n_spill;   // reads handed from the fused kernel to the per-stage kernels
+    unsigned int overflow;  // OVF_* bits
 };
 
-enum : unsigned { OVF_HITS = 1, OVF_RHITS = 2, OVF_NAMPOOL = 4, OVF_OUT = 8 };
+enum : unsigned { OVF_HITS = 1, OVF_RHITS = 2, OVF_OUT = 8 };
 
 // the public GIS_FLAG_SPILLED bit (include/gis.h)
 enum : uint8_t { FLAG_SPILL = GIS_FLAG_SPILLED };
@@ -89,10 +86,10 @@ struct DevBatch {
     float *frac;
     uint8_t *flags;
     uint32_t *nam_cnt;
-    uint64_t *hit_off;        // bump-allocated region of the read's hits (units)
-    uint64_t *rhit_off;
-    uint64_t *nam_src;        // fused kernel: offset of the read's NAMs in nam_pool
-    // hit pools: a read with h hits takes h + 1 units; keys and aux take 2 per unit
+    uint64_t *hit_off;        // [n+1] scan of hit_total: the read's hits start here
+    uint64_t *rhit_off;       // [n+1] scan of rhit_total
+    // hit arrays (capacity hit_cap / rhit_cap hits); merge scratch per read r
+    // with h hits at hit offset o: keys[2o .. 2o+2h), aux[2o+r .. 2o+r+2h+1)
     DevHit *hits;
     DevHit *rhits;
     uint64_t hit_cap, rhit_cap;
@@ -100,9 +97,8 @@ struct DevBatch {
     uint32_t *aux, *raux;
     NamWork *nam_ws, *rnam_ws;
     gis_nam *nam_fin, *rnam_fin;
-    // fused kernel NAM pool
-    gis_nam *nam_pool;
-    uint64_t nam_pool_cap;
+    // fused kernel: NAM slots of F_NAMCAP per read
+    gis_nam *nam_slots;
     // dense output
     uint64_t *nam_off;        // [n+1]
     gis_nam *nams_out;
@@ -118,6 +114,7 @@ constexpr int LMAX_LDS = 320;  // reads up to this length seed from LDS
 constexpr int SEED_WPB = 4;    // waves per block of the seeding kernel
 constexpr int WPB = 4;         // waves per block of the per-read kernels
 constexpr int MERGE_LDS = 128; // hits a read may have for the LDS merge
+constexpr int F_NAMCAP = 32;   // NAMs a read may have in the fused kernel
 
 // launch wrappers (kernels.hip)
 void launch_reset_ctl(DevCtl *ctl, hipStream_t st);

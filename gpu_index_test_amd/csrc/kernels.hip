@@ -3,8 +3,9 @@
 // One wavefront (64 lanes) owns one read in every per-read kernel. Lanes
 // work on positions, windows, seeds or hits of that read in parallel and the
 // wave synchronises through LDS or through its own scratch regions. Reads are
-// independent: waves only meet in three atomic bump allocators and the spill
-// list, whose order never changes a read's result.
+// independent: variable-size per-read outputs are placed by scans of per-read
+// counts, and waves only meet in the fused kernel's spill list, whose order
+// never changes a read's result.
 //
 // Two forms share the device functions below:
 //  - per-stage: k_seed -> k_lookup -> k_hits -> k_rescue_count ->
@@ -495,26 +496,24 @@ __global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, ReadSet set, De
             const uint8_t keep = keep_spill ? (uint8_t)(b.flags[r] & FLAG_SPILL) : (uint8_t)0;
             b.flags[r] = (uint8_t)(keep | (rescue ? GIS_FLAG_RESCUED : 0) |
                                    (n == 0 ? GIS_FLAG_NO_RANDSTROBES : 0));
-            uint32_t keep_hits = rescue ? 0u : hits;  // rescued reads discard these hits
-            uint64_t off = 0;
-            if (keep_hits) {
-                off = atomicAdd(&b.ctl->hit_cursor, (unsigned long long)keep_hits + 1);
-                if (off + keep_hits + 1 > b.hit_cap) {
-                    atomicOr(&b.ctl->overflow, OVF_HITS);
-                    keep_hits = 0;
-                }
-            }
-            b.hit_total[r] = keep_hits;
-            b.hit_off[r] = off;
+            b.hit_total[r] = rescue ? 0u : hits;  // rescued reads discard these hits
             b.rhit_total[r] = 0;
-            b.rhit_off[r] = 0;
         }
     }
 }
 
 // write the hits of non-rescued reads into their region (same walk as k_lookup)
+// capacity check of a hit array against the scan total (wave-uniform)
+__device__ __forceinline__ bool over_capacity(const uint64_t *off, uint32_t n, uint64_t cap,
+                                              DevCtl *ctl, unsigned bit) {
+    if (off[n] <= cap) return false;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&ctl->overflow, bit);
+    return true;
+}
+
 __global__ __launch_bounds__(64 * WPB) void k_hits(DevBatch b, ReadSet set, DevIndex ix) {
     const uint32_t lane = lane_id();
+    if (over_capacity(b.hit_off, b.n_reads, b.hit_cap, b.ctl, OVF_HITS)) return;
     const uint32_t cnt = set_count(set);
     for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
         const uint32_t r = set_at(set, w);
@@ -597,23 +596,13 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_count(DevBatch b, ReadSet s
             }
             total += wave_sum(rh);
         }
-        if (lane == 0) {
-            uint64_t off = 0;
-            if (total) {
-                off = atomicAdd(&b.ctl->rhit_cursor, (unsigned long long)total + 1);
-                if (off + total + 1 > b.rhit_cap) {
-                    atomicOr(&b.ctl->overflow, OVF_RHITS);
-                    total = 0;
-                }
-            }
-            b.rhit_total[r] = total;
-            b.rhit_off[r] = off;
-        }
+        if (lane == 0) b.rhit_total[r] = total;
     }
 }
 
 __global__ __launch_bounds__(64 * WPB) void k_rescue_hits(DevBatch b, ReadSet set, DevIndex ix) {
     const uint32_t lane = lane_id();
+    if (over_capacity(b.rhit_off, b.n_reads, b.rhit_cap, b.ctl, OVF_RHITS)) return;
     const uint32_t cnt = set_count(set);
     for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
         const uint32_t r = set_at(set, w);
@@ -847,6 +836,9 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
     __shared__ MergeLds lds[WPB];
     const uint32_t lane = lane_id();
     const int wid = (int)(threadIdx.x >> 6);
+    if (rescue ? over_capacity(b.rhit_off, b.n_reads, b.rhit_cap, b.ctl, OVF_RHITS)
+               : over_capacity(b.hit_off, b.n_reads, b.hit_cap, b.ctl, OVF_HITS))
+        return;
     const uint32_t cnt = set_count(set);
     for (uint32_t w = blockIdx.x * WPB + (uint32_t)wid; w < cnt; w += gridDim.x * WPB) {
         const uint32_t r = set_at(set, w);
@@ -870,7 +862,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
             merge_emit(nh, G, S.keys, S.aux, S.ws, fin);
         } else {
             uint64_t *keys = (rescue ? b.rkeys : b.keys) + 2 * hb;
-            uint32_t *aux = (rescue ? b.raux : b.aux) + 2 * hb;
+            uint32_t *aux = (rescue ? b.raux : b.aux) + 2 * hb + r;
             NamWork *ws = (rescue ? b.rnam_ws : b.nam_ws) + hb;
             const uint32_t G = merge_build(nh, ghits, keys, aux, ws);
             total = merge_count(nh, G, aux);
@@ -905,6 +897,20 @@ union alignas(8) FusedLds {
     FusedSeedLds seed;
     MergeLds merge;
 };
+
+// hand read r to the per-stage kernels (reads spill rarely, so one atomic
+// per spilled read does not contend)
+__device__ __forceinline__ void spill_read(const DevBatch &b, uint32_t r) {
+    if (lane_id() == 0) {
+        b.flags[r] = FLAG_SPILL;
+        b.nam_cnt[r] = 0;
+        b.hit_total[r] = 0;
+        b.rhit_total[r] = 0;
+        const unsigned int at = atomicAdd(&b.ctl->n_spill, 1u);
+        b.spill_ids[at] = r;
+    }
+    wave_sync_mem();
+}
 
 __global__ __launch_bounds__(64 * F_WPB) void k_fused(DevBatch b, DevIndex ix, SeedParams P,
                                                       uint32_t rescue_cutoff, uint32_t hard_cutoff) {
@@ -980,37 +986,24 @@ __global__ __launch_bounds__(64 * F_WPB) void k_fused(DevBatch b, DevIndex ix, S
             }
         }
         if (spill) {
-            if (lane == 0) {
-                b.flags[r] = FLAG_SPILL;
-                b.nam_cnt[r] = 0;
-                const unsigned int at = atomicAdd(&b.ctl->n_spill, 1u);
-                b.spill_ids[at] = r;
-            }
-            wave_sync_mem();
+            spill_read(b, r);
             continue;
         }
         uint32_t total = 0, G = 0;
-        uint64_t base = 0;
         if (nh > 0) {
             G = merge_build(nh, S.merge.hits, S.merge.keys, S.merge.aux, S.merge.ws);
             total = merge_count(nh, G, S.merge.aux);
         }
-        if (total > 0) {
-            unsigned long long bl = 0;
-            if (lane == 0) bl = atomicAdd(&b.ctl->nam_cursor, (unsigned long long)total);
-            base = __shfl(bl, 0, 64);
-            if (base + total > b.nam_pool_cap) {
-                if (lane == 0) atomicOr(&b.ctl->overflow, OVF_NAMPOOL);
-                total = 0;
-            } else {
-                merge_emit(nh, G, S.merge.keys, S.merge.aux, S.merge.ws, b.nam_pool + base);
-            }
+        if (total > (uint32_t)F_NAMCAP) {
+            spill_read(b, r);
+            continue;
         }
+        if (total > 0)
+            merge_emit(nh, G, S.merge.keys, S.merge.aux, S.merge.ws, b.nam_slots + (uint64_t)r * F_NAMCAP);
         if (lane == 0) {
             b.frac[r] = frac;
             b.flags[r] = (uint8_t)((rescue ? GIS_FLAG_RESCUED : 0) | (n_fwd == 0 ? GIS_FLAG_NO_RANDSTROBES : 0));
             b.nam_cnt[r] = total;
-            b.nam_src[r] = base;
             b.hit_total[r] = 0;
             b.rhit_total[r] = 0;
         }
@@ -1032,7 +1025,7 @@ __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
         if (c == 0) continue;
         const uint8_t fl = b.flags[r];
         const gis_nam *srcn;
-        if (fused && !(fl & FLAG_SPILL)) srcn = b.nam_pool + b.nam_src[r];
+        if (fused && !(fl & FLAG_SPILL)) srcn = b.nam_slots + (uint64_t)r * F_NAMCAP;
         else if (fl & GIS_FLAG_RESCUED) srcn = b.rnam_fin + b.rhit_off[r];
         else srcn = b.nam_fin + b.hit_off[r];
         const uint32_t *src = reinterpret_cast<const uint32_t *>(srcn);
@@ -1044,9 +1037,6 @@ __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
 
 __global__ void k_reset_ctl(DevCtl *ctl) {
     if (threadIdx.x == 0) {
-        ctl->hit_cursor = 0;
-        ctl->rhit_cursor = 0;
-        ctl->nam_cursor = 0;
         ctl->n_spill = 0;
         ctl->overflow = 0;
     }

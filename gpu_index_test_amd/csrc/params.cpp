@@ -79,5 +79,5 @@ extern "C" void gis_map_options_default(gis_map_options *opt) {
     opt->rescue_level = 2;   // CommandLineOptions::rescue_level, step2/cmdline.hpp:56
     opt->hard_cutoff = 1000; // find_nams_rescue, step2/nam.cu:964
     opt->mode = GIS_MODE_PLAIN;
-    opt->reserved_ = 0;
+    opt->flags = 0;
 }

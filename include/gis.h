@@ -127,10 +127,16 @@ typedef struct {
     int32_t rescue_level;  /* default 2 */
     uint32_t hard_cutoff;  /* default 1000 */
     int32_t mode;          /* GIS_MODE_* */
-    int32_t reserved_;
+    int32_t flags;         /* GIS_RUN_* */
 } gis_map_options;
 
+/* GIS_MODE_PLAIN: one launch per stage (seed, lookup, hits, rescue, merge).
+ * GIS_MODE_FUSED: one wave-per-read kernel for the whole path; reads beyond
+ * its LDS budget go through the per-stage kernels. Same NAMs either way. */
 enum { GIS_MODE_PLAIN = 0, GIS_MODE_FUSED = 1 };
+/* GIS_RUN_NO_GRAPH: enqueue the launches one by one instead of replaying the
+ * batch's captured HIP graph. */
+enum { GIS_RUN_NO_GRAPH = 1 };
 
 typedef struct gis_ctx gis_ctx;
 

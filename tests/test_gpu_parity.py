@@ -298,3 +298,28 @@ def test_fused_spills_and_mapper(setup):
             assert (~spilled).sum() > 0.8 * len(l)
             assert spilled[np.asarray(l) > 320].all()
     m.close()
+
+
+@pytest.mark.parametrize("mode", [G.MODE_PLAIN, G.MODE_FUSED])
+def test_graph_replays_match_eager(setup, mode):
+    """A batch's captured launch sequence replayed several times, with eager
+    (non-graph) work of the same context and synchronisations in between;
+    every replay equals the eager result and the oracle."""
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 500, seed=66)
+    s, o, l = G.pack_reads(reads)
+    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
+    m = G.Mapper(idx, len(l), len(s))
+    m.upload(s, o, l)
+    other = _read_set(g, lens, 200, seed=67)
+    s2, o2, l2 = G.pack_reads(other)
+    for rep in range(4):
+        m.run(mode=mode, graph=True)
+        res = m.download()
+        _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
+                      res.flags, efl)
+        eager = G.find_nams_batch(idx, s2, o2, l2, mode=mode, graph=False)
+        assert len(eager.nam_offsets) == len(l2) + 1
+    eager_same = G.find_nams_batch(idx, s, o, l, mode=mode, graph=False)
+    assert eager_same.nams.tobytes() == en.tobytes()
+    m.close()
