@@ -40,7 +40,20 @@ def main():
                     help="bounded sample of the workload timed on the CPU oracle")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_probe} MiB device to device")
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the read-sharded "
@@ -77,36 +90,46 @@ def main():
         m.upload(seqs[base:end], offs[lo:hi] - base, lens[lo:hi])
         mappers.append(m)
 
-    def timed(mode):
+    def timed(mode, graph):
         # set-up: one untimed run per batch so every device buffer has been
-        # allocated and touched before timing
+        # allocated and touched (and its launch graph captured) before timing
         for m in mappers:
-            m.run(mode=mode)
+            m.run(mode=mode, graph=graph)
         for i in range(args.warmup):
-            mappers[i % nb].run(mode=mode)
+            mappers[i % nb].run(mode=mode, graph=graph)
         step_ms, reads_done = [], 0
         t_start = time.perf_counter()
         for i in range(args.steps):
             m = mappers[i % nb]
             ts = time.perf_counter()
-            m.run(mode=mode)  # ends with a stream synchronisation
+            m.run(mode=mode, graph=graph)  # ends with a stream synchronisation
             step_ms.append((time.perf_counter() - ts) * 1e3)
             reads_done += m.n
         elapsed = time.perf_counter() - t_start
         return reads_done / elapsed, elapsed, step_ms
 
+    def kernel_times(mode):
+        # the same runs enqueued launch by launch, with HIP events between the
+        # kernels on the batch's stream (graph replays carry no events)
+        out = []
+        for i in range(min(args.steps, nb)):
+            mappers[i].run(mode=mode, graph=False)
+            out.append(mappers[i].stage_times())
+        return out
+
     # the fused form (one wave-per-read kernel, spills to the per-stage
     # kernels), reported beside the plain per-stage drop-in
-    fused_rps, fused_elapsed, fused_ms = timed(G.MODE_FUSED)
-    fused_stage = mappers[(args.steps - 1) % nb].stage_times()
-    log(f"fused form: {fused_rps / 1e6:.1f} M reads/s; stage ms " +
-        ", ".join(f"{k}={v:.3f}" for k, v in fused_stage.items()))
+    fused_rps, fused_elapsed, fused_ms = timed(G.MODE_FUSED, True)
+    fused_eager_rps, _, _ = timed(G.MODE_FUSED, False)
+    fused_stage = kernel_times(G.MODE_FUSED)[-1]
+    log(f"fused form: {fused_rps / 1e6:.1f} M reads/s (graph), {fused_eager_rps / 1e6:.1f} M reads/s "
+        "(eager); stage ms " + ", ".join(f"{k}={v:.3f}" for k, v in fused_stage.items()))
     # the plain per-stage drop-in: the configuration's form and the value
-    reads_per_s, elapsed, step_ms = timed(G.MODE_PLAIN)
-    # per-kernel times of the timed runs (HIP events on the batch stream; each
-    # batch keeps the times of its last run)
-    lookup_ms = [mappers[i % nb].stage_times().get("lookup", float("nan"))
-                 for i in range(min(args.steps, nb))]
+    plain_eager_rps, _, _ = timed(G.MODE_PLAIN, False)
+    reads_per_s, elapsed, step_ms = timed(G.MODE_PLAIN, True)
+    log(f"plain form: {reads_per_s / 1e6:.1f} M reads/s (graph), {plain_eager_rps / 1e6:.1f} M reads/s (eager)")
+    stages = kernel_times(G.MODE_PLAIN)
+    lookup_ms = [st["lookup"] for st in stages]
 
     # algorithmic bytes of the lookup kernel per launch (per batch), counted
     # on the device during the run
@@ -119,8 +142,8 @@ def main():
         lk_bytes.append(b.value)
         lk_n.append(q.value)
     achieved_gbs = float(np.sum(lk_bytes) / (np.sum(lookup_ms) * 1e-3) / 1e9)
-    stage_last = mappers[(args.steps - 1) % nb].stage_times()
-    log("stage ms (last step): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_last.items()))
+    stage_last = stages[-1]
+    log("stage ms (last batch): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_last.items()))
     log(f"lookup: {np.mean(lk_bytes) / 1e6:.1f} MB algorithmic per launch, "
         f"{np.mean(lookup_ms):.3f} ms avg, {np.mean(lk_n):.0f} lookups per launch")
 
@@ -180,7 +203,9 @@ def main():
             "avg_launch_ms": float(np.mean(lookup_ms)),
         },
         "cpu_baseline": cpu,
-        "fused_form": {"value": fused_rps, "unit": "reads/s",
+        "plain_eager_value": plain_eager_rps,
+        "stage_ms": {k: float(np.mean([st[k] for st in stages])) for k in stages[0]},
+        "fused_form": {"value": fused_rps, "unit": "reads/s", "eager_value": fused_eager_rps,
                        "ms_per_step": fused_elapsed * 1e3 / args.steps,
                        "p50_batch_ms": float(np.percentile(fused_ms, 50)),
                        "p99_batch_ms": float(np.percentile(fused_ms, 99))},

@@ -38,6 +38,7 @@ struct gis_batch {
     uint64_t tot_nams = 0;
     uint32_t n_spill = 0;
     int last_mode = GIS_MODE_PLAIN;
+    bool last_graph = false;
     bool ran = false;
     // pinned landing area for the end-of-run readback
     DevCtl *h_ctl = nullptr;
@@ -374,6 +375,7 @@ void batch_run(gis_batch *b, const gis_map_options &o, int rescue_mode, int stop
                      (ovf & OVF_OUT) ? b->tot_nams + b->tot_nams / 8 + 1024 : 0);
     }
     b->last_mode = mode;
+    b->last_graph = use_graph;
     b->ran = stop == STOP_NONE;
 }
 
@@ -732,6 +734,8 @@ extern "C" int gis_batch_stage_times(gis_batch *b, double *ms, int cap, int *n, 
     if (!b) return GIS_ERR_BAD_PARAMETER;
     gis_ctx *ctx = b->ctx;
     API_BEGIN(ctx)
+    if (b->last_graph)
+        return fail(ctx, GIS_ERR_STATE, "stage times need a run with GIS_RUN_NO_GRAPH (graph replays carry no events)");
     std::string nm;
     int k = 0;
     for (int i = 1; i < b->n_ev && k < cap; i++, k++) {
