@@ -1,0 +1,74 @@
+// Drives the C++ façade (include/gis.hpp) the way a reference caller would:
+// StrobemerIndex::populate, then per read randstrobes_query -> find_nams ->
+// the caller's rescue decision -> find_nams_rescue (step2/main.cu:3209-3216),
+// and the same reads again through one device-resident Batch.
+//
+// usage: facade_check <refs.fa> <reads.txt> <out_prefix>
+// writes <out_prefix>.single and <out_prefix>.batch, each:
+//   u64 n_reads, u64 offsets[n_reads + 1], gis_nam nams[offsets[n_reads]]
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "../../include/gis.hpp"
+
+static void write_out(const std::string &path, const std::vector<uint64_t> &offs,
+                      const std::vector<gis::Nam> &nams) {
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) throw std::runtime_error("cannot write " + path);
+    const uint64_t n = offs.size() - 1;
+    std::fwrite(&n, 8, 1, f);
+    std::fwrite(offs.data(), 8, offs.size(), f);
+    if (!nams.empty()) std::fwrite(nams.data(), sizeof(gis::Nam), nams.size(), f);
+    std::fclose(f);
+}
+
+int main(int argc, char **argv) {
+    if (argc != 4) {
+        std::cerr << "usage: facade_check <refs.fa> <reads.txt> <out_prefix>\n";
+        return 2;
+    }
+    try {
+        const gis::References refs = gis::References::from_fasta(argv[1]);
+        const auto params = gis::IndexParameters::from_read_length(150);
+        gis::StrobemerIndex index(refs, params);
+        index.populate(0.0002f, 4);
+        std::vector<std::string> reads;
+        {
+            std::ifstream in(argv[2]);
+            std::string line;
+            while (std::getline(in, line))
+                if (!line.empty()) reads.push_back(line);
+        }
+        const unsigned rescue_cutoff = 2 * index.filter_cutoff();
+        std::vector<uint64_t> offs{0};
+        std::vector<gis::Nam> all;
+        for (const auto &r : reads) {
+            const auto q = gis::randstrobes_query(r, index);
+            auto [frac, nams] = gis::find_nams(q, index);
+            if ((double)frac < 0.7 || nams.empty()) nams = gis::find_nams_rescue(q, index, rescue_cutoff);
+            all.insert(all.end(), nams.begin(), nams.end());
+            offs.push_back(all.size());
+        }
+        write_out(std::string(argv[3]) + ".single", offs, all);
+
+        uint64_t bases = 0;
+        std::vector<std::string_view> views;
+        for (const auto &r : reads) {
+            bases += r.size();
+            views.emplace_back(r);
+        }
+        gis::Batch batch(index, (uint32_t)reads.size(), bases);
+        batch.upload(views);
+        batch.run();
+        const gis::BatchNams res = batch.download();
+        write_out(std::string(argv[3]) + ".batch", res.offsets, res.nams);
+        std::cout << "facade_check: " << reads.size() << " reads, " << all.size() << " NAMs\n";
+    } catch (const gis::Error &e) {
+        std::cerr << "gis error " << e.code() << ": " << e.what() << "\n";
+        return 1;
+    }
+    return 0;
+}

@@ -85,3 +85,16 @@ def test_error_free_synthetic_read_is_a_genome_substring():
     for i in range(50):
         read = s[int(o[i]): int(o[i]) + int(ln[i])].tobytes()
         assert gb[int(t["pos"][i]): int(t["pos"][i]) + int(ln[i])] == read
+
+
+def test_cxx_facade_compiles():
+    """The C++ façade (include/gis.hpp) compiles on its own with g++ and the
+    façade check program links against libgis.so."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cxx", "facade_check.cpp")
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", src],
+                   check=True, timeout=120)
+    exe = os.path.join(ROOT, "gpu_index_test_amd", "_lib", "facade_check")
+    assert os.path.exists(exe), "build() compiles tests/cxx/facade_check.cpp"
+    out = subprocess.run(["ldd", exe], capture_output=True, text=True, check=True).stdout
+    assert "libgis.so" in out and "not found" not in out

@@ -323,3 +323,38 @@ def test_graph_replays_match_eager(setup, mode):
     eager_same = G.find_nams_batch(idx, s, o, l, mode=mode, graph=False)
     assert eager_same.nams.tobytes() == en.tobytes()
     m.close()
+
+
+def test_cxx_facade_matches_oracle(setup, tmp_path):
+    """A C++ caller of include/gis.hpp (per-read randstrobes_query ->
+    find_nams -> rescue decision -> find_nams_rescue, and one Batch) gets
+    the oracle's NAMs."""
+    import os
+    import subprocess
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 150, seed=21)
+    reads = [r for r in reads if len(r) > 0 and b"\n" not in r]
+    fa = tmp_path / "refs.fa"
+    with open(fa, "wb") as fh:
+        pos = 0
+        for i, L in enumerate(lens):
+            fh.write(b">contig%d description\n" % i)
+            seq = g[pos: pos + int(L)].tobytes()
+            for j in range(0, len(seq), 80):
+                fh.write(seq[j: j + 80] + b"\n")
+            pos += int(L)
+    rd = tmp_path / "reads.txt"
+    rd.write_bytes(b"\n".join(reads) + b"\n")
+    exe = os.path.join(os.path.dirname(G.__file__), "_lib", "facade_check")
+    out = str(tmp_path / "res")
+    subprocess.run([exe, str(fa), str(rd), out], check=True, timeout=120)
+    s, o, l = G.pack_reads(reads)
+    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 4)
+    for suffix in ("single", "batch"):
+        raw = open(out + "." + suffix, "rb").read()
+        n = int(np.frombuffer(raw[:8], dtype=np.uint64)[0])
+        offs = np.frombuffer(raw[8: 8 + 8 * (n + 1)], dtype=np.uint64)
+        nams = np.frombuffer(raw[8 + 8 * (n + 1):], dtype=G.NAM)
+        assert n == len(reads)
+        assert np.array_equal(offs, eo), suffix
+        assert nams.tobytes() == en.tobytes(), suffix
