@@ -367,13 +367,18 @@ void batch_run(gis_batch *b, const gis_map_options &o, int rescue_mode, int stop
     const uint32_t rcut = rescue_cutoff_of(b->ctx, o);
     const int mode = o.mode;
     const bool use_graph = stop == STOP_NONE && (o.flags & GIS_RUN_NO_GRAPH) == 0;
+    unsigned ovf = 0;
     for (int attempt = 0; attempt < 4; attempt++) {
-        const unsigned ovf = run_once(b, mode, rescue_mode, rcut, o.hard_cutoff, stop, use_graph);
+        ovf = run_once(b, mode, rescue_mode, rcut, o.hard_cutoff, stop, use_graph);
         if (!ovf) break;
+        // after a hit overflow the merges did not run, so the NAM total is
+        // meaningless: grow the hit arrays now and size the output next time
+        if (ovf & (OVF_HITS | OVF_RHITS)) ovf &= ~(unsigned)OVF_OUT;
         ensure_pools(b, (ovf & OVF_HITS) ? b->h_total[0] + b->h_total[0] / 8 + 4096 : 0,
                      (ovf & OVF_RHITS) ? b->h_total[1] + b->h_total[1] / 8 + 4096 : 0,
                      (ovf & OVF_OUT) ? b->tot_nams + b->tot_nams / 8 + 1024 : 0);
     }
+    if (ovf) throw HipError{hipErrorOutOfMemory, "batch arrays still too small after regrowth"};
     b->last_mode = mode;
     b->last_graph = use_graph;
     b->ran = stop == STOP_NONE;
@@ -664,8 +669,10 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(st));
         b->tot_nams = b->h_total[2];
-        const unsigned ovf = b->h_ctl->overflow;
+        unsigned ovf = b->h_ctl->overflow;
         if (!ovf) break;
+        if (attempt == 3) throw HipError{hipErrorOutOfMemory, "batch arrays still too small after regrowth"};
+        if (ovf & (OVF_HITS | OVF_RHITS)) ovf &= ~(unsigned)OVF_OUT;
         ensure_pools(b, (ovf & OVF_HITS) ? b->h_total[0] + 4096 : 0,
                      (ovf & OVF_RHITS) ? b->h_total[1] + 4096 : 0,
                      (ovf & OVF_OUT) ? b->tot_nams + 1024 : 0);

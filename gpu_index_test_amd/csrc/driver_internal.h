@@ -19,7 +19,7 @@ struct DevBuf {
 
 struct HipError {
     hipError_t e;
-    const char *what;
+    std::string what;
 };
 
 #define HIP_CHECK(call)                                  \
@@ -37,7 +37,11 @@ T *ensure(DevBuf &b, size_t n) {
         if (b.p) HIP_CHECK(hipFree(b.p));
         b.p = nullptr;
         b.bytes = 0;
-        HIP_CHECK(hipMalloc(&b.p, alloc));
+        const hipError_t e = hipMalloc(&b.p, alloc);
+        if (e != hipSuccess) {
+            b.p = nullptr;
+            throw HipError{e, "hipMalloc of " + std::to_string(alloc) + " bytes"};
+        }
         b.bytes = alloc;
     }
     return static_cast<T *>(b.p);
@@ -80,7 +84,7 @@ inline int fail(gis_ctx *ctx, int code, const std::string &msg) {
 }
 
 inline int hip_fail(gis_ctx *ctx, const HipError &e) {
-    return fail(ctx, GIS_ERR_HIP, std::string(e.what) + ": " + hipGetErrorString(e.e));
+    return fail(ctx, GIS_ERR_HIP, e.what + ": " + hipGetErrorString(e.e));
 }
 
 inline DevIndex dev_index(const gis_ctx *ctx) {

@@ -825,15 +825,185 @@ __device__ __forceinline__ void merge_emit(uint32_t nh, uint32_t G, const uint64
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-cooperative merge. The hits of a read are taken in (strand/ref group,
+// arrival) order -- within a group that is (query_start, ref_start) order --
+// and broadcast one at a time to the wave. Lane j holds the group's j-th open
+// NAM in creation order, so "the first open NAM that accepts the hit"
+// (step2/nam.cu:391-447) is the lowest lane of a ballot, and a hit nobody
+// accepts opens a NAM in the next free lane. Creation order inside a group is
+// canonical order, so NAM c of a group is written at (NAMs of earlier
+// groups) + c with nam_id = that rank. When all 64 lanes are open, NAMs whose
+// query_end lies before the hit's query_start (they can never accept a later
+// hit of the group) are written out and the rest are compacted in order; a
+// group that still needs more than 64 open NAMs reports MERGE_OVERFLOW and
+// the caller merges the read with merge_build instead.
+
+constexpr uint32_t MERGE_OVERFLOW = 0xffffffffu;
+
+struct LaneNam {
+    int qs, qe, qp, rs, re, rp, n;
+    uint32_t cid;
+};
+
+__device__ __forceinline__ void lane_nam_store(gis_nam *out, uint32_t idx, uint32_t rsd,
+                                               const LaneNam &o) {
+    gis_nam x;
+    x.nam_id = (int32_t)idx;
+    x.query_start = o.qs;
+    x.query_end = o.qe;
+    x.query_prev_hit_startpos = o.qp;
+    x.ref_start = o.rs;
+    x.ref_end = o.re;
+    x.ref_prev_hit_startpos = o.rp;
+    x.n_hits = o.n;
+    x.ref_id = (int32_t)(rsd & 0x7fffffffu);
+    x.score = nam_score(o.qs, o.qe, o.rs, o.re, o.n);
+    x.is_rc = (uint8_t)(rsd >> 31);
+    x.pad_[0] = x.pad_[1] = x.pad_[2] = 0;
+    out[idx] = x;
+}
+
+__device__ __forceinline__ int permute_push(int v, uint32_t dst) {
+    return __builtin_amdgcn_ds_permute((int)(dst << 2), v);
+}
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+
+// Merge the nh hits of a read into NAMs written to out[0 .. total) in
+// canonical order (entries at index >= cap are not written). keys: scratch
+// of at least the next power of two >= nh entries (used when nh > 64).
+// Returns the NAM count or MERGE_OVERFLOW.
+__device__ __forceinline__ uint32_t merge_wave(uint32_t nh, const DevHit *hits, uint64_t *keys,
+                                               gis_nam *out, uint32_t cap) {
+    const uint32_t lane = lane_id();
+    // chunk registers: hit (x0 + lane) of the grouped order
+    uint32_t c_rsd = 0, c_rs = 0, c_re = 0, c_q = 0;
+    bool by_keys = false;
+    if (nh <= 64) {
+        DevHit h{};
+        if (lane < nh) h = hits[lane];
+        const uint32_t rsd = lane < nh ? h.ref_strand : 0xffffffffu;
+        const uint32_t prev = __shfl_up(rsd, 1, 64);
+        const bool uns = __ballot(lane > 0 && lane < nh && prev > rsd) != 0;
+        c_rsd = rsd;
+        c_rs = (uint32_t)h.ref_start;
+        c_re = (uint32_t)h.ref_end;
+        c_q = (uint32_t)h.query_start | ((uint32_t)h.query_end << 16);
+        if (uns) {
+            uint64_t key = lane < nh ? (((uint64_t)rsd << 32) | lane) : UINT64_MAX;
+            bitonic_regs(key);
+            const int src = (int)((uint32_t)key & 63u);
+            c_rsd = (uint32_t)__shfl((int)c_rsd, src, 64);
+            c_rs = (uint32_t)__shfl((int)c_rs, src, 64);
+            c_re = (uint32_t)__shfl((int)c_re, src, 64);
+            c_q = (uint32_t)__shfl((int)c_q, src, 64);
+        }
+    } else {
+        uint32_t unsorted = 0;
+        for (uint32_t i0 = 0; i0 < nh; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            if (i < nh) {
+                const uint32_t rsd = hits[i].ref_strand;
+                keys[i] = ((uint64_t)rsd << 32) | i;
+                if (i > 0 && hits[i - 1].ref_strand > rsd) unsorted = 1;
+            }
+        }
+        by_keys = __ballot(unsorted != 0) != 0;
+        wave_sync_mem();
+        if (by_keys) {
+            uint32_t P = 1;
+            while (P < nh) P <<= 1;
+            for (uint32_t i = nh + lane; i < P; i += 64) keys[i] = UINT64_MAX;
+            wave_sync_mem();
+            bitonic_mem(keys, P);
+        }
+    }
+
+    LaneNam o{0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t n_open = 0, grp = 0xffffffffu, base = 0, gcnt = 0;
+    bool ovf = false;
+    for (uint32_t x0 = 0; x0 < nh && !ovf; x0 += 64) {
+        if (nh > 64) {
+            const uint32_t i = x0 + lane;
+            if (i < nh) {
+                const DevHit h = hits[by_keys ? (uint32_t)keys[i] : i];
+                c_rsd = h.ref_strand;
+                c_rs = (uint32_t)h.ref_start;
+                c_re = (uint32_t)h.ref_end;
+                c_q = (uint32_t)h.query_start | ((uint32_t)h.query_end << 16);
+            }
+        }
+        const uint32_t m = nh - x0 < 64 ? nh - x0 : 64;
+        for (uint32_t t = 0; t < m; t++) {
+            const uint32_t rsd = readlane_u32(c_rsd, t);
+            const int rs = (int)readlane_u32(c_rs, t);
+            const int re = (int)readlane_u32(c_re, t);
+            const uint32_t q = readlane_u32(c_q, t);
+            const int qs = (int)(q & 0xffffu), qe = (int)(q >> 16);
+            if (rsd != grp) {
+                if (lane < n_open && base + o.cid < cap) lane_nam_store(out, base + o.cid, grp, o);
+                base += gcnt;
+                gcnt = 0;
+                n_open = 0;
+                grp = rsd;
+            }
+            const bool live = lane < n_open;
+            const bool near = live && o.qp < qs && qs <= o.qe && o.rp < rs && rs <= o.re;
+            const bool ext = qe > o.qe && re > o.re;
+            const bool con = qe <= o.qe && re <= o.re;
+            const uint64_t bal = __ballot(near && (ext || con));
+            if (bal) {
+                if (lane == (uint32_t)__builtin_ctzll(bal)) {
+                    if (ext) {
+                        o.qe = qe;
+                        o.re = re;
+                    }
+                    o.qp = qs;
+                    o.rp = rs;
+                    o.n++;
+                }
+                continue;
+            }
+            if (n_open == 64) {
+                const bool keep = o.qe >= qs;
+                const uint64_t km = __ballot(keep);
+                if (!keep && base + o.cid < cap) lane_nam_store(out, base + o.cid, grp, o);
+                const uint32_t nk = (uint32_t)__popcll(km);
+                if (nk == 64) {
+                    ovf = true;
+                    break;
+                }
+                const uint32_t dst = keep ? mbcnt(km) : nk + mbcnt(~km);
+                o.qs = permute_push(o.qs, dst);
+                o.qe = permute_push(o.qe, dst);
+                o.qp = permute_push(o.qp, dst);
+                o.rs = permute_push(o.rs, dst);
+                o.re = permute_push(o.re, dst);
+                o.rp = permute_push(o.rp, dst);
+                o.n = permute_push(o.n, dst);
+                o.cid = (uint32_t)permute_push((int)o.cid, dst);
+                n_open = nk;
+            }
+            if (lane == n_open) o = LaneNam{qs, qe, qs, rs, re, rs, 1, gcnt};
+            n_open++;
+            gcnt++;
+        }
+    }
+    if (ovf) return MERGE_OVERFLOW;
+    if (lane < n_open && base + o.cid < cap) lane_nam_store(out, base + o.cid, grp, o);
+    return base + gcnt;
+}
+
 struct alignas(8) MergeLds {
     DevHit hits[MERGE_LDS];
     uint64_t keys[MERGE_LDS];
-    uint32_t aux[2 * MERGE_LDS + 2];
-    NamWork ws[MERGE_LDS];
 };
 
 __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int rescue) {
-    __shared__ MergeLds lds[WPB];
+    __shared__ uint64_t lkeys[WPB][MERGE_LDS];
     const uint32_t lane = lane_id();
     const int wid = (int)(threadIdx.x >> 6);
     if (rescue ? over_capacity(b.rhit_off, b.n_reads, b.rhit_cap, b.ctl, OVF_RHITS)
@@ -852,21 +1022,14 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
         }
         const DevHit *ghits = (rescue ? b.rhits : b.hits) + hb;
         gis_nam *fin = (rescue ? b.rnam_fin : b.nam_fin) + hb;
-        uint32_t total;
-        if (nh <= (uint32_t)MERGE_LDS) {
-            MergeLds &S = lds[wid];
-            for (uint32_t i = lane; i < nh; i += 64) S.hits[i] = ghits[i];
-            wave_sync_mem();
-            const uint32_t G = merge_build(nh, S.hits, S.keys, S.aux, S.ws);
-            total = merge_count(nh, G, S.aux);
-            merge_emit(nh, G, S.keys, S.aux, S.ws, fin);
-        } else {
-            uint64_t *keys = (rescue ? b.rkeys : b.keys) + 2 * hb;
+        uint64_t *gkeys = (rescue ? b.rkeys : b.keys) + 2 * hb;
+        uint32_t total = merge_wave(nh, ghits, nh <= (uint32_t)MERGE_LDS ? lkeys[wid] : gkeys, fin, nh);
+        if (total == MERGE_OVERFLOW) {
             uint32_t *aux = (rescue ? b.raux : b.aux) + 2 * hb + r;
             NamWork *ws = (rescue ? b.rnam_ws : b.nam_ws) + hb;
-            const uint32_t G = merge_build(nh, ghits, keys, aux, ws);
+            const uint32_t G = merge_build(nh, ghits, gkeys, aux, ws);
             total = merge_count(nh, G, aux);
-            merge_emit(nh, G, keys, aux, ws, fin);
+            merge_emit(nh, G, gkeys, aux, ws, fin);
         }
         if (lane == 0) b.nam_cnt[r] = total;
         wave_sync_mem();
@@ -989,17 +1152,14 @@ __global__ __launch_bounds__(64 * F_WPB) void k_fused(DevBatch b, DevIndex ix, S
             spill_read(b, r);
             continue;
         }
-        uint32_t total = 0, G = 0;
-        if (nh > 0) {
-            G = merge_build(nh, S.merge.hits, S.merge.keys, S.merge.aux, S.merge.ws);
-            total = merge_count(nh, G, S.merge.aux);
-        }
-        if (total > (uint32_t)F_NAMCAP) {
+        uint32_t total = 0;
+        if (nh > 0)
+            total = merge_wave(nh, S.merge.hits, S.merge.keys, b.nam_slots + (uint64_t)r * F_NAMCAP,
+                               (uint32_t)F_NAMCAP);
+        if (total > (uint32_t)F_NAMCAP) {  // includes MERGE_OVERFLOW
             spill_read(b, r);
             continue;
         }
-        if (total > 0)
-            merge_emit(nh, G, S.merge.keys, S.merge.aux, S.merge.ws, b.nam_slots + (uint64_t)r * F_NAMCAP);
         if (lane == 0) {
             b.frac[r] = frac;
             b.flags[r] = (uint8_t)((rescue ? GIS_FLAG_RESCUED : 0) | (n_fwd == 0 ? GIS_FLAG_NO_RANDSTROBES : 0));
@@ -1016,6 +1176,8 @@ __global__ __launch_bounds__(64 * F_WPB) void k_fused(DevBatch b, DevIndex ix, S
 
 __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
     const uint32_t lane = lane_id();
+    // a hit array overflowed: the merges did not run and the counts are stale
+    if (b.ctl->overflow & (OVF_HITS | OVF_RHITS)) return;
     if (b.nam_off[b.n_reads] > b.out_cap) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&b.ctl->overflow, OVF_OUT);
         return;

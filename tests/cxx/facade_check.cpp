@@ -30,11 +30,14 @@ int main(int argc, char **argv) {
         std::cerr << "usage: facade_check <refs.fa> <reads.txt> <out_prefix>\n";
         return 2;
     }
+    const char *stage = "references";
     try {
         const gis::References refs = gis::References::from_fasta(argv[1]);
         const auto params = gis::IndexParameters::from_read_length(150);
+        stage = "populate";
         gis::StrobemerIndex index(refs, params);
         index.populate(0.0002f, 4);
+        stage = "per-read find_nams";
         std::vector<std::string> reads;
         {
             std::ifstream in(argv[2]);
@@ -60,6 +63,7 @@ int main(int argc, char **argv) {
             bases += r.size();
             views.emplace_back(r);
         }
+        stage = "batch";
         gis::Batch batch(index, (uint32_t)reads.size(), bases);
         batch.upload(views);
         batch.run();
@@ -67,7 +71,7 @@ int main(int argc, char **argv) {
         write_out(std::string(argv[3]) + ".batch", res.offsets, res.nams);
         std::cout << "facade_check: " << reads.size() << " reads, " << all.size() << " NAMs\n";
     } catch (const gis::Error &e) {
-        std::cerr << "gis error " << e.code() << ": " << e.what() << "\n";
+        std::cerr << "gis error " << e.code() << " in " << stage << ": " << e.what() << "\n";
         return 1;
     }
     return 0;
