@@ -75,11 +75,11 @@ __device__ __forceinline__ uint32_t set_at(const ReadSet &s, uint32_t w) { retur
 // seeding: canonical syncmers and query randstrobes of one read
 
 struct SeedBufs {
-    uint8_t *codes;     // 2-bit base codes, 4 = not ACGT
+    uint64_t *pk;       // bases 2 bits each, LSB first: base i at bits 2(i%32) of pk[i/32]
     uint8_t *st;        // per k-mer window: anchor state / ST_UNKNOWN / ST_INVALID
     uint64_t *sh;       // canonical s-mer hash per s-mer start
     uint64_t *sy_hash;  // syncmer list
-    uint32_t *sy_pos;
+    uint16_t *sy_pos;   // read positions fit 16 bits (reads are at most 65535 bp)
     uint64_t *nmask;    // bit i set when base i is not ACGT
 };
 
@@ -95,6 +95,38 @@ __device__ __forceinline__ bool any_n(const uint64_t *nm, int a, int b) {
     return (m1 | m2) != 0;
 }
 
+// bit m of v to bit 2m
+__device__ __forceinline__ uint64_t spread32(uint32_t v) {
+    uint64_t x = v;
+    x = (x | (x << 16)) & 0x0000FFFF0000FFFFULL;
+    x = (x | (x << 8)) & 0x00FF00FF00FF00FFULL;
+    x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0FULL;
+    x = (x | (x << 2)) & 0x3333333333333333ULL;
+    x = (x | (x << 1)) & 0x5555555555555555ULL;
+    return x;
+}
+
+// the 2-bit groups of x in reverse order
+__device__ __forceinline__ uint64_t rev_pairs(uint64_t x) {
+    x = __builtin_bitreverse64(x);
+    return ((x >> 1) & 0x5555555555555555ULL) | ((x & 0x5555555555555555ULL) << 1);
+}
+
+// Canonical code of the n-mer (n <= 32) starting at base j: the smaller of
+// the forward code (first base most significant, the rolling xs[0] of
+// step2/randstrobes.cu:25-27) and the reverse-complement code (first base's
+// complement least significant, xs[1]). In the LSB-first packing the window
+// is already the reverse complement up to complementing every base.
+__device__ __forceinline__ uint64_t canonical_code(const uint64_t *pk, int j, int n, uint64_t mask) {
+    const int q = j >> 5, sh = (j & 31) * 2;
+    uint64_t g = pk[q] >> sh;
+    if (sh) g |= pk[q + 1] << (64 - sh);
+    g &= mask;
+    const uint64_t r = g ^ mask;
+    const uint64_t f = rev_pairs(g) >> (64 - 2 * n);
+    return f < r ? f : r;
+}
+
 // Canonical syncmers of one read (SyncmerIterator, step2/randstrobes.cu:18-79),
 // the whole wave working on the same read. The first `cap` syncmers go to
 // B.sy_hash/B.sy_pos (and out_sy_* when given); returns the total count.
@@ -105,17 +137,25 @@ __device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, 
     const int lane = (int)lane_id();
     const int k = P.k, s = P.s, w = P.w;
 
-    // base codes and the N mask (step2/randstrobes.cu:21-22, 70-76)
-    for (int b0 = 0; b0 < L; b0 += 64) {
+    // 2-bit packed bases and the N mask from three ballots per 64 bases
+    // (step2/randstrobes.cu:21-22, 70-76)
+    int nch = 0;
+    for (int b0 = 0; b0 < L; b0 += 64, nch++) {
         const int i = b0 + lane;
         int c = 4;
-        if (i < L) {
-            c = nt4((unsigned char)seq[i]);
-            B.codes[i] = (uint8_t)c;
-        }
+        if (i < L) c = nt4((unsigned char)seq[i]);
+        const bool ok = i < L && c < 4;
+        const uint64_t lo = __ballot(ok && (c & 1));
+        const uint64_t hi = __ballot(ok && (c & 2));
         const uint64_t m = __ballot(i < L && c > 3);
-        if (lane == 0) B.nmask[b0 >> 6] = m;
+        if (lane == 0) {
+            B.nmask[b0 >> 6] = m;
+            B.pk[2 * nch] = spread32((uint32_t)lo) | (spread32((uint32_t)hi) << 1);
+        } else if (lane == 1) {
+            B.pk[2 * nch + 1] = spread32((uint32_t)(lo >> 32)) | (spread32((uint32_t)(hi >> 32)) << 1);
+        }
     }
+    if (lane == 0) B.pk[2 * nch] = 0;  // pad word read by windows ending in the last word
     wave_sync();
 
     // canonical s-mer hashes, one s-mer per lane (step2/randstrobes.cu:23-32)
@@ -124,15 +164,7 @@ __device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, 
         const int j = j0 + lane;
         if (j < ns) {
             uint64_t h = 0;
-            if (!any_n(B.nmask, j, j + s - 1)) {
-                uint64_t f = 0, r = 0;
-                for (int m = 0; m < s; m++) {
-                    const uint64_t c = B.codes[j + m];
-                    f = (f << 2) | c;
-                    r |= (3 - c) << (2 * m);
-                }
-                h = xxh64_u64(f < r ? f : r);
-            }
+            if (!any_n(B.nmask, j, j + s - 1)) h = xxh64_u64(canonical_code(B.pk, j, s, P.smask));
             B.sh[j] = h;
         }
     }
@@ -147,7 +179,7 @@ __device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, 
             if (any_n(B.nmask, p, p + k - 1)) {
                 st = ST_INVALID;
             } else {
-                const bool seg_first = (p == 0) || B.codes[p - 1] > 3;
+                const bool seg_first = (p == 0) || ((B.nmask[(p - 1) >> 6] >> ((p - 1) & 63)) & 1);
                 st = anchor_state(B.sh, p, w, seg_first);
             }
             B.st[p] = st;
@@ -155,8 +187,8 @@ __device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, 
     }
     wave_sync();
 
-    // resolve tied windows from the nearest anchor and compact syncmers; a
-    // window is a syncmer when its minimum sits at offset t-1
+    // resolve tied windows from the nearest anchor and compact syncmer
+    // positions; a window is a syncmer when its minimum sits at offset t-1
     // (step2/randstrobes.cu:64-69)
     uint32_t nsync = 0;
     for (int p0 = 0; p0 < nk; p0 += 64) {
@@ -179,24 +211,22 @@ __device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, 
         const uint64_t bal = __ballot(is_sync);
         if (is_sync) {
             const uint32_t idx = nsync + mbcnt(bal);
-            uint64_t f = 0, r = 0;
-            for (int m = 0; m < k; m++) {
-                const uint64_t c = B.codes[p + m];
-                f = (f << 2) | c;
-                r |= (3 - c) << (2 * m);
-            }
-            f &= P.kmask;
-            const uint64_t hk = xxh64_u64(f < r ? f : r);
-            if (idx < cap) {
-                B.sy_hash[idx] = hk;
-                B.sy_pos[idx] = (uint32_t)p;
-            }
-            if (out_sy_hash) {
-                out_sy_hash[idx] = hk;
-                out_sy_pos[idx] = (uint64_t)p;
-            }
+            if (idx < cap) B.sy_pos[idx] = (uint16_t)p;
+            if (out_sy_pos) out_sy_pos[idx] = (uint64_t)p;
         }
         nsync += (uint32_t)__popcll(bal);
+    }
+    wave_sync();
+
+    // k-mer hashes over the compacted list (xxh64 of the canonical k-mer)
+    const uint32_t nh = nsync < cap ? nsync : cap;
+    for (uint32_t i0 = 0; i0 < nh; i0 += 64) {
+        const uint32_t i = i0 + (uint32_t)lane;
+        if (i < nh) {
+            const uint64_t hk = xxh64_u64(canonical_code(B.pk, (int)B.sy_pos[i], k, P.kmask));
+            B.sy_hash[i] = hk;
+            if (out_sy_hash) out_sy_hash[i] = hk;
+        }
     }
     wave_sync();
     return nsync;
@@ -205,7 +235,11 @@ __device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, 
 // Forward and reverse-complement randstrobe number i of a read with n
 // syncmers (RandstrobeIterator::get, step2/randstrobes.cu:136-159); the
 // reverse-complement pass walks the forward syncmers backwards with mirrored
-// positions L - pos - k (step2/randstrobes.cu:221-239).
+// positions L - pos - k (step2/randstrobes.cu:221-239). Candidates lie at
+// index offsets w_min..w_max; positions grow with the index, so the
+// reference's break at the first candidate beyond max_dist is the same as
+// skipping every candidate beyond it, which lets the candidate loads issue
+// independently.
 struct RsPair {
     uint64_t fh, rh;
     uint32_t fs, fe, rs, re;
@@ -215,34 +249,43 @@ __device__ __forceinline__ RsPair randstrobe_pair(const SeedBufs &B, uint32_t n,
                                                   const SeedParams &P) {
     RsPair o;
     const uint32_t k = (uint32_t)P.k;
-    const uint32_t wend = (i + P.w_max < n - 1) ? i + P.w_max : n - 1;
+    const uint32_t last = n - 1;
     {
         const uint64_t h1 = B.sy_hash[i];
         const uint32_t p1 = B.sy_pos[i];
         const uint64_t maxp = (uint64_t)p1 + P.max_dist_u;
         uint32_t best = i;
         uint64_t minv = UINT64_MAX;
-        for (uint32_t j = i + P.w_min; j <= wend && (uint64_t)B.sy_pos[j] <= maxp; j++) {
-            const uint64_t c = (uint64_t)__popcll((h1 ^ B.sy_hash[j]) & P.q);
-            if (c < minv) { minv = c; best = j; }
+#pragma unroll 1
+        for (uint32_t d = P.w_min; d <= P.w_max; d++) {
+            const uint32_t j = i + d;
+            if (j <= last) {
+                const uint64_t hj = B.sy_hash[j];
+                const uint32_t pj = B.sy_pos[j];
+                const uint64_t c = (uint64_t)__popcll((h1 ^ hj) & P.q);
+                if ((uint64_t)pj <= maxp && c < minv) { minv = c; best = j; }
+            }
         }
         o.fh = h1 + B.sy_hash[best];
         o.fs = p1;
         o.fe = B.sy_pos[best] + k;
     }
     {
-        const uint32_t a = n - 1 - i;
+        const uint32_t a = last - i;
         const uint64_t h1 = B.sy_hash[a];
         const uint32_t p1 = (uint32_t)L - B.sy_pos[a] - k;
         const uint64_t maxp = (uint64_t)p1 + P.max_dist_u;
         uint32_t best = a;
         uint64_t minv = UINT64_MAX;
-        for (uint32_t jj = i + P.w_min; jj <= wend; jj++) {
-            const uint32_t bidx = n - 1 - jj;
-            const uint32_t pb = (uint32_t)L - B.sy_pos[bidx] - k;
-            if ((uint64_t)pb > maxp) break;
-            const uint64_t c = (uint64_t)__popcll((h1 ^ B.sy_hash[bidx]) & P.q);
-            if (c < minv) { minv = c; best = bidx; }
+#pragma unroll 1
+        for (uint32_t d = P.w_min; d <= P.w_max; d++) {
+            if (i + d <= last) {
+                const uint32_t bidx = last - (i + d);
+                const uint64_t hb = B.sy_hash[bidx];
+                const uint32_t pb = (uint32_t)L - B.sy_pos[bidx] - k;
+                const uint64_t c = (uint64_t)__popcll((h1 ^ hb) & P.q);
+                if ((uint64_t)pb <= maxp && c < minv) { minv = c; best = bidx; }
+            }
         }
         o.rh = h1 + B.sy_hash[best];
         o.rs = p1;
@@ -279,8 +322,8 @@ struct alignas(8) SeedLds {
     uint64_t sh[LMAX_LDS];
     uint64_t sy_hash[LMAX_LDS];
     uint64_t nmask[LMAX_LDS / 64 + 2];
-    uint32_t sy_pos[LMAX_LDS];
-    uint8_t codes[LMAX_LDS];
+    uint64_t pk[2 * (LMAX_LDS / 64) + 1];
+    uint16_t sy_pos[LMAX_LDS];
     uint8_t st[LMAX_LDS];
 };
 
@@ -292,14 +335,15 @@ __device__ __forceinline__ SeedBufs long_bufs(uint8_t *base, uint64_t L) {
     B.sh = (uint64_t *)base;
     B.sy_hash = (uint64_t *)(base + 8 * L);
     B.nmask = (uint64_t *)(base + 16 * L);
-    B.sy_pos = (uint32_t *)(base + 16 * L + 8 * (L / 64 + 2));
-    B.codes = base + 16 * L + 8 * (L / 64 + 2) + align8(4 * L);
-    B.st = B.codes + align8(L);
+    B.sy_pos = (uint16_t *)(base + 16 * L + 8 * (L / 64 + 2));
+    // the 8 (2 ceil(L / 64) + 1) bytes of packed bases fit the L bytes given to them (L > 320)
+    B.pk = (uint64_t *)(base + 16 * L + 8 * (L / 64 + 2) + align8(4 * L));
+    B.st = (uint8_t *)B.pk + align8(L);
     return B;
 }
 
 template <bool LONG>
-__global__ __launch_bounds__(64 * SEED_WPB) void k_seed(DevBatch b, ReadSet set, SeedParams P,
+__global__ __launch_bounds__(64 * SEED_WPB) __attribute__((amdgpu_waves_per_eu(6))) void k_seed(DevBatch b, ReadSet set, SeedParams P,
                                                         int want_sy) {
     __shared__ SeedLds lds[LONG ? 1 : SEED_WPB];
     const int wid = (int)(threadIdx.x >> 6);
@@ -313,7 +357,7 @@ __global__ __launch_bounds__(64 * SEED_WPB) void k_seed(DevBatch b, ReadSet set,
             B = long_bufs(b.long_scratch + b.long_off[r], (uint64_t)L);
         } else {
             SeedLds &S = lds[wid];
-            B = SeedBufs{S.codes, S.st, S.sh, S.sy_hash, S.sy_pos, S.nmask};
+            B = SeedBufs{S.pk, S.st, S.sh, S.sy_hash, S.sy_pos, S.nmask};
         }
         uint64_t *osh = want_sy ? b.sy_hash + b.sy_off[r] : nullptr;
         uint64_t *osp = want_sy ? b.sy_pos + b.sy_off[r] : nullptr;
@@ -1051,8 +1095,8 @@ struct alignas(8) FusedSeedLds {
     uint64_t sh[F_LMAX];
     uint64_t nmask[F_LMAX / 64 + 2];
     uint64_t sy_hash[F_SYCAP];
-    uint32_t sy_pos[F_SYCAP];
-    uint8_t codes[F_LMAX];
+    uint64_t pk[2 * (F_LMAX / 64) + 1];
+    uint16_t sy_pos[F_SYCAP];
     uint8_t st[F_LMAX];
 };
 
@@ -1088,7 +1132,7 @@ __global__ __launch_bounds__(64 * F_WPB) void k_fused(DevBatch b, DevIndex ix, S
         bool rescue = false;
         float frac = 1.0f;
         if (!spill) {
-            SeedBufs B{S.seed.codes, S.seed.st, S.seed.sh, S.seed.sy_hash, S.seed.sy_pos, S.seed.nmask};
+            SeedBufs B{S.seed.pk, S.seed.st, S.seed.sh, S.seed.sy_hash, S.seed.sy_pos, S.seed.nmask};
             const uint32_t n = seed_syncmers(b.seqs + b.read_off[r], L, P, B, F_SYCAP, nullptr, nullptr);
             n_fwd = ((uint32_t)L >= P.w_max && n > P.w_min) ? n - P.w_min : 0;
             spill = n > (uint32_t)F_SYCAP || n_fwd > 64;
