@@ -153,6 +153,23 @@ class Context:
     def check(self, code):
         _check(code, self.h)
 
+    @property
+    def fused_config(self):
+        """Launch configuration of the fused form (waves per block, length
+        split, queue discipline, chunk, persistent blocks per CU)."""
+        c = N.FusedConfig()
+        self.check(self.L.gis_fused_config_get(self.h, C.byref(c)))
+        return c.as_dict()
+
+    @fused_config.setter
+    def fused_config(self, cfg):
+        c = N.FusedConfig()
+        cur = self.fused_config
+        cur.update(cfg)
+        for k, v in cur.items():
+            setattr(c, k, int(v))
+        self.check(self.L.gis_fused_config_set(self.h, C.byref(c)))
+
     def close(self):
         if getattr(self, "h", None):
             self.L.gis_ctx_destroy(self.h)
@@ -492,6 +509,15 @@ class Mapper:
         cap = self.nam_count()
         return _call_with_nam_out(self.ctx, self.n,
                                   lambda o: self.ctx.L.gis_batch_download(self.h, C.byref(o)), cap)
+
+    def autotune(self, rescue_level=2, hard_cutoff=1000):
+        """Time every fused-form configuration on the uploaded reads and keep
+        the fastest in the context -> (config dict, best ms, candidates tried)."""
+        opt = _map_options(rescue_level, hard_cutoff, N.MODE_FUSED, False)
+        best, ms, n = N.FusedConfig(), C.c_double(), C.c_uint32()
+        self.ctx.check(self.ctx.L.gis_batch_autotune(self.h, C.byref(opt), C.byref(best), C.byref(ms),
+                                                     C.byref(n)))
+        return best.as_dict(), ms.value, n.value
 
     def stage_times(self):
         ms = (C.c_double * 16)()

@@ -67,6 +67,14 @@ class MapOptions(C.Structure):
                 ("flags", C.c_int32)]
 
 
+class FusedConfig(C.Structure):
+    _fields_ = [("waves_per_block", C.c_int32), ("length_split", C.c_int32), ("queue", C.c_int32),
+                ("chunk", C.c_int32), ("blocks_per_cu", C.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class GenomeSpec(C.Structure):
     _fields_ = [("total_length", C.c_uint64), ("n_contigs", C.c_uint32), ("gc", C.c_double),
                 ("repeat_fraction", C.c_double), ("n_families", C.c_uint32),
@@ -118,6 +126,10 @@ PROTOTYPES = {
     "gis_batch_stage_times": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_int), C.c_char_p, C.c_int]),
     "gis_batch_lookup_bytes": (C.c_int, [_P, C.POINTER(_U64), C.POINTER(_U64)]),
     "gis_batch_nam_count": (C.c_int, [_P, C.POINTER(_U64)]),
+    "gis_fused_config_get": (C.c_int, [_P, C.POINTER(FusedConfig)]),
+    "gis_fused_config_set": (C.c_int, [_P, C.POINTER(FusedConfig)]),
+    "gis_batch_autotune": (C.c_int, [_P, C.POINTER(MapOptions), C.POINTER(FusedConfig),
+                                     C.POINTER(C.c_double), C.POINTER(_U32)]),
     "gis_synth_genome": (C.c_int, [C.POINTER(GenomeSpec), _P, _P]),
     "gis_synth_reads": (C.c_int, [C.POINTER(ReadSpec), _P, _P, _U32, _P, _P, _P, _P, _P, _P]),
 }

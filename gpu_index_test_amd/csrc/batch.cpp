@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -31,7 +32,9 @@ struct gis_batch {
     DevBuf pos, hcnt, rcnt, lk_bytes;
     DevBuf hit_total, rhit_total, frac, flags, nam_cnt, hit_off, rhit_off;
     DevBuf hits, rhits, keys, rkeys, aux, raux, nam_ws, rnam_ws, nam_fin, rnam_fin;
-    DevBuf nam_slots, nam_off, nams_out, scan_tmp, ctl, spill_ids;
+    DevBuf nam_slots, nam_off, nams_out, scan_tmp, ctl, spill_ids, queue;
+    std::vector<uint32_t> h_queue;  // read ids by length, longest first
+    uint32_t n_long = 0;            // queue entries longer than the short bin (160 bp)
     uint64_t hit_cap = 0, rhit_cap = 0, out_cap = 0;
     uint64_t tot_rs_cap = 0;
     // results of the last run
@@ -50,7 +53,7 @@ struct gis_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     DevBatch graph_key{};
-    int graph_cfg[6] = {-1, -1, -1, -1, -1, -1};
+    int graph_cfg[11] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
 };
 
 namespace gis {
@@ -70,7 +73,7 @@ void batch_free(gis_batch *b) {
                      &b->hit_off, &b->rhit_off, &b->hits, &b->rhits, &b->keys,
                      &b->rkeys, &b->aux, &b->raux, &b->nam_ws, &b->rnam_ws, &b->nam_fin,
                      &b->rnam_fin, &b->nam_slots, &b->nam_off, &b->nams_out, &b->scan_tmp, &b->ctl,
-                     &b->spill_ids};
+                     &b->spill_ids, &b->queue};
     for (DevBuf *x : all) release(*x);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
@@ -84,6 +87,9 @@ void batch_free(gis_batch *b) {
 }  // namespace gis
 
 namespace {
+
+// reads up to this length form the fused kernel's short bin
+constexpr int SHORT_BIN = 160;
 
 // long-read scratch bytes of the seeding stage for one read of length L
 // (layout of long_bufs in kernels.hip)
@@ -235,7 +241,19 @@ int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const 
     }
     b->n_reads = n;
     b->want_sy = want_sy;
+    // the fused kernel's work queue: read ids by length, longest first
+    // (counting sort over lengths capped at 1023; stable within a length)
+    {
+        std::vector<uint32_t> cnt(1025, 0);
+        for (uint32_t r = 0; r < n; r++) cnt[1023 - std::min<uint32_t>(lens[r], 1023u) + 1]++;
+        for (int i = 1; i < 1025; i++) cnt[i] += cnt[i - 1];
+        b->h_queue.assign(n, 0);
+        for (uint32_t r = 0; r < n; r++) b->h_queue[cnt[1023 - std::min<uint32_t>(lens[r], 1023u)]++] = r;
+        b->n_long = 0;
+        for (uint32_t r = 0; r < n; r++) b->n_long += lens[r] > (uint32_t)SHORT_BIN ? 1u : 0u;
+    }
     ensure_read_arrays(b, n);
+    ensure<uint32_t>(b->queue, n);
     b->tot_rs_cap = b->h_rs_off[n];
     ensure_slot_arrays(b, b->tot_rs_cap);
     if (want_sy) {
@@ -260,6 +278,7 @@ int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const 
     HIP_CHECK(hipMemcpyAsync(b->rs_off.p, b->h_rs_off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipMemcpyAsync(b->sy_off.p, b->h_sy_off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipMemcpyAsync(b->long_off.p, b->h_long_off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
+    if (n) HIP_CHECK(hipMemcpyAsync(b->queue.p, b->h_queue.data(), 4ull * n, hipMemcpyHostToDevice, st));
     HIP_CHECK(hipStreamSynchronize(st));
     b->ran = false;
     return GIS_OK;
@@ -288,7 +307,21 @@ void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_
     ReadSet set{nullptr, nullptr, n};
     const bool fused = mode == GIS_MODE_FUSED && stop == STOP_NONE;
     if (fused) {
-        launch_fused(d, ix, sp, rcut, hard, st);
+        const gis_fused_config &fc = ctx->fcfg;
+        const unsigned grid = fc.blocks_per_cu > 0 ? (unsigned)(ctx->n_cu * fc.blocks_per_cu) : 0u;
+        const uint32_t chunk = fc.queue ? (uint32_t)fc.chunk : 0u;
+        const uint32_t *q = static_cast<const uint32_t *>(b->queue.p);
+        bool ok;
+        if (fc.length_split) {
+            ok = launch_fused(d, ix, sp, rcut, hard, q, b->n_long, 320, fc.waves_per_block, grid, chunk,
+                              &d.ctl->q_head[0], st) &&
+                 launch_fused(d, ix, sp, rcut, hard, q + b->n_long, n - b->n_long, SHORT_BIN,
+                              fc.waves_per_block, grid, chunk, &d.ctl->q_head[1], st);
+        } else {
+            ok = launch_fused(d, ix, sp, rcut, hard, q, n, 320, fc.waves_per_block, grid, chunk,
+                              &d.ctl->q_head[0], st);
+        }
+        if (!ok) throw HipError{hipErrorInvalidValue, "unsupported fused configuration"};
         mark(b, "fused");
         set = ReadSet{d.spill_ids, &d.ctl->n_spill, n};
     }
@@ -328,8 +361,10 @@ unsigned run_once(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32
     hipStream_t st = ctx->stream;
     if (use_graph) {
         const DevBatch key = make_devbatch(b);
-        const int cfg[6] = {mode, rescue_mode, (int)rcut, (int)hard, stop,
-                            (b->want_sy ? 1 : 0) | (b->any_long ? 2 : 0)};
+        const gis_fused_config &fc = ctx->fcfg;
+        const int cfg[11] = {mode, rescue_mode, (int)rcut, (int)hard, stop,
+                             (b->want_sy ? 1 : 0) | (b->any_long ? 2 : 0), fc.waves_per_block,
+                             fc.length_split, fc.chunk, fc.blocks_per_cu, (int)b->n_long};
         const bool same = b->graph_exec && std::memcmp(&key, &b->graph_key, sizeof key) == 0 &&
                           std::memcmp(cfg, b->graph_cfg, sizeof cfg) == 0;
         if (!same) {
@@ -777,6 +812,57 @@ extern "C" int gis_batch_lookup_bytes(gis_batch *b, uint64_t *bytes, uint64_t *n
     }
     if (bytes) *bytes = tb;
     if (n_lookups) *n_lookups = tq;
+    return GIS_OK;
+    API_END(ctx)
+}
+
+// ---------------- autotuner of the fused form ----------------
+
+extern "C" int gis_batch_autotune(gis_batch *b, const gis_map_options *opt, gis_fused_config *best,
+                                  double *best_ms, uint32_t *n_tried) {
+    if (!b) return GIS_ERR_BAD_PARAMETER;
+    gis_ctx *ctx = b->ctx;
+    API_BEGIN(ctx)
+    if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
+    if (b->n_reads == 0) return fail(ctx, GIS_ERR_STATE, "no reads uploaded");
+    gis_map_options o = opts_or_default(opt);
+    o.mode = GIS_MODE_FUSED;
+    o.flags |= GIS_RUN_NO_GRAPH;
+    // candidates: waves per block x length split x queue discipline
+    struct Q { int queue, chunk, bpc; };
+    const Q queues[] = {{0, 0, 0}, {0, 0, 4}, {0, 0, 8}, {1, 4, 8}, {1, 16, 8}, {1, 64, 4}};
+    std::vector<gis_fused_config> cands;
+    for (int w : {2, 4, 8})
+        for (int split : {0, 1})
+            for (const Q &q : queues) cands.push_back(gis_fused_config{w, split, q.queue, q.chunk, q.bpc});
+    const gis_fused_config saved = ctx->fcfg;
+    double bt = 1e300;
+    gis_fused_config bc = saved;
+    try {
+        for (const gis_fused_config &c : cands) {
+            ctx->fcfg = c;
+            batch_run(b, o, 0, STOP_NONE);  // first launch of the variant; pools sized
+            double t = 1e300;
+            for (int rep = 0; rep < 3; rep++) {
+                const auto t0 = std::chrono::steady_clock::now();
+                batch_run(b, o, 0, STOP_NONE);
+                const double ms =
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                t = std::min(t, ms);
+            }
+            if (t < bt) {
+                bt = t;
+                bc = c;
+            }
+        }
+    } catch (...) {
+        ctx->fcfg = saved;
+        throw;
+    }
+    ctx->fcfg = bc;
+    if (best) *best = bc;
+    if (best_ms) *best_ms = bt;
+    if (n_tried) *n_tried = (uint32_t)cands.size();
     return GIS_OK;
     API_END(ctx)
 }

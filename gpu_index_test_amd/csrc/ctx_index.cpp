@@ -68,6 +68,9 @@ extern "C" int gis_ctx_create(int device, gis_ctx **out) {
     if (hipSetDevice(device) != hipSuccess) return GIS_ERR_HIP;
     gis_ctx *ctx = new gis_ctx();
     ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->n_cu = prop.multiProcessorCount;
     // default-priority stream
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
@@ -192,4 +195,31 @@ extern "C" int gis_index_download(gis_ctx *ctx, gis_ref_randstrobe *randstrobes,
         HIP_CHECK(hipMemcpy(starts, ix.starts.p, ((1ULL << ix.bits) + 1) * 8, hipMemcpyDeviceToHost));
     return GIS_OK;
     API_END(ctx)
+}
+
+extern "C" int gis_fused_config_get(gis_ctx *ctx, gis_fused_config *out) {
+    if (!ctx || !out) return GIS_ERR_BAD_PARAMETER;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    *out = ctx->fcfg;
+    return GIS_OK;
+}
+
+extern "C" int gis_fused_config_set(gis_ctx *ctx, const gis_fused_config *cfg) {
+    if (!ctx || !cfg) return GIS_ERR_BAD_PARAMETER;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    const gis_fused_config c = *cfg;
+    if (c.waves_per_block != 2 && c.waves_per_block != 4 && c.waves_per_block != 8)
+        return fail(ctx, GIS_ERR_BAD_PARAMETER, "waves_per_block must be 2, 4 or 8");
+    if (c.length_split != 0 && c.length_split != 1)
+        return fail(ctx, GIS_ERR_BAD_PARAMETER, "length_split must be 0 or 1");
+    if (c.queue != 0 && c.queue != 1) return fail(ctx, GIS_ERR_BAD_PARAMETER, "queue must be 0 or 1");
+    if (c.queue == 1 && (c.chunk < 1 || c.chunk > 4096))
+        return fail(ctx, GIS_ERR_BAD_PARAMETER, "chunk must be in 1..4096");
+    if (c.blocks_per_cu < 0 || c.blocks_per_cu > 64)
+        return fail(ctx, GIS_ERR_BAD_PARAMETER, "blocks_per_cu must be in 0..64");
+    if (c.queue == 1 && c.blocks_per_cu == 0)
+        return fail(ctx, GIS_ERR_BAD_PARAMETER, "a dynamic queue needs a persistent grid (blocks_per_cu > 0)");
+    ctx->fcfg = c;
+    if (c.queue == 0) ctx->fcfg.chunk = 0;
+    return GIS_OK;
 }

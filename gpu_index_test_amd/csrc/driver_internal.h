@@ -74,6 +74,8 @@ struct gis_ctx {
     std::string err;
     gis::DevIndexStore ix;
     gis_batch *scratch = nullptr;  // batch reused by the host-buffer entry points
+    int n_cu = 256;                // compute units of the device (persistent grids)
+    gis_fused_config fcfg{4, 0, 0, 0, 0};
 };
 
 namespace gis {

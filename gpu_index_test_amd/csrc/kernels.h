@@ -41,6 +41,7 @@ struct NamWork {
 struct DevCtl {
     unsigned int n_spill;   // reads handed from the fused kernel to the per-stage kernels
     unsigned int overflow;  // OVF_* bits
+    unsigned int q_head[2]; // dynamic work-queue heads of the fused kernel's two bins
 };
 
 enum : unsigned { OVF_HITS = 1, OVF_RHITS = 2, OVF_OUT = 8 };
@@ -129,8 +130,11 @@ void launch_rescue_count(const DevBatch &b, const ReadSet &set, const DevIndex &
                          uint32_t rescue_cutoff, uint32_t hard_cutoff, hipStream_t st);
 void launch_rescue_hits(const DevBatch &b, const ReadSet &set, const DevIndex &ix, hipStream_t st);
 void launch_merge(const DevBatch &b, const ReadSet &set, int rescue, hipStream_t st);
-void launch_fused(const DevBatch &b, const DevIndex &ix, const SeedParams &sp,
-                  uint32_t rescue_cutoff, uint32_t hard_cutoff, hipStream_t st);
+// fused kernel over queue[0 .. count); lmax in {160, 320}, wpb in {2, 4, 8};
+// returns false for an unsupported combination
+bool launch_fused(const DevBatch &b, const DevIndex &ix, const SeedParams &sp, uint32_t rescue_cutoff,
+                  uint32_t hard_cutoff, const uint32_t *queue, uint32_t count, int lmax, int wpb,
+                  unsigned grid, uint32_t chunk, unsigned int *head, hipStream_t st);
 void launch_compact(const DevBatch &b, int fused, hipStream_t st);
 void launch_find_flat(const DevIndex &ix, const uint64_t *hashes, uint64_t n, uint64_t *positions,
                       hipStream_t st);

@@ -1086,22 +1086,29 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
 // it has at most 64 + w_min syncmers; hits and the merge live in LDS. Reads
 // beyond these limits are appended to the spill list for the per-stage
 // kernels.
+//
+// The kernel is persistent over a device work queue: a list of read ids
+// ordered by length (longest first), cut by the host into a long and a short
+// bin that run as separate launches with their own LDS tiling (LMAX). Waves
+// take reads either statically (wave w takes queue entries w, w + W, ...,
+// which deals the length-sorted queue round robin) or dynamically (a wave
+// grabs `chunk` entries at a time from a counter in the control block).
 
-constexpr int F_LMAX = 320;
 constexpr int F_SYCAP = 96;
-constexpr int F_WPB = 4;
 
+template <int LMAX>
 struct alignas(8) FusedSeedLds {
-    uint64_t sh[F_LMAX];
-    uint64_t nmask[F_LMAX / 64 + 2];
+    uint64_t sh[LMAX];
+    uint64_t nmask[LMAX / 64 + 2];
     uint64_t sy_hash[F_SYCAP];
-    uint64_t pk[2 * (F_LMAX / 64) + 1];
+    uint64_t pk[2 * ((LMAX + 63) / 64) + 1];
     uint16_t sy_pos[F_SYCAP];
-    uint8_t st[F_LMAX];
+    uint8_t st[LMAX];
 };
 
+template <int LMAX>
 union alignas(8) FusedLds {
-    FusedSeedLds seed;
+    FusedSeedLds<LMAX> seed;
     MergeLds merge;
 };
 
@@ -1119,99 +1126,121 @@ __device__ __forceinline__ void spill_read(const DevBatch &b, uint32_t r) {
     wave_sync_mem();
 }
 
-__global__ __launch_bounds__(64 * F_WPB) void k_fused(DevBatch b, DevIndex ix, SeedParams P,
-                                                      uint32_t rescue_cutoff, uint32_t hard_cutoff) {
-    __shared__ FusedLds lds[F_WPB];
+template <int LMAX>
+__device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix, const SeedParams &P,
+                                           FusedLds<LMAX> &S, uint32_t r, uint32_t rescue_cutoff,
+                                           uint32_t hard_cutoff) {
     const uint32_t lane = lane_id();
-    const int wid = (int)(threadIdx.x >> 6);
-    FusedLds &S = lds[wid];
-    for (uint32_t r = blockIdx.x * F_WPB + (uint32_t)wid; r < b.n_reads; r += gridDim.x * F_WPB) {
-        const int L = (int)b.read_len[r];
-        bool spill = L > F_LMAX;
-        uint32_t nh = 0, n_fwd = 0;
-        bool rescue = false;
-        float frac = 1.0f;
+    const int L = (int)b.read_len[r];
+    bool spill = L > LMAX;
+    uint32_t nh = 0, n_fwd = 0;
+    bool rescue = false;
+    float frac = 1.0f;
+    if (!spill) {
+        SeedBufs B{S.seed.pk, S.seed.st, S.seed.sh, S.seed.sy_hash, S.seed.sy_pos, S.seed.nmask};
+        const uint32_t n = seed_syncmers(b.seqs + b.read_off[r], L, P, B, F_SYCAP, nullptr, nullptr);
+        n_fwd = ((uint32_t)L >= P.w_max && n > P.w_min) ? n - P.w_min : 0;
+        spill = n > (uint32_t)F_SYCAP || n_fwd > 64;
+        RsPair rp{};
+        const bool act = !spill && lane < n_fwd;
+        if (act) rp = randstrobe_pair(B, n, lane, L, P);
+        wave_sync();  // the syncmer lists are dead from here; LDS becomes MergeLds
         if (!spill) {
-            SeedBufs B{S.seed.pk, S.seed.st, S.seed.sh, S.seed.sy_hash, S.seed.sy_pos, S.seed.nmask};
-            const uint32_t n = seed_syncmers(b.seqs + b.read_off[r], L, P, B, F_SYCAP, nullptr, nullptr);
-            n_fwd = ((uint32_t)L >= P.w_max && n > P.w_min) ? n - P.w_min : 0;
-            spill = n > (uint32_t)F_SYCAP || n_fwd > 64;
-            RsPair rp{};
-            const bool act = !spill && lane < n_fwd;
-            if (act) rp = randstrobe_pair(B, n, lane, L, P);
-            wave_sync();  // the syncmer lists are dead from here; LDS becomes MergeLds
-            if (!spill) {
-                // both randstrobes of the lane through find / filter / run walk
-                uint64_t pf = GIS_NOT_FOUND, pr = GIS_NOT_FOUND;
-                uint32_t hf = 0, hr = 0;
-                bool gf = false, gr = false;
-                if (act) {
-                    pf = dev_find(ix, rp.fh);
-                    pr = dev_find(ix, rp.rh);
-                    if (pf != GIS_NOT_FOUND && !dev_filtered(ix, pf, rp.fh)) {
-                        gf = true;
-                        hf = walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, nullptr);
-                    }
-                    if (pr != GIS_NOT_FOUND && !dev_filtered(ix, pr, rp.rh)) {
-                        gr = true;
-                        hr = walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, nullptr);
-                    }
+            // both randstrobes of the lane through find / filter / run walk
+            uint64_t pf = GIS_NOT_FOUND, pr = GIS_NOT_FOUND;
+            uint32_t hf = 0, hr = 0;
+            bool gf = false, gr = false;
+            if (act) {
+                pf = dev_find(ix, rp.fh);
+                pr = dev_find(ix, rp.rh);
+                if (pf != GIS_NOT_FOUND && !dev_filtered(ix, pf, rp.fh)) {
+                    gf = true;
+                    hf = walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, nullptr);
                 }
-                const uint32_t found = wave_sum((pf != GIS_NOT_FOUND ? 1u : 0u) + (pr != GIS_NOT_FOUND ? 1u : 0u));
-                const uint32_t good = wave_sum((gf ? 1u : 0u) + (gr ? 1u : 0u));
-                const uint32_t hits_norm = wave_sum(hf + hr);
-                frac = found > 0 ? (float)good / (float)found : 1.0f;
-                rescue = (double)frac < 0.7 || hits_norm == 0;
-                if (rescue) {
-                    // run lengths and the taken prefix of each strand
-                    const uint32_t cf = act && pf != GIS_NOT_FOUND ? dev_get_count(ix, pf) : 0u;
-                    const uint32_t cr = act && pr != GIS_NOT_FOUND ? dev_get_count(ix, pr) : 0u;
-                    uint32_t rankf = 0, rankr = 0;
-                    for (uint32_t j = 0; j < n_fwd; j++) {
-                        const uint32_t cjf = __shfl(cf, (int)j, 64), sjf = __shfl(rp.fs, (int)j, 64),
-                                       ejf = __shfl(rp.fe, (int)j, 64);
-                        const uint32_t cjr = __shfl(cr, (int)j, 64), sjr = __shfl(rp.rs, (int)j, 64),
-                                       ejr = __shfl(rp.re, (int)j, 64);
-                        rankf += (cjf > 0 && rh_less(cjf, sjf, ejf, cf, rp.fs, rp.fe)) ? 1u : 0u;
-                        rankr += (cjr > 0 && rh_less(cjr, sjr, ejr, cr, rp.rs, rp.re)) ? 1u : 0u;
-                    }
-                    const bool tf = cf > 0 && cf <= hard_cutoff && (cf <= rescue_cutoff || rankf < 5);
-                    const bool tr = cr > 0 && cr <= hard_cutoff && (cr <= rescue_cutoff || rankr < 5);
-                    hf = tf ? walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, nullptr) : 0u;
-                    hr = tr ? walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, nullptr) : 0u;
-                }
-                // hits into LDS: forward seeds in lane order, then reverse ones
-                const uint32_t tot_f = wave_sum(hf);
-                nh = tot_f + wave_sum(hr);
-                spill = nh > (uint32_t)MERGE_LDS;
-                if (!spill) {
-                    const uint32_t of = wave_excl_scan(hf), orr = tot_f + wave_excl_scan(hr);
-                    if (hf) walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, S.merge.hits + of);
-                    if (hr) walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, S.merge.hits + orr);
-                    wave_sync_mem();
+                if (pr != GIS_NOT_FOUND && !dev_filtered(ix, pr, rp.rh)) {
+                    gr = true;
+                    hr = walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, nullptr);
                 }
             }
+            const uint32_t found = wave_sum((pf != GIS_NOT_FOUND ? 1u : 0u) + (pr != GIS_NOT_FOUND ? 1u : 0u));
+            const uint32_t good = wave_sum((gf ? 1u : 0u) + (gr ? 1u : 0u));
+            const uint32_t hits_norm = wave_sum(hf + hr);
+            frac = found > 0 ? (float)good / (float)found : 1.0f;
+            rescue = (double)frac < 0.7 || hits_norm == 0;
+            if (rescue) {
+                // run lengths and the taken prefix of each strand
+                const uint32_t cf = act && pf != GIS_NOT_FOUND ? dev_get_count(ix, pf) : 0u;
+                const uint32_t cr = act && pr != GIS_NOT_FOUND ? dev_get_count(ix, pr) : 0u;
+                uint32_t rankf = 0, rankr = 0;
+                for (uint32_t j = 0; j < n_fwd; j++) {
+                    const uint32_t cjf = __shfl(cf, (int)j, 64), sjf = __shfl(rp.fs, (int)j, 64),
+                                   ejf = __shfl(rp.fe, (int)j, 64);
+                    const uint32_t cjr = __shfl(cr, (int)j, 64), sjr = __shfl(rp.rs, (int)j, 64),
+                                   ejr = __shfl(rp.re, (int)j, 64);
+                    rankf += (cjf > 0 && rh_less(cjf, sjf, ejf, cf, rp.fs, rp.fe)) ? 1u : 0u;
+                    rankr += (cjr > 0 && rh_less(cjr, sjr, ejr, cr, rp.rs, rp.re)) ? 1u : 0u;
+                }
+                const bool tf = cf > 0 && cf <= hard_cutoff && (cf <= rescue_cutoff || rankf < 5);
+                const bool tr = cr > 0 && cr <= hard_cutoff && (cr <= rescue_cutoff || rankr < 5);
+                hf = tf ? walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, nullptr) : 0u;
+                hr = tr ? walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, nullptr) : 0u;
+            }
+            // hits into LDS: forward seeds in lane order, then reverse ones
+            const uint32_t tot_f = wave_sum(hf);
+            nh = tot_f + wave_sum(hr);
+            spill = nh > (uint32_t)MERGE_LDS;
+            if (!spill) {
+                const uint32_t of = wave_excl_scan(hf), orr = tot_f + wave_excl_scan(hr);
+                if (hf) walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, S.merge.hits + of);
+                if (hr) walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, S.merge.hits + orr);
+                wave_sync_mem();
+            }
         }
-        if (spill) {
-            spill_read(b, r);
-            continue;
-        }
-        uint32_t total = 0;
-        if (nh > 0)
-            total = merge_wave(nh, S.merge.hits, S.merge.keys, b.nam_slots + (uint64_t)r * F_NAMCAP,
-                               (uint32_t)F_NAMCAP);
-        if (total > (uint32_t)F_NAMCAP) {  // includes MERGE_OVERFLOW
-            spill_read(b, r);
-            continue;
-        }
-        if (lane == 0) {
-            b.frac[r] = frac;
-            b.flags[r] = (uint8_t)((rescue ? GIS_FLAG_RESCUED : 0) | (n_fwd == 0 ? GIS_FLAG_NO_RANDSTROBES : 0));
-            b.nam_cnt[r] = total;
-            b.hit_total[r] = 0;
-            b.rhit_total[r] = 0;
-        }
-        wave_sync_mem();
+    }
+    if (spill) {
+        spill_read(b, r);
+        return;
+    }
+    uint32_t total = 0;
+    if (nh > 0)
+        total = merge_wave(nh, S.merge.hits, S.merge.keys, b.nam_slots + (uint64_t)r * F_NAMCAP,
+                           (uint32_t)F_NAMCAP);
+    if (total > (uint32_t)F_NAMCAP) {  // includes MERGE_OVERFLOW
+        spill_read(b, r);
+        return;
+    }
+    if (lane == 0) {
+        b.frac[r] = frac;
+        b.flags[r] = (uint8_t)((rescue ? GIS_FLAG_RESCUED : 0) | (n_fwd == 0 ? GIS_FLAG_NO_RANDSTROBES : 0));
+        b.nam_cnt[r] = total;
+        b.hit_total[r] = 0;
+        b.rhit_total[r] = 0;
+    }
+    wave_sync_mem();
+}
+
+template <int LMAX, int FWPB>
+__global__ __launch_bounds__(64 * FWPB) void k_fused(DevBatch b, DevIndex ix, SeedParams P,
+                                                     uint32_t rescue_cutoff, uint32_t hard_cutoff,
+                                                     const uint32_t *__restrict__ queue, uint32_t count,
+                                                     uint32_t chunk, unsigned int *head) {
+    __shared__ FusedLds<LMAX> lds[FWPB];
+    const int wid = (int)(threadIdx.x >> 6);
+    FusedLds<LMAX> &S = lds[wid];
+    if (chunk == 0) {
+        for (uint32_t w = blockIdx.x * FWPB + (uint32_t)wid; w < count; w += gridDim.x * FWPB)
+            fused_read<LMAX>(b, ix, P, S, queue[w], rescue_cutoff, hard_cutoff);
+        return;
+    }
+    // dynamic: every wave leaves once a grab starts at or past the end
+    for (;;) {
+        uint32_t start = 0;
+        if (lane_id() == 0) start = atomicAdd(head, chunk);
+        start = (uint32_t)__builtin_amdgcn_readfirstlane((int)start);
+        if (start >= count) break;
+        const uint32_t end = count - start < chunk ? count : start + chunk;
+        for (uint32_t w = start; w < end; w++)
+            fused_read<LMAX>(b, ix, P, S, queue[w], rescue_cutoff, hard_cutoff);
     }
 }
 
@@ -1245,6 +1274,8 @@ __global__ void k_reset_ctl(DevCtl *ctl) {
     if (threadIdx.x == 0) {
         ctl->n_spill = 0;
         ctl->overflow = 0;
+        ctl->q_head[0] = 0;
+        ctl->q_head[1] = 0;
     }
 }
 
@@ -1380,11 +1411,32 @@ void launch_merge(const DevBatch &b, const ReadSet &set, int rescue, hipStream_t
     hipLaunchKernelGGL(k_merge, dim3(grid_for_set(b, set, WPB)), dim3(64 * WPB), 0, st, b, set, rescue);
 }
 
-void launch_fused(const DevBatch &b, const DevIndex &ix, const SeedParams &sp,
-                  uint32_t rescue_cutoff, uint32_t hard_cutoff, hipStream_t st) {
-    if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_fused, dim3(grid_for(b.n_reads, F_WPB)), dim3(64 * F_WPB), 0, st, b, ix, sp,
-                       rescue_cutoff, hard_cutoff);
+// fused kernel over queue[0 .. count): FWPB waves per block, LMAX the LDS
+// tiling (reads longer than LMAX spill), grid 0 = one wave per entry,
+// chunk 0 = static round robin, else dynamic grabs of chunk entries.
+template <int LMAX, int FWPB>
+static void launch_fused_t(const DevBatch &b, const DevIndex &ix, const SeedParams &sp, uint32_t rcut,
+                           uint32_t hard, const uint32_t *queue, uint32_t count, unsigned grid,
+                           uint32_t chunk, unsigned int *head, hipStream_t st) {
+    const unsigned g1 = grid_for(count, FWPB);
+    const unsigned g = grid == 0 || grid > g1 ? g1 : grid;
+    hipLaunchKernelGGL((k_fused<LMAX, FWPB>), dim3(g), dim3(64 * FWPB), 0, st, b, ix, sp, rcut, hard,
+                       queue, count, chunk, head);
+}
+
+bool launch_fused(const DevBatch &b, const DevIndex &ix, const SeedParams &sp, uint32_t rcut,
+                  uint32_t hard, const uint32_t *queue, uint32_t count, int lmax, int wpb, unsigned grid,
+                  uint32_t chunk, unsigned int *head, hipStream_t st) {
+    if (count == 0) return true;
+#define GIS_FUSED_CASE(LM, W)                                                                      \
+    if (lmax == LM && wpb == W) {                                                                  \
+        launch_fused_t<LM, W>(b, ix, sp, rcut, hard, queue, count, grid, chunk, head, st);         \
+        return true;                                                                               \
+    }
+    GIS_FUSED_CASE(160, 2) GIS_FUSED_CASE(160, 4) GIS_FUSED_CASE(160, 8)
+    GIS_FUSED_CASE(320, 2) GIS_FUSED_CASE(320, 4) GIS_FUSED_CASE(320, 8)
+#undef GIS_FUSED_CASE
+    return false;
 }
 
 void launch_compact(const DevBatch &b, int fused, hipStream_t st) {

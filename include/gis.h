@@ -140,6 +140,21 @@ enum { GIS_RUN_NO_GRAPH = 1 };
 
 typedef struct gis_ctx gis_ctx;
 
+/* Launch configuration of the fused form (GIS_MODE_FUSED), kept per context.
+ * The fused kernel is persistent over a device work queue of read ids sorted
+ * by length, longest first. */
+typedef struct {
+    int32_t waves_per_block; /* 2, 4 or 8 reads in flight per workgroup */
+    int32_t length_split;    /* 1: reads of at most 160 bp run in a second launch with 160-base LDS tiles */
+    int32_t queue;           /* 0: static round robin over the queue; 1: dynamic grabs from a device counter */
+    int32_t chunk;           /* queue entries per grab when queue = 1 (1..4096) */
+    int32_t blocks_per_cu;   /* 0: one wave per read; 1..64: a persistent grid of that many blocks per CU */
+} gis_fused_config;
+
+/* ---------------- fused-form configuration (no reference counterpart) ---------------- */
+int gis_fused_config_get(gis_ctx *ctx, gis_fused_config *out);
+int gis_fused_config_set(gis_ctx *ctx, const gis_fused_config *cfg);
+
 /* ---------------- parameters ---------------- */
 
 /* IndexParameters::from_read_length, step2/indexparameters.hpp:87-89,
@@ -230,6 +245,12 @@ int gis_batch_stage_times(gis_batch *b, double *ms, int cap, int *n, char *names
 int gis_batch_lookup_bytes(gis_batch *b, uint64_t *bytes, uint64_t *n_lookups);
 /* total NAMs of the last run (available after gis_batch_run, before download) */
 int gis_batch_nam_count(gis_batch *b, uint64_t *n_nams);
+/* Autotuner of the fused form: runs the uploaded reads under every candidate
+ * gis_fused_config (eager launches, best of three wall-clock runs each),
+ * keeps the fastest in the context and reports it. Results are the same
+ * under every configuration. */
+int gis_batch_autotune(gis_batch *b, const gis_map_options *opt, gis_fused_config *best,
+                       double *best_ms, uint32_t *n_tried);
 
 /* ---------------- synthetic inputs (deterministic) ---------------- */
 typedef struct {

@@ -358,3 +358,63 @@ def test_cxx_facade_matches_oracle(setup, tmp_path):
         assert n == len(reads)
         assert np.array_equal(offs, eo), suffix
         assert nams.tobytes() == en.tobytes(), suffix
+
+
+FUSED_CONFIGS = [dict(waves_per_block=w, length_split=sp, queue=q, chunk=ch, blocks_per_cu=bpc)
+                 for w in (2, 4, 8) for sp in (0, 1)
+                 for (q, ch, bpc) in ((0, 0, 0), (0, 0, 2), (1, 3, 2))]
+
+
+def test_fused_configs_match_oracle(setup):
+    """Every launch configuration of the fused form (waves per block, length
+    split into a 160-base and a 320-base launch, static or dynamic work
+    queue, persistent grid) gives the oracle's NAMs."""
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 600, seed=71)
+    s, o, l = G.pack_reads(reads)
+    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
+    saved = idx.ctx.fused_config
+    try:
+        for cfg in FUSED_CONFIGS:
+            idx.ctx.fused_config = cfg
+            for graph in (False, True):
+                res = G.find_nams_batch(idx, s, o, l, mode=G.MODE_FUSED, graph=graph)
+                _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
+                              res.flags, efl)
+    finally:
+        idx.ctx.fused_config = saved
+
+
+def test_fused_config_validation(setup):
+    g, lens, p, idx, op, oidx = setup
+    saved = idx.ctx.fused_config
+    bad = [dict(waves_per_block=3), dict(length_split=2), dict(queue=1, chunk=0, blocks_per_cu=2),
+           dict(queue=1, chunk=8, blocks_per_cu=0), dict(blocks_per_cu=65), dict(queue=2)]
+    for b in bad:
+        with pytest.raises(G.GisError):
+            idx.ctx.fused_config = dict(saved, **b)
+        assert idx.ctx.fused_config == saved
+
+
+def test_autotune_keeps_results(setup):
+    """The autotuner tries every candidate on the uploaded reads and keeps the
+    fastest; NAMs stay the oracle's under the chosen configuration."""
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 800, seed=72)
+    s, o, l = G.pack_reads(reads)
+    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
+    saved = idx.ctx.fused_config
+    m = G.Mapper(idx, len(l), len(s))
+    try:
+        m.upload(s, o, l)
+        best, ms, n = m.autotune()
+        assert n >= 18 and ms > 0
+        assert idx.ctx.fused_config == best
+        for graph in (False, True):
+            m.run(mode=G.MODE_FUSED, graph=graph)
+            res = m.download()
+            _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
+                          res.flags, efl)
+    finally:
+        m.close()
+        idx.ctx.fused_config = saved
