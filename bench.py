@@ -1,13 +1,20 @@
 """Benchmark of the strobealign seeding path on one MI355X.
 
-Workload (the GPU drop-in configuration): 100 Mbp synthetic genome (8 contigs,
-GC 41%, 5% repeat content from 500 families of 1-10 kbp copied 2-50x at 1-5%
-divergence), 1M single-end 150 bp reads (0.5% substitutions, 0.05% indels),
-fixed 65,536-read batches, one launch per stage. A step is one batch through
-seeding -> lookup -> hits -> NAMs (+ rescue), reads already resident in HBM.
+Default workload (the GPU drop-in configuration, C2): 100 Mbp synthetic
+genome (8 contigs, GC 41%, 5% repeat content from 500 families of 1-10 kbp
+copied 2-50x at 1-5% divergence), 1M single-end 150 bp reads (0.5%
+substitutions, 0.05% indels), fixed 65,536-read batches, one launch per
+stage. A step is one batch through seeding -> lookup -> hits -> NAMs
+(+ rescue), reads already resident in HBM.
+
+--config c3 runs the batched uneven-read form instead: the same genome, 2M
+read pairs with lengths drawn from 50/100/150/250/300 bp (weights
+0.1/0.2/0.4/0.2/0.1, insert N(400, 50)), 131,072-read batches through the
+persistent fused kernel under the configuration the autotuner picks, beside
+the per-stage launches.
 
 Prints one JSON line: reads/s (whole job), p50/p99 batch latency, the
-roofline of the dominant kernel and the CPU oracle's rate on the host cores.
+roofline of the lookup kernel and the CPU oracle's rate on the host cores.
 """
 import argparse
 import json
@@ -28,13 +35,256 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def c2_genome(G, genome_bp):
+    return G.synth_genome(genome_bp, n_contigs=8, gc=0.41, repeat_fraction=0.05, n_families=500,
+                          family_len=(1000, 10000), copies=(2, 50), divergence=(0.01, 0.05), seed=1)
+
+
+def build_index(G, genome, clens, threads):
+    p = G.IndexParameters.from_read_length(150)
+    idx = G.StrobemerIndex(None, p)
+    t0 = time.time()
+    idx.populate_arrays(genome, clens, f=0.0002, n_threads=threads)
+    info = idx.info()
+    log(f"index: {info.n_randstrobes} randstrobes, bits {info.bits}, filter_cutoff "
+        f"{info.filter_cutoff}, built+uploaded in {time.time() - t0:.1f}s")
+    return idx, info
+
+
+def make_batches(G, idx, seqs, offs, lens, batch):
+    n = len(lens)
+    nb = math.ceil(n / batch)
+    mappers = []
+    for i in range(nb):
+        lo, hi = i * batch, min(n, (i + 1) * batch)
+        base = int(offs[lo])
+        end = int(offs[hi - 1] + lens[hi - 1])
+        m = G.Mapper(idx, hi - lo, end - base)
+        m.upload(seqs[base:end], offs[lo:hi] - base, lens[lo:hi])
+        mappers.append(m)
+    return mappers
+
+
+def timed(mappers, steps, warmup, mode, graph):
+    """K timed batch runs after an untimed pass over every batch (buffers
+    allocated, graphs captured) and W warm-up runs."""
+    nb = len(mappers)
+    for m in mappers:
+        m.run(mode=mode, graph=graph)
+    for i in range(warmup):
+        mappers[i % nb].run(mode=mode, graph=graph)
+    step_ms, reads_done = [], 0
+    t_start = time.perf_counter()
+    for i in range(steps):
+        m = mappers[i % nb]
+        ts = time.perf_counter()
+        m.run(mode=mode, graph=graph)  # ends with a stream synchronisation
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+        reads_done += m.n
+    elapsed = time.perf_counter() - t_start
+    return reads_done / elapsed, elapsed, step_ms
+
+
+def kernel_times(G, mappers, k, mode):
+    """The same runs enqueued launch by launch, with HIP events between the
+    kernels on the batch's stream (graph replays carry no events)."""
+    out = []
+    for i in range(min(k, len(mappers))):
+        mappers[i].run(mode=mode, graph=False)
+        out.append(mappers[i].stage_times())
+    return out
+
+
+def lookup_roofline(G, idx, mappers, k, stages):
+    """Algorithmic bytes of k_lookup per launch (counted on the device in the
+    plain eager runs just made) over its HIP-event time."""
+    lk_bytes, lk_n = [], []
+    for i in range(min(k, len(mappers))):
+        b, q = G._native.C.c_uint64(), G._native.C.c_uint64()
+        idx.ctx.check(idx.ctx.L.gis_batch_lookup_bytes(mappers[i].h, G._native.C.byref(b),
+                                                       G._native.C.byref(q)))
+        lk_bytes.append(b.value)
+        lk_n.append(q.value)
+    lookup_ms = [st["lookup"] for st in stages]
+    achieved = float(np.sum(lk_bytes) / (np.sum(lookup_ms) * 1e-3) / 1e9)
+    log(f"lookup: {np.mean(lk_bytes) / 1e6:.1f} MB algorithmic per launch, "
+        f"{np.mean(lookup_ms):.3f} ms avg, {np.mean(lk_n):.0f} lookups per launch")
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "lookup_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    return {"kernel": "k_lookup", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "algorithmic_bytes_per_launch": float(np.mean(lk_bytes)),
+            "avg_launch_ms": float(np.mean(lookup_ms))}
+
+
+def cpu_baseline(genome, clens, seqs, offs, lens, n_cpu, threads, what):
+    """The CPU oracle on the host cores: one warm-up pass, then the median of
+    five timed passes over the first n_cpu reads, and a 1-thread figure on a
+    sixteenth of them."""
+    from oracle import oracle_lib as O
+    t0 = time.time()
+    oidx = O.Index.build_arrays(genome, clens, O.params_from_read_length(150), n_threads=threads)
+    log(f"oracle index built in {time.time() - t0:.1f}s")
+    n_cpu = min(n_cpu, len(lens))
+    end = int(offs[n_cpu - 1] + lens[n_cpu - 1])
+    rc = 2 * oidx.filter_cutoff
+    O.time_map_batch(oidx, seqs[:end], offs[:n_cpu], lens[:n_cpu], rc, 1000, threads)
+    runs = [O.time_map_batch(oidx, seqs[:end], offs[:n_cpu], lens[:n_cpu], rc, 1000, threads)
+            for _ in range(5)]
+    secs = float(np.median(runs))
+    n1 = max(1, n_cpu // 16)
+    end1 = int(offs[n1 - 1] + lens[n1 - 1])
+    s1 = O.time_map_batch(oidx, seqs[:end1], offs[:n1], lens[:n1], rc, 1000, 1)
+    oidx.free()
+    log(f"cpu oracle: {n_cpu / secs / 1e6:.2f} M reads/s on {threads} threads, "
+        f"{n1 / s1 / 1e3:.0f} k reads/s on 1 thread")
+    return {"value": n_cpu / secs, "unit": "reads/s", "cores": threads, "kind": "port",
+            "sample": f"first {n_cpu} reads of the {what} workload, seeding through NAMs + rescue; "
+                      f"median of 5 passes ({secs:.2f} s each) after a warm-up pass",
+            "one_thread_value": n1 / s1}
+
+
+def run_c2(args, G):
+    genome, clens = c2_genome(G, args.genome)
+    threads = min(16, os.cpu_count() or 1)
+    idx, info = build_index(G, genome, clens, threads)
+    t0 = time.time()
+    seqs, offs, lens, _ = G.synth_reads(genome, clens, args.reads, lengths=(150,), weights=(1.0,),
+                                        sub_rate=0.005, indel_rate=0.0005, seed=2)
+    log(f"reads {args.reads} in {time.time() - t0:.1f}s")
+    batch = args.batch or 65536
+    mappers = make_batches(G, idx, seqs, offs, lens, batch)
+
+    # the fused form (one wave-per-read kernel, spills to the per-stage
+    # kernels), reported beside the plain per-stage drop-in
+    fused_rps, fused_elapsed, fused_ms = timed(mappers, args.steps, args.warmup, G.MODE_FUSED, True)
+    fused_eager_rps, _, _ = timed(mappers, args.steps, args.warmup, G.MODE_FUSED, False)
+    log(f"fused form: {fused_rps / 1e6:.1f} M reads/s (graph), {fused_eager_rps / 1e6:.1f} M reads/s (eager)")
+    # the plain per-stage drop-in: the configuration's form and the value
+    plain_eager_rps, _, _ = timed(mappers, args.steps, args.warmup, G.MODE_PLAIN, False)
+    reads_per_s, elapsed, step_ms = timed(mappers, args.steps, args.warmup, G.MODE_PLAIN, True)
+    log(f"plain form: {reads_per_s / 1e6:.1f} M reads/s (graph), {plain_eager_rps / 1e6:.1f} M reads/s (eager)")
+    stages = kernel_times(G, mappers, args.steps, G.MODE_PLAIN)
+    roof = lookup_roofline(G, idx, mappers, args.steps, stages)
+    log("stage ms (mean): " + ", ".join(f"{k}={np.mean([st[k] for st in stages]):.3f}" for k in stages[0]))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(genome, clens, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C2")
+
+    return {
+        "metric": "reads/sec to NAMs at 150bp (1 GPU, bit-exact) vs CPU; p50/p99 batch latency",
+        "value": reads_per_s,
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "p50_batch_ms": float(np.percentile(step_ms, 50)),
+        "p99_batch_ms": float(np.percentile(step_ms, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded genome and reads)",
+        "config": {
+            "workload": "plain GPU drop-in: 100 Mbp synthetic genome (8 contigs, GC 41%, 5% "
+                        "repeats), 1M SE 150 bp reads, 65,536-read batches, one launch per stage",
+            "genome_bp": args.genome, "reads": args.reads, "batch_reads": batch,
+            "batches": len(mappers), "index_randstrobes": int(info.n_randstrobes),
+            "bits": int(info.bits), "filter_cutoff": int(info.filter_cutoff),
+            "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "plain_eager_value": plain_eager_rps,
+        "stage_ms": {k: float(np.mean([st[k] for st in stages])) for k in stages[0]},
+        "fused_form": {"value": fused_rps, "unit": "reads/s", "eager_value": fused_eager_rps,
+                       "ms_per_step": fused_elapsed * 1e3 / args.steps,
+                       "p50_batch_ms": float(np.percentile(fused_ms, 50)),
+                       "p99_batch_ms": float(np.percentile(fused_ms, 99))},
+    }
+
+
+def run_c3(args, G):
+    genome, clens = c2_genome(G, args.genome)
+    threads = min(16, os.cpu_count() or 1)
+    idx, info = build_index(G, genome, clens, threads)
+    pairs = args.pairs
+    t0 = time.time()
+    seqs, offs, lens, _ = G.synth_reads(genome, clens, pairs, lengths=(50, 100, 150, 250, 300),
+                                        weights=(0.1, 0.2, 0.4, 0.2, 0.1), paired=True,
+                                        insert=(400.0, 50.0), sub_rate=0.005, indel_rate=0.0005,
+                                        seed=2)
+    log(f"reads {len(lens)} ({pairs} pairs, mean length {lens.mean():.1f}) in {time.time() - t0:.1f}s")
+    batch = args.batch or 131072
+    mappers = make_batches(G, idx, seqs, offs, lens, batch)
+    steps = args.steps
+
+    best, best_ms, n_tried = mappers[0].autotune()
+    log(f"autotune: {n_tried} configurations, best {best} at {best_ms:.3f} ms per batch")
+    fused_rps, fused_elapsed, fused_ms = timed(mappers, steps, args.warmup, G.MODE_FUSED, True)
+    log(f"fused (autotuned): {fused_rps / 1e6:.1f} M reads/s")
+    plain_rps, plain_elapsed, plain_ms = timed(mappers, steps, args.warmup, G.MODE_PLAIN, True)
+    log(f"plain per-stage: {plain_rps / 1e6:.1f} M reads/s")
+    fstage = kernel_times(G, mappers, steps, G.MODE_FUSED)
+    spilled = [m.download().flags for m in mappers[:1]]
+    spill_frac = float(np.mean((spilled[0] & 0x80) != 0))
+    stages = kernel_times(G, mappers, steps, G.MODE_PLAIN)
+    roof = lookup_roofline(G, idx, mappers, steps, stages)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(genome, clens, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C3")
+
+    return {
+        "metric": "reads/sec to NAMs, 50-300 bp paired mixture (1 GPU, bit-exact) vs CPU; "
+                  "p50/p99 batch latency",
+        "value": fused_rps,
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": fused_elapsed * 1e3 / steps,
+        "p50_batch_ms": float(np.percentile(fused_ms, 50)),
+        "p99_batch_ms": float(np.percentile(fused_ms, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded genome and reads)",
+        "config": {
+            "workload": "batched uneven reads: 100 Mbp synthetic genome, 2M read pairs of "
+                        "50/100/150/250/300 bp (0.1/0.2/0.4/0.2/0.1), insert N(400, 50), "
+                        "131,072-read batches, persistent fused kernel (autotuned)",
+            "genome_bp": args.genome, "pairs": pairs, "reads": int(len(lens)),
+            "batch_reads": batch, "batches": len(mappers), "fused_config": best,
+            "autotune_candidates": n_tried, "spilled_fraction": spill_frac,
+            "index_randstrobes": int(info.n_randstrobes), "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fused_stage_ms": {k: float(np.mean([st[k] for st in fstage])) for k in fstage[0]},
+        "plain_form": {"value": plain_rps, "unit": "reads/s",
+                       "ms_per_step": plain_elapsed * 1e3 / steps,
+                       "p50_batch_ms": float(np.percentile(plain_ms, 50)),
+                       "p99_batch_ms": float(np.percentile(plain_ms, 99)),
+                       "stage_ms": {k: float(np.mean([st[k] for st in stages])) for k in stages[0]}},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--reads", type=int, default=1_000_000)
-    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--config", choices=("c2", "c3"), default="c2")
+    ap.add_argument("--reads", type=int, default=1_000_000, help="c2: single-end reads")
+    ap.add_argument("--pairs", type=int, default=2_000_000, help="c3: read pairs")
+    ap.add_argument("--batch", type=int, default=0, help="reads per batch (c2 65536, c3 131072)")
     ap.add_argument("--genome", type=int, default=100_000_000)
     ap.add_argument("--cpu-reads", type=int, default=1_000_000,
                     help="bounded sample of the workload timed on the CPU oracle")
@@ -61,155 +311,7 @@ def main():
         sys.exit(2)
 
     import gpu_index_test_amd as G
-
-    t0 = time.time()
-    genome, clens = G.synth_genome(args.genome, n_contigs=8, gc=0.41, repeat_fraction=0.05,
-                                   n_families=500, family_len=(1000, 10000), copies=(2, 50),
-                                   divergence=(0.01, 0.05), seed=1)
-    log(f"genome {args.genome} bp in {time.time() - t0:.1f}s")
-    p = G.IndexParameters.from_read_length(150)
-    idx = G.StrobemerIndex(None, p)
-    t0 = time.time()
-    idx.populate_arrays(genome, clens, f=0.0002, n_threads=min(16, os.cpu_count() or 1))
-    info = idx.info()
-    log(f"index: {info.n_randstrobes} randstrobes, bits {info.bits}, filter_cutoff "
-        f"{info.filter_cutoff}, built+uploaded in {time.time() - t0:.1f}s")
-    t0 = time.time()
-    seqs, offs, lens, _ = G.synth_reads(genome, clens, args.reads, lengths=(150,), weights=(1.0,),
-                                        sub_rate=0.005, indel_rate=0.0005, seed=2)
-    log(f"reads {args.reads} in {time.time() - t0:.1f}s")
-
-    # one device batch per 65,536 reads, all uploaded before timing
-    nb = math.ceil(args.reads / args.batch)
-    mappers = []
-    for i in range(nb):
-        lo, hi = i * args.batch, min(args.reads, (i + 1) * args.batch)
-        base = int(offs[lo])
-        end = int(offs[hi - 1] + lens[hi - 1])
-        m = G.Mapper(idx, hi - lo, end - base)
-        m.upload(seqs[base:end], offs[lo:hi] - base, lens[lo:hi])
-        mappers.append(m)
-
-    def timed(mode, graph):
-        # set-up: one untimed run per batch so every device buffer has been
-        # allocated and touched (and its launch graph captured) before timing
-        for m in mappers:
-            m.run(mode=mode, graph=graph)
-        for i in range(args.warmup):
-            mappers[i % nb].run(mode=mode, graph=graph)
-        step_ms, reads_done = [], 0
-        t_start = time.perf_counter()
-        for i in range(args.steps):
-            m = mappers[i % nb]
-            ts = time.perf_counter()
-            m.run(mode=mode, graph=graph)  # ends with a stream synchronisation
-            step_ms.append((time.perf_counter() - ts) * 1e3)
-            reads_done += m.n
-        elapsed = time.perf_counter() - t_start
-        return reads_done / elapsed, elapsed, step_ms
-
-    def kernel_times(mode):
-        # the same runs enqueued launch by launch, with HIP events between the
-        # kernels on the batch's stream (graph replays carry no events)
-        out = []
-        for i in range(min(args.steps, nb)):
-            mappers[i].run(mode=mode, graph=False)
-            out.append(mappers[i].stage_times())
-        return out
-
-    # the fused form (one wave-per-read kernel, spills to the per-stage
-    # kernels), reported beside the plain per-stage drop-in
-    fused_rps, fused_elapsed, fused_ms = timed(G.MODE_FUSED, True)
-    fused_eager_rps, _, _ = timed(G.MODE_FUSED, False)
-    fused_stage = kernel_times(G.MODE_FUSED)[-1]
-    log(f"fused form: {fused_rps / 1e6:.1f} M reads/s (graph), {fused_eager_rps / 1e6:.1f} M reads/s "
-        "(eager); stage ms " + ", ".join(f"{k}={v:.3f}" for k, v in fused_stage.items()))
-    # the plain per-stage drop-in: the configuration's form and the value
-    plain_eager_rps, _, _ = timed(G.MODE_PLAIN, False)
-    reads_per_s, elapsed, step_ms = timed(G.MODE_PLAIN, True)
-    log(f"plain form: {reads_per_s / 1e6:.1f} M reads/s (graph), {plain_eager_rps / 1e6:.1f} M reads/s (eager)")
-    stages = kernel_times(G.MODE_PLAIN)
-    lookup_ms = [st["lookup"] for st in stages]
-
-    # algorithmic bytes of the lookup kernel per launch (per batch), counted
-    # on the device during the run
-    lk_bytes, lk_n = [], []
-    for i in range(min(args.steps, nb)):
-        m = mappers[i % nb]
-        b, q = G._native.C.c_uint64(), G._native.C.c_uint64()
-        idx.ctx.check(idx.ctx.L.gis_batch_lookup_bytes(m.h, G._native.C.byref(b),
-                                                       G._native.C.byref(q)))
-        lk_bytes.append(b.value)
-        lk_n.append(q.value)
-    achieved_gbs = float(np.sum(lk_bytes) / (np.sum(lookup_ms) * 1e-3) / 1e9)
-    stage_last = stages[-1]
-    log("stage ms (last batch): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_last.items()))
-    log(f"lookup: {np.mean(lk_bytes) / 1e6:.1f} MB algorithmic per launch, "
-        f"{np.mean(lookup_ms):.3f} ms avg, {np.mean(lk_n):.0f} lookups per launch")
-
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "lookup_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
-
-    cpu = None
-    if not args.no_cpu:
-        from oracle import oracle_lib as O
-        cores = args.cpu_threads or min(16, os.cpu_count() or 1)
-        t0 = time.time()
-        oidx = O.Index.build_arrays(genome, clens, O.params_from_read_length(150), n_threads=cores)
-        log(f"oracle index built in {time.time() - t0:.1f}s")
-        n_cpu = min(args.cpu_reads, args.reads)
-        end = int(offs[n_cpu - 1] + lens[n_cpu - 1])
-        secs = O.time_map_batch(oidx, seqs[:end], offs[:n_cpu], lens[:n_cpu],
-                                2 * oidx.filter_cutoff, 1000, cores)
-        cpu = {"value": n_cpu / secs, "unit": "reads/s", "cores": cores, "kind": "port",
-               "sample": f"first {n_cpu} reads of the workload, seeding through NAMs + rescue, "
-                         f"{secs:.1f} s wall on {cores} threads"}
-        oidx.free()
-
-    out = {
-        "metric": "reads/sec to NAMs at 150bp (1 GPU, bit-exact) vs CPU; p50/p99 batch latency",
-        "value": reads_per_s,
-        "unit": "reads/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
-        "p50_batch_ms": float(np.percentile(step_ms, 50)),
-        "p99_batch_ms": float(np.percentile(step_ms, 99)),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64",
-        "data": "synthetic (seeded genome and reads)",
-        "config": {
-            "workload": "plain GPU drop-in: 100 Mbp synthetic genome (8 contigs, GC 41%, 5% "
-                        "repeats), 1M SE 150 bp reads, 65,536-read batches, one launch per stage",
-            "genome_bp": args.genome, "reads": args.reads, "batch_reads": args.batch,
-            "batches": nb, "index_randstrobes": int(info.n_randstrobes), "bits": int(info.bits),
-            "filter_cutoff": int(info.filter_cutoff), "parallelism": "single GPU",
-        },
-        "roofline": {
-            "kernel": "k_lookup",
-            "bound": "hbm",
-            "achieved": achieved_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": float(np.mean(lk_bytes)),
-            "avg_launch_ms": float(np.mean(lookup_ms)),
-        },
-        "cpu_baseline": cpu,
-        "plain_eager_value": plain_eager_rps,
-        "stage_ms": {k: float(np.mean([st[k] for st in stages])) for k in stages[0]},
-        "fused_form": {"value": fused_rps, "unit": "reads/s", "eager_value": fused_eager_rps,
-                       "ms_per_step": fused_elapsed * 1e3 / args.steps,
-                       "p50_batch_ms": float(np.percentile(fused_ms, 50)),
-                       "p99_batch_ms": float(np.percentile(fused_ms, 99))},
-    }
+    out = run_c3(args, G) if args.config == "c3" else run_c2(args, G)
     print(json.dumps(out), flush=True)
 
 
