@@ -2,11 +2,13 @@
 // step2/index.hpp:37-183).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "driver_internal.h"
+#include "index_build.h"
 #include "index_host.h"
 
 using namespace gis;
@@ -97,6 +99,13 @@ extern "C" void gis_ctx_destroy(gis_ctx *ctx) {
 
 extern "C" const char *gis_last_error(const gis_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+// bases per seeding segment (a knob for tests: GIS_INDEX_SEGMENT)
+uint64_t seeding_segment() {
+    const char *e = std::getenv("GIS_INDEX_SEGMENT");
+    const long long v = e ? std::atoll(e) : 0;
+    return v > 0 ? (uint64_t)v : 4000000ull;
+}
+
 extern "C" int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint64_t *lens,
                                uint32_t n_refs, const gis_params *params, int bits, float f,
                                int n_threads) {
@@ -105,13 +114,61 @@ extern "C" int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint
     uint64_t total = 0;
     for (uint32_t j = 0; j < n_refs; j++) total += lens[j];
     if (total == 0) return fail(ctx, GIS_ERR_INVALID_FASTA, "No reference sequences found");
-    HostIndex h;
+    if (bits == -1) bits = pick_bits(total, *params);
+    if (bits < 8 || bits > 31) return fail(ctx, GIS_ERR_BAD_PARAMETER, "Bits must be between 8 and 31");
+    std::vector<gis_ref_randstrobe> recs;
     std::string err;
-    const int rc = build_host_index(seqs, lens, n_refs, *params, bits, f, n_threads, h, err);
+    const int rc = seed_reference(seqs, lens, n_refs, *params, n_threads, seeding_segment(), recs, err);
     if (rc) return fail(ctx, rc, err);
-    upload_index(ctx, h);
+    DevIndexStore &ix = ctx->ix;
+    ix.loaded = false;
+    release(ix.rs);
+    release(ix.starts);
+    const uint64_t n = recs.size();
+    ensure<gis_ref_randstrobe>(ix.rs, n);
+    ensure<uint64_t>(ix.starts, (1ull << bits) + 1);
+    if (n) HIP_CHECK(hipMemcpy(ix.rs.p, recs.data(), n * sizeof(gis_ref_randstrobe), hipMemcpyHostToDevice));
+    std::vector<gis_ref_randstrobe>().swap(recs);
+    uint64_t hist[RUN_BINS] = {};
+    try {
+        device_sort_records(static_cast<uint4 *>(ix.rs.p), n, ctx->stream);
+        device_finish_index(static_cast<const uint4 *>(ix.rs.p), n, bits, static_cast<uint64_t *>(ix.starts.p),
+                            hist, ctx->stream);
+    } catch (const BuildError &e) {
+        throw HipError{e.e, e.what};
+    }
+    HostIndex st;
+    filter_stats(hist, f, st);
+    ix.n = n;
+    ix.bits = bits;
+    ix.filter_cutoff = st.filter_cutoff;
+    ix.params = *params;
+    ix.distinct = st.distinct_strobemers;
+    ix.once = st.tot_occur_once;
+    ix.high = st.tot_high_ab;
+    ix.mid = st.tot_mid_ab;
+    ix.index_cutoff = st.index_cutoff;
+    ix.loaded = true;
     return GIS_OK;
     API_END(ctx)
+}
+
+extern "C" int gis_reference_randstrobes(const char *const *seqs, const uint64_t *lens, uint32_t n_refs,
+                                         const gis_params *params, int n_threads, uint64_t segment,
+                                         gis_ref_randstrobe *out, uint64_t capacity, uint64_t *n_out) {
+    if (!params || !n_out || (n_refs && (!seqs || !lens))) return GIS_ERR_BAD_PARAMETER;
+    try {
+        std::vector<gis_ref_randstrobe> recs;
+        std::string err;
+        const int rc = seed_reference(seqs, lens, n_refs, *params, n_threads, segment, recs, err);
+        if (rc) return rc;
+        *n_out = recs.size();
+        if (recs.size() > capacity || (!recs.empty() && !out)) return GIS_ERR_CAPACITY;
+        if (!recs.empty()) std::memcpy(out, recs.data(), recs.size() * sizeof(gis_ref_randstrobe));
+        return GIS_OK;
+    } catch (const std::bad_alloc &) {
+        return GIS_ERR_NOMEM;
+    }
 }
 
 extern "C" int gis_index_upload(gis_ctx *ctx, const gis_ref_randstrobe *randstrobes, uint64_t n,

@@ -1,10 +1,12 @@
-// Strobemer index construction and the .sti v2 file format.
+// Host side of strobemer index construction, and the .sti v2 file format.
 //
-// Construction follows StrobemerIndex::populate (step2/index.cu:143-244):
-// randstrobes of every contig (packed = ref_index << 8 | strobe2 offset),
-// sorted by (hash, position) -- ties broken by packed so the order is
-// deterministic -- then bucket starts over the top `bits` bits and the
-// repetitive-seed filter cutoff from the count histogram.
+// StrobemerIndex::populate (step2/index.cu:143-244) is split between the
+// host and the GPU: the host seeds the reference (randstrobes of every
+// contig, packed = ref_index << 8 | strobe2 offset) in parallel segments;
+// the device sorts the records by (hash, position, packed), fills the
+// bucket starts and counts runs of equal hashes (index_build.hip); the host
+// turns that run-length histogram into the repetition statistics and the
+// filter cutoff.
 #include "index_host.h"
 
 #include <algorithm>
@@ -36,24 +38,37 @@ struct Ring {
     uint64_t operator[](int64_t i) const { return v[i & 63]; }
 };
 
-// forward randstrobes of one contig (the reference's RandstrobeGenerator,
-// step2/randstrobes.cu:161-190, over the contig's syncmers)
-void contig_randstrobes(const char *seq, uint64_t len, uint32_t ref_index, const gis_params &gp,
-                        const SeedParams &P, std::vector<gis_ref_randstrobe> &out) {
+// Randstrobes (RefRandstrobe, packed = ref_index << 8 | strobe2 offset) of
+// one contig whose strobe1 syncmer starts in [A, B): the reference's
+// RandstrobeGenerator (step2/randstrobes.cu:161-190) over the contig's
+// syncmers, restricted to a segment so that one contig can be seeded by
+// several threads. The syncmer scan starts `back` bases before A and is
+// exact once it has passed a window whose state needs no history (a unique
+// minimum below ~0, or the first window after a contig start or an N);
+// returns false when no such window lies before A, and the caller retries
+// with a longer run-in. The scan continues past B until the partner windows
+// of the last strobe1 are complete (max_dist) and w_min syncmers follow it.
+bool segment_randstrobes(const char *seq, uint64_t len, uint32_t ref_index, uint64_t A, uint64_t B,
+                         uint64_t back, const gis_params &gp, const SeedParams &P,
+                         std::vector<gis_ref_randstrobe> &out) {
+    const int k = P.k, s = P.s, w = P.w;
+    const uint64_t S = A > back ? A - back : 0;
     std::vector<uint64_t> sy_hash;
     std::vector<uint32_t> sy_pos;
-    sy_hash.reserve(len / 4 + 16);
-    sy_pos.reserve(len / 4 + 16);
+    uint64_t first_owned = UINT64_MAX;  // index in sy_* of the first syncmer at >= A
+    uint64_t after_B = 0;
     Ring ring;
-    const int k = P.k, s = P.s, w = P.w;
-    uint64_t run = 0;  // ACGT bases since the last non-ACGT
+    uint64_t run = 0;  // ACGT bases since the last non-ACGT or the scan start
     uint64_t fs = 0, rs = 0, fk = 0, rk = 0;
     uint8_t state = ST_INVALID;
-    for (uint64_t i = 0; i < len; i++) {
+    bool exact = S == 0;  // at the contig start the first window is a true segment start
+    const uint64_t stop_pos = B + P.max_dist_u + 1;
+    for (uint64_t i = S; i < len; i++) {
         const int c = nt4((unsigned char)seq[i]);
         if (c > 3) {
             run = 0;
             state = ST_INVALID;
+            exact = true;  // the next window starts a segment
             continue;
         }
         run++;
@@ -65,184 +80,191 @@ void contig_randstrobes(const char *seq, uint64_t len, uint32_t ref_index, const
         const int64_t j = (int64_t)(i + 1 - (uint64_t)s);
         ring.v[j & 63] = xxh64_u64(fs < rs ? fs : rs);
         if (run < (uint64_t)k) continue;
-        const int64_t p = (int64_t)(i + 1 - (uint64_t)k);
-        // window p holds s-mers p .. p+w-1; the templates index h[p + o]
+        const uint64_t p = i + 1 - (uint64_t)k;
         struct Shift {
             const Ring *r;
             int64_t base;
-            uint64_t operator[](int64_t i) const { return (*r)[base + i]; }
-        } view{&ring, p};
-        if (run == (uint64_t)k) state = anchor_state(view, 0, w, true);
-        else state = slide(state, Shift{&ring, p - 1}, 1, w);
+            uint64_t operator[](int64_t x) const { return (*r)[base + x]; }
+        };
+        if (run == (uint64_t)k) {
+            state = anchor_state(Shift{&ring, (int64_t)p}, 0, w, true);
+        } else {
+            state = slide(state, Shift{&ring, (int64_t)p - 1}, 1, w);
+            if (!exact && anchor_state(Shift{&ring, (int64_t)p}, 0, w, false) != ST_UNKNOWN) exact = true;
+        }
+        if (p == A && !exact) return false;
+        if (p >= A && !exact) return false;  // (A itself may be inside an N run)
         if (state == (uint8_t)(P.t - 1)) {
+            if (p >= A && first_owned == UINT64_MAX) first_owned = sy_hash.size();
+            if (p >= B) after_B++;
             sy_hash.push_back(xxh64_u64(fk < rk ? fk : rk));
             sy_pos.push_back((uint32_t)p);
         }
+        if (p >= stop_pos && after_B >= gp.w_min) break;
     }
+    if (first_owned == UINT64_MAX) return true;
     const size_t n = sy_hash.size();
-    const size_t n_out = n > gp.w_min ? n - gp.w_min : 0;
-    const size_t o0 = out.size();
-    out.resize(o0 + n_out);
-    if (len < gp.w_max) {
-        // assign_randstrobes returns before writing (step2/index.cu:282-284)
-        // while count_randstrobes still counted these slots: they stay zero
-        for (size_t i = 0; i < n_out; i++) out[o0 + i] = gis_ref_randstrobe{0, 0, 0};
-        return;
-    }
-    for (size_t i = 0; i < n_out; i++) {
+    for (size_t i = first_owned; i < n && sy_pos[i] < B; i++) {
+        if (i + gp.w_min > n - 1) break;  // fewer than w_min syncmers follow: no randstrobe
         const size_t wend = std::min(i + (size_t)gp.w_max, n - 1);
         const uint64_t maxp = (uint64_t)sy_pos[i] + P.max_dist_u;
         size_t best = i;
         uint64_t minv = UINT64_MAX;
-        for (size_t j = i + gp.w_min; j <= wend && (uint64_t)sy_pos[j] <= maxp; j++) {
-            const uint64_t cnt = (uint64_t)__builtin_popcountll((sy_hash[i] ^ sy_hash[j]) & gp.q);
+        for (size_t jj = i + gp.w_min; jj <= wend && (uint64_t)sy_pos[jj] <= maxp; jj++) {
+            const uint64_t cnt = (uint64_t)__builtin_popcountll((sy_hash[i] ^ sy_hash[jj]) & gp.q);
             if (cnt < minv) {
                 minv = cnt;
-                best = j;
+                best = jj;
             }
         }
         gis_ref_randstrobe e;
         e.hash = sy_hash[i] + sy_hash[best];
         e.position = sy_pos[i];
         e.packed = (ref_index << 8) + (sy_pos[best] - sy_pos[i]);
-        out[o0 + i] = e;
+        out.push_back(e);
     }
+    return true;
 }
 
-inline bool rs_less(const gis_ref_randstrobe &a, const gis_ref_randstrobe &b) {
-    if (a.hash != b.hash) return a.hash < b.hash;
-    if (a.position != b.position) return a.position < b.position;
-    return a.packed < b.packed;
-}
-
-void parallel_sort(std::vector<gis_ref_randstrobe> &v, int n_threads) {
-    const size_t n = v.size();
-    int T = std::max(1, n_threads);
-    while (T > 1 && n / (size_t)T < (1u << 16)) T >>= 1;
-    if (T <= 1) {
-        std::sort(v.begin(), v.end(), rs_less);
-        return;
-    }
-    std::vector<size_t> bounds(T + 1);
-    for (int t = 0; t <= T; t++) bounds[t] = n * (size_t)t / (size_t)T;
-    std::vector<std::thread> th;
-    for (int t = 0; t < T; t++)
-        th.emplace_back([&, t]() { std::sort(v.begin() + bounds[t], v.begin() + bounds[t + 1], rs_less); });
-    for (auto &x : th) x.join();
-    std::vector<gis_ref_randstrobe> tmp(n);
-    auto *src = &v, *dst = &tmp;
-    for (int width = 1; width < T; width *= 2) {
-        th.clear();
-        for (int t = 0; t < T; t += 2 * width) {
-            const size_t lo = bounds[t], mid = bounds[std::min(T, t + width)],
-                         hi = bounds[std::min(T, t + 2 * width)];
-            th.emplace_back([=]() {
-                std::merge(src->begin() + lo, src->begin() + mid, src->begin() + mid,
-                           src->begin() + hi, dst->begin() + lo, rs_less);
-            });
+// count of randstrobe slots of a contig shorter than w_max: the reference
+// counts them (count_randstrobes) but assign_randstrobes returns before
+// writing (step2/index.cu:282-284), so they stay zero
+uint64_t short_contig_slots(const char *seq, uint64_t len, const gis_params &gp, const SeedParams &P) {
+    // count syncmers with a full scan (the contig is tiny)
+    uint64_t nsy = 0;
+    {
+        const int k = P.k, s = P.s, w = P.w;
+        Ring ring;
+        uint64_t run = 0, fs = 0, rs = 0;
+        uint8_t state = ST_INVALID;
+        for (uint64_t i = 0; i < len; i++) {
+            const int c = nt4((unsigned char)seq[i]);
+            if (c > 3) {
+                run = 0;
+                state = ST_INVALID;
+                continue;
+            }
+            run++;
+            fs = ((fs << 2) | (uint64_t)c) & P.smask;
+            rs = (rs >> 2) | ((uint64_t)(3 - c) << (2 * (s - 1)));
+            if (run < (uint64_t)s) continue;
+            const int64_t j = (int64_t)(i + 1 - (uint64_t)s);
+            ring.v[j & 63] = xxh64_u64(fs < rs ? fs : rs);
+            if (run < (uint64_t)k) continue;
+            const int64_t p = (int64_t)(i + 1 - (uint64_t)k);
+            struct Shift {
+                const Ring *r;
+                int64_t base;
+                uint64_t operator[](int64_t x) const { return (*r)[base + x]; }
+            };
+            if (run == (uint64_t)k) state = anchor_state(Shift{&ring, p}, 0, w, true);
+            else state = slide(state, Shift{&ring, p - 1}, 1, w);
+            if (state == (uint8_t)(P.t - 1)) nsy++;
         }
-        for (auto &x : th) x.join();
-        std::swap(src, dst);
     }
-    if (src != &v) v.swap(tmp);
+    return nsy > gp.w_min ? nsy - gp.w_min : 0;
 }
 
 }  // namespace
 
-void finish_index(HostIndex &ix, float f) {
-    const auto &rs = ix.rs;
-    const uint64_t n = rs.size();
-    const uint64_t nb = (1ULL << ix.bits) + 1;
-    ix.starts.clear();
-    ix.starts.reserve(nb);
-    std::vector<uint64_t> counts;
-    ix.tot_occur_once = ix.tot_high_ab = ix.tot_mid_ab = 0;
-    uint64_t unique = n == 0 ? 0 : 1;
-    uint64_t prev = n == 0 ? 0 : rs[0].hash;
-    uint32_t run = 0;
-    auto close_run = [&](uint32_t c) {
-        if (c == 1) {
-            ix.tot_occur_once++;
-        } else {
-            (c > 100 ? ix.tot_high_ab : ix.tot_mid_ab)++;
-            counts.push_back(c);
-        }
-    };
-    // A bucket start is recorded only when the hash changes, so the buckets up
-    // to the second distinct hash all start after the first hash's run
-    // (step2/index.cu:187-212); lookups reproduce that.
-    for (uint64_t i = 0; i < n; i++) {
-        const uint64_t h = rs[i].hash;
-        if (h == prev) {
-            run++;
-            continue;
-        }
-        unique++;
-        close_run(run);
-        run = 1;
-        const uint64_t top = h >> (64 - ix.bits);
-        while (ix.starts.size() <= top) ix.starts.push_back(i);
-        prev = h;
-    }
-    close_run(run);
-    while (ix.starts.size() < nb) ix.starts.push_back(n);
-    std::sort(counts.begin(), counts.end(), std::greater<uint64_t>());
-    // index_cutoff = unique_mers * f in float (step2/index.cu:232)
-    const uint64_t index_cutoff = (uint64_t)((float)unique * f);
-    uint32_t cutoff = 30;
-    if (!counts.empty()) {
-        cutoff = (uint32_t)(index_cutoff < counts.size() ? counts[index_cutoff] : counts.back());
-        cutoff = std::min(100u, std::max(30u, cutoff));
-    }
-    ix.filter_cutoff = cutoff;
-    ix.distinct_strobemers = unique;
-    ix.index_cutoff = index_cutoff;
-}
-
-int build_host_index(const char *const *seqs, const uint64_t *lens, uint32_t n_refs,
-                     const gis_params &p, int bits, float f, int n_threads, HostIndex &out,
-                     std::string &err) {
-    uint64_t total = 0;
-    for (uint32_t j = 0; j < n_refs; j++) total += lens[j];
-    if (bits == -1) bits = pick_bits(total, p);
-    if (bits < 8 || bits > 31) {
-        err = "Bits must be between 8 and 31";
-        return GIS_ERR_BAD_PARAMETER;
-    }
+int seed_reference(const char *const *seqs, const uint64_t *lens, uint32_t n_refs, const gis_params &p,
+                   int n_threads, uint64_t segment, std::vector<gis_ref_randstrobe> &out, std::string &err) {
     if (n_refs >= (1u << 24)) {
         err = "too many reference sequences for the 24-bit ref index";
         return GIS_ERR_BAD_PARAMETER;
     }
+    if (segment == 0) segment = 4000000;
     const SeedParams P = make_seed_params(p);
-    const int T = std::max(1, n_threads);
-    std::vector<std::vector<gis_ref_randstrobe>> per(n_refs);
-    std::atomic<uint32_t> next{0};
+    // jobs: [ref, A, B) segments in (contig, position) order
+    struct Job {
+        uint32_t ref;
+        uint64_t A, B;
+    };
+    std::vector<Job> jobs;
+    for (uint32_t j = 0; j < n_refs; j++) {
+        if (lens[j] < p.w_max) {
+            jobs.push_back(Job{j, 0, 0});  // zero slots of a contig shorter than w_max
+            continue;
+        }
+        for (uint64_t a = 0; a < lens[j]; a += segment) jobs.push_back(Job{j, a, std::min(lens[j], a + segment)});
+    }
+    std::vector<std::vector<gis_ref_randstrobe>> per(jobs.size());
+    std::atomic<size_t> next{0};
     std::vector<std::thread> th;
+    const int T = std::max(1, std::min<int>(n_threads, (int)jobs.size()));
     for (int t = 0; t < T; t++) {
         th.emplace_back([&]() {
             for (;;) {
-                const uint32_t j = next.fetch_add(1);
-                if (j >= n_refs) break;
-                contig_randstrobes(seqs[j], lens[j], j, p, P, per[j]);
+                const size_t x = next.fetch_add(1);
+                if (x >= jobs.size()) break;
+                const Job &jb = jobs[x];
+                auto &o = per[x];
+                if (jb.B == 0) {
+                    o.assign(short_contig_slots(seqs[jb.ref], lens[jb.ref], p, P), gis_ref_randstrobe{0, 0, 0});
+                    continue;
+                }
+                uint64_t back = 1024;
+                while (!segment_randstrobes(seqs[jb.ref], lens[jb.ref], jb.ref, jb.A, jb.B, back, p, P, o)) {
+                    o.clear();
+                    back *= 4;
+                }
             }
         });
     }
     for (auto &x : th) x.join();
     uint64_t n = 0;
     for (auto &v : per) n += v.size();
-    out = HostIndex();
-    out.rs.resize(n);
+    out.resize(n);
     uint64_t o = 0;
     for (auto &v : per) {
-        std::copy(v.begin(), v.end(), out.rs.begin() + (int64_t)o);
+        std::copy(v.begin(), v.end(), out.begin() + (int64_t)o);
         o += v.size();
         std::vector<gis_ref_randstrobe>().swap(v);
     }
-    parallel_sort(out.rs, T);
-    out.bits = bits;
-    out.params = p;
-    finish_index(out, f);
     return GIS_OK;
+}
+
+void filter_stats(const uint64_t *hist, float f, HostIndex &ix) {
+    // hist[c] = runs of c equal hashes for c = 1..100, hist[101] = runs of
+    // more than 100 (step2/index.cu:214-244: counts of the repeated hashes
+    // sorted in descending order, index_cutoff = unique * f in float, the
+    // count at that rank clamped to [30, 100]).
+    uint64_t unique = 0, repeated = 0;
+    for (int c = 1; c <= 101; c++) unique += hist[c];
+    for (int c = 2; c <= 101; c++) repeated += hist[c];
+    ix.tot_occur_once = hist[1];
+    ix.tot_high_ab = hist[101];
+    ix.tot_mid_ab = repeated - hist[101];
+    ix.distinct_strobemers = unique;
+    const uint64_t index_cutoff = (uint64_t)((float)unique * f);
+    ix.index_cutoff = index_cutoff;
+    uint32_t cutoff = 30;
+    if (repeated > 0) {
+        uint64_t v = 0;
+        if (index_cutoff < repeated) {
+            if (index_cutoff < hist[101]) {
+                v = 101;
+            } else {
+                uint64_t cum = hist[101];
+                for (int c = 100; c >= 2; c--) {
+                    cum += hist[c];
+                    if (index_cutoff < cum) {
+                        v = (uint64_t)c;
+                        break;
+                    }
+                }
+            }
+        } else {
+            v = 101;  // the smallest repeated count
+            for (int c = 2; c <= 100; c++)
+                if (hist[c]) {
+                    v = (uint64_t)c;
+                    break;
+                }
+        }
+        cutoff = (uint32_t)std::min<uint64_t>(100, std::max<uint64_t>(30, v));
+    }
+    ix.filter_cutoff = cutoff;
 }
 
 bool params_equal(const gis_params &a, const gis_params &b) {

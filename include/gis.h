@@ -176,10 +176,22 @@ int gis_device_count(int *count);
 
 /* StrobemerIndex::populate(f, n_threads), step2/index.hpp:54 (body
  * step2/index.cu:143-311). bits = -1 picks bits from the total length
- * (step2/index.cu:137-141). The index is left resident in HBM. */
+ * (step2/index.cu:137-141). The host seeds the reference with n_threads
+ * threads; the records are sorted, bucketed and counted on the GPU. The
+ * index is left resident in HBM. */
 int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint64_t *lens,
                     uint32_t n_refs, const gis_params *params, int bits, float f,
                     int n_threads);
+/* The unsorted RefRandstrobe records of a reference in (contig, position)
+ * order: the reference's RandstrobeGenerator over every contig
+ * (step2/randstrobes.cu:161-190, count/assign_randstrobes at
+ * step2/index.cu:246-311). Host computation; contigs are cut into segments
+ * of `segment` bases (0: 4,000,000) seeded by n_threads threads, with the
+ * same records as one sequential pass. n_out receives the record count;
+ * GIS_ERR_CAPACITY if it exceeds capacity. */
+int gis_reference_randstrobes(const char *const *seqs, const uint64_t *lens, uint32_t n_refs,
+                              const gis_params *params, int n_threads, uint64_t segment,
+                              gis_ref_randstrobe *out, uint64_t capacity, uint64_t *n_out);
 /* Upload a prebuilt table (e.g. from another builder). The table must be
  * sorted by (hash, position) and starts must have 2^bits + 1 entries. */
 int gis_index_upload(gis_ctx *ctx, const gis_ref_randstrobe *randstrobes, uint64_t n,
