@@ -11,7 +11,10 @@ stage. A step is one batch through seeding -> lookup -> hits -> NAMs
 read pairs with lengths drawn from 50/100/150/250/300 bp (weights
 0.1/0.2/0.4/0.2/0.1, insert N(400, 50)), 131,072-read batches through the
 persistent fused kernel under the configuration the autotuner picks, beside
-the per-stage launches.
+the per-stage launches. --config c4 runs the human-scale index: a 3.1 Gbp
+synthetic genome (24 contigs, 45% repeats, 2% tandem repeats) indexed on the
+GPU, 10M pairs of 150 bp in 262,144-read batches, fused against per-stage,
+with a bit-exact sample against the oracle over the same table.
 
 Prints one JSON line: reads/s (whole job), p50/p99 batch latency, the
 roofline of the lookup kernel and the CPU oracle's rate on the host cores.
@@ -120,14 +123,19 @@ def lookup_roofline(G, idx, mappers, k, stages):
             "avg_launch_ms": float(np.mean(lookup_ms))}
 
 
-def cpu_baseline(genome, clens, seqs, offs, lens, n_cpu, threads, what):
-    """The CPU oracle on the host cores: one warm-up pass, then the median of
-    five timed passes over the first n_cpu reads, and a 1-thread figure on a
-    sixteenth of them."""
+def oracle_index(genome, clens, threads):
     from oracle import oracle_lib as O
     t0 = time.time()
     oidx = O.Index.build_arrays(genome, clens, O.params_from_read_length(150), n_threads=threads)
     log(f"oracle index built in {time.time() - t0:.1f}s")
+    return oidx
+
+
+def cpu_baseline(oidx, seqs, offs, lens, n_cpu, threads, what):
+    """The CPU oracle on the host cores: one warm-up pass, then the median of
+    five timed passes over the first n_cpu reads, and a 1-thread figure on a
+    sixteenth of them."""
+    from oracle import oracle_lib as O
     n_cpu = min(n_cpu, len(lens))
     end = int(offs[n_cpu - 1] + lens[n_cpu - 1])
     rc = 2 * oidx.filter_cutoff
@@ -138,7 +146,6 @@ def cpu_baseline(genome, clens, seqs, offs, lens, n_cpu, threads, what):
     n1 = max(1, n_cpu // 16)
     end1 = int(offs[n1 - 1] + lens[n1 - 1])
     s1 = O.time_map_batch(oidx, seqs[:end1], offs[:n1], lens[:n1], rc, 1000, 1)
-    oidx.free()
     log(f"cpu oracle: {n_cpu / secs / 1e6:.2f} M reads/s on {threads} threads, "
         f"{n1 / s1 / 1e3:.0f} k reads/s on 1 thread")
     return {"value": n_cpu / secs, "unit": "reads/s", "cores": threads, "kind": "port",
@@ -173,7 +180,9 @@ def run_c2(args, G):
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(genome, clens, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C2")
+        oidx = oracle_index(genome, clens, threads)
+        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C2")
+        oidx.free()
 
     return {
         "metric": "reads/sec to NAMs at 150bp (1 GPU, bit-exact) vs CPU; p50/p99 batch latency",
@@ -213,7 +222,7 @@ def run_c3(args, G):
     genome, clens = c2_genome(G, args.genome)
     threads = min(16, os.cpu_count() or 1)
     idx, info = build_index(G, genome, clens, threads)
-    pairs = args.pairs
+    pairs = args.pairs or 2_000_000
     t0 = time.time()
     seqs, offs, lens, _ = G.synth_reads(genome, clens, pairs, lengths=(50, 100, 150, 250, 300),
                                         weights=(0.1, 0.2, 0.4, 0.2, 0.1), paired=True,
@@ -238,7 +247,9 @@ def run_c3(args, G):
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(genome, clens, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C3")
+        oidx = oracle_index(genome, clens, threads)
+        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C3")
+        oidx.free()
 
     return {
         "metric": "reads/sec to NAMs, 50-300 bp paired mixture (1 GPU, bit-exact) vs CPU; "
@@ -276,16 +287,112 @@ def run_c3(args, G):
     }
 
 
+def c4_genome(G, genome_bp):
+    return G.synth_genome(genome_bp, n_contigs=24, gc=0.41, repeat_fraction=0.45, n_families=5000,
+                          family_len=(300, 6000), copies=(2, 50), divergence=(0.02, 0.20),
+                          tandem_fraction=0.02, tandem_unit=(2, 50),
+                          contig_len=(50_000_000, 250_000_000), seed=1)
+
+
+def run_c4(args, G):
+    """Human-scale index: 3.1 Gbp synthetic genome (24 contigs of 50-250 Mbp,
+    45% repeats, 2% tandem repeats), 10M pairs of 150 bp; the batched fused
+    form (autotuned) against the per-stage launches, NAMs bit-exact."""
+    from oracle import oracle_lib as O
+    genome_bp = args.genome or 3_100_000_000
+    t0 = time.time()
+    genome, clens = c4_genome(G, genome_bp)
+    log(f"genome {genome_bp} bp, {len(clens)} contigs in {time.time() - t0:.1f}s")
+    threads = min(16, os.cpu_count() or 1)
+    idx, info = build_index(G, genome, clens, threads)
+    build_s = time.time() - t0
+    pairs = args.pairs or 10_000_000
+    t0 = time.time()
+    seqs, offs, lens, _ = G.synth_reads(genome, clens, pairs, lengths=(150,), weights=(1.0,),
+                                        paired=True, insert=(400.0, 50.0), sub_rate=0.005,
+                                        indel_rate=0.0005, seed=2)
+    log(f"reads {len(lens)} ({pairs} pairs) in {time.time() - t0:.1f}s")
+    batch = args.batch or 262144
+    t0 = time.time()
+    mappers = make_batches(G, idx, seqs, offs, lens, batch)
+    log(f"{len(mappers)} batches resident in {time.time() - t0:.1f}s")
+    steps = args.steps
+    best, best_ms, n_tried = mappers[0].autotune()
+    log(f"autotune: {n_tried} configurations, best {best} at {best_ms:.3f} ms per batch")
+    fused_rps, fused_elapsed, fused_ms = timed(mappers, steps, args.warmup, G.MODE_FUSED, True)
+    log(f"fused (autotuned): {fused_rps / 1e6:.1f} M reads/s")
+    fres = mappers[0].download()
+    plain_rps, plain_elapsed, plain_ms = timed(mappers, steps, args.warmup, G.MODE_PLAIN, True)
+    log(f"plain per-stage: {plain_rps / 1e6:.1f} M reads/s")
+    pres = mappers[0].download()
+    same = bool(np.array_equal(fres.nam_offsets, pres.nam_offsets) and fres.nams.tobytes() == pres.nams.tobytes())
+    stages = kernel_times(G, mappers, steps, G.MODE_PLAIN)
+    roof = lookup_roofline(G, idx, mappers, steps, stages)
+
+    # the oracle over the same (GPU-built) table: parity on a sample, then
+    # the CPU baseline
+    t0 = time.time()
+    oidx = O.Index.wrap(idx.randstrobes, idx.randstrobe_start_indices, info.bits, info.filter_cutoff,
+                        O.params_from_read_length(150))
+    log(f"table copied to the host for the oracle in {time.time() - t0:.1f}s")
+    n_par = min(20000, int(mappers[0].n))
+    end = int(offs[n_par - 1] + lens[n_par - 1])
+    eo, en, ef, efl = O.map_batch(oidx, seqs[:end], offs[:n_par], lens[:n_par], 2 * info.filter_cutoff, 1000,
+                                  threads)
+    po = pres.nam_offsets[: n_par + 1]
+    exact = bool(np.array_equal(po, eo) and pres.nams[: int(po[-1])].tobytes() == en.tobytes())
+    log(f"parity sample: {n_par} reads bit-exact vs the oracle: {exact}; fused == plain: {same}")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C4")
+    oidx.free()
+    return {
+        "metric": "reads/sec to NAMs at 150bp, 3.1 Gbp index (1 GPU, bit-exact) vs CPU; "
+                  "p50/p99 batch latency",
+        "value": fused_rps,
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": fused_elapsed * 1e3 / steps,
+        "p50_batch_ms": float(np.percentile(fused_ms, 50)),
+        "p99_batch_ms": float(np.percentile(fused_ms, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded genome and reads)",
+        "config": {
+            "workload": "human-scale index: 3.1 Gbp synthetic genome (24 contigs of 50-250 Mbp, "
+                        "45% repeats, 2% tandem repeats), 10M pairs of 150 bp, 262,144-read batches, "
+                        "batched fused form (autotuned) vs per-stage launches",
+            "genome_bp": genome_bp, "pairs": pairs, "reads": int(len(lens)), "batch_reads": batch,
+            "batches": len(mappers), "fused_config": best, "autotune_candidates": n_tried,
+            "index_randstrobes": int(info.n_randstrobes), "bits": int(info.bits),
+            "filter_cutoff": int(info.filter_cutoff), "index_gb": info.device_bytes / 1e9,
+            "index_build_s": build_s, "parallelism": "single GPU",
+        },
+        "parity_sample": {"reads": n_par, "bit_exact_vs_oracle": exact, "fused_equals_plain": same},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "plain_form": {"value": plain_rps, "unit": "reads/s",
+                       "ms_per_step": plain_elapsed * 1e3 / steps,
+                       "p50_batch_ms": float(np.percentile(plain_ms, 50)),
+                       "p99_batch_ms": float(np.percentile(plain_ms, 99)),
+                       "stage_ms": {k: float(np.mean([st[k] for st in stages])) for k in stages[0]}},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", choices=("c2", "c3"), default="c2")
+    ap.add_argument("--config", choices=("c2", "c3", "c4"), default="c2")
     ap.add_argument("--reads", type=int, default=1_000_000, help="c2: single-end reads")
-    ap.add_argument("--pairs", type=int, default=2_000_000, help="c3: read pairs")
-    ap.add_argument("--batch", type=int, default=0, help="reads per batch (c2 65536, c3 131072)")
-    ap.add_argument("--genome", type=int, default=100_000_000)
+    ap.add_argument("--pairs", type=int, default=0, help="c3/c4: read pairs (2M / 10M)")
+    ap.add_argument("--batch", type=int, default=0, help="reads per batch (c2 65536, c3 131072, c4 262144)")
+    ap.add_argument("--genome", type=int, default=0, help="genome bp (c2/c3 100M, c4 3.1G)")
     ap.add_argument("--cpu-reads", type=int, default=1_000_000,
                     help="bounded sample of the workload timed on the CPU oracle")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -311,7 +418,11 @@ def main():
         sys.exit(2)
 
     import gpu_index_test_amd as G
-    out = run_c3(args, G) if args.config == "c3" else run_c2(args, G)
+    if args.config == "c4":
+        out = run_c4(args, G)
+    else:
+        args.genome = args.genome or 100_000_000
+        out = run_c3(args, G) if args.config == "c3" else run_c2(args, G)
     print(json.dumps(out), flush=True)
 
 

@@ -31,11 +31,12 @@ struct gis_batch {
     DevBuf rs_cnt, rs_nfwd, rs_hash, rs_start, rs_end, sy_cnt, sy_hash, sy_pos;
     DevBuf pos, hcnt, rcnt, lk_bytes;
     DevBuf hit_total, rhit_total, frac, flags, nam_cnt, hit_off, rhit_off;
-    DevBuf hits, rhits, keys, rkeys, aux, raux, nam_ws, rnam_ws, nam_fin, rnam_fin;
+    DevBuf hits, rhits, keys, rkeys, nam_ws, rnam_ws, nam_fin, rnam_fin;
     DevBuf nam_slots, nam_off, nams_out, scan_tmp, ctl, spill_ids, queue;
     std::vector<uint32_t> h_queue;  // read ids by length, longest first
     uint32_t n_long = 0;            // queue entries longer than the short bin (160 bp)
     uint64_t hit_cap = 0, rhit_cap = 0, out_cap = 0;
+    bool merge_scratch = false;  // keys/nam_ws allocated (reads with many hits need them)
     uint64_t tot_rs_cap = 0;
     // results of the last run
     uint64_t tot_nams = 0;
@@ -71,7 +72,7 @@ void batch_free(gis_batch *b) {
                      &b->rs_end, &b->sy_cnt, &b->sy_hash, &b->sy_pos, &b->pos, &b->hcnt, &b->rcnt,
                      &b->lk_bytes, &b->hit_total, &b->rhit_total, &b->frac, &b->flags, &b->nam_cnt,
                      &b->hit_off, &b->rhit_off, &b->hits, &b->rhits, &b->keys,
-                     &b->rkeys, &b->aux, &b->raux, &b->nam_ws, &b->rnam_ws, &b->nam_fin,
+                     &b->rkeys, &b->nam_ws, &b->rnam_ws, &b->nam_fin,
                      &b->rnam_fin, &b->nam_slots, &b->nam_off, &b->nams_out, &b->scan_tmp, &b->ctl,
                      &b->spill_ids, &b->queue};
     for (DevBuf *x : all) release(*x);
@@ -131,10 +132,9 @@ DevBatch make_devbatch(gis_batch *b) {
     d.rhit_cap = b->rhit_cap;
     d.keys = static_cast<uint64_t *>(b->keys.p);
     d.rkeys = static_cast<uint64_t *>(b->rkeys.p);
-    d.aux = static_cast<uint32_t *>(b->aux.p);
-    d.raux = static_cast<uint32_t *>(b->raux.p);
-    d.nam_ws = static_cast<NamWork *>(b->nam_ws.p);
-    d.rnam_ws = static_cast<NamWork *>(b->rnam_ws.p);
+    d.nam_ws = static_cast<WideNam *>(b->nam_ws.p);
+    d.rnam_ws = static_cast<WideNam *>(b->rnam_ws.p);
+    d.merge_scratch = b->merge_scratch ? 1u : 0u;
     d.nam_fin = static_cast<gis_nam *>(b->nam_fin.p);
     d.rnam_fin = static_cast<gis_nam *>(b->rnam_fin.p);
     d.nam_slots = static_cast<gis_nam *>(b->nam_slots.p);
@@ -165,7 +165,6 @@ void ensure_read_arrays(gis_batch *b, uint32_t n) {
     ensure<uint8_t>(b->flags, n);
     ensure<uint64_t>(b->hit_off, n + 1);
     ensure<uint64_t>(b->rhit_off, n + 1);
-    ensure<gis_nam>(b->nam_slots, (size_t)n * F_NAMCAP);
     ensure<uint64_t>(b->nam_off, n + 1);
     ensure<uint64_t>(b->scan_tmp, scan_tmp_words(n) + 8);
     ensure<DevCtl>(b->ctl, 1);
@@ -188,17 +187,18 @@ void ensure_pools(gis_batch *b, uint64_t hit_cap, uint64_t rhit_cap, uint64_t ou
     b->hit_cap = std::max(b->hit_cap, hit_cap);
     b->rhit_cap = std::max(b->rhit_cap, rhit_cap);
     b->out_cap = std::max(b->out_cap, out_cap);
-    const uint64_t n = b->n_reads;
     ensure<DevHit>(b->hits, b->hit_cap);
-    ensure<uint64_t>(b->keys, 2 * b->hit_cap);
-    ensure<uint32_t>(b->aux, 2 * b->hit_cap + n + 1);
-    ensure<NamWork>(b->nam_ws, b->hit_cap);
     ensure<gis_nam>(b->nam_fin, b->hit_cap);
     ensure<DevHit>(b->rhits, b->rhit_cap);
-    ensure<uint64_t>(b->rkeys, 2 * b->rhit_cap);
-    ensure<uint32_t>(b->raux, 2 * b->rhit_cap + n + 1);
-    ensure<NamWork>(b->rnam_ws, b->rhit_cap);
     ensure<gis_nam>(b->rnam_fin, b->rhit_cap);
+    if (b->merge_scratch) {
+        // merge scratch of reads beyond the merge's LDS budget, allocated
+        // once a run has met such a read
+        ensure<uint64_t>(b->keys, 2 * b->hit_cap);
+        ensure<WideNam>(b->nam_ws, b->hit_cap);
+        ensure<uint64_t>(b->rkeys, 2 * b->rhit_cap);
+        ensure<WideNam>(b->rnam_ws, b->rhit_cap);
+    }
     ensure<gis_nam>(b->nams_out, b->out_cap);
 }
 
@@ -261,13 +261,14 @@ int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const 
         ensure<uint64_t>(b->sy_pos, b->h_sy_off[n]);
     }
     if (b->any_long) ensure<uint8_t>(b->long_scratch, b->h_long_off[n]);
-    // initial pools: ~1.5x the typical hits of a 150 bp read, grown on demand.
+    // initial pools: about the hits of typical reads (~0.16 per base), grown
+    // on demand to what a run's scans report.
     // GIS_TEST_SMALL_POOLS starts them tiny so tests exercise the overflow path.
     const uint64_t bases = span;
     if (const char *e = std::getenv("GIS_TEST_SMALL_POOLS"); e && *e == '1')
         ensure_pools(b, 64, 64, 16);
     else
-        ensure_pools(b, 48ull * n + bases / 8 + 4096, 8ull * n + 4096, 16ull * n + 1024);
+        ensure_pools(b, 24ull * n + bases / 16 + 4096, 4ull * n + 4096, 12ull * n + 1024);
     ensure<char>(b->seqs, span);
     hipStream_t st = ctx->stream;
     if (span) HIP_CHECK(hipMemcpyAsync(b->seqs.p, seqs, span, hipMemcpyHostToDevice, st));
@@ -402,13 +403,16 @@ void batch_run(gis_batch *b, const gis_map_options &o, int rescue_mode, int stop
     const uint32_t rcut = rescue_cutoff_of(b->ctx, o);
     const int mode = o.mode;
     const bool use_graph = stop == STOP_NONE && (o.flags & GIS_RUN_NO_GRAPH) == 0;
+    if (mode == GIS_MODE_FUSED && stop == STOP_NONE) ensure<gis_nam>(b->nam_slots, (size_t)b->n_reads * F_NAMCAP);
     unsigned ovf = 0;
-    for (int attempt = 0; attempt < 4; attempt++) {
+    for (int attempt = 0; attempt < 6; attempt++) {
         ovf = run_once(b, mode, rescue_mode, rcut, o.hard_cutoff, stop, use_graph);
         if (!ovf) break;
-        // after a hit overflow the merges did not run, so the NAM total is
-        // meaningless: grow the hit arrays now and size the output next time
-        if (ovf & (OVF_HITS | OVF_RHITS)) ovf &= ~(unsigned)OVF_OUT;
+        // after a hit overflow the merges did not run, and after a merge
+        // without its scratch some reads are missing, so the NAM total is
+        // not final: grow what overflowed now and size the output next time
+        if (ovf & (OVF_HITS | OVF_RHITS | OVF_MERGE)) ovf &= ~(unsigned)OVF_OUT;
+        if (ovf & OVF_MERGE) b->merge_scratch = true;
         ensure_pools(b, (ovf & OVF_HITS) ? b->h_total[0] + b->h_total[0] / 8 + 4096 : 0,
                      (ovf & OVF_RHITS) ? b->h_total[1] + b->h_total[1] / 8 + 4096 : 0,
                      (ovf & OVF_OUT) ? b->tot_nams + b->tot_nams / 8 + 1024 : 0);
@@ -684,7 +688,7 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
     const DevIndex ix = dev_index(ctx);
     const ReadSet set{nullptr, nullptr, n};
     uint64_t *tmp = static_cast<uint64_t *>(b->scan_tmp.p);
-    for (int attempt = 0; attempt < 4; attempt++) {
+    for (int attempt = 0; attempt < 6; attempt++) {
         const DevBatch d = make_devbatch(b);
         launch_reset_ctl(d.ctl, st);
         launch_lookup(d, set, ix, rescue ? 2 : 1, st);
@@ -706,8 +710,9 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
         b->tot_nams = b->h_total[2];
         unsigned ovf = b->h_ctl->overflow;
         if (!ovf) break;
-        if (attempt == 3) throw HipError{hipErrorOutOfMemory, "batch arrays still too small after regrowth"};
-        if (ovf & (OVF_HITS | OVF_RHITS)) ovf &= ~(unsigned)OVF_OUT;
+        if (attempt == 5) throw HipError{hipErrorOutOfMemory, "batch arrays still too small after regrowth"};
+        if (ovf & (OVF_HITS | OVF_RHITS | OVF_MERGE)) ovf &= ~(unsigned)OVF_OUT;
+        if (ovf & OVF_MERGE) b->merge_scratch = true;
         ensure_pools(b, (ovf & OVF_HITS) ? b->h_total[0] + 4096 : 0,
                      (ovf & OVF_RHITS) ? b->h_total[1] + 4096 : 0,
                      (ovf & OVF_OUT) ? b->tot_nams + 1024 : 0);

@@ -30,10 +30,12 @@ struct DevHit {
     uint16_t query_end;
 };
 
-// NAM under construction (the fields merge_hits_into_nams updates)
-struct NamWork {
-    int32_t ref_start, ref_end, ref_prev;
-    uint16_t query_start, query_end, query_prev, n_hits;
+// NAM under construction on the merge's global list (groups with more than
+// 64 open NAMs): query start/end/previous hit start, ref start/end/previous
+// hit start, hits so far, creation index within the group
+struct WideNam {
+    int32_t qs, qe, qp, rs, re, rp;
+    uint32_t n, cid;
 };
 
 // Device control block of one batch run: the spill list length of the fused
@@ -44,7 +46,7 @@ struct DevCtl {
     unsigned int q_head[2]; // dynamic work-queue heads of the fused kernel's two bins
 };
 
-enum : unsigned { OVF_HITS = 1, OVF_RHITS = 2, OVF_OUT = 8 };
+enum : unsigned { OVF_HITS = 1, OVF_RHITS = 2, OVF_MERGE = 4, OVF_OUT = 8 };
 
 // the public GIS_FLAG_SPILLED bit (include/gis.h)
 enum : uint8_t { FLAG_SPILL = GIS_FLAG_SPILLED };
@@ -90,13 +92,13 @@ struct DevBatch {
     uint64_t *hit_off;        // [n+1] scan of hit_total: the read's hits start here
     uint64_t *rhit_off;       // [n+1] scan of rhit_total
     // hit arrays (capacity hit_cap / rhit_cap hits); merge scratch per read r
-    // with h hits at hit offset o: keys[2o .. 2o+2h), aux[2o+r .. 2o+r+2h+1)
+    // with h hits at hit offset o: keys[2o .. 2o+2h), nam_ws[o .. o+h)
     DevHit *hits;
     DevHit *rhits;
     uint64_t hit_cap, rhit_cap;
     uint64_t *keys, *rkeys;
-    uint32_t *aux, *raux;
-    NamWork *nam_ws, *rnam_ws;
+    WideNam *nam_ws, *rnam_ws;
+    uint32_t merge_scratch;   // keys/nam_ws allocated; else reads needing them set OVF_MERGE
     gis_nam *nam_fin, *rnam_fin;
     // fused kernel: NAM slots of F_NAMCAP per read
     gis_nam *nam_slots;

@@ -669,21 +669,24 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_hits(DevBatch b, ReadSet se
 }
 
 // ---------------------------------------------------------------------------
-// NAM merge. Hits of a read arrive strand by strand, seeds in query_start
-// order and each run in table order; within one (strand, ref) group that is
-// (query_start, ref_start) order, the order merge_hits_into_nams sorts into
-// (step2/nam.cu:373). Hits are grouped by (strand, ref_id) with a stable
-// sort, then each group is merged by one lane: every hit extends or is
-// contained in the first open NAM (in creation order) that accepts it, or
-// opens a new NAM (step2/nam.cu:391-447). NAMs whose query_end lies before
-// the current query_start can never accept a hit again and are dropped from
-// the open list; when they are written out does not change any field.
-// Creation order inside a group is (query_start, ref_start) order, so
-// writing groups in key order gives the canonical NAM order directly.
+// NAM merge (merge_hits_into_nams, step2/nam.cu:347-513). Hits of a read
+// arrive strand by strand, seeds in query_start order and each run in table
+// order; within one (strand, ref) group that is (query_start, ref_start)
+// order, the order the reference sorts each group into (step2/nam.cu:373).
+// The hits are grouped by (strand, ref_id) with a stable sort and each group
+// is merged in that order: every hit extends or is contained in the first
+// open NAM (in creation order) that accepts it, or opens a new NAM
+// (step2/nam.cu:391-447). A NAM whose query_end lies before the current
+// query_start can never accept a hit again; when it leaves the open list does
+// not change any field. Creation order inside a group is canonical order, so
+// NAM c of a group is written at (NAMs of earlier groups) + c with nam_id =
+// that rank.
 //
-// Scratch of a read with nh hits (LDS or global): keys[2 nh] (sort keys,
-// padded to a power of two), aux[2 nh + 2] (group starts, then the groups'
-// open lists), ws[nh] (NAMs under construction).
+// The open NAMs of the current group live one per lane; the hit is broadcast
+// and a ballot picks the first accepting lane. When 64 lanes are open and
+// none can be retired, the group continues on a NAM list in global scratch
+// scanned 64 entries at a time (the "wide" form), retiring dead NAMs as the
+// list grows.
 
 __device__ __forceinline__ void bitonic_regs(uint64_t &key) {
     const uint32_t lane = lane_id();
@@ -717,171 +720,53 @@ __device__ __forceinline__ void bitonic_mem(uint64_t *keys, uint32_t P) {
     }
 }
 
-// Group and merge; returns the number of groups. After it, aux[0..G] hold the
-// group starts and aux[nh + 1 + gs] the NAM count of the group starting at gs.
-__device__ __forceinline__ uint32_t merge_build(uint32_t nh, const DevHit *hits, uint64_t *keys,
-                                                uint32_t *aux, NamWork *ws) {
+// Stable wave radix sort of n keys (group << 32 | hit index) by the group
+// field, 4 bits a pass over key_bits bits; src and alt hold n entries each.
+// Returns the buffer that holds the sorted keys.
+__device__ __forceinline__ uint64_t *radix_sort_groups(uint64_t *src, uint64_t *alt, uint32_t n,
+                                                       uint32_t key_bits) {
     const uint32_t lane = lane_id();
-    uint32_t unsorted = 0;
-    for (uint32_t i0 = 0; i0 < nh; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        if (i < nh) {
-            const uint32_t rsd = hits[i].ref_strand;
-            keys[i] = ((uint64_t)rsd << 32) | i;
-            if (i > 0 && hits[i - 1].ref_strand > rsd) unsorted = 1;
+    for (uint32_t shift = 32; shift < 32 + key_bits; shift += 4) {
+        uint32_t off[16];
+#pragma unroll
+        for (int v = 0; v < 16; v++) off[v] = 0;
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            const uint32_t d = i < n ? (uint32_t)(src[i] >> shift) & 15u : 16u;
+#pragma unroll
+            for (int v = 0; v < 16; v++) off[v] += (uint32_t)__popcll(__ballot(d == (uint32_t)v));
         }
-    }
-    unsorted = wave_max(unsorted);
-    wave_sync_mem();
-    if (unsorted) {
-        if (nh <= 64) {
-            uint64_t key = lane < nh ? keys[lane] : UINT64_MAX;
-            bitonic_regs(key);
-            if (lane < nh) keys[lane] = key;
-        } else {
-            uint32_t P = 1;
-            while (P < nh) P <<= 1;
-            for (uint32_t i = nh + lane; i < P; i += 64) keys[i] = UINT64_MAX;
-            wave_sync_mem();
-            bitonic_mem(keys, P);
+        uint32_t run = 0;
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const uint32_t c = off[v];
+            off[v] = run;
+            run += c;
+        }
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+            const uint32_t i = i0 + lane;
+            uint64_t key = 0;
+            uint32_t d = 16u;
+            if (i < n) {
+                key = src[i];
+                d = (uint32_t)(key >> shift) & 15u;
+            }
+            uint32_t pos = 0;
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const uint64_t bal = __ballot(d == (uint32_t)v);
+                if (d == (uint32_t)v) pos = off[v] + mbcnt(bal);
+                off[v] += (uint32_t)__popcll(bal);
+            }
+            if (i < n) alt[pos] = key;
         }
         wave_sync_mem();
+        uint64_t *t = src;
+        src = alt;
+        alt = t;
     }
-    uint32_t G = 0;
-    for (uint32_t i0 = 0; i0 < nh; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        bool head = false;
-        if (i < nh) head = (i == 0) || ((keys[i] >> 32) != (keys[i - 1] >> 32));
-        const uint64_t bal = __ballot(head);
-        if (head) aux[G + mbcnt(bal)] = i;
-        G += (uint32_t)__popcll(bal);
-    }
-    if (lane == 0) aux[G] = nh;
-    wave_sync_mem();
-    uint32_t *open_base = aux + nh + 1;
-    for (uint32_t g0 = 0; g0 < G; g0 += 64) {
-        const uint32_t g = g0 + lane;
-        if (g < G) {
-            const uint32_t gs = aux[g], ge = aux[g + 1];
-            uint32_t *open = open_base + gs;
-            NamWork *gw = ws + gs;
-            uint32_t n_open = 0, c = 0;
-            int cur_qs = -1;
-            for (uint32_t x = gs; x < ge; x++) {
-                const DevHit h = hits[(uint32_t)keys[x]];
-                const int qs = h.query_start, qe = h.query_end, rs = h.ref_start, re = h.ref_end;
-                if (qs != cur_qs) {
-                    uint32_t w2 = 0;
-                    for (uint32_t oi = 0; oi < n_open; oi++) {
-                        const uint32_t o = open[oi];
-                        if ((int)gw[o].query_end >= qs) open[w2++] = o;
-                    }
-                    n_open = w2;
-                    cur_qs = qs;
-                }
-                bool added = false;
-                for (uint32_t oi = 0; oi < n_open; oi++) {
-                    NamWork &o = gw[open[oi]];
-                    if ((int)o.query_prev < qs && qs <= (int)o.query_end && o.ref_prev < rs &&
-                        rs <= o.ref_end) {
-                        if (qe > (int)o.query_end && re > o.ref_end) {
-                            o.query_end = (uint16_t)qe;
-                            o.ref_end = re;
-                            o.query_prev = (uint16_t)qs;
-                            o.ref_prev = rs;
-                            o.n_hits++;
-                            added = true;
-                            break;
-                        } else if (qe <= (int)o.query_end && re <= o.ref_end) {
-                            o.query_prev = (uint16_t)qs;
-                            o.ref_prev = rs;
-                            o.n_hits++;
-                            added = true;
-                            break;
-                        }
-                    }
-                }
-                if (!added) {
-                    NamWork &o = gw[c];
-                    o.query_start = (uint16_t)qs;
-                    o.query_end = (uint16_t)qe;
-                    o.query_prev = (uint16_t)qs;
-                    o.ref_start = rs;
-                    o.ref_end = re;
-                    o.ref_prev = rs;
-                    o.n_hits = 1;
-                    open[n_open++] = c;
-                    c++;
-                }
-            }
-            open[0] = c;  // the group's NAM count (its open list is no longer needed)
-        }
-    }
-    wave_sync_mem();
-    return G;
+    return src;
 }
-
-// total NAMs of the groups (wave-uniform)
-__device__ __forceinline__ uint32_t merge_count(uint32_t nh, uint32_t G, const uint32_t *aux) {
-    uint32_t total = 0;
-    for (uint32_t g0 = 0; g0 < G; g0 += 64) {
-        const uint32_t g = g0 + lane_id();
-        total += wave_sum(g < G ? aux[nh + 1 + aux[g]] : 0u);
-    }
-    return total;
-}
-
-// final NAMs in canonical order with nam_id = rank and the score filled in
-__device__ __forceinline__ void merge_emit(uint32_t nh, uint32_t G, const uint64_t *keys,
-                                           const uint32_t *aux, const NamWork *ws, gis_nam *out) {
-    const uint32_t lane = lane_id();
-    const uint32_t *open_base = aux + nh + 1;
-    uint32_t carry = 0;
-    for (uint32_t g0 = 0; g0 < G; g0 += 64) {
-        const uint32_t g = g0 + lane;
-        uint32_t c = 0, gs = 0;
-        if (g < G) {
-            gs = aux[g];
-            c = open_base[gs];
-        }
-        const uint32_t rank = carry + wave_excl_scan(c);
-        if (g < G) {
-            const uint32_t rsd = (uint32_t)(keys[gs] >> 32);
-            for (uint32_t m = 0; m < c; m++) {
-                const NamWork &o = ws[gs + m];
-                gis_nam x;
-                x.nam_id = (int32_t)(rank + m);
-                x.query_start = o.query_start;
-                x.query_end = o.query_end;
-                x.query_prev_hit_startpos = o.query_prev;
-                x.ref_start = o.ref_start;
-                x.ref_end = o.ref_end;
-                x.ref_prev_hit_startpos = o.ref_prev;
-                x.n_hits = o.n_hits;
-                x.ref_id = (int32_t)(rsd & 0x7fffffffu);
-                x.score = nam_score(o.query_start, o.query_end, o.ref_start, o.ref_end, o.n_hits);
-                x.is_rc = (uint8_t)(rsd >> 31);
-                x.pad_[0] = x.pad_[1] = x.pad_[2] = 0;
-                out[rank + m] = x;
-            }
-        }
-        carry += wave_sum(c);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Wave-cooperative merge. The hits of a read are taken in (strand/ref group,
-// arrival) order -- within a group that is (query_start, ref_start) order --
-// and broadcast one at a time to the wave. Lane j holds the group's j-th open
-// NAM in creation order, so "the first open NAM that accepts the hit"
-// (step2/nam.cu:391-447) is the lowest lane of a ballot, and a hit nobody
-// accepts opens a NAM in the next free lane. Creation order inside a group is
-// canonical order, so NAM c of a group is written at (NAMs of earlier
-// groups) + c with nam_id = that rank. When all 64 lanes are open, NAMs whose
-// query_end lies before the hit's query_start (they can never accept a later
-// hit of the group) are written out and the rest are compacted in order; a
-// group that still needs more than 64 open NAMs reports MERGE_OVERFLOW and
-// the caller merges the read with merge_build instead.
 
 constexpr uint32_t MERGE_OVERFLOW = 0xffffffffu;
 
@@ -908,6 +793,10 @@ __device__ __forceinline__ void lane_nam_store(gis_nam *out, uint32_t idx, uint3
     out[idx] = x;
 }
 
+__device__ __forceinline__ LaneNam from_wide(const WideNam &w) {
+    return LaneNam{w.qs, w.qe, w.qp, w.rs, w.re, w.rp, (int)w.n, w.cid};
+}
+
 __device__ __forceinline__ int permute_push(int v, uint32_t dst) {
     return __builtin_amdgcn_ds_permute((int)(dst << 2), v);
 }
@@ -916,16 +805,40 @@ __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
 }
 
+// does open NAM o accept hit (qs, qe, rs, re), and does the hit extend it
+__device__ __forceinline__ bool nam_accepts(int o_qp, int o_qe, int o_rp, int o_re, int qs, int qe,
+                                            int rs, int re, bool &ext) {
+    const bool near = o_qp < qs && qs <= o_qe && o_rp < rs && rs <= o_re;
+    ext = qe > o_qe && re > o_re;
+    const bool con = qe <= o_qe && re <= o_re;
+    return near && (ext || con);
+}
+
+// write the wide list's NAMs of the group that ends (all of them are final)
+__device__ __forceinline__ void wide_flush(const WideNam *wide, uint32_t nw, gis_nam *out, uint32_t base,
+                                           uint32_t grp, uint32_t cap) {
+    for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
+        const uint32_t e = c0 + lane_id();
+        if (e < nw) {
+            const WideNam w = wide[e];
+            if (base + w.cid < cap) lane_nam_store(out, base + w.cid, grp, from_wide(w));
+        }
+    }
+}
+
 // Merge the nh hits of a read into NAMs written to out[0 .. total) in
 // canonical order (entries at index >= cap are not written). keys: scratch
-// of at least the next power of two >= nh entries (used when nh > 64).
-// Returns the NAM count or MERGE_OVERFLOW.
+// of the next power of two >= nh entries when nh <= MERGE_LDS (bitonic
+// grouping), else of nh entries with keys_alt another nh (radix grouping).
+// wide: a list of nh entries for groups with more than 64 open NAMs, or
+// nullptr (then such a read returns MERGE_OVERFLOW).
 __device__ __forceinline__ uint32_t merge_wave(uint32_t nh, const DevHit *hits, uint64_t *keys,
-                                               gis_nam *out, uint32_t cap) {
+                                               uint64_t *keys_alt, WideNam *wide, gis_nam *out,
+                                               uint32_t cap) {
     const uint32_t lane = lane_id();
     // chunk registers: hit (x0 + lane) of the grouped order
     uint32_t c_rsd = 0, c_rs = 0, c_re = 0, c_q = 0;
-    bool by_keys = false;
+    const uint64_t *order = nullptr;  // grouped order of hit indices (nh > 64), or identity
     if (nh <= 64) {
         DevHit h{};
         if (lane < nh) h = hits[lane];
@@ -946,34 +859,48 @@ __device__ __forceinline__ uint32_t merge_wave(uint32_t nh, const DevHit *hits, 
             c_q = (uint32_t)__shfl((int)c_q, src, 64);
         }
     } else {
-        uint32_t unsorted = 0;
+        uint32_t unsorted = 0, max_ref = 0;
         for (uint32_t i0 = 0; i0 < nh; i0 += 64) {
             const uint32_t i = i0 + lane;
             if (i < nh) {
                 const uint32_t rsd = hits[i].ref_strand;
-                keys[i] = ((uint64_t)rsd << 32) | i;
+                max_ref = max_ref > (rsd & 0x7fffffffu) ? max_ref : (rsd & 0x7fffffffu);
                 if (i > 0 && hits[i - 1].ref_strand > rsd) unsorted = 1;
             }
         }
-        by_keys = __ballot(unsorted != 0) != 0;
-        wave_sync_mem();
-        if (by_keys) {
-            uint32_t P = 1;
-            while (P < nh) P <<= 1;
-            for (uint32_t i = nh + lane; i < P; i += 64) keys[i] = UINT64_MAX;
-            wave_sync_mem();
-            bitonic_mem(keys, P);
+        if (__ballot(unsorted != 0) != 0) {
+            if (nh <= (uint32_t)MERGE_LDS) {
+                uint32_t P = 1;
+                while (P < nh) P <<= 1;
+                for (uint32_t i = lane; i < P; i += 64)
+                    keys[i] = i < nh ? (((uint64_t)hits[i].ref_strand << 32) | i) : UINT64_MAX;
+                wave_sync_mem();
+                bitonic_mem(keys, P);
+                order = keys;
+            } else {
+                // compact group key: strand above the bits of the largest ref_id
+                const uint32_t mr = wave_max(max_ref);
+                const uint32_t rbits = mr ? 32u - (uint32_t)__builtin_clz(mr) : 0u;
+                for (uint32_t i = lane; i < nh; i += 64) {
+                    const uint32_t rsd = hits[i].ref_strand;
+                    const uint64_t g = ((uint64_t)(rsd >> 31) << rbits) | (rsd & 0x7fffffffu);
+                    keys[i] = (g << 32) | i;
+                }
+                wave_sync_mem();
+                order = radix_sort_groups(keys, keys_alt, nh, rbits + 1);
+            }
         }
     }
 
     LaneNam o{0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t n_open = 0, grp = 0xffffffffu, base = 0, gcnt = 0;
-    bool ovf = false;
-    for (uint32_t x0 = 0; x0 < nh && !ovf; x0 += 64) {
+    bool wide_mode = false;
+    uint32_t nw = 0, wide_mark = 0;
+    for (uint32_t x0 = 0; x0 < nh; x0 += 64) {
         if (nh > 64) {
             const uint32_t i = x0 + lane;
             if (i < nh) {
-                const DevHit h = hits[by_keys ? (uint32_t)keys[i] : i];
+                const DevHit h = hits[order ? (uint32_t)order[i] : i];
                 c_rsd = h.ref_strand;
                 c_rs = (uint32_t)h.ref_start;
                 c_re = (uint32_t)h.ref_end;
@@ -988,56 +915,127 @@ __device__ __forceinline__ uint32_t merge_wave(uint32_t nh, const DevHit *hits, 
             const uint32_t q = readlane_u32(c_q, t);
             const int qs = (int)(q & 0xffffu), qe = (int)(q >> 16);
             if (rsd != grp) {
-                if (lane < n_open && base + o.cid < cap) lane_nam_store(out, base + o.cid, grp, o);
+                if (wide_mode) {
+                    wide_flush(wide, nw, out, base, grp, cap);
+                    wide_mode = false;
+                    nw = 0;
+                } else if (lane < n_open && base + o.cid < cap) {
+                    lane_nam_store(out, base + o.cid, grp, o);
+                }
                 base += gcnt;
                 gcnt = 0;
                 n_open = 0;
                 grp = rsd;
             }
-            const bool live = lane < n_open;
-            const bool near = live && o.qp < qs && qs <= o.qe && o.rp < rs && rs <= o.re;
-            const bool ext = qe > o.qe && re > o.re;
-            const bool con = qe <= o.qe && re <= o.re;
-            const uint64_t bal = __ballot(near && (ext || con));
-            if (bal) {
-                if (lane == (uint32_t)__builtin_ctzll(bal)) {
-                    if (ext) {
-                        o.qe = qe;
-                        o.re = re;
+            if (!wide_mode) {
+                bool ext;
+                const bool acc = lane < n_open && nam_accepts(o.qp, o.qe, o.rp, o.re, qs, qe, rs, re, ext);
+                const uint64_t bal = __ballot(acc);
+                if (bal) {
+                    if (lane == (uint32_t)__builtin_ctzll(bal)) {
+                        if (ext) {
+                            o.qe = qe;
+                            o.re = re;
+                        }
+                        o.qp = qs;
+                        o.rp = rs;
+                        o.n++;
                     }
-                    o.qp = qs;
-                    o.rp = rs;
-                    o.n++;
+                    continue;
                 }
-                continue;
-            }
-            if (n_open == 64) {
-                const bool keep = o.qe >= qs;
-                const uint64_t km = __ballot(keep);
-                if (!keep && base + o.cid < cap) lane_nam_store(out, base + o.cid, grp, o);
-                const uint32_t nk = (uint32_t)__popcll(km);
-                if (nk == 64) {
-                    ovf = true;
-                    break;
+                if (n_open == 64) {
+                    const bool keep = o.qe >= qs;
+                    const uint64_t km = __ballot(keep);
+                    const uint32_t nk = (uint32_t)__popcll(km);
+                    if (nk == 64) {
+                        if (!wide) return MERGE_OVERFLOW;
+                        // the group continues on the wide list, in creation order
+                        wide[lane] = WideNam{o.qs, o.qe, o.qp, o.rs, o.re, o.rp, (uint32_t)o.n, o.cid};
+                        nw = 64;
+                        wide_mark = 64;
+                        wide_mode = true;
+                        n_open = 0;
+                    } else {
+                        if (!keep && base + o.cid < cap) lane_nam_store(out, base + o.cid, grp, o);
+                        const uint32_t dst = keep ? mbcnt(km) : nk + mbcnt(~km);
+                        o.qs = permute_push(o.qs, dst);
+                        o.qe = permute_push(o.qe, dst);
+                        o.qp = permute_push(o.qp, dst);
+                        o.rs = permute_push(o.rs, dst);
+                        o.re = permute_push(o.re, dst);
+                        o.rp = permute_push(o.rp, dst);
+                        o.n = permute_push(o.n, dst);
+                        o.cid = (uint32_t)permute_push((int)o.cid, dst);
+                        n_open = nk;
+                    }
                 }
-                const uint32_t dst = keep ? mbcnt(km) : nk + mbcnt(~km);
-                o.qs = permute_push(o.qs, dst);
-                o.qe = permute_push(o.qe, dst);
-                o.qp = permute_push(o.qp, dst);
-                o.rs = permute_push(o.rs, dst);
-                o.re = permute_push(o.re, dst);
-                o.rp = permute_push(o.rp, dst);
-                o.n = permute_push(o.n, dst);
-                o.cid = (uint32_t)permute_push((int)o.cid, dst);
-                n_open = nk;
+                if (!wide_mode) {
+                    if (lane == n_open) o = LaneNam{qs, qe, qs, rs, re, rs, 1, gcnt};
+                    n_open++;
+                    gcnt++;
+                    continue;
+                }
+                // (the hit was accepted by no open NAM: it opens one on the wide list)
+            } else {
+                // wide form: the first accepting entry of the list
+                bool done = false;
+                for (uint32_t c0 = 0; c0 < nw && !done; c0 += 64) {
+                    const uint32_t e = c0 + lane;
+                    WideNam w{};
+                    bool ext = false, acc = false;
+                    if (e < nw) {
+                        w = wide[e];
+                        acc = nam_accepts(w.qp, w.qe, w.rp, w.re, qs, qe, rs, re, ext);
+                    }
+                    const uint64_t bal = __ballot(acc);
+                    if (bal) {
+                        if (lane == (uint32_t)__builtin_ctzll(bal)) {
+                            if (ext) {
+                                w.qe = qe;
+                                w.re = re;
+                            }
+                            w.qp = qs;
+                            w.rp = rs;
+                            w.n++;
+                            wide[e] = w;
+                        }
+                        done = true;
+                    }
+                }
+                if (done) {
+                    wave_sync_mem();
+                    continue;
+                }
+                if (nw >= 2 * wide_mark) {
+                    // retire the dead entries (query_end before this hit), keeping the order
+                    uint32_t kept = 0;
+                    for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
+                        const uint32_t e = c0 + lane;
+                        WideNam w{};
+                        bool keep = false;
+                        if (e < nw) {
+                            w = wide[e];
+                            keep = w.qe >= qs;
+                            if (!keep && base + w.cid < cap) lane_nam_store(out, base + w.cid, grp, from_wide(w));
+                        }
+                        const uint64_t kb = __ballot(keep);
+                        wave_sync_mem();
+                        if (keep) wide[kept + mbcnt(kb)] = w;
+                        kept += (uint32_t)__popcll(kb);
+                        wave_sync_mem();
+                    }
+                    nw = kept;
+                    wide_mark = kept > 64 ? kept : 64;
+                }
             }
-            if (lane == n_open) o = LaneNam{qs, qe, qs, rs, re, rs, 1, gcnt};
-            n_open++;
+            if (lane == 0) wide[nw] = WideNam{qs, qe, qs, rs, re, rs, 1u, gcnt};
+            nw++;
             gcnt++;
+            wave_sync_mem();
         }
     }
-    if (ovf) return MERGE_OVERFLOW;
-    if (lane < n_open && base + o.cid < cap) lane_nam_store(out, base + o.cid, grp, o);
+    if (wide_mode) wide_flush(wide, nw, out, base, grp, cap);
+    else if (lane < n_open && base + o.cid < cap) lane_nam_store(out, base + o.cid, grp, o);
     return base + gcnt;
 }
 
@@ -1066,14 +1064,20 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
         }
         const DevHit *ghits = (rescue ? b.rhits : b.hits) + hb;
         gis_nam *fin = (rescue ? b.rnam_fin : b.nam_fin) + hb;
-        uint64_t *gkeys = (rescue ? b.rkeys : b.keys) + 2 * hb;
-        uint32_t total = merge_wave(nh, ghits, nh <= (uint32_t)MERGE_LDS ? lkeys[wid] : gkeys, fin, nh);
+        uint64_t *gkeys = b.merge_scratch ? (rescue ? b.rkeys : b.keys) + 2 * hb : nullptr;
+        WideNam *gwide = b.merge_scratch ? (rescue ? b.rnam_ws : b.nam_ws) + hb : nullptr;
+        uint32_t total = MERGE_OVERFLOW;
+        if (nh <= (uint32_t)MERGE_LDS)
+            total = merge_wave(nh, ghits, lkeys[wid], nullptr, gwide, fin, nh);
+        else if (b.merge_scratch)
+            total = merge_wave(nh, ghits, gkeys, gkeys + nh, gwide, fin, nh);
         if (total == MERGE_OVERFLOW) {
-            uint32_t *aux = (rescue ? b.raux : b.aux) + 2 * hb + r;
-            NamWork *ws = (rescue ? b.rnam_ws : b.nam_ws) + hb;
-            const uint32_t G = merge_build(nh, ghits, gkeys, aux, ws);
-            total = merge_count(nh, G, aux);
-            merge_emit(nh, G, gkeys, aux, ws, fin);
+            // needs the global scratch, which this run does not have yet
+            if (lane == 0) {
+                atomicOr(&b.ctl->overflow, OVF_MERGE);
+                b.nam_cnt[r] = 0;
+            }
+            continue;
         }
         if (lane == 0) b.nam_cnt[r] = total;
         wave_sync_mem();
@@ -1203,8 +1207,8 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
     }
     uint32_t total = 0;
     if (nh > 0)
-        total = merge_wave(nh, S.merge.hits, S.merge.keys, b.nam_slots + (uint64_t)r * F_NAMCAP,
-                           (uint32_t)F_NAMCAP);
+        total = merge_wave(nh, S.merge.hits, S.merge.keys, nullptr, nullptr,
+                           b.nam_slots + (uint64_t)r * F_NAMCAP, (uint32_t)F_NAMCAP);
     if (total > (uint32_t)F_NAMCAP) {  // includes MERGE_OVERFLOW
         spill_read(b, r);
         return;

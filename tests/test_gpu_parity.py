@@ -418,3 +418,57 @@ def test_autotune_keeps_results(setup):
     finally:
         m.close()
         idx.ctx.fused_config = saved
+
+
+def _repeat_genome(rng):
+    """Random sequence with a long tandem repeat (11 bp unit, 1% mutations)
+    and 60 near-identical copies of a 1.5 kbp element on both contigs: seeds
+    there have hundreds of hits, so rescued reads have many hits per
+    (strand, ref) group and many NAMs open at once."""
+    unit = rand_seq(rng, 11)
+    tandem = bytearray((unit * 3000)[:30000])
+    for i in range(len(tandem)):
+        if rng.random() < 0.01:
+            tandem[i] = b"ACGT"[rng.integers(4)]
+    elem = rand_seq(rng, 1500)
+    parts0, parts1 = [rand_seq(rng, 20000), bytes(tandem)], [rand_seq(rng, 20000)]
+    for c in range(60):
+        (parts0 if c % 2 else parts1).append(mutate(rng, elem, 0.002))
+        (parts0 if c % 2 else parts1).append(rand_seq(rng, 700))
+    return [b"".join(parts0), b"".join(parts1)]
+
+
+def test_heavy_repeat_reads(ctx):
+    """Reads inside a tandem repeat and a high-copy element: hit lists far
+    beyond the merge's LDS budget (radix grouping) and groups with more than
+    64 NAMs open at once (the wide list), in both forms."""
+    rng = np.random.default_rng(81)
+    seqs = _repeat_genome(rng)
+    p = G.IndexParameters.from_read_length(150)
+    refs = G.References(seqs)
+    idx = G.StrobemerIndex(refs, p)
+    idx.populate(f=0.0002, n_threads=4)
+    oidx = O.Index.build(seqs, O.params_from_read_length(150), n_threads=4)
+    reads = []
+    for _ in range(300):
+        c = int(rng.integers(2))
+        st = int(rng.integers(0, len(seqs[c]) - 150))
+        r = mutate(rng, seqs[c][st: st + 150], 0.005)
+        reads.append(revcomp(r) if rng.integers(2) else r)
+    s, o, l = G.pack_reads(reads)
+    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 4)
+    for mode in (G.MODE_PLAIN, G.MODE_FUSED):
+        res = G.find_nams_batch(idx, s, o, l, mode=mode)
+        _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
+                      res.flags, efl)
+    # the read set must reach the wide form: some group has more than 64
+    # NAMs spanning one query position
+    most = 0
+    for r in range(len(reads)):
+        nams = en[int(eo[r]): int(eo[r + 1])]
+        for key in set(zip(nams["is_rc"].tolist(), nams["ref_id"].tolist())):
+            g = nams[(nams["is_rc"] == key[0]) & (nams["ref_id"] == key[1])]
+            for x in np.unique(g["query_start"]):
+                most = max(most, int(((g["query_start"] <= x) & (g["query_end"] >= x)).sum()))
+    assert most > 64, most
+    assert (efl & 1).sum() > 10
