@@ -20,7 +20,7 @@ from ._native import (FLAG_NO_RANDSTROBES, FLAG_RESCUED, GIS_DEFAULT, GIS_NOT_FO
                       GisError)
 
 __all__ = [
-    "IndexParameters", "References", "Context", "StrobemerIndex", "Mapper", "MapResult",
+    "IndexParameters", "References", "Context", "StrobemerIndex", "Mapper", "MapResult", "Server",
     "randstrobes_query", "find_nams", "find_nams_rescue", "synth_genome", "synth_reads",
     "pack_reads", "GisError", "NAM", "QUERY_RANDSTROBE", "REF_RANDSTROBE", "SYNCMER", "HIT",
     "GIS_NOT_FOUND", "FLAG_RESCUED", "FLAG_NO_RANDSTROBES", "MODE_PLAIN", "MODE_FUSED",
@@ -530,6 +530,74 @@ class Mapper:
     def close(self):
         if getattr(self, "h", None):
             self.ctx.L.gis_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (continuous batching) over a resident index: submit
+    requests of any size, wait for their NAMs, or replay a timed trace.
+    Requests are admitted by an HBM budget, coalesced into steps replayed
+    from captured HIP graphs, with up to `slots` steps in flight."""
+
+    def __init__(self, index, slots=4, max_step_reads=65536, hbm_budget=0, mode=N.MODE_PLAIN,
+                 graph=True, rescue_level=2, hard_cutoff=1000):
+        index._check_owner()
+        self.index, self.ctx = index, index.ctx
+        cfg = N.ServerConfig()
+        cfg.slots, cfg.max_step_reads, cfg.hbm_budget = slots, max_step_reads, hbm_budget
+        cfg.options = _map_options(rescue_level, hard_cutoff, mode, graph)
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.L.gis_server_create(self.ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def submit(self, seqs, offsets, lens):
+        seqs, offsets, lens = _seq_args(seqs, offsets, lens)
+        rid = C.c_uint64()
+        _check(self.ctx.L.gis_server_submit(self.h, N.ptr(seqs), N.ptr(offsets), N.ptr(lens), len(lens),
+                                            C.byref(rid)), None)
+        return rid.value, len(lens)
+
+    def wait(self, request, timeout_ms=-1):
+        """-> (MapResult, latency ms) of a request returned by submit()."""
+        rid, n = request
+        lat = C.c_double()
+        cap = max(16, 16 * n)
+        while True:
+            o, offs, nams, frac, flags = _nam_out(n, cap)
+            code = self.ctx.L.gis_server_wait(self.h, rid, timeout_ms, C.byref(o), C.byref(lat))
+            if code == 5 and o.n_nams > cap:
+                cap = int(o.n_nams)
+                continue
+            _check(code, None)
+            return MapResult(offs, nams[: o.n_nams], frac[:n], flags[:n]), lat.value
+
+    def replay(self, seqs, offsets, lens, arrival_s, first, count):
+        """Timed trace -> (latency ms per request, wall seconds)."""
+        seqs, offsets, lens = _seq_args(seqs, offsets, lens)
+        arrival_s = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        first = np.ascontiguousarray(first, dtype=np.uint32)
+        count = np.ascontiguousarray(count, dtype=np.uint32)
+        lat = np.empty(max(1, len(arrival_s)), dtype=np.float64)
+        wall = C.c_double()
+        _check(self.ctx.L.gis_server_replay(self.h, N.ptr(seqs), N.ptr(offsets), N.ptr(lens), len(arrival_s),
+                                            N.ptr(arrival_s), N.ptr(first), N.ptr(count), N.ptr(lat),
+                                            C.byref(wall)), None)
+        return lat[: len(arrival_s)], wall.value
+
+    def stats(self):
+        st = N.ServerStats()
+        _check(self.ctx.L.gis_server_stats_get(self.h, C.byref(st)), None)
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.L.gis_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -75,6 +75,16 @@ class FusedConfig(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("max_step_reads", C.c_uint32), ("hbm_budget", C.c_uint64),
+                ("options", MapOptions)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("reads", C.c_uint64), ("steps", C.c_uint64),
+                ("busy_s", C.c_double), ("peak_bytes", C.c_uint64)]
+
+
 class GenomeSpec(C.Structure):
     _fields_ = [("total_length", C.c_uint64), ("n_contigs", C.c_uint32), ("gc", C.c_double),
                 ("repeat_fraction", C.c_double), ("n_families", C.c_uint32),
@@ -132,6 +142,12 @@ PROTOTYPES = {
     "gis_fused_config_set": (C.c_int, [_P, C.POINTER(FusedConfig)]),
     "gis_batch_autotune": (C.c_int, [_P, C.POINTER(MapOptions), C.POINTER(FusedConfig),
                                      C.POINTER(C.c_double), C.POINTER(_U32)]),
+    "gis_server_create": (C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    "gis_server_destroy": (None, [_P]),
+    "gis_server_submit": (C.c_int, [_P, _P, _P, _P, _U32, C.POINTER(_U64)]),
+    "gis_server_wait": (C.c_int, [_P, _U64, C.c_int, C.POINTER(NamOut), C.POINTER(C.c_double)]),
+    "gis_server_replay": (C.c_int, [_P, _P, _P, _P, _U32, _P, _P, _P, _P, C.POINTER(C.c_double)]),
+    "gis_server_stats_get": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "gis_synth_genome": (C.c_int, [C.POINTER(GenomeSpec), _P, _P]),
     "gis_synth_reads": (C.c_int, [C.POINTER(ReadSpec), _P, _P, _U32, _P, _P, _P, _P, _P, _P]),
 }

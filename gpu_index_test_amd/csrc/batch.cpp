@@ -17,51 +17,27 @@
 #include <string>
 #include <vector>
 
-#include "driver_internal.h"
+#include "batch_internal.h"
 
 using namespace gis;
-
-struct gis_batch {
-    gis_ctx *ctx = nullptr;
-    uint32_t n_reads = 0;
-    bool want_sy = false;
-    bool any_long = false;
-    std::vector<uint64_t> h_rs_off, h_sy_off, h_long_off;
-    DevBuf seqs, read_off, read_len, rs_off, sy_off, long_off, long_scratch;
-    DevBuf rs_cnt, rs_nfwd, rs_hash, rs_start, rs_end, sy_cnt, sy_hash, sy_pos;
-    DevBuf pos, hcnt, rcnt, lk_bytes;
-    DevBuf hit_total, rhit_total, frac, flags, nam_cnt, hit_off, rhit_off;
-    DevBuf hits, rhits, keys, rkeys, nam_ws, rnam_ws, nam_fin, rnam_fin;
-    DevBuf nam_slots, nam_off, nams_out, scan_tmp, ctl, spill_ids, queue;
-    std::vector<uint32_t> h_queue;  // read ids by length, longest first
-    uint32_t n_long = 0;            // queue entries longer than the short bin (160 bp)
-    uint64_t hit_cap = 0, rhit_cap = 0, out_cap = 0;
-    bool merge_scratch = false;  // keys/nam_ws allocated (reads with many hits need them)
-    uint64_t tot_rs_cap = 0;
-    // results of the last run
-    uint64_t tot_nams = 0;
-    uint32_t n_spill = 0;
-    int last_mode = GIS_MODE_PLAIN;
-    bool last_graph = false;
-    bool ran = false;
-    // pinned landing area for the end-of-run readback
-    DevCtl *h_ctl = nullptr;
-    uint64_t *h_total = nullptr;  // [3]: hits, rescue hits, NAMs
-    hipEvent_t ev[16] = {};
-    int n_ev = 0;
-    std::vector<std::string> ev_names;
-    // captured launch sequence of the last configuration
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    DevBatch graph_key{};
-    int graph_cfg[11] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
-};
 
 namespace gis {
 
 gis_batch *batch_new(gis_ctx *ctx) {
     gis_batch *b = new gis_batch();
     b->ctx = ctx;
+    b->stream = ctx->stream;
+    return b;
+}
+
+gis_batch *batch_new_stream(gis_ctx *ctx) {
+    gis_batch *b = batch_new(ctx);
+    // default priority, like the context's stream
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        throw HipError{hipErrorOutOfMemory, "hipStreamCreateWithFlags"};
+    }
+    b->own_stream = true;
     return b;
 }
 
@@ -80,14 +56,17 @@ void batch_free(gis_batch *b) {
         if (e) (void)hipEventDestroy(e);
     if (b->h_ctl) (void)hipHostFree(b->h_ctl);
     if (b->h_total) (void)hipHostFree(b->h_total);
-    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto &g : b->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
 }  // namespace gis
 
-namespace {
+namespace gis {
 
 // reads up to this length form the fused kernel's short bin
 constexpr int SHORT_BIN = 160;
@@ -205,7 +184,7 @@ void ensure_pools(gis_batch *b, uint64_t hit_cap, uint64_t rhit_cap, uint64_t ou
 void mark(gis_batch *b, const char *name) {
     if (b->n_ev >= 16) return;
     if (!b->ev[b->n_ev]) HIP_CHECK(hipEventCreate(&b->ev[b->n_ev]));
-    HIP_CHECK(hipEventRecord(b->ev[b->n_ev], b->ctx->stream));
+    HIP_CHECK(hipEventRecord(b->ev[b->n_ev], b->stream));
     b->ev_names.push_back(name);
     b->n_ev++;
 }
@@ -270,7 +249,7 @@ int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const 
     else
         ensure_pools(b, 24ull * n + bases / 16 + 4096, 4ull * n + 4096, 12ull * n + 1024);
     ensure<char>(b->seqs, span);
-    hipStream_t st = ctx->stream;
+    hipStream_t st = b->stream;
     if (span) HIP_CHECK(hipMemcpyAsync(b->seqs.p, seqs, span, hipMemcpyHostToDevice, st));
     if (n) {
         HIP_CHECK(hipMemcpyAsync(b->read_off.p, offsets, 8ull * n, hipMemcpyHostToDevice, st));
@@ -290,12 +269,10 @@ uint32_t rescue_cutoff_of(const gis_ctx *ctx, const gis_map_options &o) {
     return o.rescue_level < 100 ? (uint32_t)(o.rescue_level * (int)ctx->ix.filter_cutoff) : 100u;
 }
 
-enum { STOP_SEED = 0, STOP_HITS = 1, STOP_NONE = 2 };
-
 // The launch sequence of one run, enqueued on the context's stream.
 void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_t hard, int stop) {
     gis_ctx *ctx = b->ctx;
-    hipStream_t st = ctx->stream;
+    hipStream_t st = b->stream;
     const DevIndex ix = dev_index(ctx);
     const SeedParams sp = make_seed_params(ctx->ix.params);
     const DevBatch d = make_devbatch(b);
@@ -354,60 +331,88 @@ void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_
     HIP_CHECK(hipMemcpyAsync(b->h_ctl, d.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, st));
 }
 
-// One run; with use_graph the sequence is captured once per configuration
-// (buffers, mode, cutoffs) and replayed. Returns the overflow bits.
-unsigned run_once(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_t hard, int stop,
-                  bool use_graph) {
+RunOpts run_opts(const gis_batch *b, const gis_map_options &o, int rescue_mode, int stop) {
+    RunOpts r;
+    r.mode = o.mode;
+    r.rescue_mode = rescue_mode;
+    r.rcut = rescue_cutoff_of(b->ctx, o);
+    r.hard = o.hard_cutoff;
+    r.stop = stop;
+    r.graph = stop == STOP_NONE && (o.flags & GIS_RUN_NO_GRAPH) == 0;
+    return r;
+}
+
+// Enqueue one run on the batch's stream. With r.graph the launch sequence is
+// captured once per configuration (buffers, mode, cutoffs, fused launch
+// configuration) and replayed; a batch keeps the graphs of its last 8
+// configurations.
+void batch_launch(gis_batch *b, const RunOpts &r) {
     gis_ctx *ctx = b->ctx;
-    hipStream_t st = ctx->stream;
-    if (use_graph) {
+    hipStream_t st = b->stream;
+    if (r.mode == GIS_MODE_FUSED && r.stop == STOP_NONE)
+        ensure<gis_nam>(b->nam_slots, (size_t)b->n_reads * F_NAMCAP);
+    if (r.graph) {
         const DevBatch key = make_devbatch(b);
         const gis_fused_config &fc = ctx->fcfg;
-        const int cfg[11] = {mode, rescue_mode, (int)rcut, (int)hard, stop,
+        const int cfg[11] = {r.mode, r.rescue_mode, (int)r.rcut, (int)r.hard, r.stop,
                              (b->want_sy ? 1 : 0) | (b->any_long ? 2 : 0), fc.waves_per_block,
                              fc.length_split, fc.chunk, fc.blocks_per_cu, (int)b->n_long};
-        const bool same = b->graph_exec && std::memcmp(&key, &b->graph_key, sizeof key) == 0 &&
-                          std::memcmp(cfg, b->graph_cfg, sizeof cfg) == 0;
-        if (!same) {
-            if (b->graph_exec) HIP_CHECK(hipGraphExecDestroy(b->graph_exec));
-            if (b->graph) HIP_CHECK(hipGraphDestroy(b->graph));
-            b->graph_exec = nullptr;
-            b->graph = nullptr;
+        gis_batch::GraphEntry *hit = nullptr;
+        for (auto &g : b->graphs)
+            if (std::memcmp(&key, &g.key, sizeof key) == 0 && std::memcmp(cfg, g.cfg, sizeof cfg) == 0) hit = &g;
+        if (!hit) {
+            if (b->graphs.size() >= 8) {
+                auto old = b->graphs.begin();
+                for (auto it = b->graphs.begin(); it != b->graphs.end(); ++it)
+                    if (it->used < old->used) old = it;
+                HIP_CHECK(hipStreamSynchronize(st));
+                HIP_CHECK(hipGraphExecDestroy(old->exec));
+                HIP_CHECK(hipGraphDestroy(old->graph));
+                b->graphs.erase(old);
+            }
+            gis_batch::GraphEntry e{};
+            e.key = key;
+            std::memcpy(e.cfg, cfg, sizeof cfg);
             HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                enqueue_run(b, mode, rescue_mode, rcut, hard, stop);
+                enqueue_run(b, r.mode, r.rescue_mode, r.rcut, r.hard, r.stop);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(st, &g);
                 if (g) (void)hipGraphDestroy(g);
                 throw;
             }
-            HIP_CHECK(hipStreamEndCapture(st, &b->graph));
-            HIP_CHECK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
-            b->graph_key = key;
-            std::memcpy(b->graph_cfg, cfg, sizeof cfg);
+            HIP_CHECK(hipStreamEndCapture(st, &e.graph));
+            const hipError_t ie = hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hipGraphDestroy(e.graph);
+                throw HipError{ie, "hipGraphInstantiate"};
+            }
+            b->graphs.push_back(e);
+            hit = &b->graphs.back();
         }
-        HIP_CHECK(hipGraphLaunch(b->graph_exec, st));
+        hit->used = ++b->graph_clock;
+        HIP_CHECK(hipGraphLaunch(hit->exec, st));
     } else {
-        enqueue_run(b, mode, rescue_mode, rcut, hard, stop);
+        enqueue_run(b, r.mode, r.rescue_mode, r.rcut, r.hard, r.stop);
     }
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamSynchronize(st));
-    const bool fused = mode == GIS_MODE_FUSED && stop == STOP_NONE;
+}
+
+bool batch_ready(const gis_batch *b) { return hipStreamQuery(b->stream) == hipSuccess; }
+
+// wait for the run on the stream; returns the overflow bits
+static unsigned finish_once(gis_batch *b, const RunOpts &r) {
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    const bool fused = r.mode == GIS_MODE_FUSED && r.stop == STOP_NONE;
     b->n_spill = fused ? b->h_ctl->n_spill : 0;
-    b->tot_nams = stop == STOP_NONE ? b->h_total[2] : 0;
+    b->tot_nams = r.stop == STOP_NONE ? b->h_total[2] : 0;
     return b->h_ctl->overflow;
 }
 
-void batch_run(gis_batch *b, const gis_map_options &o, int rescue_mode, int stop) {
-    const uint32_t rcut = rescue_cutoff_of(b->ctx, o);
-    const int mode = o.mode;
-    const bool use_graph = stop == STOP_NONE && (o.flags & GIS_RUN_NO_GRAPH) == 0;
-    if (mode == GIS_MODE_FUSED && stop == STOP_NONE) ensure<gis_nam>(b->nam_slots, (size_t)b->n_reads * F_NAMCAP);
-    unsigned ovf = 0;
-    for (int attempt = 0; attempt < 6; attempt++) {
-        ovf = run_once(b, mode, rescue_mode, rcut, o.hard_cutoff, stop, use_graph);
-        if (!ovf) break;
+void batch_complete(gis_batch *b, const RunOpts &r) {
+    unsigned ovf = finish_once(b, r);
+    for (int attempt = 1; ovf && attempt < 6; attempt++) {
         // after a hit overflow the merges did not run, and after a merge
         // without its scratch some reads are missing, so the NAM total is
         // not final: grow what overflowed now and size the output next time
@@ -416,11 +421,19 @@ void batch_run(gis_batch *b, const gis_map_options &o, int rescue_mode, int stop
         ensure_pools(b, (ovf & OVF_HITS) ? b->h_total[0] + b->h_total[0] / 8 + 4096 : 0,
                      (ovf & OVF_RHITS) ? b->h_total[1] + b->h_total[1] / 8 + 4096 : 0,
                      (ovf & OVF_OUT) ? b->tot_nams + b->tot_nams / 8 + 1024 : 0);
+        batch_launch(b, r);
+        ovf = finish_once(b, r);
     }
     if (ovf) throw HipError{hipErrorOutOfMemory, "batch arrays still too small after regrowth"};
-    b->last_mode = mode;
-    b->last_graph = use_graph;
-    b->ran = stop == STOP_NONE;
+    b->last_mode = r.mode;
+    b->last_graph = r.graph;
+    b->ran = r.stop == STOP_NONE;
+}
+
+void batch_run(gis_batch *b, const gis_map_options &o, int rescue_mode, int stop) {
+    const RunOpts r = run_opts(b, o, rescue_mode, stop);
+    batch_launch(b, r);
+    batch_complete(b, r);
 }
 
 int batch_download(gis_batch *b, gis_nam_out *out) {
@@ -432,7 +445,7 @@ int batch_download(gis_batch *b, gis_nam_out *out) {
     if (out->nam_capacity < b->tot_nams || (b->tot_nams && !out->nams))
         return fail(ctx, GIS_ERR_CAPACITY,
                     "NAM capacity " + std::to_string(out->nam_capacity) + " < " + std::to_string(b->tot_nams));
-    hipStream_t st = ctx->stream;
+    hipStream_t st = b->stream;
     if (out->nam_offsets)
         HIP_CHECK(hipMemcpyAsync(out->nam_offsets, b->nam_off.p, 8ull * (n + 1), hipMemcpyDeviceToHost, st));
     if (b->tot_nams)
@@ -458,7 +471,7 @@ gis_map_options opts_or_default(const gis_map_options *opt) {
     return o;
 }
 
-}  // namespace
+}  // namespace gis
 
 // ---------------- stage entry points ----------------
 
@@ -740,7 +753,7 @@ extern "C" void gis_batch_destroy(gis_batch *b) {
     gis_ctx *ctx = b->ctx;
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(b->stream);
     batch_free(b);
 }
 

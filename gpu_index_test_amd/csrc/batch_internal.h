@@ -1,0 +1,88 @@
+// The device batch shared by the C-ABI batch entry points (batch.cpp) and
+// the serving loop (server.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "driver_internal.h"
+
+struct gis_batch {
+    using DevBuf = gis::DevBuf;
+    using DevCtl = gis::DevCtl;
+    using DevBatch = gis::DevBatch;
+    gis_ctx *ctx = nullptr;
+    uint32_t n_reads = 0;
+    bool want_sy = false;
+    bool any_long = false;
+    std::vector<uint64_t> h_rs_off, h_sy_off, h_long_off;
+    DevBuf seqs, read_off, read_len, rs_off, sy_off, long_off, long_scratch;
+    DevBuf rs_cnt, rs_nfwd, rs_hash, rs_start, rs_end, sy_cnt, sy_hash, sy_pos;
+    DevBuf pos, hcnt, rcnt, lk_bytes;
+    DevBuf hit_total, rhit_total, frac, flags, nam_cnt, hit_off, rhit_off;
+    DevBuf hits, rhits, keys, rkeys, nam_ws, rnam_ws, nam_fin, rnam_fin;
+    DevBuf nam_slots, nam_off, nams_out, scan_tmp, ctl, spill_ids, queue;
+    std::vector<uint32_t> h_queue;  // read ids by length, longest first
+    uint32_t n_long = 0;            // queue entries longer than the short bin (160 bp)
+    uint64_t hit_cap = 0, rhit_cap = 0, out_cap = 0;
+    bool merge_scratch = false;  // keys/nam_ws allocated (reads with many hits need them)
+    uint64_t tot_rs_cap = 0;
+    // results of the last run
+    uint64_t tot_nams = 0;
+    uint32_t n_spill = 0;
+    int last_mode = GIS_MODE_PLAIN;
+    bool last_graph = false;
+    bool ran = false;
+    // pinned landing area for the end-of-run readback
+    DevCtl *h_ctl = nullptr;
+    uint64_t *h_total = nullptr;  // [3]: hits, rescue hits, NAMs
+    hipEvent_t ev[16] = {};
+    int n_ev = 0;
+    std::vector<std::string> ev_names;
+    // captured launch sequences, one per configuration (buffers, mode, cutoffs)
+    struct GraphEntry {
+        DevBatch key;
+        int cfg[11];
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    std::vector<GraphEntry> graphs;
+    uint64_t graph_clock = 0;
+    // the stream the batch runs on (the context's, or its own for a serving slot)
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+};
+
+
+namespace gis {
+
+enum { STOP_SEED = 0, STOP_HITS = 1, STOP_NONE = 2 };
+
+// the launch configuration of one run
+struct RunOpts {
+    int mode;          // GIS_MODE_*
+    int rescue_mode;   // 0 the caller's decision, 1 never, 2 always
+    uint32_t rcut, hard;
+    int stop;          // STOP_*
+    bool graph;        // replay a captured graph
+};
+
+gis_batch *batch_new_stream(gis_ctx *ctx);  // a batch with a stream of its own
+int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
+                 uint32_t n, bool want_sy);
+RunOpts run_opts(const gis_batch *b, const gis_map_options &o, int rescue_mode, int stop);
+// enqueue one run on the batch's stream without waiting
+void batch_launch(gis_batch *b, const RunOpts &r);
+// the launched run has finished on the device
+bool batch_ready(const gis_batch *b);
+// wait for the launched run, grow what overflowed and run again until the
+// results are complete
+void batch_complete(gis_batch *b, const RunOpts &r);
+void batch_run(gis_batch *b, const gis_map_options &o, int rescue_mode, int stop);
+int batch_download(gis_batch *b, gis_nam_out *out);
+gis_map_options opts_or_default(const gis_map_options *opt);
+
+}  // namespace gis

@@ -1,0 +1,454 @@
+// C ABI: the serving loop. Requests (sets of reads) are queued by the
+// caller; a worker thread admits them into device slots by an HBM budget,
+// coalesces the queued requests into one launch sequence per slot (a step,
+// replayed from the slot's captured hipGraph for its padded size), keeps up
+// to `slots` steps in flight on separate streams, and hands each request its
+// per-read NAMs when its step completes. Requests that arrive while steps
+// are on the GPU join the next step on a free slot (continuous batching);
+// a request's NAMs do not depend on the requests it was batched with.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch_internal.h"
+
+using namespace gis;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Request {
+    uint64_t id = 0;
+    std::vector<char> seqs;
+    std::vector<uint64_t> offs;
+    std::vector<uint32_t> lens;
+    uint64_t bytes = 0;  // estimated device bytes while in flight
+    Clock::time_point t_submit, t_done;
+    bool done = false;
+    int status = GIS_OK;
+    std::string err;
+    std::vector<uint64_t> nam_off;
+    std::vector<gis_nam> nams;
+    std::vector<float> frac;
+    std::vector<uint8_t> flags;
+};
+
+struct Slot {
+    gis_batch *b = nullptr;
+    std::vector<Request *> reqs;
+    std::vector<uint32_t> first;  // first read of each request in the step
+    uint32_t n_reads = 0;         // reads of the step before padding
+    uint64_t bytes = 0;
+    bool busy = false;
+    RunOpts r{};
+};
+
+// device bytes a request holds while in flight: randstrobe slots and
+// lookup results (~64 B per base), hits, NAMs and per-read arrays (~2 KB
+// per read)
+uint64_t estimate_bytes(uint64_t reads, uint64_t bases) { return 64 * bases + 2048 * reads + 4096; }
+
+// padded step sizes, so that a slot replays one of a few captured graphs
+uint32_t padded(uint32_t n) {
+    uint32_t p = 256;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+struct gis_server {
+    gis_ctx *ctx = nullptr;
+    gis_server_config cfg{};
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> pending;
+    std::unordered_map<uint64_t, Request *> reqs;
+    uint64_t next_id = 1;
+    uint64_t bytes_in_flight = 0;
+    bool stop = false;
+    std::thread worker;
+    gis_server_stats stats{};
+    Clock::time_point busy_since;
+    int n_busy = 0;
+    // host staging of the worker
+    std::vector<char> st_seqs;
+    std::vector<uint64_t> st_offs;
+    std::vector<uint32_t> st_lens;
+    std::vector<uint64_t> h_off;
+    std::vector<gis_nam> h_nams;
+    std::vector<float> h_frac;
+    std::vector<uint8_t> h_flags;
+};
+
+namespace {
+
+// (the worker marks the requests done under the server lock afterwards)
+void fail_step(gis_server *s, Slot &sl, const std::string &msg) {
+    (void)s;
+    for (Request *q : sl.reqs) {
+        q->status = GIS_ERR_HIP;
+        q->err = msg;
+        q->t_done = Clock::now();
+    }
+}
+
+// stage, upload and launch the requests of a slot (the worker holds the
+// context lock)
+void launch_step(gis_server *s, Slot &sl) {
+    const uint32_t n = sl.n_reads, np = padded(n);
+    s->st_seqs.clear();
+    s->st_offs.resize(np);
+    s->st_lens.resize(np);
+    sl.first.clear();
+    uint32_t r = 0;
+    for (Request *q : sl.reqs) {
+        sl.first.push_back(r);
+        for (size_t i = 0; i < q->lens.size(); i++, r++) {
+            s->st_offs[r] = s->st_seqs.size();
+            s->st_lens[r] = q->lens[i];
+            s->st_seqs.insert(s->st_seqs.end(), q->seqs.begin() + (int64_t)q->offs[i],
+                              q->seqs.begin() + (int64_t)(q->offs[i] + q->lens[i]));
+        }
+    }
+    for (; r < np; r++) {  // empty reads: no randstrobes, no NAMs
+        s->st_offs[r] = s->st_seqs.size();
+        s->st_lens[r] = 0;
+    }
+    if (s->st_seqs.empty()) s->st_seqs.push_back('\0');
+    const int rc = batch_upload(sl.b, s->st_seqs.data(), s->st_offs.data(), s->st_lens.data(), np, false);
+    if (rc) throw HipError{hipErrorInvalidValue, "server upload failed"};
+    sl.r = run_opts(sl.b, s->cfg.options, 0, STOP_NONE);
+    batch_launch(sl.b, sl.r);
+}
+
+// results of a finished slot into its requests (the worker holds the context lock)
+void complete_step(gis_server *s, Slot &sl) {
+    batch_complete(sl.b, sl.r);
+    const uint32_t np = sl.b->n_reads;
+    s->h_off.resize(np + 1);
+    s->h_nams.resize(std::max<uint64_t>(1, sl.b->tot_nams));
+    s->h_frac.resize(np);
+    s->h_flags.resize(np);
+    gis_nam_out o{s->h_off.data(), s->h_nams.data(), s->h_nams.size(), s->h_frac.data(), s->h_flags.data(), 0};
+    if (batch_download(sl.b, &o)) throw HipError{hipErrorInvalidValue, "server download failed"};
+    const Clock::time_point now = Clock::now();
+    for (size_t j = 0; j < sl.reqs.size(); j++) {
+        Request *q = sl.reqs[j];
+        const uint32_t f = sl.first[j], k = (uint32_t)q->lens.size();
+        const uint64_t a = s->h_off[f], e = s->h_off[f + k];
+        q->nam_off.resize(k + 1);
+        for (uint32_t i = 0; i <= k; i++) q->nam_off[i] = s->h_off[f + i] - a;
+        q->nams.assign(s->h_nams.begin() + (int64_t)a, s->h_nams.begin() + (int64_t)e);
+        q->frac.assign(s->h_frac.begin() + f, s->h_frac.begin() + f + k);
+        q->flags.assign(s->h_flags.begin() + f, s->h_flags.begin() + f + k);
+        q->t_done = now;
+    }
+}
+
+void worker_main(gis_server *s) {
+    (void)hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        bool progress = false;
+        // completions
+        for (Slot &sl : s->slots) {
+            if (!sl.busy || !batch_ready(sl.b)) continue;
+            lk.unlock();
+            try {
+                std::lock_guard<std::recursive_mutex> cl(s->ctx->mu);
+                complete_step(s, sl);
+            } catch (const HipError &e) {
+                fail_step(s, sl, e.what + ": " + hipGetErrorString(e.e));
+            } catch (const std::bad_alloc &) {
+                fail_step(s, sl, "host allocation failed");
+            }
+            lk.lock();
+            for (Request *q : sl.reqs) q->done = true;
+            sl.busy = false;
+            sl.reqs.clear();
+            s->bytes_in_flight -= sl.bytes;
+            if (--s->n_busy == 0)
+                s->stats.busy_s += std::chrono::duration<double>(Clock::now() - s->busy_since).count();
+            progress = true;
+            s->cv_done.notify_all();
+        }
+        // admission: free slots take queued requests in arrival order while
+        // the step stays within max_step_reads and the HBM budget
+        for (Slot &sl : s->slots) {
+            if (sl.busy || s->pending.empty()) continue;
+            uint32_t n = 0;
+            uint64_t bytes = 0;
+            std::vector<Request *> take;
+            while (!s->pending.empty()) {
+                Request *q = s->pending.front();
+                const uint32_t k = (uint32_t)q->lens.size();
+                if (!take.empty() && n + k > s->cfg.max_step_reads) break;
+                if (s->bytes_in_flight + bytes + q->bytes > s->cfg.hbm_budget) break;
+                take.push_back(q);
+                n += k;
+                bytes += q->bytes;
+                s->pending.pop_front();
+            }
+            if (take.empty()) break;  // the budget is held by steps in flight
+            sl.reqs = std::move(take);
+            sl.n_reads = n;
+            sl.bytes = bytes;
+            sl.busy = true;
+            s->bytes_in_flight += bytes;
+            s->stats.peak_bytes = std::max(s->stats.peak_bytes, s->bytes_in_flight);
+            if (s->n_busy++ == 0) s->busy_since = Clock::now();
+            s->stats.steps++;
+            lk.unlock();
+            bool failed = false;
+            try {
+                std::lock_guard<std::recursive_mutex> cl(s->ctx->mu);
+                launch_step(s, sl);
+            } catch (const HipError &e) {
+                failed = true;
+                fail_step(s, sl, e.what + ": " + hipGetErrorString(e.e));
+                (void)hipStreamSynchronize(sl.b->stream);
+            } catch (const std::bad_alloc &) {
+                failed = true;
+                fail_step(s, sl, "host allocation failed");
+            }
+            lk.lock();
+            if (failed) {
+                for (Request *q : sl.reqs) q->done = true;
+                sl.busy = false;
+                sl.reqs.clear();
+                s->bytes_in_flight -= sl.bytes;
+                if (--s->n_busy == 0)
+                    s->stats.busy_s += std::chrono::duration<double>(Clock::now() - s->busy_since).count();
+                s->cv_done.notify_all();
+            }
+            progress = true;
+        }
+        if (s->stop && s->n_busy == 0) break;
+        if (s->n_busy == 0 && s->pending.empty()) {
+            s->cv_work.wait(lk);
+        } else if (!progress) {
+            // steps in flight: poll their streams
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            lk.lock();
+        }
+    }
+}
+
+Request *make_request(gis_server *s, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
+                      uint32_t n_reads) {
+    Request *q = new Request();
+    uint64_t bases = 0;
+    q->offs.resize(n_reads);
+    q->lens.assign(lens, lens + n_reads);
+    for (uint32_t i = 0; i < n_reads; i++) bases += lens[i];
+    q->seqs.reserve(bases);
+    for (uint32_t i = 0; i < n_reads; i++) {
+        q->offs[i] = q->seqs.size();
+        q->seqs.insert(q->seqs.end(), seqs + offsets[i], seqs + offsets[i] + lens[i]);
+    }
+    q->bytes = estimate_bytes(n_reads, bases);
+    (void)s;
+    return q;
+}
+
+}  // namespace
+
+extern "C" int gis_server_create(gis_ctx *ctx, const gis_server_config *cfg, gis_server **out) {
+    if (!out) return GIS_ERR_BAD_PARAMETER;
+    *out = nullptr;
+    API_BEGIN(ctx)
+    if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
+    gis_server_config c{};
+    if (cfg) {
+        c = *cfg;
+    } else {
+        c.slots = 4;
+        c.max_step_reads = 65536;
+        c.hbm_budget = 0;
+        gis_map_options_default(&c.options);
+    }
+    if (c.slots < 1 || c.slots > 16) return fail(ctx, GIS_ERR_BAD_PARAMETER, "slots must be in 1..16");
+    if (c.max_step_reads < 1 || c.max_step_reads > (1u << 22))
+        return fail(ctx, GIS_ERR_BAD_PARAMETER, "max_step_reads must be in 1..4194304");
+    if (c.hbm_budget == 0) c.hbm_budget = 8ull << 30;
+    if (c.options.mode != GIS_MODE_FUSED) c.options.mode = GIS_MODE_PLAIN;
+    gis_server *s = new gis_server();
+    s->ctx = ctx;
+    s->cfg = c;
+    try {
+        for (uint32_t i = 0; i < c.slots; i++) {
+            Slot sl;
+            sl.b = batch_new_stream(ctx);
+            s->slots.push_back(sl);
+        }
+    } catch (...) {
+        for (Slot &sl : s->slots) batch_free(sl.b);
+        delete s;
+        throw;
+    }
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return GIS_OK;
+    API_END(ctx)
+}
+
+extern "C" void gis_server_destroy(gis_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        // queued requests that never started are dropped
+        for (Request *q : s->pending) {
+            q->status = GIS_ERR_CANCELLED;
+            q->done = true;
+        }
+        s->pending.clear();
+    }
+    s->cv_work.notify_all();
+    s->worker.join();
+    {
+        std::lock_guard<std::recursive_mutex> cl(s->ctx->mu);
+        (void)hipSetDevice(s->ctx->device);
+        for (Slot &sl : s->slots) {
+            (void)hipStreamSynchronize(sl.b->stream);
+            batch_free(sl.b);
+        }
+    }
+    for (auto &kv : s->reqs) delete kv.second;
+    delete s;
+}
+
+extern "C" int gis_server_submit(gis_server *s, const char *seqs, const uint64_t *offsets,
+                                 const uint32_t *lens, uint32_t n_reads, uint64_t *id) {
+    if (!s || !id || (n_reads && (!seqs || !offsets || !lens))) return GIS_ERR_BAD_PARAMETER;
+    for (uint32_t i = 0; i < n_reads; i++)
+        if (lens[i] > GIS_MAX_READ_LEN) return GIS_ERR_BAD_PARAMETER;
+    Request *q = nullptr;
+    try {
+        q = make_request(s, seqs, offsets, lens, n_reads);
+    } catch (const std::bad_alloc &) {
+        return GIS_ERR_NOMEM;
+    }
+    if (q->bytes > s->cfg.hbm_budget) {
+        delete q;
+        return GIS_ERR_BUDGET;
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) {
+            delete q;
+            return GIS_ERR_CANCELLED;
+        }
+        q->id = s->next_id++;
+        q->t_submit = Clock::now();
+        s->reqs[q->id] = q;
+        s->pending.push_back(q);
+        s->stats.requests++;
+        s->stats.reads += n_reads;
+        *id = q->id;
+    }
+    s->cv_work.notify_one();
+    return GIS_OK;
+}
+
+extern "C" int gis_server_wait(gis_server *s, uint64_t id, int timeout_ms, gis_nam_out *out,
+                               double *latency_ms) {
+    if (!s) return GIS_ERR_BAD_PARAMETER;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return GIS_ERR_BAD_PARAMETER;
+    Request *q = it->second;
+    auto ready = [&] { return q->done; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) return GIS_ERR_STATE;
+    if (q->status != GIS_OK) {
+        const int st = q->status;
+        s->reqs.erase(it);
+        delete q;
+        return st;
+    }
+    const uint32_t n = (uint32_t)q->lens.size();
+    if (out) {
+        out->n_nams = q->nams.size();
+        if (out->nam_capacity < q->nams.size() || (!q->nams.empty() && !out->nams)) return GIS_ERR_CAPACITY;
+        if (out->nam_offsets) std::memcpy(out->nam_offsets, q->nam_off.data(), 8ull * (n + 1));
+        if (!q->nams.empty()) std::memcpy(out->nams, q->nams.data(), sizeof(gis_nam) * q->nams.size());
+        if (out->nonrepetitive_fraction && n) std::memcpy(out->nonrepetitive_fraction, q->frac.data(), 4ull * n);
+        if (out->flags && n) std::memcpy(out->flags, q->flags.data(), n);
+    }
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(q->t_done - q->t_submit).count();
+    s->reqs.erase(it);
+    delete q;
+    return GIS_OK;
+}
+
+extern "C" int gis_server_replay(gis_server *s, const char *seqs, const uint64_t *offsets,
+                                 const uint32_t *lens, uint32_t n_requests, const double *arrival_s,
+                                 const uint32_t *first, const uint32_t *count, double *latency_ms,
+                                 double *wall_s) {
+    if (!s || (n_requests && (!seqs || !offsets || !lens || !arrival_s || !first || !count)))
+        return GIS_ERR_BAD_PARAMETER;
+    std::vector<uint64_t> ids(n_requests, 0);
+    std::vector<int> sub_rc(n_requests, GIS_OK);
+    std::mutex m;
+    std::condition_variable cv;
+    uint32_t submitted = 0;
+    const Clock::time_point t0 = Clock::now();
+    // the client: submits each request at its arrival time
+    std::thread client([&] {
+        for (uint32_t i = 0; i < n_requests; i++) {
+            const Clock::time_point due = t0 + std::chrono::duration_cast<Clock::duration>(
+                                                   std::chrono::duration<double>(arrival_s[i]));
+            std::this_thread::sleep_until(due);
+            uint64_t id = 0;
+            const int r = gis_server_submit(s, seqs, offsets + first[i], lens + first[i], count[i], &id);
+            {
+                std::lock_guard<std::mutex> lk(m);
+                ids[i] = id;
+                sub_rc[i] = r;
+                submitted = i + 1;
+            }
+            cv.notify_one();
+        }
+    });
+    // this thread collects the requests in arrival order while the client
+    // runs, so finished results do not pile up
+    int rc = GIS_OK;
+    for (uint32_t i = 0; i < n_requests; i++) {
+        {
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return submitted > i; });
+        }
+        if (sub_rc[i] != GIS_OK) {
+            if (rc == GIS_OK) rc = sub_rc[i];
+            if (latency_ms) latency_ms[i] = -1.0;
+            continue;
+        }
+        double lat = 0;
+        const int w = gis_server_wait(s, ids[i], -1, nullptr, &lat);
+        if (w != GIS_OK && rc == GIS_OK) rc = w;
+        if (latency_ms) latency_ms[i] = w == GIS_OK ? lat : -1.0;
+    }
+    client.join();
+    if (wall_s) *wall_s = std::chrono::duration<double>(Clock::now() - t0).count();
+    return rc;
+}
+
+extern "C" int gis_server_stats_get(gis_server *s, gis_server_stats *out) {
+    if (!s || !out) return GIS_ERR_BAD_PARAMETER;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    return GIS_OK;
+}

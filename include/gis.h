@@ -264,6 +264,53 @@ int gis_batch_nam_count(gis_batch *b, uint64_t *n_nams);
 int gis_batch_autotune(gis_batch *b, const gis_map_options *opt, gis_fused_config *best,
                        double *best_ms, uint32_t *n_tried);
 
+/* ---------------- serving (no reference counterpart) ---------------- */
+/* A request loop on one context: requests (sets of reads) are queued by the
+ * caller; a worker thread admits them into device slots while the estimated
+ * device bytes of the steps in flight stay within hbm_budget, coalesces the
+ * queued requests into one launch sequence per slot (a step, replayed from
+ * the slot's captured hipGraph), keeps up to `slots` steps in flight on
+ * separate streams, and completes each request with its per-read NAMs.
+ * Requests arriving while steps run join the next step on a free slot. A
+ * request's NAMs are the same alone and batched with others. The context's
+ * index must stay loaded while the server exists. */
+typedef struct {
+    uint32_t slots;            /* steps in flight, each on its own stream (1..16) */
+    uint32_t max_step_reads;   /* reads coalesced into one step (a larger request runs alone) */
+    uint64_t hbm_budget;       /* estimated device bytes of the steps in flight (0: 8 GiB) */
+    gis_map_options options;   /* mode, rescue level, hard cutoff; graph replays unless GIS_RUN_NO_GRAPH */
+} gis_server_config;
+
+typedef struct {
+    uint64_t requests, reads, steps;
+    double busy_s;             /* time with at least one step in flight */
+    uint64_t peak_bytes;       /* largest estimated device bytes in flight */
+} gis_server_stats;
+
+typedef struct gis_server gis_server;
+/* cfg may be NULL: 4 slots, 65,536 reads per step, 8 GiB, default options */
+int gis_server_create(gis_ctx *ctx, const gis_server_config *cfg, gis_server **out);
+/* Stops the worker; requests still queued are cancelled. No call on the
+ * server may be in progress. */
+void gis_server_destroy(gis_server *s);
+/* Queue a request (the reads are copied) and return its id. GIS_ERR_BUDGET
+ * if the request alone exceeds the HBM budget. */
+int gis_server_submit(gis_server *s, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
+                      uint32_t n_reads, uint64_t *id);
+/* Wait for request id (timeout_ms < 0: no limit; GIS_ERR_STATE on timeout),
+ * copy its results (out may be NULL; GIS_ERR_CAPACITY with n_nams set if
+ * nam_capacity is too small, the request stays) and release it. latency_ms:
+ * submission to completion. */
+int gis_server_wait(gis_server *s, uint64_t id, int timeout_ms, gis_nam_out *out, double *latency_ms);
+/* Replay a timed trace: request i (reads first[i] .. first[i] + count[i] - 1
+ * of the given pool) is submitted arrival_s[i] seconds after the start by a
+ * client thread; returns when all are done, with each request's latency
+ * (ms, -1 if it failed) and the wall time. */
+int gis_server_replay(gis_server *s, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
+                      uint32_t n_requests, const double *arrival_s, const uint32_t *first,
+                      const uint32_t *count, double *latency_ms, double *wall_s);
+int gis_server_stats_get(gis_server *s, gis_server_stats *out);
+
 /* ---------------- synthetic inputs (deterministic) ---------------- */
 typedef struct {
     uint64_t total_length;
