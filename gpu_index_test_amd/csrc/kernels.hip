@@ -1044,7 +1044,11 @@ struct alignas(8) MergeLds {
     uint64_t keys[MERGE_LDS];
 };
 
-__global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int rescue) {
+// reads handed from k_merge4 to k_merge (more than 64 hits, or more than
+// 16 NAMs on one strand)
+constexpr uint32_t MERGE_DEFER = 0xfffffffeu;
+
+__global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int rescue, int only_deferred) {
     __shared__ uint64_t lkeys[WPB][MERGE_LDS];
     const uint32_t lane = lane_id();
     const int wid = (int)(threadIdx.x >> 6);
@@ -1056,6 +1060,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
         const uint32_t r = set_at(set, w);
         const bool rescued = (b.flags[r] & GIS_FLAG_RESCUED) != 0;
         if (rescued != (rescue != 0)) continue;
+        if (only_deferred && b.nam_cnt[r] != MERGE_DEFER) continue;
         const uint64_t hb = rescue ? b.rhit_off[r] : b.hit_off[r];
         const uint32_t nh = rescue ? b.rhit_total[r] : b.hit_total[r];
         if (nh == 0) {
@@ -1080,6 +1085,123 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
             continue;
         }
         if (lane == 0) b.nam_cnt[r] = total;
+        wave_sync_mem();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Merge of four reads per wave (16 lanes each) for reads with at most 64
+// hits. Each lane holds one NAM of its read; NAMs of every (strand, ref)
+// group of the current strand share the segment, so the hits are taken in
+// arrival order without grouping them: within one group that is
+// (query_start, ref_start) order, and a NAM only accepts hits of its own
+// group. Each segment broadcasts its read's next hit from LDS, a ballot
+// restricted to the segment picks the first accepting NAM (creation order),
+// and a hit nobody accepts opens a NAM in the segment's next lane. When a
+// strand's hits end, its NAMs are final: each lane ranks its NAM among the
+// segment's by (ref_id, creation index), which is canonical order within the
+// strand, and writes it. A read with more than 16 NAMs on one strand, or
+// more than 64 hits, is marked MERGE_DEFER for k_merge.
+
+constexpr uint32_t SEG = 16;
+constexpr uint32_t RPW = 64 / SEG;
+
+__device__ __forceinline__ void seg_finalize(gis_nam *out, uint32_t base, uint32_t j, uint32_t n_open,
+                                             uint32_t seg, uint32_t rsd, const LaneNam &o, bool on) {
+    // rank by (ref_id, creation index) among the segment's open lanes
+    const uint32_t key = ((rsd & 0x7fffffffu) << 8) | (o.cid & 0xffu);
+    uint32_t rank = 0;
+    for (uint32_t k = 0; k < SEG; k++) {
+        const uint32_t ok = (uint32_t)__shfl((int)key, (int)(seg * SEG + k), 64);
+        rank += (on && k < n_open && ok < key) ? 1u : 0u;
+    }
+    if (on && j < n_open) lane_nam_store(out, base + rank, rsd, o);
+}
+
+__global__ __launch_bounds__(64 * WPB) void k_merge4(DevBatch b, ReadSet set, int rescue) {
+    __shared__ DevHit lh[WPB][RPW][64];
+    const uint32_t lane = lane_id(), seg = lane / SEG, j = lane % SEG;
+    const int wid = (int)(threadIdx.x >> 6);
+    if (rescue ? over_capacity(b.rhit_off, b.n_reads, b.rhit_cap, b.ctl, OVF_RHITS)
+               : over_capacity(b.hit_off, b.n_reads, b.hit_cap, b.ctl, OVF_HITS))
+        return;
+    const uint32_t cnt = set_count(set);
+    const uint32_t stride = gridDim.x * WPB * RPW;
+    for (uint32_t w0 = (blockIdx.x * WPB + (uint32_t)wid) * RPW; w0 < cnt; w0 += stride) {
+        const uint32_t widx = w0 + seg;
+        bool act = widx < cnt;
+        uint32_t r = 0, nh = 0;
+        uint64_t hb = 0;
+        if (act) {
+            r = set_at(set, widx);
+            const bool rescued = (b.flags[r] & GIS_FLAG_RESCUED) != 0;
+            act = rescued == (rescue != 0);
+        }
+        if (act) {
+            hb = rescue ? b.rhit_off[r] : b.hit_off[r];
+            nh = rescue ? b.rhit_total[r] : b.hit_total[r];
+            if (nh == 0 || nh > 64) {
+                if (j == 0) b.nam_cnt[r] = nh == 0 ? 0u : MERGE_DEFER;
+                act = false;
+            }
+        }
+        if (act) {
+            const DevHit *gh = (rescue ? b.rhits : b.hits) + hb;
+            for (uint32_t k = j; k < nh; k += SEG) lh[wid][seg][k] = gh[k];
+        }
+        wave_sync_mem();
+        gis_nam *out = (rescue ? b.rnam_fin : b.nam_fin) + hb;
+        LaneNam o{0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t rsd_o = 0;        // this lane's NAM group
+        uint32_t n_open = 0;       // NAMs of the segment on the current strand
+        uint32_t strand = 0, base = 0;
+        bool ovf = false;
+        const uint32_t maxh = wave_max(act ? nh : 0u);
+        for (uint32_t t = 0; t < maxh; t++) {
+            const bool valid = act && !ovf && t < nh;
+            DevHit h{};
+            if (valid) h = lh[wid][seg][t];
+            const uint32_t hs = h.ref_strand >> 31;
+            // the strand changes: the segment's forward NAMs are final
+            const bool sw = valid && hs != strand;
+            if (__ballot(sw)) {
+                seg_finalize(out, base, j, n_open, seg, rsd_o, o, sw);
+                if (sw) {
+                    base += n_open;
+                    n_open = 0;
+                    strand = hs;
+                }
+            }
+            const int qs = h.query_start, qe = h.query_end, rs = h.ref_start, re = h.ref_end;
+            bool ext = false;
+            const bool acc = valid && j < n_open && rsd_o == h.ref_strand &&
+                             nam_accepts(o.qp, o.qe, o.rp, o.re, qs, qe, rs, re, ext);
+            const uint64_t bal = __ballot(acc);
+            const uint32_t mine = (uint32_t)(bal >> (seg * SEG)) & 0xffffu;
+            if (mine && j == (uint32_t)__builtin_ctz(mine)) {
+                if (ext) {
+                    o.qe = qe;
+                    o.re = re;
+                }
+                o.qp = qs;
+                o.rp = rs;
+                o.n++;
+            }
+            if (valid && !mine) {
+                if (n_open == SEG) {
+                    ovf = true;
+                } else {
+                    if (j == n_open) {
+                        o = LaneNam{qs, qe, qs, rs, re, rs, 1, n_open};
+                        rsd_o = h.ref_strand;
+                    }
+                    n_open++;
+                }
+            }
+        }
+        const bool fin = act && !ovf;
+        if (__ballot(fin)) seg_finalize(out, base, j, n_open, seg, rsd_o, o, fin);
+        if (act && j == 0) b.nam_cnt[r] = ovf ? MERGE_DEFER : base + n_open;
         wave_sync_mem();
     }
 }
@@ -1412,7 +1534,9 @@ void launch_rescue_hits(const DevBatch &b, const ReadSet &set, const DevIndex &i
 
 void launch_merge(const DevBatch &b, const ReadSet &set, int rescue, hipStream_t st) {
     if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_merge, dim3(grid_for_set(b, set, WPB)), dim3(64 * WPB), 0, st, b, set, rescue);
+    const unsigned g = grid_for_set(b, set, WPB);
+    hipLaunchKernelGGL(k_merge4, dim3((g + RPW - 1) / RPW), dim3(64 * WPB), 0, st, b, set, rescue);
+    hipLaunchKernelGGL(k_merge, dim3(g), dim3(64 * WPB), 0, st, b, set, rescue, 1);
 }
 
 // fused kernel over queue[0 .. count): FWPB waves per block, LMAX the LDS
