@@ -14,7 +14,8 @@ persistent fused kernel under the configuration the autotuner picks, beside
 the per-stage launches. --config c4 runs the human-scale index: a 3.1 Gbp
 synthetic genome (24 contigs, 45% repeats, 2% tandem repeats) indexed on the
 GPU, 10M pairs of 150 bp in 262,144-read batches, fused against per-stage,
-with a bit-exact sample against the oracle over the same table.
+with a bit-exact sample against the oracle over the same table. --config c5
+serves timed request traces on that index (Poisson arrivals at five rates).
 
 Prints one JSON line: reads/s (whole job), p50/p99 batch latency, the
 roofline of the lookup kernel and the CPU oracle's rate on the host cores.
@@ -62,7 +63,9 @@ def make_batches(G, idx, seqs, offs, lens, batch):
         lo, hi = i * batch, min(n, (i + 1) * batch)
         base = int(offs[lo])
         end = int(offs[hi - 1] + lens[hi - 1])
-        m = G.Mapper(idx, hi - lo, end - base)
+        # every batch runs on the first batch's device workspace (the batches
+        # run one after another); only the reads stay resident per batch
+        m = G.Mapper(idx, hi - lo, end - base, share=mappers[0] if mappers else None)
         m.upload(seqs[base:end], offs[lo:hi] - base, lens[lo:hi])
         mappers.append(m)
     return mappers
@@ -90,24 +93,29 @@ def timed(mappers, steps, warmup, mode, graph):
 
 def kernel_times(G, mappers, k, mode):
     """The same runs enqueued launch by launch, with HIP events between the
-    kernels on the batch's stream (graph replays carry no events)."""
+    kernels on the batch's stream (graph replays carry no events); with each
+    run, the algorithmic bytes k_lookup counted on the device."""
     out = []
     for i in range(min(k, len(mappers))):
-        mappers[i].run(mode=mode, graph=False)
-        out.append(mappers[i].stage_times())
+        m = mappers[i]
+        m.run(mode=mode, graph=False)
+        st = m.stage_times()
+        b, q = G._native.C.c_uint64(), G._native.C.c_uint64()
+        m.ctx.check(m.ctx.L.gis_batch_lookup_bytes(m.h, G._native.C.byref(b), G._native.C.byref(q)))
+        st["_lookup_bytes"], st["_lookups"] = b.value, q.value
+        out.append(st)
     return out
 
 
-def lookup_roofline(G, idx, mappers, k, stages):
+def stage_means(stages):
+    return {k: float(np.mean([st[k] for st in stages])) for k in stages[0] if not k.startswith("_")}
+
+
+def lookup_roofline(stages):
     """Algorithmic bytes of k_lookup per launch (counted on the device in the
-    plain eager runs just made) over its HIP-event time."""
-    lk_bytes, lk_n = [], []
-    for i in range(min(k, len(mappers))):
-        b, q = G._native.C.c_uint64(), G._native.C.c_uint64()
-        idx.ctx.check(idx.ctx.L.gis_batch_lookup_bytes(mappers[i].h, G._native.C.byref(b),
-                                                       G._native.C.byref(q)))
-        lk_bytes.append(b.value)
-        lk_n.append(q.value)
+    plain eager runs) over its HIP-event time."""
+    lk_bytes = [st["_lookup_bytes"] for st in stages]
+    lk_n = [st["_lookups"] for st in stages]
     lookup_ms = [st["lookup"] for st in stages]
     achieved = float(np.sum(lk_bytes) / (np.sum(lookup_ms) * 1e-3) / 1e9)
     log(f"lookup: {np.mean(lk_bytes) / 1e6:.1f} MB algorithmic per launch, "
@@ -175,8 +183,8 @@ def run_c2(args, G):
     reads_per_s, elapsed, step_ms = timed(mappers, args.steps, args.warmup, G.MODE_PLAIN, True)
     log(f"plain form: {reads_per_s / 1e6:.1f} M reads/s (graph), {plain_eager_rps / 1e6:.1f} M reads/s (eager)")
     stages = kernel_times(G, mappers, args.steps, G.MODE_PLAIN)
-    roof = lookup_roofline(G, idx, mappers, args.steps, stages)
-    log("stage ms (mean): " + ", ".join(f"{k}={np.mean([st[k] for st in stages]):.3f}" for k in stages[0]))
+    roof = lookup_roofline(stages)
+    log("stage ms (mean): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_means(stages).items()))
 
     cpu = None
     if not args.no_cpu:
@@ -210,7 +218,7 @@ def run_c2(args, G):
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_eager_value": plain_eager_rps,
-        "stage_ms": {k: float(np.mean([st[k] for st in stages])) for k in stages[0]},
+        "stage_ms": stage_means(stages),
         "fused_form": {"value": fused_rps, "unit": "reads/s", "eager_value": fused_eager_rps,
                        "ms_per_step": fused_elapsed * 1e3 / args.steps,
                        "p50_batch_ms": float(np.percentile(fused_ms, 50)),
@@ -240,10 +248,10 @@ def run_c3(args, G):
     plain_rps, plain_elapsed, plain_ms = timed(mappers, steps, args.warmup, G.MODE_PLAIN, True)
     log(f"plain per-stage: {plain_rps / 1e6:.1f} M reads/s")
     fstage = kernel_times(G, mappers, steps, G.MODE_FUSED)
-    spilled = [m.download().flags for m in mappers[:1]]
-    spill_frac = float(np.mean((spilled[0] & 0x80) != 0))
+    mappers[0].run(mode=G.MODE_FUSED)
+    spill_frac = float(np.mean((mappers[0].download().flags & 0x80) != 0))
     stages = kernel_times(G, mappers, steps, G.MODE_PLAIN)
-    roof = lookup_roofline(G, idx, mappers, steps, stages)
+    roof = lookup_roofline(stages)
 
     cpu = None
     if not args.no_cpu:
@@ -278,12 +286,12 @@ def run_c3(args, G):
         },
         "roofline": roof,
         "cpu_baseline": cpu,
-        "fused_stage_ms": {k: float(np.mean([st[k] for st in fstage])) for k in fstage[0]},
+        "fused_stage_ms": stage_means(fstage),
         "plain_form": {"value": plain_rps, "unit": "reads/s",
                        "ms_per_step": plain_elapsed * 1e3 / steps,
                        "p50_batch_ms": float(np.percentile(plain_ms, 50)),
                        "p99_batch_ms": float(np.percentile(plain_ms, 99)),
-                       "stage_ms": {k: float(np.mean([st[k] for st in stages])) for k in stages[0]}},
+                       "stage_ms": stage_means(stages)},
     }
 
 
@@ -321,13 +329,15 @@ def run_c4(args, G):
     log(f"autotune: {n_tried} configurations, best {best} at {best_ms:.3f} ms per batch")
     fused_rps, fused_elapsed, fused_ms = timed(mappers, steps, args.warmup, G.MODE_FUSED, True)
     log(f"fused (autotuned): {fused_rps / 1e6:.1f} M reads/s")
+    mappers[0].run(mode=G.MODE_FUSED)
     fres = mappers[0].download()
     plain_rps, plain_elapsed, plain_ms = timed(mappers, steps, args.warmup, G.MODE_PLAIN, True)
     log(f"plain per-stage: {plain_rps / 1e6:.1f} M reads/s")
+    mappers[0].run(mode=G.MODE_PLAIN)
     pres = mappers[0].download()
     same = bool(np.array_equal(fres.nam_offsets, pres.nam_offsets) and fres.nams.tobytes() == pres.nams.tobytes())
     stages = kernel_times(G, mappers, steps, G.MODE_PLAIN)
-    roof = lookup_roofline(G, idx, mappers, steps, stages)
+    roof = lookup_roofline(stages)
 
     # the oracle over the same (GPU-built) table: parity on a sample, then
     # the CPU baseline
@@ -379,7 +389,107 @@ def run_c4(args, G):
                        "ms_per_step": plain_elapsed * 1e3 / steps,
                        "p50_batch_ms": float(np.percentile(plain_ms, 50)),
                        "p99_batch_ms": float(np.percentile(plain_ms, 99)),
-                       "stage_ms": {k: float(np.mean([st[k] for st in stages])) for k in stages[0]}},
+                       "stage_ms": stage_means(stages)},
+    }
+
+
+def run_c5(args, G):
+    """Serving on the 3.1 Gbp index: Poisson request arrivals at five rates
+    (fractions of the saturated throughput), each request 1-8,192 reads
+    (log-uniform) with lengths from the 50-300 bp mixture, paired and single
+    mixed 50/50; HBM-budgeted admission, graph-replayed steps, several steps
+    in flight. Reports p50/p99 request latency and sustained reads/s per rate."""
+    genome_bp = args.genome or 3_100_000_000
+    t0 = time.time()
+    genome, clens = c4_genome(G, genome_bp)
+    threads = min(16, os.cpu_count() or 1)
+    idx, info = build_index(G, genome, clens, threads)
+    log(f"genome + index in {time.time() - t0:.1f}s")
+    mix = dict(lengths=(50, 100, 150, 250, 300), weights=(0.1, 0.2, 0.4, 0.2, 0.1),
+               sub_rate=0.005, indel_rate=0.0005)
+    pool_pairs = args.pairs or 1_000_000
+    s_pe, o_pe, l_pe, _ = G.synth_reads(genome, clens, pool_pairs, paired=True, insert=(400.0, 50.0),
+                                        seed=2, **mix)
+    s_se, o_se, l_se, _ = G.synth_reads(genome, clens, 2 * pool_pairs, seed=3, **mix)
+    # one pool: single-end reads then pairs (mates adjacent)
+    seqs = np.concatenate([s_se[: int(o_se[-1] + l_se[-1])], s_pe[: int(o_pe[-1] + l_pe[-1])]])
+    offs = np.concatenate([o_se, o_pe + np.uint64(int(o_se[-1] + l_se[-1]))]).astype(np.uint64)
+    lens = np.concatenate([l_se, l_pe]).astype(np.uint32)
+    n_se = len(l_se)
+    del s_pe, s_se
+    rng = np.random.default_rng(5)
+
+    def requests(n):
+        sizes = np.clip(np.exp(rng.uniform(0, np.log(8192), size=n)).astype(np.int64), 1, 8192)
+        paired = rng.random(n) < 0.5
+        first = np.empty(n, dtype=np.int64)
+        for i in range(n):
+            k = int(sizes[i])
+            if paired[i]:
+                k += k & 1  # whole pairs
+                sizes[i] = k
+                start = n_se + 2 * int(rng.integers(0, (len(lens) - n_se - k) // 2))
+            else:
+                start = int(rng.integers(0, n_se - k))
+            first[i] = start
+        return first, sizes
+
+    mode = G.MODE_FUSED if args.server_mode == "fused" else G.MODE_PLAIN
+    srv = G.Server(idx, slots=args.slots, max_step_reads=args.step_reads, hbm_budget=32 << 30, mode=mode)
+    # saturation: a burst of requests at t = 0 (after one untimed burst that
+    # sizes every slot's buffers and captures its graphs)
+    fw, kw = requests(args.burst)
+    srv.replay(seqs, offs, lens, np.zeros(len(fw)), fw, kw)
+    f0, k0 = requests(args.burst)
+    lat0, wall0 = srv.replay(seqs, offs, lens, np.zeros(len(f0)), f0, k0)
+    sat = float(k0.sum()) / wall0
+    log(f"saturation: {len(f0)} requests ({int(k0.sum())} reads) in {wall0:.2f}s = {sat / 1e6:.1f} M reads/s")
+    mean_size = float(np.mean(np.clip(np.exp(rng.uniform(0, np.log(8192), size=200000)), 1, 8192)))
+    rates = []
+    for frac in (0.2, 0.4, 0.6, 0.8, 0.95):
+        target = frac * sat
+        lam = target / mean_size
+        n = max(20, int(lam * args.rate_seconds))
+        fr, kr = requests(n)
+        arrival = np.cumsum(rng.exponential(1.0 / lam, size=n))
+        lat, wall = srv.replay(seqs, offs, lens, arrival, fr, kr)
+        done = float(kr.sum())
+        row = {"target_reads_per_s": target, "offered_reads_per_s": done / float(arrival[-1]),
+               "sustained_reads_per_s": done / wall, "requests": n,
+               "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+               "failed": int((lat < 0).sum())}
+        log(f"rate {frac:.0%}: {row['sustained_reads_per_s'] / 1e6:.1f} M reads/s sustained, "
+            f"p50 {row['p50_ms']:.2f} ms, p99 {row['p99_ms']:.2f} ms over {n} requests")
+        rates.append(row)
+    st = srv.stats()
+    srv.close()
+    best = max(rates, key=lambda r: r["sustained_reads_per_s"])
+    return {
+        "metric": "served reads/sec to NAMs, 3.1 Gbp index, Poisson request arrivals (1 GPU, "
+                  "bit-exact); p50/p99 request latency",
+        "value": best["sustained_reads_per_s"],
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": int(st["steps"]),
+        "warmup": 0,
+        "ms_per_step": st["busy_s"] * 1e3 / max(1, st["steps"]),
+        "p50_batch_ms": best["p50_ms"],
+        "p99_batch_ms": best["p99_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded genome, reads and arrivals)",
+        "config": {
+            "workload": "serving: 3.1 Gbp index resident, Poisson arrivals at 5 rates (20-95% of "
+                        "saturation), requests of 1-8,192 reads (log-uniform), 50-300 bp mixture, "
+                        "paired and single 50/50, HBM-budgeted admission, graph-replayed steps",
+            "genome_bp": genome_bp, "pool_reads": int(len(lens)), "slots": args.slots,
+            "max_step_reads": args.step_reads, "server_mode": args.server_mode,
+            "saturation_reads_per_s": sat, "parallelism": "single GPU",
+        },
+        "rates": rates,
+        "server_stats": st,
     }
 
 
@@ -388,7 +498,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", choices=("c2", "c3", "c4"), default="c2")
+    ap.add_argument("--config", choices=("c2", "c3", "c4", "c5"), default="c2")
+    ap.add_argument("--slots", type=int, default=8, help="c5: steps in flight")
+    ap.add_argument("--step-reads", type=int, default=65536, help="c5: reads per step")
+    ap.add_argument("--server-mode", choices=("plain", "fused"), default="plain")
+    ap.add_argument("--burst", type=int, default=6000, help="c5: requests of the saturation burst")
+    ap.add_argument("--rate-seconds", type=float, default=3.0, help="c5: seconds of arrivals per rate")
     ap.add_argument("--reads", type=int, default=1_000_000, help="c2: single-end reads")
     ap.add_argument("--pairs", type=int, default=0, help="c3/c4: read pairs (2M / 10M)")
     ap.add_argument("--batch", type=int, default=0, help="reads per batch (c2 65536, c3 131072, c4 262144)")
@@ -420,6 +535,8 @@ def main():
     import gpu_index_test_amd as G
     if args.config == "c4":
         out = run_c4(args, G)
+    elif args.config == "c5":
+        out = run_c5(args, G)
     else:
         args.genome = args.genome or 100_000_000
         out = run_c3(args, G) if args.config == "c3" else run_c2(args, G)

@@ -481,11 +481,18 @@ def find_nams_rescue(query_randstrobes, index, rescue_cutoff, hard_cutoff=1000):
 class Mapper:
     """Device-resident batches: upload reads, run the path, download NAMs."""
 
-    def __init__(self, index, max_reads, max_bases):
+    def __init__(self, index, max_reads, max_bases, share=None):
+        """share: another Mapper whose device workspace this one runs on
+        (batches run one at a time; download before running another)."""
         index._check_owner()
         self.index, self.ctx = index, index.ctx
         h = C.c_void_p()
-        self.ctx.check(self.ctx.L.gis_batch_create(self.ctx.h, max_reads, max_bases, C.byref(h)))
+        if share is None:
+            self.ctx.check(self.ctx.L.gis_batch_create(self.ctx.h, max_reads, max_bases, C.byref(h)))
+        else:
+            self._share = share
+            self.ctx.check(self.ctx.L.gis_batch_create_shared(self.ctx.h, max_reads, max_bases, share.h,
+                                                              C.byref(h)))
         self.h = h
 
     def upload(self, seqs, offsets, lens):
@@ -502,7 +509,7 @@ class Mapper:
 
     def nam_count(self):
         v = C.c_uint64()
-        self.ctx.check(self.ctx.L.gis_batch_nam_count(self.h, C.byref(v)))
+        _check(self.ctx.L.gis_batch_nam_count(self.h, C.byref(v)), None)
         return v.value
 
     def download(self):

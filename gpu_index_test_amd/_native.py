@@ -131,6 +131,7 @@ PROTOTYPES = {
                                       C.POINTER(NamOut)]),
     "gis_find_nams_qrs": (C.c_int, [_P, _P, _P, _U32, C.c_int, _U32, _U32, C.POINTER(NamOut)]),
     "gis_batch_create": (C.c_int, [_P, _U32, _U64, C.POINTER(_P)]),
+    "gis_batch_create_shared": (C.c_int, [_P, _U32, _U64, _P, C.POINTER(_P)]),
     "gis_batch_destroy": (None, [_P]),
     "gis_batch_upload": (C.c_int, [_P, _P, _P, _P, _U32]),
     "gis_batch_run": (C.c_int, [_P, C.POINTER(MapOptions)]),

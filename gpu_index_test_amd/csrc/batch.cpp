@@ -23,42 +23,52 @@ using namespace gis;
 
 namespace gis {
 
-gis_batch *batch_new(gis_ctx *ctx) {
+gis_batch *batch_new(gis_ctx *ctx, Workspace *share) {
     gis_batch *b = new gis_batch();
     b->ctx = ctx;
     b->stream = ctx->stream;
+    b->ws = share ? share : new Workspace();
+    b->ws->refs++;
     return b;
 }
 
 gis_batch *batch_new_stream(gis_ctx *ctx) {
-    gis_batch *b = batch_new(ctx);
+    gis_batch *b = batch_new(ctx, nullptr);
     // default priority, like the context's stream
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete b;
+        batch_free(b);
         throw HipError{hipErrorOutOfMemory, "hipStreamCreateWithFlags"};
     }
     b->own_stream = true;
     return b;
 }
 
+static void workspace_release(Workspace *w) {
+    DevBuf *all[] = {&w->long_scratch, &w->rs_cnt, &w->rs_nfwd, &w->rs_hash, &w->rs_start, &w->rs_end,
+                     &w->sy_cnt, &w->sy_hash, &w->sy_pos, &w->pos, &w->hcnt, &w->rcnt, &w->lk_bytes,
+                     &w->hit_total, &w->rhit_total, &w->frac, &w->flags, &w->nam_cnt, &w->hit_off,
+                     &w->rhit_off, &w->hits, &w->rhits, &w->keys, &w->rkeys, &w->nam_ws, &w->rnam_ws,
+                     &w->nam_fin, &w->rnam_fin, &w->nam_slots, &w->nam_off, &w->nams_out, &w->scan_tmp,
+                     &w->ctl, &w->spill_ids, &w->gwork, &w->split_list};
+    for (DevBuf *x : all) release(*x);
+    if (w->h_ctl) (void)hipHostFree(w->h_ctl);
+    if (w->h_total) (void)hipHostFree(w->h_total);
+    delete w;
+}
+
 void batch_free(gis_batch *b) {
     if (!b) return;
-    DevBuf *all[] = {&b->seqs, &b->read_off, &b->read_len, &b->rs_off, &b->sy_off, &b->long_off,
-                     &b->long_scratch, &b->rs_cnt, &b->rs_nfwd, &b->rs_hash, &b->rs_start,
-                     &b->rs_end, &b->sy_cnt, &b->sy_hash, &b->sy_pos, &b->pos, &b->hcnt, &b->rcnt,
-                     &b->lk_bytes, &b->hit_total, &b->rhit_total, &b->frac, &b->flags, &b->nam_cnt,
-                     &b->hit_off, &b->rhit_off, &b->hits, &b->rhits, &b->keys,
-                     &b->rkeys, &b->nam_ws, &b->rnam_ws, &b->nam_fin,
-                     &b->rnam_fin, &b->nam_slots, &b->nam_off, &b->nams_out, &b->scan_tmp, &b->ctl,
-                     &b->spill_ids, &b->queue};
+    DevBuf *all[] = {&b->seqs, &b->read_off, &b->read_len, &b->rs_off, &b->sy_off, &b->long_off, &b->queue};
     for (DevBuf *x : all) release(*x);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
-    if (b->h_ctl) (void)hipHostFree(b->h_ctl);
-    if (b->h_total) (void)hipHostFree(b->h_total);
     for (auto &g : b->graphs) {
         (void)hipGraphExecDestroy(g.exec);
         (void)hipGraphDestroy(g.graph);
+    }
+    if (b->ws) {
+        if (b->ws->owner == b) b->ws->owner = nullptr;
+        if (--b->ws->refs == 0) workspace_release(b->ws);
     }
     if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -85,100 +95,110 @@ DevBatch make_devbatch(gis_batch *b) {
     d.read_len = static_cast<const uint32_t *>(b->read_len.p);
     d.n_reads = b->n_reads;
     d.rs_off = static_cast<const uint64_t *>(b->rs_off.p);
-    d.rs_cnt = static_cast<uint32_t *>(b->rs_cnt.p);
-    d.rs_nfwd = static_cast<uint32_t *>(b->rs_nfwd.p);
-    d.rs_hash = static_cast<uint64_t *>(b->rs_hash.p);
-    d.rs_start = static_cast<uint32_t *>(b->rs_start.p);
-    d.rs_end = static_cast<uint32_t *>(b->rs_end.p);
+    d.rs_cnt = static_cast<uint32_t *>(b->ws->rs_cnt.p);
+    d.rs_nfwd = static_cast<uint32_t *>(b->ws->rs_nfwd.p);
+    d.rs_hash = static_cast<uint64_t *>(b->ws->rs_hash.p);
+    d.rs_start = static_cast<uint32_t *>(b->ws->rs_start.p);
+    d.rs_end = static_cast<uint32_t *>(b->ws->rs_end.p);
     d.sy_off = static_cast<const uint64_t *>(b->sy_off.p);
-    d.sy_cnt = static_cast<uint32_t *>(b->sy_cnt.p);
-    d.sy_hash = static_cast<uint64_t *>(b->sy_hash.p);
-    d.sy_pos = static_cast<uint64_t *>(b->sy_pos.p);
-    d.pos = static_cast<uint64_t *>(b->pos.p);
-    d.hcnt = static_cast<uint32_t *>(b->hcnt.p);
-    d.rcnt = static_cast<uint32_t *>(b->rcnt.p);
-    d.lk_bytes = static_cast<uint32_t *>(b->lk_bytes.p);
-    d.hit_total = static_cast<uint32_t *>(b->hit_total.p);
-    d.rhit_total = static_cast<uint32_t *>(b->rhit_total.p);
-    d.frac = static_cast<float *>(b->frac.p);
-    d.flags = static_cast<uint8_t *>(b->flags.p);
-    d.nam_cnt = static_cast<uint32_t *>(b->nam_cnt.p);
-    d.hit_off = static_cast<uint64_t *>(b->hit_off.p);
-    d.rhit_off = static_cast<uint64_t *>(b->rhit_off.p);
-    d.hits = static_cast<DevHit *>(b->hits.p);
-    d.rhits = static_cast<DevHit *>(b->rhits.p);
-    d.hit_cap = b->hit_cap;
-    d.rhit_cap = b->rhit_cap;
-    d.keys = static_cast<uint64_t *>(b->keys.p);
-    d.rkeys = static_cast<uint64_t *>(b->rkeys.p);
-    d.nam_ws = static_cast<WideNam *>(b->nam_ws.p);
-    d.rnam_ws = static_cast<WideNam *>(b->rnam_ws.p);
-    d.merge_scratch = b->merge_scratch ? 1u : 0u;
-    d.nam_fin = static_cast<gis_nam *>(b->nam_fin.p);
-    d.rnam_fin = static_cast<gis_nam *>(b->rnam_fin.p);
-    d.nam_slots = static_cast<gis_nam *>(b->nam_slots.p);
-    d.nam_off = static_cast<uint64_t *>(b->nam_off.p);
-    d.nams_out = static_cast<gis_nam *>(b->nams_out.p);
-    d.out_cap = b->out_cap;
+    d.sy_cnt = static_cast<uint32_t *>(b->ws->sy_cnt.p);
+    d.sy_hash = static_cast<uint64_t *>(b->ws->sy_hash.p);
+    d.sy_pos = static_cast<uint64_t *>(b->ws->sy_pos.p);
+    d.pos = static_cast<uint64_t *>(b->ws->pos.p);
+    d.hcnt = static_cast<uint32_t *>(b->ws->hcnt.p);
+    d.rcnt = static_cast<uint32_t *>(b->ws->rcnt.p);
+    d.lk_bytes = static_cast<uint32_t *>(b->ws->lk_bytes.p);
+    d.hit_total = static_cast<uint32_t *>(b->ws->hit_total.p);
+    d.rhit_total = static_cast<uint32_t *>(b->ws->rhit_total.p);
+    d.frac = static_cast<float *>(b->ws->frac.p);
+    d.flags = static_cast<uint8_t *>(b->ws->flags.p);
+    d.nam_cnt = static_cast<uint32_t *>(b->ws->nam_cnt.p);
+    d.hit_off = static_cast<uint64_t *>(b->ws->hit_off.p);
+    d.rhit_off = static_cast<uint64_t *>(b->ws->rhit_off.p);
+    d.hits = static_cast<DevHit *>(b->ws->hits.p);
+    d.rhits = static_cast<DevHit *>(b->ws->rhits.p);
+    d.hit_cap = b->ws->hit_cap;
+    d.rhit_cap = b->ws->rhit_cap;
+    d.keys = static_cast<uint64_t *>(b->ws->keys.p);
+    d.rkeys = static_cast<uint64_t *>(b->ws->rkeys.p);
+    d.nam_ws = static_cast<WideNam *>(b->ws->nam_ws.p);
+    d.rnam_ws = static_cast<WideNam *>(b->ws->rnam_ws.p);
+    d.merge_scratch = b->ws->merge_scratch ? 1u : 0u;
+    d.gwork = static_cast<uint4 *>(b->ws->gwork.p);
+    d.split_list = static_cast<uint4 *>(b->ws->split_list.p);
+    d.nam_fin = static_cast<gis_nam *>(b->ws->nam_fin.p);
+    d.rnam_fin = static_cast<gis_nam *>(b->ws->rnam_fin.p);
+    d.nam_slots = static_cast<gis_nam *>(b->ws->nam_slots.p);
+    d.nam_off = static_cast<uint64_t *>(b->ws->nam_off.p);
+    d.nams_out = static_cast<gis_nam *>(b->ws->nams_out.p);
+    d.out_cap = b->ws->out_cap;
     d.long_off = static_cast<const uint64_t *>(b->long_off.p);
-    d.long_scratch = static_cast<uint8_t *>(b->long_scratch.p);
-    d.ctl = static_cast<DevCtl *>(b->ctl.p);
-    d.spill_ids = static_cast<uint32_t *>(b->spill_ids.p);
+    d.long_scratch = static_cast<uint8_t *>(b->ws->long_scratch.p);
+    d.ctl = static_cast<DevCtl *>(b->ws->ctl.p);
+    d.spill_ids = static_cast<uint32_t *>(b->ws->spill_ids.p);
     return d;
 }
 
-void ensure_read_arrays(gis_batch *b, uint32_t n) {
+// the batch's own per-read input arrays
+void ensure_input_arrays(gis_batch *b, uint32_t n) {
     ensure<uint64_t>(b->read_off, n);
     ensure<uint32_t>(b->read_len, n);
     ensure<uint64_t>(b->rs_off, n + 1);
     ensure<uint64_t>(b->sy_off, n + 1);
     ensure<uint64_t>(b->long_off, n + 1);
-    ensure<uint32_t>(b->rs_cnt, n);
-    ensure<uint32_t>(b->rs_nfwd, n);
-    ensure<uint32_t>(b->sy_cnt, n);
-    ensure<uint32_t>(b->lk_bytes, n);
-    ensure<uint32_t>(b->hit_total, n);
-    ensure<uint32_t>(b->rhit_total, n);
-    ensure<uint32_t>(b->nam_cnt, n);
-    ensure<float>(b->frac, n);
-    ensure<uint8_t>(b->flags, n);
-    ensure<uint64_t>(b->hit_off, n + 1);
-    ensure<uint64_t>(b->rhit_off, n + 1);
-    ensure<uint64_t>(b->nam_off, n + 1);
-    ensure<uint64_t>(b->scan_tmp, scan_tmp_words(n) + 8);
-    ensure<DevCtl>(b->ctl, 1);
-    ensure<uint32_t>(b->spill_ids, n);
-    if (!b->h_ctl) HIP_CHECK(hipHostMalloc((void **)&b->h_ctl, sizeof(DevCtl)));
-    if (!b->h_total) HIP_CHECK(hipHostMalloc((void **)&b->h_total, 3 * sizeof(uint64_t)));
+    ensure<uint32_t>(b->queue, n);
+}
+
+// per-read arrays of the workspace
+void ensure_read_arrays(gis_batch *b, uint32_t n) {
+    ensure<uint32_t>(b->ws->rs_cnt, n);
+    ensure<uint32_t>(b->ws->rs_nfwd, n);
+    ensure<uint32_t>(b->ws->sy_cnt, n);
+    ensure<uint32_t>(b->ws->lk_bytes, n);
+    ensure<uint32_t>(b->ws->hit_total, n);
+    ensure<uint32_t>(b->ws->rhit_total, n);
+    ensure<uint32_t>(b->ws->nam_cnt, n);
+    ensure<float>(b->ws->frac, n);
+    ensure<uint8_t>(b->ws->flags, n);
+    ensure<uint64_t>(b->ws->hit_off, n + 1);
+    ensure<uint64_t>(b->ws->rhit_off, n + 1);
+    ensure<uint64_t>(b->ws->nam_off, n + 1);
+    ensure<uint64_t>(b->ws->scan_tmp, scan_tmp_words(n) + 8);
+    ensure<DevCtl>(b->ws->ctl, 1);
+    ensure<uint32_t>(b->ws->spill_ids, n);
+    ensure<uint4>(b->ws->split_list, n);
+    if (!b->ws->h_ctl) HIP_CHECK(hipHostMalloc((void **)&b->ws->h_ctl, sizeof(DevCtl)));
+    if (!b->ws->h_total) HIP_CHECK(hipHostMalloc((void **)&b->ws->h_total, 3 * sizeof(uint64_t)));
 }
 
 void ensure_slot_arrays(gis_batch *b, uint64_t slots) {
-    ensure<uint64_t>(b->rs_hash, slots);
-    ensure<uint32_t>(b->rs_start, slots);
-    ensure<uint32_t>(b->rs_end, slots);
-    ensure<uint64_t>(b->pos, slots);
-    ensure<uint32_t>(b->hcnt, slots);
-    ensure<uint32_t>(b->rcnt, slots);
+    ensure<uint64_t>(b->ws->rs_hash, slots);
+    ensure<uint32_t>(b->ws->rs_start, slots);
+    ensure<uint32_t>(b->ws->rs_end, slots);
+    ensure<uint64_t>(b->ws->pos, slots);
+    ensure<uint32_t>(b->ws->hcnt, slots);
+    ensure<uint32_t>(b->ws->rcnt, slots);
 }
 
 // hit arrays, merge scratch and the NAM output (capacities in hits / NAMs)
 void ensure_pools(gis_batch *b, uint64_t hit_cap, uint64_t rhit_cap, uint64_t out_cap) {
-    b->hit_cap = std::max(b->hit_cap, hit_cap);
-    b->rhit_cap = std::max(b->rhit_cap, rhit_cap);
-    b->out_cap = std::max(b->out_cap, out_cap);
-    ensure<DevHit>(b->hits, b->hit_cap);
-    ensure<gis_nam>(b->nam_fin, b->hit_cap);
-    ensure<DevHit>(b->rhits, b->rhit_cap);
-    ensure<gis_nam>(b->rnam_fin, b->rhit_cap);
-    if (b->merge_scratch) {
+    b->ws->hit_cap = std::max(b->ws->hit_cap, hit_cap);
+    b->ws->rhit_cap = std::max(b->ws->rhit_cap, rhit_cap);
+    b->ws->out_cap = std::max(b->ws->out_cap, out_cap);
+    ensure<DevHit>(b->ws->hits, b->ws->hit_cap);
+    ensure<gis_nam>(b->ws->nam_fin, b->ws->hit_cap);
+    ensure<DevHit>(b->ws->rhits, b->ws->rhit_cap);
+    ensure<gis_nam>(b->ws->rnam_fin, b->ws->rhit_cap);
+    if (b->ws->merge_scratch) {
         // merge scratch of reads beyond the merge's LDS budget, allocated
         // once a run has met such a read
-        ensure<uint64_t>(b->keys, 2 * b->hit_cap);
-        ensure<WideNam>(b->nam_ws, b->hit_cap);
-        ensure<uint64_t>(b->rkeys, 2 * b->rhit_cap);
-        ensure<WideNam>(b->rnam_ws, b->rhit_cap);
+        ensure<uint64_t>(b->ws->keys, 2 * b->ws->hit_cap);
+        ensure<WideNam>(b->ws->nam_ws, b->ws->hit_cap);
+        ensure<uint64_t>(b->ws->rkeys, 2 * b->ws->rhit_cap);
+        ensure<WideNam>(b->ws->rnam_ws, b->ws->rhit_cap);
+        ensure<uint4>(b->ws->gwork, std::max(b->ws->hit_cap, b->ws->rhit_cap) + 64);
     }
-    ensure<gis_nam>(b->nams_out, b->out_cap);
+    ensure<gis_nam>(b->ws->nams_out, b->ws->out_cap);
 }
 
 void mark(gis_batch *b, const char *name) {
@@ -231,23 +251,9 @@ int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const 
         b->n_long = 0;
         for (uint32_t r = 0; r < n; r++) b->n_long += lens[r] > (uint32_t)SHORT_BIN ? 1u : 0u;
     }
-    ensure_read_arrays(b, n);
-    ensure<uint32_t>(b->queue, n);
+    ensure_input_arrays(b, n);
     b->tot_rs_cap = b->h_rs_off[n];
-    ensure_slot_arrays(b, b->tot_rs_cap);
-    if (want_sy) {
-        ensure<uint64_t>(b->sy_hash, b->h_sy_off[n]);
-        ensure<uint64_t>(b->sy_pos, b->h_sy_off[n]);
-    }
-    if (b->any_long) ensure<uint8_t>(b->long_scratch, b->h_long_off[n]);
-    // initial pools: about the hits of typical reads (~0.16 per base), grown
-    // on demand to what a run's scans report.
-    // GIS_TEST_SMALL_POOLS starts them tiny so tests exercise the overflow path.
-    const uint64_t bases = span;
-    if (const char *e = std::getenv("GIS_TEST_SMALL_POOLS"); e && *e == '1')
-        ensure_pools(b, 64, 64, 16);
-    else
-        ensure_pools(b, 24ull * n + bases / 16 + 4096, 4ull * n + 4096, 12ull * n + 1024);
+    b->span = span;
     ensure<char>(b->seqs, span);
     hipStream_t st = b->stream;
     if (span) HIP_CHECK(hipMemcpyAsync(b->seqs.p, seqs, span, hipMemcpyHostToDevice, st));
@@ -277,7 +283,7 @@ void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_
     const SeedParams sp = make_seed_params(ctx->ix.params);
     const DevBatch d = make_devbatch(b);
     const uint32_t n = b->n_reads;
-    uint64_t *tmp = static_cast<uint64_t *>(b->scan_tmp.p);
+    uint64_t *tmp = static_cast<uint64_t *>(b->ws->scan_tmp.p);
     b->n_ev = 0;
     b->ev_names.clear();
     mark(b, "start");
@@ -315,8 +321,8 @@ void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_
         launch_scan(d.rhit_total, d.rhit_off, n, tmp, st);
         launch_rescue_hits(d, set, ix, st);
         mark(b, "rescue");
-        HIP_CHECK(hipMemcpyAsync(b->h_total, d.hit_off + n, 8, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(b->h_total + 1, d.rhit_off + n, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->h_total, d.hit_off + n, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->h_total + 1, d.rhit_off + n, 8, hipMemcpyDeviceToHost, st));
     }
     if (stop == STOP_NONE) {
         launch_merge(d, set, 0, st);
@@ -326,9 +332,29 @@ void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_
         launch_scan(d.nam_cnt, d.nam_off, n, tmp, st);
         launch_compact(d, fused ? 1 : 0, st);
         mark(b, "output");
-        HIP_CHECK(hipMemcpyAsync(b->h_total + 2, d.nam_off + n, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->h_total + 2, d.nam_off + n, 8, hipMemcpyDeviceToHost, st));
     }
-    HIP_CHECK(hipMemcpyAsync(b->h_ctl, d.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(b->ws->h_ctl, d.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, st));
+}
+
+// size the workspace for this batch's reads (grow-only; cheap when it
+// already fits)
+void prepare_workspace(gis_batch *b) {
+    const uint32_t n = b->n_reads;
+    ensure_read_arrays(b, n);
+    ensure_slot_arrays(b, b->tot_rs_cap);
+    if (b->want_sy) {
+        ensure<uint64_t>(b->ws->sy_hash, b->h_sy_off[n]);
+        ensure<uint64_t>(b->ws->sy_pos, b->h_sy_off[n]);
+    }
+    if (b->any_long) ensure<uint8_t>(b->ws->long_scratch, b->h_long_off[n]);
+    // initial pools: about the hits of typical reads (~0.16 per base), grown
+    // on demand to what a run's scans report.
+    // GIS_TEST_SMALL_POOLS starts them tiny so tests exercise the overflow path.
+    if (const char *e = std::getenv("GIS_TEST_SMALL_POOLS"); e && *e == '1')
+        ensure_pools(b, 64, 64, 16);
+    else
+        ensure_pools(b, 24ull * n + b->span / 16 + 4096, 4ull * n + 4096, 12ull * n + 1024);
 }
 
 RunOpts run_opts(const gis_batch *b, const gis_map_options &o, int rescue_mode, int stop) {
@@ -349,8 +375,11 @@ RunOpts run_opts(const gis_batch *b, const gis_map_options &o, int rescue_mode, 
 void batch_launch(gis_batch *b, const RunOpts &r) {
     gis_ctx *ctx = b->ctx;
     hipStream_t st = b->stream;
+    prepare_workspace(b);
+    b->ws->owner = b;
+    b->ran = false;
     if (r.mode == GIS_MODE_FUSED && r.stop == STOP_NONE)
-        ensure<gis_nam>(b->nam_slots, (size_t)b->n_reads * F_NAMCAP);
+        ensure<gis_nam>(b->ws->nam_slots, (size_t)b->n_reads * F_NAMCAP);
     if (r.graph) {
         const DevBatch key = make_devbatch(b);
         const gis_fused_config &fc = ctx->fcfg;
@@ -405,9 +434,9 @@ bool batch_ready(const gis_batch *b) { return hipStreamQuery(b->stream) == hipSu
 static unsigned finish_once(gis_batch *b, const RunOpts &r) {
     HIP_CHECK(hipStreamSynchronize(b->stream));
     const bool fused = r.mode == GIS_MODE_FUSED && r.stop == STOP_NONE;
-    b->n_spill = fused ? b->h_ctl->n_spill : 0;
-    b->tot_nams = r.stop == STOP_NONE ? b->h_total[2] : 0;
-    return b->h_ctl->overflow;
+    b->ws->n_spill = fused ? b->ws->h_ctl->n_spill : 0;
+    b->ws->tot_nams = r.stop == STOP_NONE ? b->ws->h_total[2] : 0;
+    return b->ws->h_ctl->overflow;
 }
 
 void batch_complete(gis_batch *b, const RunOpts &r) {
@@ -417,10 +446,10 @@ void batch_complete(gis_batch *b, const RunOpts &r) {
         // without its scratch some reads are missing, so the NAM total is
         // not final: grow what overflowed now and size the output next time
         if (ovf & (OVF_HITS | OVF_RHITS | OVF_MERGE)) ovf &= ~(unsigned)OVF_OUT;
-        if (ovf & OVF_MERGE) b->merge_scratch = true;
-        ensure_pools(b, (ovf & OVF_HITS) ? b->h_total[0] + b->h_total[0] / 8 + 4096 : 0,
-                     (ovf & OVF_RHITS) ? b->h_total[1] + b->h_total[1] / 8 + 4096 : 0,
-                     (ovf & OVF_OUT) ? b->tot_nams + b->tot_nams / 8 + 1024 : 0);
+        if (ovf & OVF_MERGE) b->ws->merge_scratch = true;
+        ensure_pools(b, (ovf & OVF_HITS) ? b->ws->h_total[0] + b->ws->h_total[0] / 8 + 4096 : 0,
+                     (ovf & OVF_RHITS) ? b->ws->h_total[1] + b->ws->h_total[1] / 8 + 4096 : 0,
+                     (ovf & OVF_OUT) ? b->ws->tot_nams + b->ws->tot_nams / 8 + 1024 : 0);
         batch_launch(b, r);
         ovf = finish_once(b, r);
     }
@@ -440,26 +469,28 @@ int batch_download(gis_batch *b, gis_nam_out *out) {
     gis_ctx *ctx = b->ctx;
     if (!out) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null output");
     if (!b->ran) return fail(ctx, GIS_ERR_STATE, "batch has not been run");
+    if (b->ws->owner != b)
+        return fail(ctx, GIS_ERR_STATE, "another batch sharing the workspace ran since this batch's run");
     const uint32_t n = b->n_reads;
-    out->n_nams = b->tot_nams;
-    if (out->nam_capacity < b->tot_nams || (b->tot_nams && !out->nams))
+    out->n_nams = b->ws->tot_nams;
+    if (out->nam_capacity < b->ws->tot_nams || (b->ws->tot_nams && !out->nams))
         return fail(ctx, GIS_ERR_CAPACITY,
-                    "NAM capacity " + std::to_string(out->nam_capacity) + " < " + std::to_string(b->tot_nams));
+                    "NAM capacity " + std::to_string(out->nam_capacity) + " < " + std::to_string(b->ws->tot_nams));
     hipStream_t st = b->stream;
     if (out->nam_offsets)
-        HIP_CHECK(hipMemcpyAsync(out->nam_offsets, b->nam_off.p, 8ull * (n + 1), hipMemcpyDeviceToHost, st));
-    if (b->tot_nams)
-        HIP_CHECK(hipMemcpyAsync(out->nams, b->nams_out.p, sizeof(gis_nam) * b->tot_nams,
+        HIP_CHECK(hipMemcpyAsync(out->nam_offsets, b->ws->nam_off.p, 8ull * (n + 1), hipMemcpyDeviceToHost, st));
+    if (b->ws->tot_nams)
+        HIP_CHECK(hipMemcpyAsync(out->nams, b->ws->nams_out.p, sizeof(gis_nam) * b->ws->tot_nams,
                                  hipMemcpyDeviceToHost, st));
     if (out->nonrepetitive_fraction && n)
-        HIP_CHECK(hipMemcpyAsync(out->nonrepetitive_fraction, b->frac.p, 4ull * n, hipMemcpyDeviceToHost, st));
-    if (out->flags && n) HIP_CHECK(hipMemcpyAsync(out->flags, b->flags.p, n, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(out->nonrepetitive_fraction, b->ws->frac.p, 4ull * n, hipMemcpyDeviceToHost, st));
+    if (out->flags && n) HIP_CHECK(hipMemcpyAsync(out->flags, b->ws->flags.p, n, hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     return GIS_OK;
 }
 
 gis_batch *scratch_batch(gis_ctx *ctx) {
-    if (!ctx->scratch) ctx->scratch = batch_new(ctx);
+    if (!ctx->scratch) ctx->scratch = batch_new(ctx, nullptr);
     return ctx->scratch;
 }
 
@@ -485,7 +516,7 @@ extern "C" int gis_syncmers_batch(gis_ctx *ctx, const char *seqs, const uint64_t
     const gis_map_options o = opts_or_default(nullptr);
     batch_run(b, o, 0, STOP_SEED);
     std::vector<uint32_t> cnt(n_reads);
-    if (n_reads) HIP_CHECK(hipMemcpy(cnt.data(), b->sy_cnt.p, 4ull * n_reads, hipMemcpyDeviceToHost));
+    if (n_reads) HIP_CHECK(hipMemcpy(cnt.data(), b->ws->sy_cnt.p, 4ull * n_reads, hipMemcpyDeviceToHost));
     uint64_t total = 0;
     for (uint32_t r = 0; r < n_reads; r++) total += cnt[r];
     if (n_out) *n_out = total;
@@ -493,8 +524,8 @@ extern "C" int gis_syncmers_batch(gis_ctx *ctx, const char *seqs, const uint64_t
     const uint64_t cap = b->h_sy_off[n_reads];
     std::vector<uint64_t> hh(cap), pp(cap);
     if (cap) {
-        HIP_CHECK(hipMemcpy(hh.data(), b->sy_hash.p, 8 * cap, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(pp.data(), b->sy_pos.p, 8 * cap, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(hh.data(), b->ws->sy_hash.p, 8 * cap, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(pp.data(), b->ws->sy_pos.p, 8 * cap, hipMemcpyDeviceToHost));
     }
     uint64_t o2 = 0;
     for (uint32_t r = 0; r < n_reads; r++) {
@@ -520,7 +551,7 @@ extern "C" int gis_randstrobes_batch(gis_ctx *ctx, const char *seqs, const uint6
     const gis_map_options o = opts_or_default(nullptr);
     batch_run(b, o, 0, STOP_SEED);
     std::vector<uint32_t> cnt(n_reads);
-    if (n_reads) HIP_CHECK(hipMemcpy(cnt.data(), b->rs_cnt.p, 4ull * n_reads, hipMemcpyDeviceToHost));
+    if (n_reads) HIP_CHECK(hipMemcpy(cnt.data(), b->ws->rs_cnt.p, 4ull * n_reads, hipMemcpyDeviceToHost));
     uint64_t total = 0;
     for (uint32_t r = 0; r < n_reads; r++) total += cnt[r];
     if (n_out) *n_out = total;
@@ -529,9 +560,9 @@ extern "C" int gis_randstrobes_batch(gis_ctx *ctx, const char *seqs, const uint6
     std::vector<uint64_t> hh(cap);
     std::vector<uint32_t> ss(cap), ee(cap);
     if (cap) {
-        HIP_CHECK(hipMemcpy(hh.data(), b->rs_hash.p, 8 * cap, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(ss.data(), b->rs_start.p, 4 * cap, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(ee.data(), b->rs_end.p, 4 * cap, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(hh.data(), b->ws->rs_hash.p, 8 * cap, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(ss.data(), b->ws->rs_start.p, 4 * cap, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(ee.data(), b->ws->rs_end.p, 4 * cap, hipMemcpyDeviceToHost));
     }
     uint64_t o2 = 0;
     for (uint32_t r = 0; r < n_reads; r++) {
@@ -557,8 +588,8 @@ extern "C" int gis_lookup_batch(gis_ctx *ctx, const uint64_t *hashes, uint64_t n
     if (n && (!hashes || !positions)) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null argument");
     if (n == 0) return GIS_OK;
     gis_batch *b = scratch_batch(ctx);
-    uint64_t *dh = ensure<uint64_t>(b->rs_hash, n);
-    uint64_t *dp = ensure<uint64_t>(b->pos, n);
+    uint64_t *dh = ensure<uint64_t>(b->ws->rs_hash, n);
+    uint64_t *dp = ensure<uint64_t>(b->ws->pos, n);
     HIP_CHECK(hipMemcpyAsync(dh, hashes, 8 * n, hipMemcpyHostToDevice, ctx->stream));
     launch_find_flat(dev_index(ctx), dh, n, dp, ctx->stream);
     HIP_CHECK(hipGetLastError());
@@ -573,12 +604,12 @@ static int copy_hits_out(gis_ctx *ctx, gis_batch *b, uint64_t *out_offsets, gis_
     const uint32_t n = b->n_reads;
     std::vector<uint8_t> flags(n);
     std::vector<uint64_t> ho(n + 1), rho(n + 1);
-    if (n) HIP_CHECK(hipMemcpy(flags.data(), b->flags.p, n, hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(ho.data(), b->hit_off.p, 8ull * (n + 1), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(rho.data(), b->rhit_off.p, 8ull * (n + 1), hipMemcpyDeviceToHost));
+    if (n) HIP_CHECK(hipMemcpy(flags.data(), b->ws->flags.p, n, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(ho.data(), b->ws->hit_off.p, 8ull * (n + 1), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(rho.data(), b->ws->rhit_off.p, 8ull * (n + 1), hipMemcpyDeviceToHost));
     std::vector<DevHit> h(ho[n]), rh(rho[n]);
-    if (ho[n]) HIP_CHECK(hipMemcpy(h.data(), b->hits.p, sizeof(DevHit) * ho[n], hipMemcpyDeviceToHost));
-    if (rho[n]) HIP_CHECK(hipMemcpy(rh.data(), b->rhits.p, sizeof(DevHit) * rho[n], hipMemcpyDeviceToHost));
+    if (ho[n]) HIP_CHECK(hipMemcpy(h.data(), b->ws->hits.p, sizeof(DevHit) * ho[n], hipMemcpyDeviceToHost));
+    if (rho[n]) HIP_CHECK(hipMemcpy(rh.data(), b->ws->rhits.p, sizeof(DevHit) * rho[n], hipMemcpyDeviceToHost));
     uint64_t total = 0;
     for (uint32_t r = 0; r < n; r++)
         total += (flags[r] & GIS_FLAG_RESCUED) ? rho[r + 1] - rho[r] : ho[r + 1] - ho[r];
@@ -683,24 +714,25 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
     b->h_rs_off = off;
     ensure_read_arrays(b, n);
     ensure_slot_arrays(b, total);
+    b->ws->owner = b;
     ensure_pools(b, 64ull * n + 4096, 16ull * n + 4096, 16ull * n + 1024);
     b->tot_rs_cap = total;
     hipStream_t st = ctx->stream;
     if (total) {
-        HIP_CHECK(hipMemcpyAsync(b->rs_hash.p, hh.data(), 8 * total, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(b->rs_start.p, ss.data(), 4 * total, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(b->rs_end.p, ee.data(), 4 * total, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->rs_hash.p, hh.data(), 8 * total, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->rs_start.p, ss.data(), 4 * total, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->rs_end.p, ee.data(), 4 * total, hipMemcpyHostToDevice, st));
     }
     HIP_CHECK(hipMemcpyAsync(b->rs_off.p, off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
     if (n) {
-        HIP_CHECK(hipMemcpyAsync(b->rs_cnt.p, cnt.data(), 4ull * n, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(b->rs_nfwd.p, nf.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->rs_cnt.p, cnt.data(), 4ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->rs_nfwd.p, nf.data(), 4ull * n, hipMemcpyHostToDevice, st));
     }
     HIP_CHECK(hipStreamSynchronize(st));
     // the stages after seeding, with the rescue decision forced either way
     const DevIndex ix = dev_index(ctx);
     const ReadSet set{nullptr, nullptr, n};
-    uint64_t *tmp = static_cast<uint64_t *>(b->scan_tmp.p);
+    uint64_t *tmp = static_cast<uint64_t *>(b->ws->scan_tmp.p);
     for (int attempt = 0; attempt < 6; attempt++) {
         const DevBatch d = make_devbatch(b);
         launch_reset_ctl(d.ctl, st);
@@ -714,21 +746,21 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
         launch_merge(d, set, 1, st);
         launch_scan(d.nam_cnt, d.nam_off, n, tmp, st);
         launch_compact(d, 0, st);
-        HIP_CHECK(hipMemcpyAsync(b->h_total, d.hit_off + n, 8, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(b->h_total + 1, d.rhit_off + n, 8, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(b->h_total + 2, d.nam_off + n, 8, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(b->h_ctl, d.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->h_total, d.hit_off + n, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->h_total + 1, d.rhit_off + n, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->h_total + 2, d.nam_off + n, 8, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->h_ctl, d.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(st));
-        b->tot_nams = b->h_total[2];
-        unsigned ovf = b->h_ctl->overflow;
+        b->ws->tot_nams = b->ws->h_total[2];
+        unsigned ovf = b->ws->h_ctl->overflow;
         if (!ovf) break;
         if (attempt == 5) throw HipError{hipErrorOutOfMemory, "batch arrays still too small after regrowth"};
         if (ovf & (OVF_HITS | OVF_RHITS | OVF_MERGE)) ovf &= ~(unsigned)OVF_OUT;
-        if (ovf & OVF_MERGE) b->merge_scratch = true;
-        ensure_pools(b, (ovf & OVF_HITS) ? b->h_total[0] + 4096 : 0,
-                     (ovf & OVF_RHITS) ? b->h_total[1] + 4096 : 0,
-                     (ovf & OVF_OUT) ? b->tot_nams + 1024 : 0);
+        if (ovf & OVF_MERGE) b->ws->merge_scratch = true;
+        ensure_pools(b, (ovf & OVF_HITS) ? b->ws->h_total[0] + 4096 : 0,
+                     (ovf & OVF_RHITS) ? b->ws->h_total[1] + 4096 : 0,
+                     (ovf & OVF_OUT) ? b->ws->tot_nams + 1024 : 0);
     }
     b->ran = true;
     return batch_download(b, out);
@@ -740,8 +772,23 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
 extern "C" int gis_batch_create(gis_ctx *ctx, uint32_t max_reads, uint64_t max_bases, gis_batch **out) {
     API_BEGIN(ctx)
     if (!out) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null output");
-    gis_batch *b = batch_new(ctx);
+    gis_batch *b = batch_new(ctx, nullptr);
+    ensure_input_arrays(b, max_reads);
     ensure_read_arrays(b, max_reads);
+    ensure<char>(b->seqs, max_bases);
+    *out = b;
+    return GIS_OK;
+    API_END(ctx)
+}
+
+extern "C" int gis_batch_create_shared(gis_ctx *ctx, uint32_t max_reads, uint64_t max_bases,
+                                       gis_batch *share_with, gis_batch **out) {
+    API_BEGIN(ctx)
+    if (!out || !share_with) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null argument");
+    if (share_with->ctx != ctx) return fail(ctx, GIS_ERR_BAD_PARAMETER, "the batch belongs to another context");
+    gis_batch *b = batch_new(ctx, share_with->ws);
+    b->stream = share_with->stream;
+    ensure_input_arrays(b, max_reads);
     ensure<char>(b->seqs, max_bases);
     *out = b;
     return GIS_OK;
@@ -786,7 +833,8 @@ extern "C" int gis_batch_download(gis_batch *b, gis_nam_out *out) {
 
 extern "C" int gis_batch_nam_count(gis_batch *b, uint64_t *n_nams) {
     if (!b || !n_nams) return GIS_ERR_BAD_PARAMETER;
-    *n_nams = b->tot_nams;
+    if (b->ws->owner != b) return GIS_ERR_STATE;
+    *n_nams = b->ws->tot_nams;
     return GIS_OK;
 }
 
@@ -815,13 +863,14 @@ extern "C" int gis_batch_lookup_bytes(gis_batch *b, uint64_t *bytes, uint64_t *n
     if (!b) return GIS_ERR_BAD_PARAMETER;
     gis_ctx *ctx = b->ctx;
     API_BEGIN(ctx)
+    if (b->ws->owner != b) return fail(ctx, GIS_ERR_STATE, "another batch ran on the shared workspace since");
     if (!b->ran) return fail(ctx, GIS_ERR_STATE, "batch has not been run");
     if (b->last_mode != GIS_MODE_PLAIN) return fail(ctx, GIS_ERR_STATE, "lookup bytes are counted by the per-stage path");
     const uint32_t n = b->n_reads;
     std::vector<uint32_t> by(n), cnt(n);
     if (n) {
-        HIP_CHECK(hipMemcpy(by.data(), b->lk_bytes.p, 4ull * n, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(cnt.data(), b->rs_cnt.p, 4ull * n, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(by.data(), b->ws->lk_bytes.p, 4ull * n, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(cnt.data(), b->ws->rs_cnt.p, 4ull * n, hipMemcpyDeviceToHost));
     }
     uint64_t tb = 0, tq = 0;
     for (uint32_t r = 0; r < n; r++) {

@@ -9,35 +9,50 @@
 
 #include "driver_internal.h"
 
-struct gis_batch {
-    using DevBuf = gis::DevBuf;
-    using DevCtl = gis::DevCtl;
-    using DevBatch = gis::DevBatch;
-    gis_ctx *ctx = nullptr;
-    uint32_t n_reads = 0;
-    bool want_sy = false;
-    bool any_long = false;
-    std::vector<uint64_t> h_rs_off, h_sy_off, h_long_off;
-    DevBuf seqs, read_off, read_len, rs_off, sy_off, long_off, long_scratch;
+namespace gis {
+
+// Device intermediates and results of a run. A workspace may be shared by
+// several batches that run one at a time (each run's results must be read
+// before another batch runs on it); each serving slot has its own.
+struct Workspace {
+    DevBuf long_scratch;
     DevBuf rs_cnt, rs_nfwd, rs_hash, rs_start, rs_end, sy_cnt, sy_hash, sy_pos;
     DevBuf pos, hcnt, rcnt, lk_bytes;
     DevBuf hit_total, rhit_total, frac, flags, nam_cnt, hit_off, rhit_off;
     DevBuf hits, rhits, keys, rkeys, nam_ws, rnam_ws, nam_fin, rnam_fin;
-    DevBuf nam_slots, nam_off, nams_out, scan_tmp, ctl, spill_ids, queue;
-    std::vector<uint32_t> h_queue;  // read ids by length, longest first
-    uint32_t n_long = 0;            // queue entries longer than the short bin (160 bp)
+    DevBuf nam_slots, nam_off, nams_out, scan_tmp, ctl, spill_ids, gwork, split_list;
     uint64_t hit_cap = 0, rhit_cap = 0, out_cap = 0;
-    bool merge_scratch = false;  // keys/nam_ws allocated (reads with many hits need them)
-    uint64_t tot_rs_cap = 0;
+    bool merge_scratch = false;  // keys/nam_ws/gwork allocated (reads with many hits need them)
     // results of the last run
     uint64_t tot_nams = 0;
     uint32_t n_spill = 0;
-    int last_mode = GIS_MODE_PLAIN;
-    bool last_graph = false;
-    bool ran = false;
+    const gis_batch *owner = nullptr;  // the batch whose run the results belong to
     // pinned landing area for the end-of-run readback
     DevCtl *h_ctl = nullptr;
     uint64_t *h_total = nullptr;  // [3]: hits, rescue hits, NAMs
+    int refs = 0;
+};
+
+}  // namespace gis
+
+struct gis_batch {
+    using DevBuf = gis::DevBuf;
+    using DevBatch = gis::DevBatch;
+    gis_ctx *ctx = nullptr;
+    // resident reads and what the host derived from their lengths
+    uint32_t n_reads = 0;
+    uint64_t span = 0;  // bases
+    bool want_sy = false;
+    bool any_long = false;
+    std::vector<uint64_t> h_rs_off, h_sy_off, h_long_off;
+    DevBuf seqs, read_off, read_len, rs_off, sy_off, long_off, queue;
+    std::vector<uint32_t> h_queue;  // read ids by length, longest first
+    uint32_t n_long = 0;            // queue entries longer than the short bin (160 bp)
+    uint64_t tot_rs_cap = 0;
+    gis::Workspace *ws = nullptr;
+    int last_mode = GIS_MODE_PLAIN;
+    bool last_graph = false;
+    bool ran = false;
     hipEvent_t ev[16] = {};
     int n_ev = 0;
     std::vector<std::string> ev_names;
@@ -56,7 +71,6 @@ struct gis_batch {
     bool own_stream = false;
 };
 
-
 namespace gis {
 
 enum { STOP_SEED = 0, STOP_HITS = 1, STOP_NONE = 2 };
@@ -70,7 +84,8 @@ struct RunOpts {
     bool graph;        // replay a captured graph
 };
 
-gis_batch *batch_new_stream(gis_ctx *ctx);  // a batch with a stream of its own
+gis_batch *batch_new(gis_ctx *ctx, Workspace *share);  // share = nullptr: a workspace of its own
+gis_batch *batch_new_stream(gis_ctx *ctx);  // a batch with a stream and a workspace of its own
 int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
                  uint32_t n, bool want_sy);
 RunOpts run_opts(const gis_batch *b, const gis_map_options &o, int rescue_mode, int stop);

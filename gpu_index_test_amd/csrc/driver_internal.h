@@ -101,7 +101,6 @@ inline DevIndex dev_index(const gis_ctx *ctx) {
 }
 
 // the batch machinery (batch.cpp)
-gis_batch *batch_new(gis_ctx *ctx);
 void batch_free(gis_batch *b);
 
 }  // namespace gis

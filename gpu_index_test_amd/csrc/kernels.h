@@ -44,6 +44,8 @@ struct DevCtl {
     unsigned int n_spill;   // reads handed from the fused kernel to the per-stage kernels
     unsigned int overflow;  // OVF_* bits
     unsigned int q_head[2]; // dynamic work-queue heads of the fused kernel's two bins
+    unsigned int n_gwork[2];  // group work items of split reads (normal, rescue merge)
+    unsigned int n_split[2];  // split reads (normal, rescue merge)
 };
 
 enum : unsigned { OVF_HITS = 1, OVF_RHITS = 2, OVF_MERGE = 4, OVF_OUT = 8 };
@@ -98,7 +100,12 @@ struct DevBatch {
     uint64_t hit_cap, rhit_cap;
     uint64_t *keys, *rkeys;
     WideNam *nam_ws, *rnam_ws;
-    uint32_t merge_scratch;   // keys/nam_ws allocated; else reads needing them set OVF_MERGE
+    uint32_t merge_scratch;   // keys/nam_ws/gwork allocated; else reads needing them set OVF_MERGE
+    // reads with many hits merge each (strand, ref) group on its own wave:
+    // gwork items {read, first, end, group | order code << 30}, split list
+    // entries {read, groups, order code, 0}
+    uint4 *gwork;
+    uint4 *split_list;
     gis_nam *nam_fin, *rnam_fin;
     // fused kernel: NAM slots of F_NAMCAP per read
     gis_nam *nam_slots;

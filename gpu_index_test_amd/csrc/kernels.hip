@@ -826,80 +826,66 @@ __device__ __forceinline__ void wide_flush(const WideNam *wide, uint32_t nw, gis
     }
 }
 
-// Merge the nh hits of a read into NAMs written to out[0 .. total) in
-// canonical order (entries at index >= cap are not written). keys: scratch
-// of the next power of two >= nh entries when nh <= MERGE_LDS (bitonic
-// grouping), else of nh entries with keys_alt another nh (radix grouping).
-// wide: a list of nh entries for groups with more than 64 open NAMs, or
-// nullptr (then such a read returns MERGE_OVERFLOW).
-__device__ __forceinline__ uint32_t merge_wave(uint32_t nh, const DevHit *hits, uint64_t *keys,
-                                               uint64_t *keys_alt, WideNam *wide, gis_nam *out,
-                                               uint32_t cap) {
+// Grouped order of nh > 64 hits: nullptr when they already are in (strand,
+// ref) order, else keys (bitonic in keys, nh <= MERGE_LDS) or one of keys /
+// keys_alt (stable radix sort) holding group << 32 | hit index.
+__device__ __forceinline__ const uint64_t *group_order(uint32_t nh, const DevHit *hits, uint64_t *keys,
+                                                      uint64_t *keys_alt) {
     const uint32_t lane = lane_id();
-    // chunk registers: hit (x0 + lane) of the grouped order
-    uint32_t c_rsd = 0, c_rs = 0, c_re = 0, c_q = 0;
-    const uint64_t *order = nullptr;  // grouped order of hit indices (nh > 64), or identity
-    if (nh <= 64) {
-        DevHit h{};
-        if (lane < nh) h = hits[lane];
-        const uint32_t rsd = lane < nh ? h.ref_strand : 0xffffffffu;
-        const uint32_t prev = __shfl_up(rsd, 1, 64);
-        const bool uns = __ballot(lane > 0 && lane < nh && prev > rsd) != 0;
-        c_rsd = rsd;
-        c_rs = (uint32_t)h.ref_start;
-        c_re = (uint32_t)h.ref_end;
-        c_q = (uint32_t)h.query_start | ((uint32_t)h.query_end << 16);
-        if (uns) {
-            uint64_t key = lane < nh ? (((uint64_t)rsd << 32) | lane) : UINT64_MAX;
-            bitonic_regs(key);
-            const int src = (int)((uint32_t)key & 63u);
-            c_rsd = (uint32_t)__shfl((int)c_rsd, src, 64);
-            c_rs = (uint32_t)__shfl((int)c_rs, src, 64);
-            c_re = (uint32_t)__shfl((int)c_re, src, 64);
-            c_q = (uint32_t)__shfl((int)c_q, src, 64);
-        }
-    } else {
-        uint32_t unsorted = 0, max_ref = 0;
-        for (uint32_t i0 = 0; i0 < nh; i0 += 64) {
-            const uint32_t i = i0 + lane;
-            if (i < nh) {
-                const uint32_t rsd = hits[i].ref_strand;
-                max_ref = max_ref > (rsd & 0x7fffffffu) ? max_ref : (rsd & 0x7fffffffu);
-                if (i > 0 && hits[i - 1].ref_strand > rsd) unsorted = 1;
-            }
-        }
-        if (__ballot(unsorted != 0) != 0) {
-            if (nh <= (uint32_t)MERGE_LDS) {
-                uint32_t P = 1;
-                while (P < nh) P <<= 1;
-                for (uint32_t i = lane; i < P; i += 64)
-                    keys[i] = i < nh ? (((uint64_t)hits[i].ref_strand << 32) | i) : UINT64_MAX;
-                wave_sync_mem();
-                bitonic_mem(keys, P);
-                order = keys;
-            } else {
-                // compact group key: strand above the bits of the largest ref_id
-                const uint32_t mr = wave_max(max_ref);
-                const uint32_t rbits = mr ? 32u - (uint32_t)__builtin_clz(mr) : 0u;
-                for (uint32_t i = lane; i < nh; i += 64) {
-                    const uint32_t rsd = hits[i].ref_strand;
-                    const uint64_t g = ((uint64_t)(rsd >> 31) << rbits) | (rsd & 0x7fffffffu);
-                    keys[i] = (g << 32) | i;
-                }
-                wave_sync_mem();
-                order = radix_sort_groups(keys, keys_alt, nh, rbits + 1);
-            }
+    const uint64_t *order = nullptr;
+    uint32_t unsorted = 0, max_ref = 0;
+    for (uint32_t i0 = 0; i0 < nh; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        if (i < nh) {
+            const uint32_t rsd = hits[i].ref_strand;
+            max_ref = max_ref > (rsd & 0x7fffffffu) ? max_ref : (rsd & 0x7fffffffu);
+            if (i > 0 && hits[i - 1].ref_strand > rsd) unsorted = 1;
         }
     }
+    if (__ballot(unsorted != 0) != 0) {
+        if (nh <= (uint32_t)MERGE_LDS) {
+            uint32_t P = 1;
+            while (P < nh) P <<= 1;
+            for (uint32_t i = lane; i < P; i += 64)
+                keys[i] = i < nh ? (((uint64_t)hits[i].ref_strand << 32) | i) : UINT64_MAX;
+            wave_sync_mem();
+            bitonic_mem(keys, P);
+            order = keys;
+        } else {
+            // compact group key: strand above the bits of the largest ref_id
+            const uint32_t mr = wave_max(max_ref);
+            const uint32_t rbits = mr ? 32u - (uint32_t)__builtin_clz(mr) : 0u;
+            for (uint32_t i = lane; i < nh; i += 64) {
+                const uint32_t rsd = hits[i].ref_strand;
+                const uint64_t g = ((uint64_t)(rsd >> 31) << rbits) | (rsd & 0x7fffffffu);
+                keys[i] = (g << 32) | i;
+            }
+            wave_sync_mem();
+            order = radix_sort_groups(keys, keys_alt, nh, rbits + 1);
+        }
+    }
+    return order;
+}
 
+// Merge hits order[xb .. xe) (hit indices in grouped order; order nullptr:
+// the hits themselves in order), or, with pre, the xe <= 64 hits already in
+// the chunk registers. NAM c of the k-th group of the range is written at
+// out[(NAMs of the range's earlier groups) + c]; entries at index >= cap are
+// not written. Returns the NAM count, or MERGE_OVERFLOW when a group needs
+// more than 64 open NAMs and there is no wide list.
+__device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64_t *order, uint32_t xb,
+                                                uint32_t xe, bool pre, uint32_t c_rsd, uint32_t c_rs,
+                                                uint32_t c_re, uint32_t c_q, WideNam *wide, gis_nam *out,
+                                                uint32_t cap) {
+    const uint32_t lane = lane_id();
     LaneNam o{0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t n_open = 0, grp = 0xffffffffu, base = 0, gcnt = 0;
     bool wide_mode = false;
     uint32_t nw = 0, wide_mark = 0;
-    for (uint32_t x0 = 0; x0 < nh; x0 += 64) {
-        if (nh > 64) {
+    for (uint32_t x0 = xb; x0 < xe; x0 += 64) {
+        if (!pre) {
             const uint32_t i = x0 + lane;
-            if (i < nh) {
+            if (i < xe) {
                 const DevHit h = hits[order ? (uint32_t)order[i] : i];
                 c_rsd = h.ref_strand;
                 c_rs = (uint32_t)h.ref_start;
@@ -907,7 +893,7 @@ __device__ __forceinline__ uint32_t merge_wave(uint32_t nh, const DevHit *hits, 
                 c_q = (uint32_t)h.query_start | ((uint32_t)h.query_end << 16);
             }
         }
-        const uint32_t m = nh - x0 < 64 ? nh - x0 : 64;
+        const uint32_t m = xe - x0 < 64 ? xe - x0 : 64;
         for (uint32_t t = 0; t < m; t++) {
             const uint32_t rsd = readlane_u32(c_rsd, t);
             const int rs = (int)readlane_u32(c_rs, t);
@@ -1039,6 +1025,37 @@ __device__ __forceinline__ uint32_t merge_wave(uint32_t nh, const DevHit *hits, 
     return base + gcnt;
 }
 
+// Merge the nh hits of a read into NAMs written to out[0 .. total) in
+// canonical order (entries at index >= cap are not written). keys: scratch
+// of the next power of two >= nh entries when nh <= MERGE_LDS (bitonic
+// grouping), else of nh entries with keys_alt another nh (radix grouping).
+// wide: a list of nh entries for groups with more than 64 open NAMs, or
+// nullptr (then such a read returns MERGE_OVERFLOW).
+__device__ __forceinline__ uint32_t merge_wave(uint32_t nh, const DevHit *hits, uint64_t *keys,
+                                               uint64_t *keys_alt, WideNam *wide, gis_nam *out,
+                                               uint32_t cap) {
+    const uint32_t lane = lane_id();
+    if (nh > 64) return merge_range(hits, group_order(nh, hits, keys, keys_alt), 0, nh, false, 0, 0, 0, 0,
+                                    wide, out, cap);
+    DevHit h{};
+    if (lane < nh) h = hits[lane];
+    const uint32_t rsd = lane < nh ? h.ref_strand : 0xffffffffu;
+    const uint32_t prev = __shfl_up(rsd, 1, 64);
+    const bool uns = __ballot(lane > 0 && lane < nh && prev > rsd) != 0;
+    uint32_t c_rsd = rsd, c_rs = (uint32_t)h.ref_start, c_re = (uint32_t)h.ref_end;
+    uint32_t c_q = (uint32_t)h.query_start | ((uint32_t)h.query_end << 16);
+    if (uns) {
+        uint64_t key = lane < nh ? (((uint64_t)rsd << 32) | lane) : UINT64_MAX;
+        bitonic_regs(key);
+        const int src = (int)((uint32_t)key & 63u);
+        c_rsd = (uint32_t)__shfl((int)c_rsd, src, 64);
+        c_rs = (uint32_t)__shfl((int)c_rs, src, 64);
+        c_re = (uint32_t)__shfl((int)c_re, src, 64);
+        c_q = (uint32_t)__shfl((int)c_q, src, 64);
+    }
+    return merge_range(hits, nullptr, 0, nh, true, c_rsd, c_rs, c_re, c_q, wide, out, cap);
+}
+
 struct alignas(8) MergeLds {
     DevHit hits[MERGE_LDS];
     uint64_t keys[MERGE_LDS];
@@ -1047,6 +1064,10 @@ struct alignas(8) MergeLds {
 // reads handed from k_merge4 to k_merge (more than 64 hits, or more than
 // 16 NAMs on one strand)
 constexpr uint32_t MERGE_DEFER = 0xfffffffeu;
+// reads whose (strand, ref) groups merge on separate waves (k_merge_groups,
+// then k_merge_split_finish), for reads with more than SPLIT_MIN hits
+constexpr uint32_t MERGE_SPLIT = 0xfffffffdu;
+constexpr uint32_t SPLIT_MIN = 512;
 
 __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int rescue, int only_deferred) {
     __shared__ uint64_t lkeys[WPB][MERGE_LDS];
@@ -1071,6 +1092,46 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
         gis_nam *fin = (rescue ? b.rnam_fin : b.nam_fin) + hb;
         uint64_t *gkeys = b.merge_scratch ? (rescue ? b.rkeys : b.keys) + 2 * hb : nullptr;
         WideNam *gwide = b.merge_scratch ? (rescue ? b.rnam_ws : b.nam_ws) + hb : nullptr;
+        if (only_deferred && nh > SPLIT_MIN && b.merge_scratch) {
+            // group the hits; with several groups, each merges on its own wave
+            const uint64_t *ord = group_order(nh, ghits, gkeys, gkeys + nh);
+            const uint32_t code = ord == nullptr ? 0u : (ord == gkeys ? 1u : 2u);
+            uint64_t *slots = code == 1u ? gkeys + nh : gkeys;
+            uint32_t G = 0;
+            for (uint32_t i0 = 0; i0 < nh; i0 += 64) {
+                const uint32_t i = i0 + lane;
+                bool head = false;
+                if (i < nh) {
+                    const uint32_t cur = ghits[ord ? (uint32_t)ord[i] : i].ref_strand;
+                    head = i == 0 || ghits[ord ? (uint32_t)ord[i - 1] : i - 1].ref_strand != cur;
+                }
+                const uint64_t bal = __ballot(head);
+                if (head) slots[G + mbcnt(bal)] = (uint64_t)i << 32;
+                G += (uint32_t)__popcll(bal);
+            }
+            wave_sync_mem();
+            if (G >= 2) {
+                uint32_t at = 0;
+                if (lane == 0) at = atomicAdd(&b.ctl->n_gwork[rescue ? 1 : 0], G);
+                at = (uint32_t)__builtin_amdgcn_readfirstlane((int)at);
+                for (uint32_t g = lane; g < G; g += 64) {
+                    const uint32_t gs = (uint32_t)(slots[g] >> 32);
+                    const uint32_t ge = g + 1 < G ? (uint32_t)(slots[g + 1] >> 32) : nh;
+                    b.gwork[at + g] = make_uint4(r, gs, ge, g | (code << 30));
+                }
+                if (lane == 0) {
+                    const uint32_t si = atomicAdd(&b.ctl->n_split[rescue ? 1 : 0], 1u);
+                    b.split_list[si] = make_uint4(r, G, code, 0u);
+                    b.nam_cnt[r] = MERGE_SPLIT;
+                }
+                wave_sync_mem();
+                continue;
+            }
+            const uint32_t t1 = merge_range(ghits, ord, 0, nh, false, 0, 0, 0, 0, gwide, fin, nh);
+            if (lane == 0) b.nam_cnt[r] = t1;
+            wave_sync_mem();
+            continue;
+        }
         uint32_t total = MERGE_OVERFLOW;
         if (nh <= (uint32_t)MERGE_LDS)
             total = merge_wave(nh, ghits, lkeys[wid], nullptr, gwide, fin, nh);
@@ -1085,6 +1146,65 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
             continue;
         }
         if (lane == 0) b.nam_cnt[r] = total;
+        wave_sync_mem();
+    }
+}
+
+// one wave per group work item: the group's hits (in grouped order) merge
+// into NAMs at the group's own hit-indexed region of the read's output; the
+// NAM count goes to the read's group slot
+__global__ __launch_bounds__(64 * WPB) void k_merge_groups(DevBatch b, int rescue) {
+    const uint32_t lane = lane_id();
+    const unsigned int cnt = b.ctl->n_gwork[rescue ? 1 : 0];
+    for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
+        const uint4 it = b.gwork[w];
+        const uint32_t r = it.x, gs = it.y, ge = it.z, g = it.w & 0x3fffffffu, code = it.w >> 30;
+        const uint64_t hb = rescue ? b.rhit_off[r] : b.hit_off[r];
+        const uint32_t nh = rescue ? b.rhit_total[r] : b.hit_total[r];
+        uint64_t *gkeys = (rescue ? b.rkeys : b.keys) + 2 * hb;
+        const uint64_t *ord = code == 0u ? nullptr : (code == 1u ? gkeys : gkeys + nh);
+        uint64_t *slots = code == 1u ? gkeys + nh : gkeys;
+        const DevHit *ghits = (rescue ? b.rhits : b.hits) + hb;
+        WideNam *gwide = (rescue ? b.rnam_ws : b.nam_ws) + hb + gs;
+        gis_nam *out = (rescue ? b.rnam_fin : b.nam_fin) + hb + gs;
+        const uint32_t c = merge_range(ghits, ord, gs, ge, false, 0, 0, 0, 0, gwide, out, ge - gs);
+        if (lane == 0) slots[g] = ((uint64_t)gs << 32) | c;
+        wave_sync_mem();
+    }
+}
+
+// one wave per split read: the groups' NAMs move to consecutive places in
+// group order (canonical order) with nam_id = rank
+__global__ __launch_bounds__(64 * WPB) void k_merge_split_finish(DevBatch b, int rescue) {
+    const uint32_t lane = lane_id();
+    const unsigned int cnt = b.ctl->n_split[rescue ? 1 : 0];
+    for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
+        const uint4 e = b.split_list[w];
+        const uint32_t r = e.x, G = e.y, code = e.z;
+        const uint64_t hb = rescue ? b.rhit_off[r] : b.hit_off[r];
+        const uint32_t nh = rescue ? b.rhit_total[r] : b.hit_total[r];
+        uint64_t *gkeys = (rescue ? b.rkeys : b.keys) + 2 * hb;
+        const uint64_t *slots = code == 1u ? gkeys + nh : gkeys;
+        gis_nam *fin = (rescue ? b.rnam_fin : b.nam_fin) + hb;
+        uint32_t base = 0;
+        for (uint32_t g = 0; g < G; g++) {
+            const uint64_t sg = slots[g];
+            const uint32_t gs = (uint32_t)(sg >> 32), c = (uint32_t)sg;
+            // base <= gs: every NAM moves towards the front, chunk by chunk
+            for (uint32_t c0 = 0; c0 < c; c0 += 64) {
+                const uint32_t k = c0 + lane;
+                gis_nam x{};
+                if (k < c) x = fin[gs + k];
+                wave_sync_mem();
+                if (k < c) {
+                    x.nam_id = (int32_t)(base + k);
+                    fin[base + k] = x;
+                }
+                wave_sync_mem();
+            }
+            base += c;
+        }
+        if (lane == 0) b.nam_cnt[r] = base;
         wave_sync_mem();
     }
 }
@@ -1402,6 +1522,8 @@ __global__ void k_reset_ctl(DevCtl *ctl) {
         ctl->overflow = 0;
         ctl->q_head[0] = 0;
         ctl->q_head[1] = 0;
+        ctl->n_gwork[0] = ctl->n_gwork[1] = 0;
+        ctl->n_split[0] = ctl->n_split[1] = 0;
     }
 }
 
@@ -1537,6 +1659,10 @@ void launch_merge(const DevBatch &b, const ReadSet &set, int rescue, hipStream_t
     const unsigned g = grid_for_set(b, set, WPB);
     hipLaunchKernelGGL(k_merge4, dim3((g + RPW - 1) / RPW), dim3(64 * WPB), 0, st, b, set, rescue);
     hipLaunchKernelGGL(k_merge, dim3(g), dim3(64 * WPB), 0, st, b, set, rescue, 1);
+    if (b.merge_scratch) {
+        hipLaunchKernelGGL(k_merge_groups, dim3(1024), dim3(64 * WPB), 0, st, b, rescue);
+        hipLaunchKernelGGL(k_merge_split_finish, dim3(256), dim3(64 * WPB), 0, st, b, rescue);
+    }
 }
 
 // fused kernel over queue[0 .. count): FWPB waves per block, LMAX the LDS

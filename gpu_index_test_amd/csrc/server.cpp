@@ -1,5 +1,5 @@
 // C ABI: the serving loop. Requests (sets of reads) are queued by the
-// caller; a worker thread admits them into device slots by an HBM budget,
+// caller; slot threads admit them into device slots by an HBM budget,
 // coalesces the queued requests into one launch sequence per slot (a step,
 // replayed from the slot's captured hipGraph for its padded size), keeps up
 // to `slots` steps in flight on separate streams, and hands each request its
@@ -26,6 +26,35 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
+// grow-only pinned host buffer (fast, asynchronous copies to and from HBM)
+template <typename T>
+struct Pinned {
+    T *p = nullptr;
+    size_t cap = 0, n = 0;
+    void reserve(size_t want) {
+        if (want <= cap) return;
+        const size_t c = std::max(want, cap + cap / 2);
+        T *q = nullptr;
+        if (hipHostMalloc((void **)&q, c * sizeof(T)) != hipSuccess) throw HipError{hipErrorOutOfMemory, "hipHostMalloc"};
+        if (p && n) std::memcpy(q, p, n * sizeof(T));
+        if (p) (void)hipHostFree(p);
+        p = q;
+        cap = c;
+    }
+    void resize(size_t m) {
+        reserve(m);
+        n = m;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = n = 0;
+    }
+    T *data() { return p; }
+    size_t size() const { return n; }
+    T &operator[](size_t i) { return p[i]; }
+};
+
 struct Request {
     uint64_t id = 0;
     std::vector<char> seqs;
@@ -48,8 +77,25 @@ struct Slot {
     std::vector<uint32_t> first;  // first read of each request in the step
     uint32_t n_reads = 0;         // reads of the step before padding
     uint64_t bytes = 0;
-    bool busy = false;
     RunOpts r{};
+    std::thread th;
+    // pinned host staging of the slot's thread
+    Pinned<char> st_seqs;
+    Pinned<uint64_t> st_offs;
+    Pinned<uint32_t> st_lens;
+    Pinned<uint64_t> h_off;
+    Pinned<gis_nam> h_nams;
+    Pinned<float> h_frac;
+    Pinned<uint8_t> h_flags;
+    void release() {
+        st_seqs.release();
+        st_offs.release();
+        st_lens.release();
+        h_off.release();
+        h_nams.release();
+        h_frac.release();
+        h_flags.release();
+    }
 };
 
 // device bytes a request holds while in flight: randstrobe slots and
@@ -69,7 +115,7 @@ uint32_t padded(uint32_t n) {
 struct gis_server {
     gis_ctx *ctx = nullptr;
     gis_server_config cfg{};
-    std::vector<Slot> slots;
+    std::vector<Slot> slots;  // sized once; each slot runs on a thread of its own
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request *> pending;
@@ -77,23 +123,14 @@ struct gis_server {
     uint64_t next_id = 1;
     uint64_t bytes_in_flight = 0;
     bool stop = false;
-    std::thread worker;
     gis_server_stats stats{};
     Clock::time_point busy_since;
     int n_busy = 0;
-    // host staging of the worker
-    std::vector<char> st_seqs;
-    std::vector<uint64_t> st_offs;
-    std::vector<uint32_t> st_lens;
-    std::vector<uint64_t> h_off;
-    std::vector<gis_nam> h_nams;
-    std::vector<float> h_frac;
-    std::vector<uint8_t> h_flags;
 };
 
 namespace {
 
-// (the worker marks the requests done under the server lock afterwards)
+// (the slot's thread marks the requests done under the server lock afterwards)
 void fail_step(gis_server *s, Slot &sl, const std::string &msg) {
     (void)s;
     for (Request *q : sl.reqs) {
@@ -103,146 +140,116 @@ void fail_step(gis_server *s, Slot &sl, const std::string &msg) {
     }
 }
 
-// stage, upload and launch the requests of a slot (the worker holds the
-// context lock)
+// stage, upload and launch the requests of a slot (on the slot's thread)
 void launch_step(gis_server *s, Slot &sl) {
     const uint32_t n = sl.n_reads, np = padded(n);
-    s->st_seqs.clear();
-    s->st_offs.resize(np);
-    s->st_lens.resize(np);
+    uint64_t bases = 0;
+    for (Request *q : sl.reqs) bases += q->seqs.size();
+    sl.st_seqs.resize(bases + 1);
+    sl.st_offs.resize(np);
+    sl.st_lens.resize(np);
     sl.first.clear();
     uint32_t r = 0;
+    uint64_t at = 0;
     for (Request *q : sl.reqs) {
         sl.first.push_back(r);
+        if (!q->seqs.empty()) std::memcpy(sl.st_seqs.data() + at, q->seqs.data(), q->seqs.size());
         for (size_t i = 0; i < q->lens.size(); i++, r++) {
-            s->st_offs[r] = s->st_seqs.size();
-            s->st_lens[r] = q->lens[i];
-            s->st_seqs.insert(s->st_seqs.end(), q->seqs.begin() + (int64_t)q->offs[i],
-                              q->seqs.begin() + (int64_t)(q->offs[i] + q->lens[i]));
+            sl.st_offs[r] = at + q->offs[i];
+            sl.st_lens[r] = q->lens[i];
         }
+        at += q->seqs.size();
     }
     for (; r < np; r++) {  // empty reads: no randstrobes, no NAMs
-        s->st_offs[r] = s->st_seqs.size();
-        s->st_lens[r] = 0;
+        sl.st_offs[r] = at;
+        sl.st_lens[r] = 0;
     }
-    if (s->st_seqs.empty()) s->st_seqs.push_back('\0');
-    const int rc = batch_upload(sl.b, s->st_seqs.data(), s->st_offs.data(), s->st_lens.data(), np, false);
+    const int rc = batch_upload(sl.b, sl.st_seqs.data(), sl.st_offs.data(), sl.st_lens.data(), np, false);
     if (rc) throw HipError{hipErrorInvalidValue, "server upload failed"};
     sl.r = run_opts(sl.b, s->cfg.options, 0, STOP_NONE);
     batch_launch(sl.b, sl.r);
 }
 
-// results of a finished slot into its requests (the worker holds the context lock)
+// wait for the slot's step and split its results into its requests (on the
+// slot's thread)
 void complete_step(gis_server *s, Slot &sl) {
+    (void)s;
     batch_complete(sl.b, sl.r);
     const uint32_t np = sl.b->n_reads;
-    s->h_off.resize(np + 1);
-    s->h_nams.resize(std::max<uint64_t>(1, sl.b->tot_nams));
-    s->h_frac.resize(np);
-    s->h_flags.resize(np);
-    gis_nam_out o{s->h_off.data(), s->h_nams.data(), s->h_nams.size(), s->h_frac.data(), s->h_flags.data(), 0};
+    sl.h_off.resize(np + 1);
+    sl.h_nams.resize(std::max<uint64_t>(1, sl.b->ws->tot_nams));
+    sl.h_frac.resize(np);
+    sl.h_flags.resize(np);
+    gis_nam_out o{sl.h_off.data(), sl.h_nams.data(), sl.h_nams.size(), sl.h_frac.data(), sl.h_flags.data(), 0};
     if (batch_download(sl.b, &o)) throw HipError{hipErrorInvalidValue, "server download failed"};
     const Clock::time_point now = Clock::now();
     for (size_t j = 0; j < sl.reqs.size(); j++) {
         Request *q = sl.reqs[j];
         const uint32_t f = sl.first[j], k = (uint32_t)q->lens.size();
-        const uint64_t a = s->h_off[f], e = s->h_off[f + k];
+        const uint64_t a = sl.h_off[f], e = sl.h_off[f + k];
         q->nam_off.resize(k + 1);
-        for (uint32_t i = 0; i <= k; i++) q->nam_off[i] = s->h_off[f + i] - a;
-        q->nams.assign(s->h_nams.begin() + (int64_t)a, s->h_nams.begin() + (int64_t)e);
-        q->frac.assign(s->h_frac.begin() + f, s->h_frac.begin() + f + k);
-        q->flags.assign(s->h_flags.begin() + f, s->h_flags.begin() + f + k);
+        for (uint32_t i = 0; i <= k; i++) q->nam_off[i] = sl.h_off[f + i] - a;
+        q->nams.assign(sl.h_nams.data() + a, sl.h_nams.data() + e);
+        q->frac.assign(sl.h_frac.data() + f, sl.h_frac.data() + f + k);
+        q->flags.assign(sl.h_flags.data() + f, sl.h_flags.data() + f + k);
         q->t_done = now;
     }
 }
 
-void worker_main(gis_server *s) {
+// One thread per slot: take the queued requests that fit (arrival order,
+// max_step_reads, HBM budget), stage and launch them as one step, wait for
+// it on the slot's stream, and complete the requests. Slots run their host
+// work and their device work independently, so steps overlap on the GPU and
+// with each other's copies.
+void slot_main(gis_server *s, size_t si) {
     (void)hipSetDevice(s->ctx->device);
+    Slot &sl = s->slots[si];
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
-        bool progress = false;
-        // completions
-        for (Slot &sl : s->slots) {
-            if (!sl.busy || !batch_ready(sl.b)) continue;
-            lk.unlock();
-            try {
-                std::lock_guard<std::recursive_mutex> cl(s->ctx->mu);
-                complete_step(s, sl);
-            } catch (const HipError &e) {
-                fail_step(s, sl, e.what + ": " + hipGetErrorString(e.e));
-            } catch (const std::bad_alloc &) {
-                fail_step(s, sl, "host allocation failed");
-            }
-            lk.lock();
-            for (Request *q : sl.reqs) q->done = true;
-            sl.busy = false;
-            sl.reqs.clear();
-            s->bytes_in_flight -= sl.bytes;
-            if (--s->n_busy == 0)
-                s->stats.busy_s += std::chrono::duration<double>(Clock::now() - s->busy_since).count();
-            progress = true;
-            s->cv_done.notify_all();
+        s->cv_work.wait(lk, [&] {
+            return s->stop || (!s->pending.empty() &&
+                               s->bytes_in_flight + s->pending.front()->bytes <= s->cfg.hbm_budget);
+        });
+        if (s->stop) break;
+        uint32_t n = 0;
+        uint64_t bytes = 0;
+        std::vector<Request *> take;
+        while (!s->pending.empty()) {
+            Request *q = s->pending.front();
+            const uint32_t k = (uint32_t)q->lens.size();
+            if (!take.empty() && n + k > s->cfg.max_step_reads) break;
+            if (s->bytes_in_flight + bytes + q->bytes > s->cfg.hbm_budget) break;
+            take.push_back(q);
+            n += k;
+            bytes += q->bytes;
+            s->pending.pop_front();
         }
-        // admission: free slots take queued requests in arrival order while
-        // the step stays within max_step_reads and the HBM budget
-        for (Slot &sl : s->slots) {
-            if (sl.busy || s->pending.empty()) continue;
-            uint32_t n = 0;
-            uint64_t bytes = 0;
-            std::vector<Request *> take;
-            while (!s->pending.empty()) {
-                Request *q = s->pending.front();
-                const uint32_t k = (uint32_t)q->lens.size();
-                if (!take.empty() && n + k > s->cfg.max_step_reads) break;
-                if (s->bytes_in_flight + bytes + q->bytes > s->cfg.hbm_budget) break;
-                take.push_back(q);
-                n += k;
-                bytes += q->bytes;
-                s->pending.pop_front();
-            }
-            if (take.empty()) break;  // the budget is held by steps in flight
-            sl.reqs = std::move(take);
-            sl.n_reads = n;
-            sl.bytes = bytes;
-            sl.busy = true;
-            s->bytes_in_flight += bytes;
-            s->stats.peak_bytes = std::max(s->stats.peak_bytes, s->bytes_in_flight);
-            if (s->n_busy++ == 0) s->busy_since = Clock::now();
-            s->stats.steps++;
-            lk.unlock();
-            bool failed = false;
-            try {
-                std::lock_guard<std::recursive_mutex> cl(s->ctx->mu);
-                launch_step(s, sl);
-            } catch (const HipError &e) {
-                failed = true;
-                fail_step(s, sl, e.what + ": " + hipGetErrorString(e.e));
-                (void)hipStreamSynchronize(sl.b->stream);
-            } catch (const std::bad_alloc &) {
-                failed = true;
-                fail_step(s, sl, "host allocation failed");
-            }
-            lk.lock();
-            if (failed) {
-                for (Request *q : sl.reqs) q->done = true;
-                sl.busy = false;
-                sl.reqs.clear();
-                s->bytes_in_flight -= sl.bytes;
-                if (--s->n_busy == 0)
-                    s->stats.busy_s += std::chrono::duration<double>(Clock::now() - s->busy_since).count();
-                s->cv_done.notify_all();
-            }
-            progress = true;
+        sl.reqs = std::move(take);
+        sl.n_reads = n;
+        sl.bytes = bytes;
+        s->bytes_in_flight += bytes;
+        s->stats.peak_bytes = std::max(s->stats.peak_bytes, s->bytes_in_flight);
+        if (s->n_busy++ == 0) s->busy_since = Clock::now();
+        s->stats.steps++;
+        lk.unlock();
+        try {
+            launch_step(s, sl);
+            complete_step(s, sl);
+        } catch (const HipError &e) {
+            fail_step(s, sl, e.what + ": " + hipGetErrorString(e.e));
+            (void)hipStreamSynchronize(sl.b->stream);
+        } catch (const std::bad_alloc &) {
+            fail_step(s, sl, "host allocation failed");
+            (void)hipStreamSynchronize(sl.b->stream);
         }
-        if (s->stop && s->n_busy == 0) break;
-        if (s->n_busy == 0 && s->pending.empty()) {
-            s->cv_work.wait(lk);
-        } else if (!progress) {
-            // steps in flight: poll their streams
-            lk.unlock();
-            std::this_thread::sleep_for(std::chrono::microseconds(20));
-            lk.lock();
-        }
+        lk.lock();
+        for (Request *q : sl.reqs) q->done = true;
+        sl.reqs.clear();
+        s->bytes_in_flight -= sl.bytes;
+        if (--s->n_busy == 0)
+            s->stats.busy_s += std::chrono::duration<double>(Clock::now() - s->busy_since).count();
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();  // budget freed
     }
 }
 
@@ -287,18 +294,16 @@ extern "C" int gis_server_create(gis_ctx *ctx, const gis_server_config *cfg, gis
     gis_server *s = new gis_server();
     s->ctx = ctx;
     s->cfg = c;
+    s->slots.resize(c.slots);
     try {
-        for (uint32_t i = 0; i < c.slots; i++) {
-            Slot sl;
-            sl.b = batch_new_stream(ctx);
-            s->slots.push_back(sl);
-        }
+        for (Slot &sl : s->slots) sl.b = batch_new_stream(ctx);
     } catch (...) {
-        for (Slot &sl : s->slots) batch_free(sl.b);
+        for (Slot &sl : s->slots)
+            if (sl.b) batch_free(sl.b);
         delete s;
         throw;
     }
-    s->worker = std::thread(worker_main, s);
+    for (size_t i = 0; i < s->slots.size(); i++) s->slots[i].th = std::thread(slot_main, s, i);
     *out = s;
     return GIS_OK;
     API_END(ctx)
@@ -317,13 +322,14 @@ extern "C" void gis_server_destroy(gis_server *s) {
         s->pending.clear();
     }
     s->cv_work.notify_all();
-    s->worker.join();
+    for (Slot &sl : s->slots) sl.th.join();
     {
         std::lock_guard<std::recursive_mutex> cl(s->ctx->mu);
         (void)hipSetDevice(s->ctx->device);
         for (Slot &sl : s->slots) {
             (void)hipStreamSynchronize(sl.b->stream);
             batch_free(sl.b);
+            sl.release();
         }
     }
     for (auto &kv : s->reqs) delete kv.second;

@@ -241,6 +241,13 @@ int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
 typedef struct gis_batch gis_batch;
 /* Capacity for up to max_reads reads with max_bases bases in total. */
 int gis_batch_create(gis_ctx *ctx, uint32_t max_reads, uint64_t max_bases, gis_batch **out);
+/* A batch that runs on the same device workspace (intermediates and results)
+ * as share_with, and on its stream: for many resident batches run one after
+ * another, the workspace is sized once instead of per batch. Batches sharing
+ * a workspace are run one at a time and a run's results are downloaded
+ * before another of them runs (GIS_ERR_STATE otherwise). */
+int gis_batch_create_shared(gis_ctx *ctx, uint32_t max_reads, uint64_t max_bases, gis_batch *share_with,
+                            gis_batch **out);
 void gis_batch_destroy(gis_batch *b);
 int gis_batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets,
                      const uint32_t *lens, uint32_t n_reads);
@@ -266,11 +273,11 @@ int gis_batch_autotune(gis_batch *b, const gis_map_options *opt, gis_fused_confi
 
 /* ---------------- serving (no reference counterpart) ---------------- */
 /* A request loop on one context: requests (sets of reads) are queued by the
- * caller; a worker thread admits them into device slots while the estimated
- * device bytes of the steps in flight stay within hbm_budget, coalesces the
- * queued requests into one launch sequence per slot (a step, replayed from
- * the slot's captured hipGraph), keeps up to `slots` steps in flight on
- * separate streams, and completes each request with its per-read NAMs.
+ * caller; one thread per slot admits them while the estimated device bytes
+ * of the steps in flight stay within hbm_budget, coalesces the queued
+ * requests into one launch sequence (a step, replayed from the slot's
+ * captured hipGraph), keeps up to `slots` steps in flight on separate
+ * streams, and completes each request with its per-read NAMs.
  * Requests arriving while steps run join the next step on a free slot. A
  * request's NAMs are the same alone and batched with others. The context's
  * index must stay loaded while the server exists. */

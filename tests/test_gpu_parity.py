@@ -472,3 +472,31 @@ def test_heavy_repeat_reads(ctx):
                 most = max(most, int(((g["query_start"] <= x) & (g["query_end"] >= x)).sum()))
     assert most > 64, most
     assert (efl & 1).sum() > 10
+
+
+def test_shared_workspace_batches(setup):
+    """Batches that share one device workspace give their own NAMs, one run
+    and download at a time; results overwritten by another batch's run are
+    refused."""
+    g, lens, p, idx, op, oidx = setup
+    sets = [_read_set(g, lens, 200, seed=s) for s in (101, 102)]
+    packs = [G.pack_reads(r) for r in sets]
+    exp = [O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8) for s, o, l in packs]
+    a = G.Mapper(idx, len(packs[0][2]), len(packs[0][0]))
+    b = G.Mapper(idx, len(packs[1][2]), len(packs[1][0]), share=a)
+    try:
+        a.upload(*packs[0])
+        b.upload(*packs[1])
+        for mode in (G.MODE_PLAIN, G.MODE_FUSED):
+            for m, e in ((a, exp[0]), (b, exp[1]), (a, exp[0])):
+                m.run(mode=mode)
+                res = m.download()
+                _compare_nams((res.nam_offsets, res.nams), (e[0], e[1]), res.nonrepetitive_fraction, e[2],
+                              res.flags, e[3])
+        a.run()
+        b.run()
+        with pytest.raises(G.GisError):
+            a.download()
+    finally:
+        b.close()
+        a.close()
