@@ -49,7 +49,7 @@ static void workspace_release(Workspace *w) {
                      &w->hit_total, &w->rhit_total, &w->frac, &w->flags, &w->nam_cnt, &w->hit_off,
                      &w->rhit_off, &w->hits, &w->rhits, &w->keys, &w->rkeys, &w->nam_ws, &w->rnam_ws,
                      &w->nam_fin, &w->rnam_fin, &w->nam_slots, &w->nam_off, &w->nams_out, &w->scan_tmp,
-                     &w->ctl, &w->spill_ids, &w->gwork, &w->split_list};
+                     &w->ctl, &w->spill_ids, &w->gwork, &w->split_list, &w->sel_ids};
     for (DevBuf *x : all) release(*x);
     if (w->h_ctl) (void)hipHostFree(w->h_ctl);
     if (w->h_total) (void)hipHostFree(w->h_total);
@@ -126,6 +126,7 @@ DevBatch make_devbatch(gis_batch *b) {
     d.merge_scratch = b->ws->merge_scratch ? 1u : 0u;
     d.gwork = static_cast<uint4 *>(b->ws->gwork.p);
     d.split_list = static_cast<uint4 *>(b->ws->split_list.p);
+    d.sel_ids = static_cast<uint32_t *>(b->ws->sel_ids.p);
     d.nam_fin = static_cast<gis_nam *>(b->ws->nam_fin.p);
     d.rnam_fin = static_cast<gis_nam *>(b->ws->rnam_fin.p);
     d.nam_slots = static_cast<gis_nam *>(b->ws->nam_slots.p);
@@ -167,6 +168,7 @@ void ensure_read_arrays(gis_batch *b, uint32_t n) {
     ensure<DevCtl>(b->ws->ctl, 1);
     ensure<uint32_t>(b->ws->spill_ids, n);
     ensure<uint4>(b->ws->split_list, n);
+    ensure<uint32_t>(b->ws->sel_ids, 2ull * n + 2);
     if (!b->ws->h_ctl) HIP_CHECK(hipHostMalloc((void **)&b->ws->h_ctl, sizeof(DevCtl)));
     if (!b->ws->h_total) HIP_CHECK(hipHostMalloc((void **)&b->ws->h_total, 3 * sizeof(uint64_t)));
 }

@@ -20,7 +20,7 @@ struct Workspace {
     DevBuf pos, hcnt, rcnt, lk_bytes;
     DevBuf hit_total, rhit_total, frac, flags, nam_cnt, hit_off, rhit_off;
     DevBuf hits, rhits, keys, rkeys, nam_ws, rnam_ws, nam_fin, rnam_fin;
-    DevBuf nam_slots, nam_off, nams_out, scan_tmp, ctl, spill_ids, gwork, split_list;
+    DevBuf nam_slots, nam_off, nams_out, scan_tmp, ctl, spill_ids, gwork, split_list, sel_ids;
     uint64_t hit_cap = 0, rhit_cap = 0, out_cap = 0;
     bool merge_scratch = false;  // keys/nam_ws/gwork allocated (reads with many hits need them)
     // results of the last run

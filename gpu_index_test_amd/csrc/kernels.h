@@ -46,6 +46,7 @@ struct DevCtl {
     unsigned int q_head[2]; // dynamic work-queue heads of the fused kernel's two bins
     unsigned int n_gwork[2];  // group work items of split reads (normal, rescue merge)
     unsigned int n_split[2];  // split reads (normal, rescue merge)
+    unsigned int n_sel[2];    // selected reads: rescued, deferred merges
 };
 
 enum : unsigned { OVF_HITS = 1, OVF_RHITS = 2, OVF_MERGE = 4, OVF_OUT = 8 };
@@ -106,6 +107,9 @@ struct DevBatch {
     // entries {read, groups, order code, 0}
     uint4 *gwork;
     uint4 *split_list;
+    // ids of the reads of the sparse passes: rescued at [0, n), deferred
+    // merges at [n, 2n)
+    uint32_t *sel_ids;
     gis_nam *nam_fin, *rnam_fin;
     // fused kernel: NAM slots of F_NAMCAP per read
     gis_nam *nam_slots;

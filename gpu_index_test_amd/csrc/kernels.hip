@@ -67,6 +67,10 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
     return x - v;
 }
 
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+
 // wave-uniform iteration over a ReadSet
 __device__ __forceinline__ uint32_t set_count(const ReadSet &s) { return s.ids ? *s.n_dev : s.n; }
 __device__ __forceinline__ uint32_t set_at(const ReadSet &s, uint32_t w) { return s.ids ? s.ids[w] : w; }
@@ -641,6 +645,7 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_count(DevBatch b, ReadSet s
             total += wave_sum(rh);
         }
         if (lane == 0) b.rhit_total[r] = total;
+        wave_sync_mem();
     }
 }
 
@@ -799,10 +804,6 @@ __device__ __forceinline__ LaneNam from_wide(const WideNam &w) {
 
 __device__ __forceinline__ int permute_push(int v, uint32_t dst) {
     return __builtin_amdgcn_ds_permute((int)(dst << 2), v);
-}
-
-__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
 }
 
 // does open NAM o accept hit (qs, qe, rs, re), and does the hit extend it
@@ -1069,7 +1070,8 @@ constexpr uint32_t MERGE_DEFER = 0xfffffffeu;
 constexpr uint32_t MERGE_SPLIT = 0xfffffffdu;
 constexpr uint32_t SPLIT_MIN = 512;
 
-__global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int rescue, int only_deferred) {
+__global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int rescue, int only_deferred,
+                                                    int split) {
     __shared__ uint64_t lkeys[WPB][MERGE_LDS];
     const uint32_t lane = lane_id();
     const int wid = (int)(threadIdx.x >> 6);
@@ -1092,7 +1094,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
         gis_nam *fin = (rescue ? b.rnam_fin : b.nam_fin) + hb;
         uint64_t *gkeys = b.merge_scratch ? (rescue ? b.rkeys : b.keys) + 2 * hb : nullptr;
         WideNam *gwide = b.merge_scratch ? (rescue ? b.rnam_ws : b.nam_ws) + hb : nullptr;
-        if (only_deferred && nh > SPLIT_MIN && b.merge_scratch) {
+        if (split && nh > SPLIT_MIN && b.merge_scratch) {
             // group the hits; with several groups, each merges on its own wave
             const uint64_t *ord = group_order(nh, ghits, gkeys, gkeys + nh);
             const uint32_t code = ord == nullptr ? 0u : (ord == gkeys ? 1u : 2u);
@@ -1516,6 +1518,41 @@ __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
     }
 }
 
+// Ids of the set's reads that a sparse pass visits, in set order within a
+// block: SEL_RESCUED (rescued reads) or SEL_DEFERRED (normal reads the
+// four-per-wave merge handed on). One atomic per block places a block's ids.
+enum { SEL_RESCUED = 0, SEL_DEFERRED = 1 };
+
+__global__ __launch_bounds__(256) void k_select(DevBatch b, ReadSet set, int kind) {
+    __shared__ uint32_t wcount[4];
+    __shared__ uint32_t block_base;
+    const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    const uint32_t cnt = set_count(set);
+    uint32_t *out = b.sel_ids + (kind ? b.n_reads : 0u);
+    for (uint32_t base = blockIdx.x * 256u; base < cnt; base += gridDim.x * 256u) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t r = 0;
+        bool m = false;
+        if (i < cnt) {
+            r = set_at(set, i);
+            const bool rescued = (b.flags[r] & GIS_FLAG_RESCUED) != 0;
+            m = kind == SEL_RESCUED ? rescued : (!rescued && b.nam_cnt[r] == MERGE_DEFER);
+        }
+        const uint64_t bal = __ballot(m);
+        if (lane == 0) wcount[wid] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t t = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+            block_base = t ? atomicAdd(&b.ctl->n_sel[kind], t) : 0u;
+        }
+        __syncthreads();
+        uint32_t off = block_base;
+        for (uint32_t w = 0; w < wid; w++) off += wcount[w];
+        if (m) out[off + mbcnt(bal)] = r;
+        __syncthreads();
+    }
+}
+
 __global__ void k_reset_ctl(DevCtl *ctl) {
     if (threadIdx.x == 0) {
         ctl->n_spill = 0;
@@ -1524,6 +1561,7 @@ __global__ void k_reset_ctl(DevCtl *ctl) {
         ctl->q_head[1] = 0;
         ctl->n_gwork[0] = ctl->n_gwork[1] = 0;
         ctl->n_split[0] = ctl->n_split[1] = 0;
+        ctl->n_sel[0] = ctl->n_sel[1] = 0;
     }
 }
 
@@ -1617,6 +1655,23 @@ static inline unsigned grid_for_set(const DevBatch &b, const ReadSet &set, unsig
     return g < 512u ? g : 512u;
 }
 
+// the reads a sparse pass visits: the ids k_select wrote
+static inline ReadSet selected(const DevBatch &b, int kind) {
+    return ReadSet{b.sel_ids + (kind ? b.n_reads : 0u), &b.ctl->n_sel[kind], b.n_reads};
+}
+
+// blocks for a per-read kernel over a selected list (its length is on the
+// device): enough waves to run every heavy read of the list at once
+static inline unsigned grid_selected(const DevBatch &b) {
+    const unsigned g = grid_for(b.n_reads, WPB);
+    return g < 2048u ? (g ? g : 1u) : 2048u;
+}
+
+static void launch_select(const DevBatch &b, const ReadSet &set, int kind, hipStream_t st) {
+    const unsigned g = grid_for(b.n_reads, 256);
+    hipLaunchKernelGGL(k_select, dim3(g < 1024u ? (g ? g : 1u) : 1024u), dim3(256), 0, st, b, set, kind);
+}
+
 void launch_reset_ctl(DevCtl *ctl, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(64), 0, st, ctl);
 }
@@ -1645,20 +1700,34 @@ void launch_hits(const DevBatch &b, const ReadSet &set, const DevIndex &ix, hipS
 void launch_rescue_count(const DevBatch &b, const ReadSet &set, const DevIndex &ix,
                          uint32_t rescue_cutoff, uint32_t hard_cutoff, hipStream_t st) {
     if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_rescue_count, dim3(grid_for_set(b, set, WPB)), dim3(64 * WPB), 0, st, b, set,
-                       ix, rescue_cutoff, hard_cutoff);
+    launch_select(b, set, SEL_RESCUED, st);
+    hipLaunchKernelGGL(k_rescue_count, dim3(grid_selected(b)), dim3(64 * WPB), 0, st, b,
+                       selected(b, SEL_RESCUED), ix, rescue_cutoff, hard_cutoff);
 }
 
+// (after launch_rescue_count, which selects the rescued reads)
 void launch_rescue_hits(const DevBatch &b, const ReadSet &set, const DevIndex &ix, hipStream_t st) {
+    (void)set;
     if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_rescue_hits, dim3(grid_for_set(b, set, WPB)), dim3(64 * WPB), 0, st, b, set, ix);
+    hipLaunchKernelGGL(k_rescue_hits, dim3(grid_selected(b)), dim3(64 * WPB), 0, st, b,
+                       selected(b, SEL_RESCUED), ix);
 }
 
+// normal pass: four reads per wave over the set, then the reads it handed
+// on, one per wave; rescue pass: the rescued reads one per wave; then the
+// (strand, ref) groups of split reads and their finish
 void launch_merge(const DevBatch &b, const ReadSet &set, int rescue, hipStream_t st) {
     if (b.n_reads == 0) return;
-    const unsigned g = grid_for_set(b, set, WPB);
-    hipLaunchKernelGGL(k_merge4, dim3((g + RPW - 1) / RPW), dim3(64 * WPB), 0, st, b, set, rescue);
-    hipLaunchKernelGGL(k_merge, dim3(g), dim3(64 * WPB), 0, st, b, set, rescue, 1);
+    if (!rescue) {
+        const unsigned g = grid_for_set(b, set, WPB);
+        hipLaunchKernelGGL(k_merge4, dim3((g + RPW - 1) / RPW), dim3(64 * WPB), 0, st, b, set, 0);
+        launch_select(b, set, SEL_DEFERRED, st);
+        hipLaunchKernelGGL(k_merge, dim3(grid_selected(b)), dim3(64 * WPB), 0, st, b,
+                           selected(b, SEL_DEFERRED), 0, 1, 1);
+    } else {
+        hipLaunchKernelGGL(k_merge, dim3(grid_selected(b)), dim3(64 * WPB), 0, st, b,
+                           selected(b, SEL_RESCUED), 1, 0, 1);
+    }
     if (b.merge_scratch) {
         hipLaunchKernelGGL(k_merge_groups, dim3(1024), dim3(64 * WPB), 0, st, b, rescue);
         hipLaunchKernelGGL(k_merge_split_finish, dim3(256), dim3(64 * WPB), 0, st, b, rescue);
