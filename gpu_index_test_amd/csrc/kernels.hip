@@ -1069,10 +1069,15 @@ constexpr uint32_t MERGE_DEFER = 0xfffffffeu;
 // then k_merge_split_finish), for reads with more than SPLIT_MIN hits
 constexpr uint32_t MERGE_SPLIT = 0xfffffffdu;
 constexpr uint32_t SPLIT_MIN = 512;
+// wide lists in LDS: per wave of k_merge for reads of at most this many
+// hits, and per group of k_merge_groups (one wave per block)
+constexpr int MERGE_WIDE_LDS = 256;
+constexpr int GROUP_WIDE_LDS = 1024;
 
 __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int rescue, int only_deferred,
                                                     int split) {
     __shared__ uint64_t lkeys[WPB][MERGE_LDS];
+    __shared__ WideNam lwide[WPB][MERGE_WIDE_LDS];  // the wide list of reads with few enough hits
     const uint32_t lane = lane_id();
     const int wid = (int)(threadIdx.x >> 6);
     if (rescue ? over_capacity(b.rhit_off, b.n_reads, b.rhit_cap, b.ctl, OVF_RHITS)
@@ -1135,6 +1140,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
             continue;
         }
         uint32_t total = MERGE_OVERFLOW;
+        if (nh <= (uint32_t)MERGE_WIDE_LDS) gwide = lwide[wid];  // NAMs <= hits fit
         if (nh <= (uint32_t)MERGE_LDS)
             total = merge_wave(nh, ghits, lkeys[wid], nullptr, gwide, fin, nh);
         else if (b.merge_scratch)
@@ -1155,10 +1161,11 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
 // one wave per group work item: the group's hits (in grouped order) merge
 // into NAMs at the group's own hit-indexed region of the read's output; the
 // NAM count goes to the read's group slot
-__global__ __launch_bounds__(64 * WPB) void k_merge_groups(DevBatch b, int rescue) {
+__global__ __launch_bounds__(64) void k_merge_groups(DevBatch b, int rescue) {
+    __shared__ WideNam lwide[GROUP_WIDE_LDS];
     const uint32_t lane = lane_id();
     const unsigned int cnt = b.ctl->n_gwork[rescue ? 1 : 0];
-    for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
+    for (uint32_t w = blockIdx.x; w < cnt; w += gridDim.x) {
         const uint4 it = b.gwork[w];
         const uint32_t r = it.x, gs = it.y, ge = it.z, g = it.w & 0x3fffffffu, code = it.w >> 30;
         const uint64_t hb = rescue ? b.rhit_off[r] : b.hit_off[r];
@@ -1167,7 +1174,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge_groups(DevBatch b, int rescu
         const uint64_t *ord = code == 0u ? nullptr : (code == 1u ? gkeys : gkeys + nh);
         uint64_t *slots = code == 1u ? gkeys + nh : gkeys;
         const DevHit *ghits = (rescue ? b.rhits : b.hits) + hb;
-        WideNam *gwide = (rescue ? b.rnam_ws : b.nam_ws) + hb + gs;
+        WideNam *gwide = ge - gs <= (uint32_t)GROUP_WIDE_LDS ? lwide : (rescue ? b.rnam_ws : b.nam_ws) + hb + gs;
         gis_nam *out = (rescue ? b.rnam_fin : b.nam_fin) + hb + gs;
         const uint32_t c = merge_range(ghits, ord, gs, ge, false, 0, 0, 0, 0, gwide, out, ge - gs);
         if (lane == 0) slots[g] = ((uint64_t)gs << 32) | c;
@@ -1729,7 +1736,7 @@ void launch_merge(const DevBatch &b, const ReadSet &set, int rescue, hipStream_t
                            selected(b, SEL_RESCUED), 1, 0, 1);
     }
     if (b.merge_scratch) {
-        hipLaunchKernelGGL(k_merge_groups, dim3(1024), dim3(64 * WPB), 0, st, b, rescue);
+        hipLaunchKernelGGL(k_merge_groups, dim3(4096), dim3(64), 0, st, b, rescue);
         hipLaunchKernelGGL(k_merge_split_finish, dim3(256), dim3(64 * WPB), 0, st, b, rescue);
     }
 }
