@@ -2,6 +2,9 @@
 // step2/index.hpp:37-183).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cerrno>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -198,27 +201,86 @@ extern "C" int gis_index_upload(gis_ctx *ctx, const gis_ref_randstrobe *randstro
     API_END(ctx)
 }
 
+// .sti v2 (step2/index.cu:73-134) mapped into memory and copied from the
+// mapping straight into HBM
 extern "C" int gis_index_read_sti(gis_ctx *ctx, const char *path, const gis_params *expected) {
     API_BEGIN(ctx)
     if (!path) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null path");
-    HostIndex h;
+    MappedSti m;
     std::string err;
-    const int rc = read_sti(path, expected, h, err);
+    const int rc = map_sti(path, expected, m, err);
     if (rc) return fail(ctx, rc, err);
-    upload_index(ctx, h);
+    try {
+        DevIndexStore &ix = ctx->ix;
+        ix.loaded = false;
+        release(ix.rs);
+        release(ix.starts);
+        ensure<gis_ref_randstrobe>(ix.rs, m.n);
+        ensure<uint64_t>(ix.starts, (1ull << m.bits) + 1);
+        const uint64_t chunk = 256ull << 20;
+        const char *src = reinterpret_cast<const char *>(m.rs);
+        char *dst = static_cast<char *>(ix.rs.p);
+        for (uint64_t o = 0, tot = m.n * sizeof(gis_ref_randstrobe); o < tot; o += chunk)
+            HIP_CHECK(hipMemcpy(dst + o, src + o, std::min(chunk, tot - o), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(ix.starts.p, m.starts, ((1ull << m.bits) + 1) * 8, hipMemcpyHostToDevice));
+        ix.n = m.n;
+        ix.bits = m.bits;
+        ix.filter_cutoff = m.filter_cutoff;
+        ix.params = m.params;
+        ix.distinct = ix.once = ix.high = ix.mid = ix.index_cutoff = 0;
+        ix.loaded = true;
+    } catch (...) {
+        unmap_sti(m);
+        throw;
+    }
+    unmap_sti(m);
     return GIS_OK;
     API_END(ctx)
 }
 
+// .sti v2 written from HBM in chunks (no whole host copy of the table)
 extern "C" int gis_index_write_sti(gis_ctx *ctx, const char *path) {
     API_BEGIN(ctx)
     if (!path) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null path");
     if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
-    HostIndex h;
-    download_index(ctx, h);
-    std::string err;
-    const int rc = write_sti(path, h, err);
-    if (rc) return fail(ctx, rc, err);
+    const DevIndexStore &ix = ctx->ix;
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return fail(ctx, GIS_ERR_IO, std::string(path) + ": " + std::strerror(errno));
+    bool ok = true;
+    auto put = [&](const void *d, size_t n) { ok = ok && std::fwrite(d, 1, n, fp) == n; };
+    put("STI\1", 4);
+    const uint32_t version = 2;
+    put(&version, 4);
+    // a reserved chunk of 8 zero bytes, as the reference writes it
+    const uint64_t reserved_size = 8, reserved = 0;
+    put(&reserved_size, 8);
+    put(&reserved, 8);
+    const gis_params &p = ix.params;
+    const int32_t v32[9] = {(int32_t)ix.filter_cutoff, ix.bits, p.canonical_read_length, p.k, p.s, p.l, p.u,
+                            (int32_t)p.q, p.max_dist};
+    put(v32, sizeof v32);
+    put(&ix.n, 8);
+    std::vector<char> buf;
+    const uint64_t chunk = 256ull << 20;
+    const uint64_t tot = ix.n * sizeof(gis_ref_randstrobe);
+    try {
+        buf.resize((size_t)std::min(chunk, std::max<uint64_t>(tot, 8)));
+        for (uint64_t o = 0; o < tot && ok; o += chunk) {
+            const uint64_t k = std::min(chunk, tot - o);
+            HIP_CHECK(hipMemcpy(buf.data(), static_cast<const char *>(ix.rs.p) + o, k, hipMemcpyDeviceToHost));
+            put(buf.data(), k);
+        }
+        const uint64_t nb = (1ull << ix.bits) + 1;
+        put(&nb, 8);
+        std::vector<uint64_t> st(nb);
+        HIP_CHECK(hipMemcpy(st.data(), ix.starts.p, nb * 8, hipMemcpyDeviceToHost));
+        put(st.data(), nb * 8);
+    } catch (...) {
+        std::fclose(fp);
+        throw;
+    }
+    if (std::fclose(fp) != 0) ok = false;
+    if (!ok) return fail(ctx, GIS_ERR_IO, std::string(path) + ": write failed");
     return GIS_OK;
     API_END(ctx)
 }

@@ -9,6 +9,11 @@
 // filter cutoff.
 #include "index_host.h"
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -276,89 +281,87 @@ bool params_equal(const gis_params &a, const gis_params &b) {
 // .sti v2: "STI\1", u32 version, u64 reserved size + bytes, i32 filter_cutoff,
 // i32 bits, 7 x i32 parameters, u64 n + n x 16 B randstrobes, u64 m + m x 8 B
 // bucket starts (step2/index.cu:73-134, step2/indexparameters.cpp:85-104)
-int write_sti(const std::string &path, const HostIndex &ix, std::string &err) {
-    FILE *fp = std::fopen(path.c_str(), "wb");
-    if (!fp) {
-        err = path + ": " + std::strerror(errno);
-        return GIS_ERR_IO;
-    }
-    auto w32 = [&](int32_t v) { std::fwrite(&v, 4, 1, fp); };
-    auto w64 = [&](uint64_t v) { std::fwrite(&v, 8, 1, fp); };
-    std::fwrite("STI\1", 1, 4, fp);
-    w32(2);
-    w64(8);
-    const char zeros[8] = {0};
-    std::fwrite(zeros, 1, 8, fp);
-    w32((int32_t)ix.filter_cutoff);
-    w32(ix.bits);
-    const gis_params &p = ix.params;
-    w32(p.canonical_read_length);
-    w32(p.k);
-    w32(p.s);
-    w32(p.l);
-    w32(p.u);
-    w32((int32_t)p.q);
-    w32(p.max_dist);
-    w64(ix.rs.size());
-    if (!ix.rs.empty()) std::fwrite(ix.rs.data(), sizeof(gis_ref_randstrobe), ix.rs.size(), fp);
-    w64(ix.starts.size());
-    if (!ix.starts.empty()) std::fwrite(ix.starts.data(), 8, ix.starts.size(), fp);
-    const bool bad = std::ferror(fp) != 0;
-    std::fclose(fp);
-    if (bad) {
-        err = path + ": write failed";
-        return GIS_ERR_IO;
-    }
-    return GIS_OK;
-}
-
-int read_sti(const std::string &path, const gis_params *expected, HostIndex &out, std::string &err) {
-    FILE *fp = std::fopen(path.c_str(), "rb");
-    if (!fp) {
+// The .sti v2 file mapped into memory: header fields parsed and checked, the
+// table and the bucket starts left in the mapping (no host copy).
+int map_sti(const std::string &path, const gis_params *expected, MappedSti &out, std::string &err) {
+    out = MappedSti();
+    const int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) {
         err = path + ": " + std::strerror(errno);
         return GIS_ERR_INVALID_INDEX;
     }
+    struct stat st;
+    if (::fstat(fd, &st) != 0) {
+        err = path + ": " + std::strerror(errno);
+        ::close(fd);
+        return GIS_ERR_INVALID_INDEX;
+    }
+    const uint64_t size = (uint64_t)st.st_size;
+    void *base = size ? ::mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0) : MAP_FAILED;
+    ::close(fd);
+    if (base == MAP_FAILED) {
+        err = "Index file has incorrect format (magic number mismatch)";
+        return GIS_ERR_INVALID_INDEX;
+    }
+    ::madvise(base, size, MADV_SEQUENTIAL);
+    out.base = base;
+    out.size = size;
+    const unsigned char *p = static_cast<const unsigned char *>(base);
+    uint64_t at = 0;
+    auto take = [&](void *dst, uint64_t n) {
+        if (at + n > size) return false;
+        std::memcpy(dst, p + at, n);
+        at += n;
+        return true;
+    };
     auto fail = [&](const std::string &m) {
-        std::fclose(fp);
+        unmap_sti(out);
         err = m;
         return GIS_ERR_INVALID_INDEX;
     };
     char magic[4];
-    int32_t v32[9];
-    uint64_t v64;
-    if (std::fread(magic, 1, 4, fp) != 4 || std::memcmp(magic, "STI\1", 4) != 0)
+    if (!take(magic, 4) || std::memcmp(magic, "STI\1", 4) != 0)
         return fail("Index file has incorrect format (magic number mismatch)");
     uint32_t version;
-    if (std::fread(&version, 4, 1, fp) != 1) return fail("truncated index file");
+    if (!take(&version, 4)) return fail("truncated index file");
     if (version != 2)
-        return fail("Can only read index file format version 2, but found version " +
-                    std::to_string(version));
-    if (std::fread(&v64, 8, 1, fp) != 1 || std::fseek(fp, (long)v64, SEEK_CUR) != 0)
-        return fail("truncated index file");
-    if (std::fread(v32, 4, 9, fp) != 9) return fail("truncated index file");
-    gis_params p;
-    if (gis_params_make(v32[2], v32[3], v32[4], v32[5], v32[6], v32[7], v32[8], &p) != GIS_OK)
+        return fail("Can only read index file format version 2, but found version " + std::to_string(version));
+    uint64_t v64;
+    if (!take(&v64, 8) || at + v64 > size) return fail("truncated index file");
+    at += v64;  // reserved chunk
+    int32_t v32[9];
+    if (!take(v32, sizeof v32)) return fail("truncated index file");
+    gis_params prm;
+    if (gis_params_make(v32[2], v32[3], v32[4], v32[5], v32[6], v32[7], v32[8], &prm) != GIS_OK)
         return fail("invalid index parameters in .sti file");
-    if (expected && !params_equal(p, *expected))
+    if (expected && !params_equal(prm, *expected))
         return fail("Index parameters in .sti file and those specified on command line differ");
-    out = HostIndex();
     out.filter_cutoff = (uint32_t)v32[0];
     out.bits = v32[1];
-    out.params = p;
+    out.params = prm;
     if (out.bits < 8 || out.bits > 31) return fail("invalid bits in .sti file");
-    if (std::fread(&v64, 8, 1, fp) != 1) return fail("truncated index file");
-    out.rs.resize(v64);
-    if (v64 && std::fread(out.rs.data(), sizeof(gis_ref_randstrobe), v64, fp) != v64)
-        return fail("truncated index file");
-    if (std::fread(&v64, 8, 1, fp) != 1) return fail("truncated index file");
-    out.starts.resize(v64);
-    if (v64 && std::fread(out.starts.data(), 8, v64, fp) != v64) return fail("truncated index file");
-    std::fclose(fp);
-    if (out.starts.size() != (1ULL << out.bits) + 1) {
-        err = "randstrobe_start_indices vector is of the wrong size";
-        return GIS_ERR_INVALID_INDEX;
+    if (!take(&v64, 8) || v64 > (size - at) / sizeof(gis_ref_randstrobe)) return fail("truncated index file");
+    out.n = v64;
+    out.rs = reinterpret_cast<const gis_ref_randstrobe *>(p + at);
+    at += v64 * sizeof(gis_ref_randstrobe);
+    if (!take(&v64, 8) || v64 > (size - at) / 8) return fail("truncated index file");
+    if (v64 != (1ULL << out.bits) + 1) return fail("randstrobe_start_indices vector is of the wrong size");
+    out.starts = reinterpret_cast<const uint64_t *>(p + at);
+    // the lookups index the table through the bucket starts: they must be
+    // non-decreasing and within the table
+    uint64_t prev = 0;
+    for (uint64_t i = 0; i < v64; i++) {
+        uint64_t v;
+        std::memcpy(&v, p + at + 8 * i, 8);
+        if (v < prev || v > out.n) return fail("bucket starts out of range");
+        prev = v;
     }
     return GIS_OK;
+}
+
+void unmap_sti(MappedSti &m) {
+    if (m.base) ::munmap(m.base, m.size);
+    m = MappedSti();
 }
 
 }  // namespace gis

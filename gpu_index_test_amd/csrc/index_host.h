@@ -27,8 +27,19 @@ int seed_reference(const char *const *seqs, const uint64_t *lens, uint32_t n_ref
 // repetition statistics and filter_cutoff from the run-length histogram
 // (hist[1..100] exact, hist[101] = runs longer than 100)
 void filter_stats(const uint64_t *hist, float f, HostIndex &ix);
-int read_sti(const std::string &path, const gis_params *expected, HostIndex &out, std::string &err);
-int write_sti(const std::string &path, const HostIndex &ix, std::string &err);
+// a .sti v2 file mapped read-only: the table and starts point into the mapping
+struct MappedSti {
+    void *base = nullptr;
+    uint64_t size = 0;
+    const gis_ref_randstrobe *rs = nullptr;
+    uint64_t n = 0;
+    const uint64_t *starts = nullptr;  // 2^bits + 1 entries (checked non-decreasing, <= n)
+    int bits = 0;
+    uint32_t filter_cutoff = 0;
+    gis_params params{};
+};
+int map_sti(const std::string &path, const gis_params *expected, MappedSti &out, std::string &err);
+void unmap_sti(MappedSti &m);
 bool params_equal(const gis_params &a, const gis_params &b);
 
 }  // namespace gis

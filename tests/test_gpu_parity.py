@@ -500,3 +500,25 @@ def test_shared_workspace_batches(setup):
     finally:
         b.close()
         a.close()
+
+
+def test_sti_rejects_damaged_files(setup, tmp_path):
+    """A .sti whose bucket starts leave the table, or that is cut short, is
+    refused on the host before anything reaches the GPU."""
+    g, lens, p, idx, op, oidx = setup
+    path = tmp_path / "ok.sti"
+    idx.write(path)
+    raw = bytearray(path.read_bytes())
+    nb = (1 << oidx.bits) + 1
+    bad = bytearray(raw)
+    pos = len(bad) - 8 * nb + 8 * (nb // 2)
+    bad[pos: pos + 8] = np.uint64(oidx.n + 5).tobytes()
+    other = G.StrobemerIndex(None, p)  # a context of its own
+    for name, data in (("starts.sti", bad), ("short.sti", raw[: len(raw) // 2]), ("magic.sti", b"XX" + raw[2:])):
+        f = tmp_path / name
+        f.write_bytes(bytes(data))
+        with pytest.raises(G.GisError) as e:
+            other.read(f)
+        assert e.value.code == 2, name
+    other.read(path)
+    assert other.randstrobes.tobytes() == oidx.randstrobes.tobytes()
