@@ -16,14 +16,15 @@ import numpy as np
 
 from . import _native as N
 from ._native import (FLAG_NO_RANDSTROBES, FLAG_RESCUED, GIS_DEFAULT, GIS_NOT_FOUND, HIT,
-                      MODE_FUSED, MODE_PLAIN, NAM, QUERY_RANDSTROBE, REF_RANDSTROBE, SYNCMER,
-                      GisError)
+                      MODE_FUSED, MODE_PLAIN, NAM, NAM_PAIR, QUERY_RANDSTROBE, REF_RANDSTROBE,
+                      SYNCMER, GisError)
 
 __all__ = [
     "IndexParameters", "References", "Context", "StrobemerIndex", "Mapper", "MapResult", "Server",
     "randstrobes_query", "find_nams", "find_nams_rescue", "synth_genome", "synth_reads",
     "pack_reads", "GisError", "NAM", "QUERY_RANDSTROBE", "REF_RANDSTROBE", "SYNCMER", "HIT",
     "GIS_NOT_FOUND", "FLAG_RESCUED", "FLAG_NO_RANDSTROBES", "MODE_PLAIN", "MODE_FUSED",
+    "PairResult", "pairs_from_nams", "NAM_PAIR",
 ]
 
 
@@ -478,6 +479,77 @@ def find_nams_rescue(query_randstrobes, index, rescue_cutoff, hard_cutoff=1000):
     return _qrs_call(index, [query_randstrobes], 1, rescue_cutoff, hard_cutoff).read_nams(0)
 
 
+def _pair_params(dropoff_threshold=0.5, max_tries=20, mu=300.0, sigma=100.0, joint=True):
+    pp = N.PairParams()
+    pp.dropoff_threshold, pp.max_tries, pp.mu, pp.sigma = dropoff_threshold, max_tries, mu, sigma
+    pp.flags = N.PAIRS_JOINT if joint else 0
+    return pp
+
+
+class PairResult:
+    """The pair step's output for n pairs (mate 1 of pair p is list p, mate 2
+    list n + p): NAM lists in the reference order, types 0..4, pair ids by
+    type, top_dropoff and MAPQ per list, joint NAM pairs per pair."""
+
+    def __init__(self, n_pairs, nam_offsets, nams, types, type_ids, type_counts, top_dropoff, mapq,
+                 joint_offsets, joint):
+        self.n_pairs = n_pairs
+        self.nam_offsets, self.nams = nam_offsets, nams
+        self.types, self.type_ids, self.type_counts = types, type_ids, type_counts
+        self.top_dropoff, self.mapq = top_dropoff, mapq
+        self.joint_offsets, self.joint = joint_offsets, joint
+
+    def list_nams(self, r):
+        return self.nams[int(self.nam_offsets[r]): int(self.nam_offsets[r + 1])]
+
+    def pair_joint(self, p):
+        return self.joint[int(self.joint_offsets[p]): int(self.joint_offsets[p + 1])]
+
+    def type_lists(self):
+        """The five per-type pair id lists (step2/main.cu:4641-4646)."""
+        out, at = [], 0
+        for c in self.type_counts:
+            out.append(self.type_ids[at: at + c])
+            at += c
+        return out
+
+
+def _call_with_pair_out(ctx, n_pairs, n_nams, fn, joint_cap):
+    while True:
+        offs = np.empty(2 * n_pairs + 1, dtype=np.uint64)
+        nams = np.empty(max(n_nams, 1), dtype=NAM)
+        types = np.empty(max(n_pairs, 1), dtype=np.uint8)
+        tids = np.empty(max(n_pairs, 1), dtype=np.uint32)
+        drop = np.empty(max(2 * n_pairs, 1), dtype=np.float32)
+        mapq = np.empty(max(2 * n_pairs, 1), dtype=np.uint8)
+        joff = np.empty(n_pairs + 1, dtype=np.uint64)
+        joint = np.empty(max(joint_cap, 1), dtype=N.NAM_PAIR)
+        o = N.PairOut()
+        o.nam_offsets, o.nams, o.nam_capacity = offs.ctypes.data, nams.ctypes.data, n_nams
+        o.types, o.type_ids = types.ctypes.data, tids.ctypes.data
+        o.top_dropoff, o.mapq = drop.ctypes.data, mapq.ctypes.data
+        o.joint_offsets, o.joint, o.joint_capacity = joff.ctypes.data, joint.ctypes.data, joint_cap
+        code = fn(o)
+        if code == 5 and (o.n_joint > joint_cap or o.n_nams > n_nams):
+            joint_cap, n_nams = max(joint_cap, int(o.n_joint)), max(n_nams, int(o.n_nams))
+            continue
+        ctx.check(code)
+        return PairResult(n_pairs, offs, nams[: o.n_nams], types[:n_pairs], tids[:n_pairs],
+                          list(o.type_counts), drop[: 2 * n_pairs], mapq[: 2 * n_pairs], joff,
+                          joint[: o.n_joint])
+
+
+def pairs_from_nams(ctx, nam_offsets, nams, n_pairs, dropoff_threshold=0.5, max_tries=20, mu=300.0,
+                    sigma=100.0, joint=True):
+    """The pair step on NAM lists in host memory (2 n_pairs lists)."""
+    nam_offsets = np.ascontiguousarray(nam_offsets, dtype=np.uint64)
+    nams = np.ascontiguousarray(nams, dtype=NAM)
+    pp = _pair_params(dropoff_threshold, max_tries, mu, sigma, joint)
+    return _call_with_pair_out(ctx, n_pairs, len(nams), lambda o: ctx.L.gis_pairs_from_nams(
+        ctx.h, N.ptr(nam_offsets), N.ptr(nams), n_pairs, C.byref(pp), C.byref(o)),
+        4 * n_pairs + 16)
+
+
 class Mapper:
     """Device-resident batches: upload reads, run the path, download NAMs."""
 
@@ -516,6 +588,18 @@ class Mapper:
         cap = self.nam_count()
         return _call_with_nam_out(self.ctx, self.n,
                                   lambda o: self.ctx.L.gis_batch_download(self.h, C.byref(o)), cap)
+
+    def pairs(self, dropoff_threshold=0.5, max_tries=20, mu=300.0, sigma=100.0, joint=True):
+        """Run the pair step on the last run's NAMs (reads [0, n) are mate 1,
+        [n, 2n) mate 2) and download it -> PairResult."""
+        pp = _pair_params(dropoff_threshold, max_tries, mu, sigma, joint)
+        self.ctx.check(self.ctx.L.gis_batch_pairs(self.h, C.byref(pp)))
+        return self.pairs_download()
+
+    def pairs_download(self):
+        n_pairs = self.n // 2
+        return _call_with_pair_out(self.ctx, n_pairs, self.nam_count(), lambda o: self.ctx.L.gis_batch_pairs_download(
+            self.h, C.byref(o)), 4 * n_pairs + 16)
 
     def autotune(self, rescue_level=2, hard_cutoff=1000):
         """Time every fused-form configuration on the uploaded reads and keep

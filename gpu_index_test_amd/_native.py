@@ -37,6 +37,8 @@ NAM = np.dtype([("nam_id", "<i4"), ("query_start", "<i4"), ("query_end", "<i4"),
                 ("score", "<f4"), ("is_rc", "u1"), ("pad", "u1", (3,))])
 HIT = np.dtype([("query_start", "<i4"), ("query_end", "<i4"), ("ref_start", "<i4"),
                 ("ref_end", "<i4"), ("ref_id", "<i4"), ("is_rc", "<i4")])
+NAM_PAIR = np.dtype([("score", "<i4"), ("i1", "<i4"), ("i2", "<i4")])
+PAIRS_JOINT = 1
 assert QUERY_RANDSTROBE.itemsize == 24 and REF_RANDSTROBE.itemsize == 16
 assert NAM.itemsize == 44 and SYNCMER.itemsize == 16 and HIT.itemsize == 24
 
@@ -103,6 +105,19 @@ class ReadSpec(C.Structure):
                 ("insert_mean", C.c_double), ("insert_sd", C.c_double), ("seed", C.c_uint64)]
 
 
+class PairParams(C.Structure):
+    _fields_ = [("dropoff_threshold", C.c_float), ("max_tries", C.c_int32), ("mu", C.c_float),
+                ("sigma", C.c_float), ("flags", C.c_uint32)]
+
+
+class PairOut(C.Structure):
+    _fields_ = [("nam_offsets", C.c_void_p), ("nams", C.c_void_p), ("nam_capacity", C.c_uint64),
+                ("types", C.c_void_p), ("type_ids", C.c_void_p), ("type_counts", C.c_uint32 * 5),
+                ("top_dropoff", C.c_void_p), ("mapq", C.c_void_p), ("joint_offsets", C.c_void_p),
+                ("joint", C.c_void_p), ("joint_capacity", C.c_uint64), ("n_nams", C.c_uint64),
+                ("n_joint", C.c_uint64)]
+
+
 # name -> (restype, argtypes); every name must be exported by libgis.so
 _P = C.c_void_p
 _U32, _U64, _I32 = C.c_uint32, C.c_uint64, C.c_int32
@@ -149,6 +164,10 @@ PROTOTYPES = {
     "gis_server_wait": (C.c_int, [_P, _U64, C.c_int, C.POINTER(NamOut), C.POINTER(C.c_double)]),
     "gis_server_replay": (C.c_int, [_P, _P, _P, _P, _U32, _P, _P, _P, _P, C.POINTER(C.c_double)]),
     "gis_server_stats_get": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "gis_pair_params_default": (None, [C.POINTER(PairParams)]),
+    "gis_batch_pairs": (C.c_int, [_P, C.POINTER(PairParams)]),
+    "gis_batch_pairs_download": (C.c_int, [_P, C.POINTER(PairOut)]),
+    "gis_pairs_from_nams": (C.c_int, [_P, _P, _P, _U32, C.POINTER(PairParams), C.POINTER(PairOut)]),
     "gis_synth_genome": (C.c_int, [C.POINTER(GenomeSpec), _P, _P]),
     "gis_synth_reads": (C.c_int, [C.POINTER(ReadSpec), _P, _P, _U32, _P, _P, _P, _P, _P, _P]),
 }

@@ -53,6 +53,7 @@ static void workspace_release(Workspace *w) {
     for (DevBuf *x : all) release(*x);
     if (w->h_ctl) (void)hipHostFree(w->h_ctl);
     if (w->h_total) (void)hipHostFree(w->h_total);
+    pairwork_free(w->pairs);
     delete w;
 }
 

@@ -31,6 +31,7 @@ struct Workspace {
     DevCtl *h_ctl = nullptr;
     uint64_t *h_total = nullptr;  // [3]: hits, rescue hits, NAMs
     int refs = 0;
+    PairWork *pairs = nullptr;    // the pair step on this workspace's NAMs (pairs.cpp)
 };
 
 }  // namespace gis

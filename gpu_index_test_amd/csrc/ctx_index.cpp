@@ -93,6 +93,8 @@ extern "C" void gis_ctx_destroy(gis_ctx *ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         batch_free(ctx->scratch);
         ctx->scratch = nullptr;
+        pairwork_free(ctx->pairs);
+        ctx->pairs = nullptr;
         release(ctx->ix.rs);
         release(ctx->ix.starts);
         (void)hipStreamDestroy(ctx->stream);

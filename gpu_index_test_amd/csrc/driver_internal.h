@@ -65,6 +65,10 @@ struct DevIndexStore {
 
 inline int fail(gis_ctx *ctx, int code, const std::string &msg);
 
+// device buffers and results of the pair step (pairs.cpp)
+struct PairWork;
+void pairwork_free(PairWork *w);
+
 }  // namespace gis
 
 struct gis_ctx {
@@ -76,6 +80,7 @@ struct gis_ctx {
     gis_batch *scratch = nullptr;  // batch reused by the host-buffer entry points
     int n_cu = 256;                // compute units of the device (persistent grids)
     gis_fused_config fcfg{4, 0, 0, 0, 0};
+    gis::PairWork *pairs = nullptr;  // the pair step on NAM lists from host memory
 };
 
 namespace gis {

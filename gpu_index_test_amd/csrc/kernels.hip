@@ -18,58 +18,9 @@
 #include <stdint.h>
 
 #include "kernels.h"
+#include "wave.h"
 
 namespace gis {
-
-// ---------------------------------------------------------------------------
-// wave helpers
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// ordering of scratch memory (LDS or global) written by other lanes of the
-// same wave; a workgroup's waves share one CU's L1
-__device__ __forceinline__ void wave_sync_mem() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
-
-__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
-    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
-    return x;
-}
-
-__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
-    for (int d = 32; d >= 1; d >>= 1) {
-        const uint32_t y = __shfl_xor(x, d, 64);
-        x = x > y ? x : y;
-    }
-    return x;
-}
-
-__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
-    const uint32_t lane = lane_id();
-    uint32_t x = v;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= (uint32_t)d) x += y;
-    }
-    return x - v;
-}
-
-__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
-}
 
 // wave-uniform iteration over a ReadSet
 __device__ __forceinline__ uint32_t set_count(const ReadSet &s) { return s.ids ? *s.n_dev : s.n; }

@@ -318,6 +318,67 @@ int gis_server_replay(gis_server *s, const char *seqs, const uint64_t *offsets, 
                       const uint32_t *count, double *latency_ms, double *wall_s);
 int gis_server_stats_get(gis_server *s, gis_server_stats *out);
 
+/* ---------------- NAM consumer of paired reads ----------------
+ * The step after the seeding path in the reference's GPU driver: each NAM
+ * list in the order gpu_sort_nams leaves it (step2/main.cu:3449-3472), the
+ * pair types of gpu_pre_cal_type (step2/main.cu:3552-3581) with top_dropoff
+ * and is_proper_nam_pair (step2/main.cu:870-898), the per-type pair lists the
+ * host builds from them (step2/main.cu:4641-4646), get_mapq
+ * (step2/main.cu:900-910) and get_best_scoring_nam_pairs
+ * (step2/main.cu:1719-1830). Pairs follow the reference's batch layout: of
+ * 2 n NAM lists, mate 1 of pair p is list p and mate 2 is list n + p. */
+
+/* dropoff_threshold and max_tries: MappingParameters (step2/aln.hpp:61-63);
+ * mu, sigma: the estimate a fresh GPUInsertSizeDistribution starts from
+ * (step2/main.cu:915-920), which gpu_pre_cal_type uses for every pair. */
+typedef struct {
+    float dropoff_threshold; /* 0.5 */
+    int32_t max_tries;       /* 20; 1 .. 256 */
+    float mu, sigma;         /* 300, 100 */
+    uint32_t flags;          /* GIS_PAIRS_* */
+} gis_pair_params;
+/* GIS_PAIRS_JOINT: also get_best_scoring_nam_pairs for every pair with NAMs
+ * on both mates (types 3 and 4). */
+enum { GIS_PAIRS_JOINT = 1 };
+void gis_pair_params_default(gis_pair_params *p);
+
+/* NamPair, step2/main.cu:957-963: the joint hit count and the positions of
+ * the two NAMs in the mates' reference-ordered lists (-1: no NAM). */
+typedef struct { int32_t score; int32_t i1; int32_t i2; } gis_nam_pair;
+
+/* Output of the pair step; every array is optional. NAM lists are in the
+ * reference order (mate 1 lists by is_rc, ref_id, n_hits descending,
+ * query_end, query_start, ref_end, ref_start; mate 2 lists by is_rc, ref_id,
+ * max(0, ref_start - query_start), then the same), with the nam_id each NAM
+ * had on input. type_ids lists the pair ids of type 0, then type 1, ...,
+ * ascending within a type. top_dropoff is 0 for an empty list. mapq is
+ * get_mapq(list, list[0]), the value the reference's type-3 path reports. */
+typedef struct {
+    uint64_t *nam_offsets;   /* [2 n + 1] */
+    gis_nam *nams;
+    uint64_t nam_capacity;
+    uint8_t *types;          /* [n] 0 .. 4 */
+    uint32_t *type_ids;      /* [n] */
+    uint32_t type_counts[5]; /* out */
+    float *top_dropoff;      /* [2 n] */
+    uint8_t *mapq;           /* [2 n] */
+    uint64_t *joint_offsets; /* [n + 1] (all 0 without GIS_PAIRS_JOINT) */
+    gis_nam_pair *joint;     /* entries of pair p at joint_offsets[p] .. joint_offsets[p+1] */
+    uint64_t joint_capacity;
+    uint64_t n_nams, n_joint; /* out */
+} gis_pair_out;
+
+/* Run the pair step on the NAMs of the batch's last run (2 n reads, mates laid
+ * out as above); results stay on the device until downloaded. */
+int gis_batch_pairs(gis_batch *b, const gis_pair_params *pp);
+/* Copy the pair step's results out. GIS_ERR_CAPACITY (with n_nams / n_joint
+ * set) if nam_capacity or joint_capacity is too small. */
+int gis_batch_pairs_download(gis_batch *b, gis_pair_out *out);
+/* The pair step on NAM lists from host memory (any order within a list;
+ * NAMs of one list must differ in (is_rc, ref_id, query_start, ref_start)). */
+int gis_pairs_from_nams(gis_ctx *ctx, const uint64_t *nam_offsets, const gis_nam *nams, uint32_t n_pairs,
+                        const gis_pair_params *pp, gis_pair_out *out);
+
 /* ---------------- synthetic inputs (deterministic) ---------------- */
 typedef struct {
     uint64_t total_length;

@@ -1049,3 +1049,269 @@ int orc_lookup_bytes(const orc_index *idx, const orc_qrs *q, size_t nq, uint64_t
     *bytes_out = b;
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* NAM consumer of paired reads (the step after find_nams in the GPU driver) */
+
+/* quick_sort_iterative, step2/main.cu:317-359: Lomuto partition around the
+ * last element, explicit stack, lower part pushed first. The comparator
+ * decides where ties land, so the order of equal keys is this algorithm's. */
+typedef int (*orc_less_fn)(const void *a, const void *b, const void *ctx);
+
+static void lomuto_sort(void *base, long n, size_t sz, orc_less_fn less, const void *ctx) {
+    if (n < 2) return;
+    char *d = (char *)base;
+    long *stack = (long *)malloc(sizeof(long) * (size_t)(2 * n));
+    char *tmp = (char *)malloc(sz);
+    char *pivot = (char *)malloc(sz);
+    long top = -1;
+    stack[++top] = 0;
+    stack[++top] = n - 1;
+    while (top >= 0) {
+        long high = stack[top--];
+        long low = stack[top--];
+        memcpy(pivot, d + high * sz, sz);
+        long i = low - 1;
+        for (long j = low; j < high; j++) {
+            if (less(d + j * sz, pivot, ctx)) {
+                ++i;
+                memcpy(tmp, d + i * sz, sz);
+                memcpy(d + i * sz, d + j * sz, sz);
+                memcpy(d + j * sz, tmp, sz);
+            }
+        }
+        memcpy(tmp, d + (i + 1) * sz, sz);
+        memcpy(d + (i + 1) * sz, d + high * sz, sz);
+        memcpy(d + high * sz, tmp, sz);
+        long p = i + 1;
+        if (p - 1 > low) {
+            stack[++top] = low;
+            stack[++top] = p - 1;
+        }
+        if (p + 1 < high) {
+            stack[++top] = p + 1;
+            stack[++top] = high;
+        }
+    }
+    free(stack);
+    free(tmp);
+    free(pivot);
+}
+
+static int imax(int a, int b) { return a > b ? a : b; }
+
+/* Mate 1: sort_nams_single2's lambda, step2/main.cu:2275-2286 (is_rc, ref_id,
+ * n_hits descending, query_end, query_start, ref_end, ref_start; true on
+ * equality). Mate 2: Nam::operator<, step2/my_struct.hpp:54-68 (is_rc,
+ * ref_id, max(0, ref_start - query_start), then as mate 1; true on equality). */
+static int nam_ref_less(const orc_nam *a, const orc_nam *b, int mate) {
+    if (a->is_rc != b->is_rc) return a->is_rc < b->is_rc;
+    if (a->ref_id != b->ref_id) return a->ref_id < b->ref_id;
+    if (mate == 2) {
+        const int d1 = imax(0, a->ref_start - a->query_start);
+        const int d2 = imax(0, b->ref_start - b->query_start);
+        if (d1 != d2) return d1 < d2;
+    }
+    if (a->n_hits != b->n_hits) return a->n_hits > b->n_hits;
+    if (a->query_end != b->query_end) return a->query_end < b->query_end;
+    if (a->query_start != b->query_start) return a->query_start < b->query_start;
+    if (a->ref_end != b->ref_end) return a->ref_end < b->ref_end;
+    if (a->ref_start != b->ref_start) return a->ref_start < b->ref_start;
+    return 1;
+}
+
+static int nam_less_mate1(const void *a, const void *b, const void *ctx) {
+    (void)ctx;
+    return nam_ref_less((const orc_nam *)a, (const orc_nam *)b, 1);
+}
+static int nam_less_mate2(const void *a, const void *b, const void *ctx) {
+    (void)ctx;
+    return nam_ref_less((const orc_nam *)a, (const orc_nam *)b, 2);
+}
+
+/* gpu_sort_nams, step2/main.cu:3449-3472: NAM lists of mate 1 (ids below
+ * the read count) by sort_nams_single2, of mate 2 by sort_nams_single. */
+void orc_sort_nams_reference(orc_nam *nams, size_t n, int mate) {
+    lomuto_sort(nams, (long)n, sizeof(orc_nam), mate == 2 ? nam_less_mate2 : nam_less_mate1, NULL);
+}
+
+/* top_dropoff, step2/main.cu:889-898 */
+float orc_top_dropoff(const orc_nam *nams, size_t n) {
+    const orc_nam *n_max = &nams[0];
+    if (n_max->n_hits <= 2) return 1.0f;
+    if (n > 1) return (float)nams[1].n_hits / (float)n_max->n_hits;
+    return 0.0f;
+}
+
+/* float to int as the GPU converts (truncation, saturating, NaN -> 0); in the
+ * reference's range the plain C conversion */
+static int f2i_sat(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return INT32_MAX;
+    if (x <= -2147483648.0f) return INT32_MIN;
+    return (int)x;
+}
+
+/* get_mapq, step2/main.cu:900-910. log is the correctly rounded natural
+ * logarithm in float, taken as log in double rounded to float. The uint8
+ * return type wraps a negative value. */
+uint8_t orc_get_mapq(const orc_nam *nams, size_t n, const orc_nam *n_max) {
+    if (n <= 1) return 60;
+    const float s1 = n_max->score;
+    const float s2 = nams[1].score;
+    const double mm = n_max->n_hits / 10.0;
+    const float min_matches = (float)(mm < 1.0 ? mm : 1.0);
+    const float lg = (float)log((double)s1);
+    const float v = 40.0f * (1.0f - s2 / s1) * min_matches * lg;
+    const int uncapped = f2i_sat(v);
+    return (uint8_t)(uncapped < 60 ? uncapped : 60);
+}
+
+/* is_proper_nam_pair, step2/main.cu:870-887 */
+int orc_is_proper_nam_pair(const orc_nam *nam1, const orc_nam *nam2, float mu, float sigma) {
+    if (nam1->ref_id != nam2->ref_id || nam1->is_rc == nam2->is_rc) return 0;
+    const int a = imax(0, nam1->ref_start - nam1->query_start);
+    const int b = imax(0, nam2->ref_start - nam2->query_start);
+    /* mu + 10 sigma as one fused multiply-add, as the reference's device
+     * compiler contracts it */
+    const float lim = fmaf(10.0f, sigma, mu);
+    const int r1_r2 = nam2->is_rc && (a <= b) && ((float)(b - a) < lim);
+    const int r2_r1 = nam1->is_rc && (b <= a) && ((float)(a - b) < lim);
+    return r1_r2 || r2_r1;
+}
+
+/* gpu_pre_cal_type's body, step2/main.cu:3565-3578: 0 no NAMs, 1 only mate 1,
+ * 2 only mate 2, 3 both dominated by one NAM forming a proper pair, 4 the
+ * rest. Lists in the reference order. */
+int orc_pair_type(const orc_nam *n1, size_t c1, const orc_nam *n2, size_t c2, float dropoff,
+                  float mu, float sigma) {
+    if (!c1 && !c2) return 0;
+    if (c1 && !c2) return 1;
+    if (!c1 && c2) return 2;
+    if (orc_top_dropoff(n1, c1) < dropoff && orc_top_dropoff(n2, c2) < dropoff &&
+        orc_is_proper_nam_pair(&n1[0], &n2[0], mu, sigma))
+        return 3;
+    return 4;
+}
+
+typedef struct {
+    const orc_nam *n1, *n2;
+} pair_ctx;
+
+/* the NamPair comparator of get_best_scoring_nam_pairs, step2/main.cu:1790-1828;
+ * an absent NAM compares as Nam{} with ref_start = -1 */
+static int nam_pair_less(const void *pa, const void *pb, const void *vctx) {
+    const orc_nam_pair *A = (const orc_nam_pair *)pa, *B = (const orc_nam_pair *)pb;
+    const pair_ctx *c = (const pair_ctx *)vctx;
+    if (A->score != B->score) return A->score > B->score;
+    orc_nam dummy;
+    memset(&dummy, 0, sizeof dummy);
+    dummy.ref_start = -1;
+    const orc_nam *a1 = A->i1 == -1 ? &dummy : &c->n1[A->i1];
+    const orc_nam *a2 = A->i2 == -1 ? &dummy : &c->n2[A->i2];
+    const orc_nam *b1 = B->i1 == -1 ? &dummy : &c->n1[B->i1];
+    const orc_nam *b2 = B->i2 == -1 ? &dummy : &c->n2[B->i2];
+    if (a1->score != b1->score) return a1->score > b1->score;
+    if (a1->is_rc != b1->is_rc) return !a1->is_rc;
+    if (a1->query_end != b1->query_end) return a1->query_end < b1->query_end;
+    if (a1->query_start != b1->query_start) return a1->query_start < b1->query_start;
+    if (a1->ref_end != b1->ref_end) return a1->ref_end < b1->ref_end;
+    if (a1->ref_start != b1->ref_start) return a1->ref_start < b1->ref_start;
+    if (a2->score != b2->score) return a2->score > b2->score;
+    if (a2->is_rc != b2->is_rc) return !a2->is_rc;
+    if (a2->query_end != b2->query_end) return a2->query_end < b2->query_end;
+    if (a2->query_start != b2->query_start) return a2->query_start < b2->query_start;
+    if (a2->ref_end != b2->ref_end) return a2->ref_end < b2->ref_end;
+    return a2->ref_start < b2->ref_start;
+}
+
+/* get_best_scoring_nam_pairs, step2/main.cu:1719-1830, on two non-empty NAM
+ * lists. Returns the entry count, or -1 if cap is too small or a list is
+ * empty. At most 4 * max_tries + 1 entries. */
+long orc_best_scoring_nam_pairs(const orc_nam *n1, size_t c1, const orc_nam *n2, size_t c2, float mu,
+                                float sigma, int max_tries, orc_nam_pair *out, size_t cap) {
+    if (!c1 || !c2 || max_tries < 1) return -1;
+    uint8_t *added1 = (uint8_t *)calloc(c1, 1), *added2 = (uint8_t *)calloc(c2, 1);
+    long size = 0;
+    int ok = 1;
+    int best = 0;
+#define PUSH(sc, a, b)                                 \
+    do {                                               \
+        if ((size_t)size >= cap) { ok = 0; goto done; } \
+        out[size].score = (sc);                        \
+        out[size].i1 = (a);                            \
+        out[size].i2 = (b);                            \
+        size++;                                        \
+    } while (0)
+    for (size_t i = 0; i < c1; i++) {
+        for (size_t j = 0; j < c2; j++) {
+            const int joint = n1[i].n_hits + n2[j].n_hits;
+            if ((double)joint < 0.5 * best || size > (long)max_tries * 2) break;
+            if (orc_is_proper_nam_pair(&n1[i], &n2[j], mu, sigma)) {
+                PUSH(joint, (int)i, (int)j);
+                added1[i] = 1;
+                added2[j] = 1;
+                if (joint > best) best = joint;
+            }
+        }
+        if (size > (long)max_tries * 2) break;
+    }
+    {
+        const int best1 = best > 0 ? best : n1[0].n_hits;
+        const size_t lim1 = c1 < (size_t)max_tries ? c1 : (size_t)max_tries;
+        for (size_t i = 0; i < lim1; i++) {
+            if (n1[i].n_hits < best1 / 2) break;
+            if (added1[i]) continue;
+            PUSH(n1[i].n_hits, (int)i, -1);
+        }
+        const int best2 = best > 0 ? best : n2[0].n_hits;
+        const size_t lim2 = c2 < (size_t)max_tries ? c2 : (size_t)max_tries;
+        for (size_t i = 0; i < lim2; i++) {
+            if (n2[i].n_hits < best2 / 2) break;
+            if (added2[i]) continue;
+            PUSH(n2[i].n_hits, -1, (int)i);
+        }
+    }
+#undef PUSH
+    {
+        pair_ctx c = {n1, n2};
+        lomuto_sort(out, size, sizeof(orc_nam_pair), nam_pair_less, &c);
+    }
+done:
+    free(added1);
+    free(added2);
+    return ok ? size : -1;
+}
+
+/* The pair step over n_pairs pairs (mate 1 of pair p is list p, mate 2 list
+ * n_pairs + p): lists copied to ord in the reference order, then per pair the
+ * type (gpu_pre_cal_type), per list top_dropoff (0 for an empty list) and
+ * get_mapq(list, list[0]), and with joint != 0 the joint entries of pairs
+ * with NAMs on both mates at joint_out + p * (4 max_tries + 1). */
+int orc_pairs_batch(const uint64_t *nam_offsets, const orc_nam *nams, uint32_t n_pairs, float dropoff,
+                    float mu, float sigma, int max_tries, int joint, orc_nam *ord, uint8_t *types,
+                    float *drop, uint8_t *mapq, uint32_t *joint_counts, orc_nam_pair *joint_out) {
+    const uint64_t n_lists = 2ull * n_pairs;
+    for (uint64_t r = 0; r < n_lists; r++) {
+        const uint64_t o = nam_offsets[r], c = nam_offsets[r + 1] - o;
+        memcpy(ord + o, nams + o, sizeof(orc_nam) * c);
+        orc_sort_nams_reference(ord + o, c, r < n_pairs ? 1 : 2);
+        drop[r] = c ? orc_top_dropoff(ord + o, c) : 0.0f;
+        mapq[r] = orc_get_mapq(ord + o, c, ord + o);
+    }
+    const size_t stride = 4 * (size_t)max_tries + 1;
+    for (uint32_t p = 0; p < n_pairs; p++) {
+        const uint64_t o1 = nam_offsets[p], c1 = nam_offsets[p + 1] - o1;
+        const uint64_t o2 = nam_offsets[n_pairs + p], c2 = nam_offsets[n_pairs + p + 1] - o2;
+        types[p] = (uint8_t)orc_pair_type(ord + o1, c1, ord + o2, c2, dropoff, mu, sigma);
+        if (!joint) continue;
+        joint_counts[p] = 0;
+        if (c1 && c2) {
+            long k = orc_best_scoring_nam_pairs(ord + o1, c1, ord + o2, c2, mu, sigma, max_tries,
+                                                joint_out + (size_t)p * stride, stride);
+            if (k < 0) return ORC_CAPACITY;
+            joint_counts[p] = (uint32_t)k;
+        }
+    }
+    return ORC_OK;
+}

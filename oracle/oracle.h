@@ -143,6 +143,23 @@ int orc_map_batch(const orc_index *idx, const char *seqs, const uint64_t *offset
 int orc_lookup_bytes(const orc_index *idx, const orc_qrs *q, size_t nq,
                      uint64_t *bytes_out);
 
+
+/* ---- NAM consumer of paired reads ---- */
+/* NamPair, step2/main.cu:957-963, with list indices (-1: no NAM) */
+typedef struct { int32_t score, i1, i2; } orc_nam_pair;
+/* gpu_sort_nams order of one NAM list (mate 1 or 2) */
+void orc_sort_nams_reference(orc_nam *nams, size_t n, int mate);
+float orc_top_dropoff(const orc_nam *nams, size_t n);
+uint8_t orc_get_mapq(const orc_nam *nams, size_t n, const orc_nam *n_max);
+int orc_is_proper_nam_pair(const orc_nam *nam1, const orc_nam *nam2, float mu, float sigma);
+int orc_pair_type(const orc_nam *n1, size_t c1, const orc_nam *n2, size_t c2, float dropoff,
+                  float mu, float sigma);
+long orc_best_scoring_nam_pairs(const orc_nam *n1, size_t c1, const orc_nam *n2, size_t c2, float mu,
+                                float sigma, int max_tries, orc_nam_pair *out, size_t cap);
+int orc_pairs_batch(const uint64_t *nam_offsets, const orc_nam *nams, uint32_t n_pairs, float dropoff,
+                    float mu, float sigma, int max_tries, int joint, orc_nam *ord, uint8_t *types,
+                    float *drop, uint8_t *mapq, uint32_t *joint_counts, orc_nam_pair *joint_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,7 @@ REF_RANDSTROBE = np.dtype([("hash", "<u8"), ("position", "<u4"), ("packed", "<u4
 SYNCMER = np.dtype([("hash", "<u8"), ("position", "<u8")])
 HIT = np.dtype([("query_start", "<i4"), ("query_end", "<i4"), ("ref_start", "<i4"),
                 ("ref_end", "<i4")])
+NAM_PAIR = np.dtype([("score", "<i4"), ("i1", "<i4"), ("i2", "<i4")])
 NAM = np.dtype([("nam_id", "<i4"), ("query_start", "<i4"), ("query_end", "<i4"),
                 ("query_prev_hit_startpos", "<i4"), ("ref_start", "<i4"), ("ref_end", "<i4"),
                 ("ref_prev_hit_startpos", "<i4"), ("n_hits", "<i4"), ("ref_id", "<i4"),
@@ -89,6 +90,16 @@ def lib():
             "orc_map_batch": (C.c_int, [C.POINTER(_Index), _P, _P, _P, C.c_size_t, C.c_uint,
                                         C.c_uint, C.c_int, _P, _P, _P, _P, _P]),
             "orc_lookup_bytes": (C.c_int, [C.POINTER(_Index), _P, C.c_size_t, C.POINTER(C.c_uint64)]),
+            "orc_sort_nams_reference": (None, [_P, C.c_size_t, C.c_int]),
+            "orc_top_dropoff": (C.c_float, [_P, C.c_size_t]),
+            "orc_get_mapq": (C.c_uint8, [_P, C.c_size_t, _P]),
+            "orc_is_proper_nam_pair": (C.c_int, [_P, _P, C.c_float, C.c_float]),
+            "orc_pair_type": (C.c_int, [_P, C.c_size_t, _P, C.c_size_t, C.c_float, C.c_float,
+                                        C.c_float]),
+            "orc_best_scoring_nam_pairs": (C.c_long, [_P, C.c_size_t, _P, C.c_size_t, C.c_float,
+                                                      C.c_float, C.c_int, _P, C.c_size_t]),
+            "orc_pairs_batch": (C.c_int, [_P, _P, C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                          C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -322,3 +333,81 @@ def lookup_bytes(idx, qrs):
     b = C.c_uint64()
     lib().orc_lookup_bytes(idx.h, _p(qrs), len(qrs), C.byref(b))
     return b.value
+
+
+# ---- NAM consumer of paired reads ----
+
+def _nam_arr(nams):
+    return np.ascontiguousarray(nams, dtype=NAM)
+
+
+def sort_nams_reference(nams, mate):
+    a = _nam_arr(nams).copy()
+    lib().orc_sort_nams_reference(_p(a), len(a), mate)
+    return a
+
+
+def top_dropoff(nams):
+    a = _nam_arr(nams)
+    return lib().orc_top_dropoff(_p(a), len(a))
+
+
+def get_mapq(nams, n_max=None):
+    a = _nam_arr(nams)
+    m = _nam_arr([n_max] if n_max is not None else a[:1])
+    return lib().orc_get_mapq(_p(a), len(a), _p(m))
+
+
+def is_proper_nam_pair(nam1, nam2, mu=300.0, sigma=100.0):
+    a, b = _nam_arr([nam1]), _nam_arr([nam2])
+    return bool(lib().orc_is_proper_nam_pair(_p(a), _p(b), mu, sigma))
+
+
+def pair_type(nams1, nams2, dropoff=0.5, mu=300.0, sigma=100.0):
+    a, b = _nam_arr(nams1), _nam_arr(nams2)
+    return lib().orc_pair_type(_p(a), len(a), _p(b), len(b), dropoff, mu, sigma)
+
+
+def best_scoring_nam_pairs(nams1, nams2, mu=300.0, sigma=100.0, max_tries=20):
+    a, b = _nam_arr(nams1), _nam_arr(nams2)
+    cap = 4 * max_tries + 1
+    out = np.empty(cap, dtype=NAM_PAIR)
+    n = lib().orc_best_scoring_nam_pairs(_p(a), len(a), _p(b), len(b), mu, sigma, max_tries,
+                                         _p(out), cap)
+    if n < 0:
+        raise ValueError("empty NAM list or bad max_tries")
+    return out[:n].copy()
+
+
+def pairs_batch(nam_offsets, nams, n_pairs, dropoff=0.5, mu=300.0, sigma=100.0, max_tries=20,
+                joint=True):
+    """The pair step on 2 n_pairs lists -> dict with ord (NAMs in the
+    reference order), types, type_lists, top_dropoff, mapq, joint_offsets,
+    joint."""
+    noff = np.ascontiguousarray(nam_offsets, dtype=np.uint64)
+    a = _nam_arr(nams)
+    ord_ = np.empty(max(len(a), 1), dtype=NAM)
+    types = np.empty(max(n_pairs, 1), dtype=np.uint8)
+    drop = np.empty(max(2 * n_pairs, 1), dtype=np.float32)
+    mapq = np.empty(max(2 * n_pairs, 1), dtype=np.uint8)
+    stride = 4 * max_tries + 1
+    jc = np.zeros(max(n_pairs, 1), dtype=np.uint32)
+    jout = np.empty(max(n_pairs * stride if joint else 1, 1), dtype=NAM_PAIR)
+    rc = lib().orc_pairs_batch(_p(noff), _p(a), n_pairs, dropoff, mu, sigma, max_tries, int(joint),
+                               _p(ord_), _p(types), _p(drop), _p(mapq), _p(jc), _p(jout))
+    if rc:
+        raise RuntimeError(f"orc_pairs_batch failed ({rc})")
+    types = types[:n_pairs]
+    joff = np.zeros(n_pairs + 1, dtype=np.uint64)
+    if joint and n_pairs:
+        joff[1:] = np.cumsum(jc[:n_pairs], dtype=np.uint64)
+        sel = np.concatenate([np.arange(p * stride, p * stride + jc[p]) for p in range(n_pairs)])
+        jflat = jout[sel.astype(np.int64)] if len(sel) else np.empty(0, dtype=NAM_PAIR)
+    else:
+        jflat = np.empty(0, dtype=NAM_PAIR)
+    return {
+        "ord": ord_[: len(a)], "types": types,
+        "type_lists": [np.nonzero(types == t)[0].astype(np.uint32) for t in range(5)],
+        "top_dropoff": drop[: 2 * n_pairs], "mapq": mapq[: 2 * n_pairs],
+        "joint_offsets": joff, "joint": jflat,
+    }
