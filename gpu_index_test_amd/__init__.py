@@ -24,7 +24,7 @@ __all__ = [
     "randstrobes_query", "find_nams", "find_nams_rescue", "synth_genome", "synth_reads",
     "pack_reads", "GisError", "NAM", "QUERY_RANDSTROBE", "REF_RANDSTROBE", "SYNCMER", "HIT",
     "GIS_NOT_FOUND", "FLAG_RESCUED", "FLAG_NO_RANDSTROBES", "MODE_PLAIN", "MODE_FUSED",
-    "PairResult", "pairs_from_nams", "NAM_PAIR",
+    "PairResult", "pairs_from_nams", "NAM_PAIR", "FastqReader", "FastqChunk",
 ]
 
 
@@ -681,6 +681,18 @@ class Server:
                                             C.byref(wall)), None)
         return lat[: len(arrival_s)], wall.value
 
+    def submit_fastq(self, reader):
+        """Take the next chunk of a FastqReader and submit it as one request
+        -> (request, FastqChunk), or (None, None) at the end of the input."""
+        rid = C.c_uint64()
+        c = N.FastqChunk()
+        code = self.ctx.L.gis_server_submit_fastq(self.h, reader.h, C.byref(rid), C.byref(c))
+        if code:
+            raise GisError(code, reader._last_error() if code in (7, 11) else "")
+        if c.n_records == 0:
+            return None, None
+        return (rid.value, c.n_reads), FastqChunk._from_c(c)
+
     def stats(self):
         st = N.ServerStats()
         _check(self.ctx.L.gis_server_stats_get(self.h, C.byref(st)), None)
@@ -690,6 +702,91 @@ class Server:
         if getattr(self, "h", None):
             self.ctx.L.gis_server_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------- FASTQ ingestion ----------------
+
+class FastqChunk:
+    """One chunk of records: concatenated reads (with two files, mate 1 of
+    record i is read i and mate 2 read n_records + i) and their names."""
+
+    def __init__(self, index, first_record, n_records, seqs, offsets, lens, names, name_offsets):
+        self.index, self.first_record, self.n_records = index, first_record, n_records
+        self.seqs, self.offsets, self.lens = seqs, offsets, lens
+        self.names, self.name_offsets = names, name_offsets
+
+    @classmethod
+    def _from_c(cls, c):
+        n = c.n_reads
+
+        def arr(p, count, dt):
+            if not count:
+                return np.zeros(0, dtype=dt)
+            return np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))),
+                                         shape=(count,)).copy()
+        noff = arr(c.name_offsets, n + 1, np.uint64)
+        return cls(c.index, c.first_record, c.n_records, arr(c.seqs, c.bases, np.uint8),
+                   arr(c.offsets, n, np.uint64), arr(c.lens, n, np.uint32),
+                   arr(c.names, int(noff[-1]) if n else 0, np.uint8), noff)
+
+    @property
+    def n_reads(self):
+        return len(self.lens)
+
+    def read(self, i):
+        o = int(self.offsets[i])
+        return self.seqs[o: o + int(self.lens[i])].tobytes()
+
+    def name(self, i):
+        return self.names[int(self.name_offsets[i]): int(self.name_offsets[i + 1])].tobytes()
+
+
+class FastqReader:
+    """Chunked FASTQ reader (one file, or two mates files): a reader thread
+    and parser threads work ahead; iterate to get FastqChunk objects in file
+    order."""
+
+    def __init__(self, path1, path2=None, chunk_bytes=0, threads=0):
+        h = C.c_void_p()
+        code = N.lib().gis_fastq_open(os.fsencode(path1), os.fsencode(path2) if path2 else None, chunk_bytes,
+                                      threads, C.byref(h))
+        if code:
+            raise GisError(code, f"cannot open {path1}" + (f" / {path2}" if path2 else ""))
+        self.h = h
+
+    def _last_error(self):
+        return N.lib().gis_fastq_last_error(self.h).decode()
+
+    def next_chunk(self):
+        c = N.FastqChunk()
+        code = N.lib().gis_fastq_next(self.h, C.byref(c))
+        if code:
+            raise GisError(code, self._last_error())
+        return FastqChunk._from_c(c) if c.n_records else None
+
+    def __iter__(self):
+        while True:
+            c = self.next_chunk()
+            if c is None:
+                return
+            yield c
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().gis_fastq_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

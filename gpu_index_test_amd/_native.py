@@ -24,6 +24,7 @@ STATUS = {
     0: "GIS_OK", 1: "GIS_ERR_BAD_PARAMETER", 2: "GIS_ERR_INVALID_INDEX", 3: "GIS_ERR_HIP",
     4: "GIS_ERR_NOMEM", 5: "GIS_ERR_CAPACITY", 6: "GIS_ERR_STATE", 7: "GIS_ERR_IO",
     8: "GIS_ERR_BUDGET", 9: "GIS_ERR_CANCELLED", 10: "GIS_ERR_INVALID_FASTA",
+    11: "GIS_ERR_INVALID_INPUT",
 }
 
 # numpy views of the C structs (byte layouts of the reference types)
@@ -118,6 +119,13 @@ class PairOut(C.Structure):
                 ("n_joint", C.c_uint64)]
 
 
+class FastqChunk(C.Structure):
+    _fields_ = [("n_records", C.c_uint32), ("n_reads", C.c_uint32), ("seqs", C.c_void_p),
+                ("offsets", C.c_void_p), ("lens", C.c_void_p), ("names", C.c_void_p),
+                ("name_offsets", C.c_void_p), ("index", C.c_uint64), ("first_record", C.c_uint64),
+                ("bases", C.c_uint64)]
+
+
 # name -> (restype, argtypes); every name must be exported by libgis.so
 _P = C.c_void_p
 _U32, _U64, _I32 = C.c_uint32, C.c_uint64, C.c_int32
@@ -168,6 +176,11 @@ PROTOTYPES = {
     "gis_batch_pairs": (C.c_int, [_P, C.POINTER(PairParams)]),
     "gis_batch_pairs_download": (C.c_int, [_P, C.POINTER(PairOut)]),
     "gis_pairs_from_nams": (C.c_int, [_P, _P, _P, _U32, C.POINTER(PairParams), C.POINTER(PairOut)]),
+    "gis_fastq_open": (C.c_int, [C.c_char_p, C.c_char_p, _U64, _U32, C.POINTER(_P)]),
+    "gis_fastq_next": (C.c_int, [_P, C.POINTER(FastqChunk)]),
+    "gis_fastq_last_error": (C.c_char_p, [_P]),
+    "gis_fastq_close": (None, [_P]),
+    "gis_server_submit_fastq": (C.c_int, [_P, _P, C.POINTER(_U64), C.POINTER(FastqChunk)]),
     "gis_synth_genome": (C.c_int, [C.POINTER(GenomeSpec), _P, _P]),
     "gis_synth_reads": (C.c_int, [C.POINTER(ReadSpec), _P, _P, _U32, _P, _P, _P, _P, _P, _P]),
 }

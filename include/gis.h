@@ -36,7 +36,8 @@ typedef enum {
     GIS_ERR_IO = 7,
     GIS_ERR_BUDGET = 8,        /* request larger than the serving HBM budget */
     GIS_ERR_CANCELLED = 9,
-    GIS_ERR_INVALID_FASTA = 10 /* InvalidFasta, step2/exceptions.hpp:12-15 */
+    GIS_ERR_INVALID_FASTA = 10, /* InvalidFasta, step2/exceptions.hpp:12-15 */
+    GIS_ERR_INVALID_INPUT = 11  /* malformed FASTQ */
 } gis_status;
 
 /* ---------------- plain data types (layouts match the reference) ---------------- */
@@ -378,6 +379,49 @@ int gis_batch_pairs_download(gis_batch *b, gis_pair_out *out);
  * NAMs of one list must differ in (is_rc, ref_id, query_start, ref_start)). */
 int gis_pairs_from_nams(gis_ctx *ctx, const uint64_t *nam_offsets, const gis_nam *nams, uint32_t n_pairs,
                         const gis_pair_params *pp, gis_pair_out *out);
+
+/* ---------------- FASTQ ingestion ----------------
+ * The chunked producer/consumer in front of the path (producer_pe_fastq_task
+ * and the chunk formatting of step2/main.cu:65-84,4212-4240; chunk reading of
+ * step2/FastxStream.h:305-364). A reader thread reads plain or gzip FASTQ in
+ * chunks of about chunk_bytes cut after complete records (two files: the same
+ * record count from each), parser threads format chunks in parallel, and
+ * chunks come out in file order.
+ * Records: lines end at '\n' (a '\r' stays in the line, as the reference
+ * reads CRLF files), the read is the second line, the name the first line
+ * without the leading '@' up to the first space (step2/main.cu:4121-4138).
+ * GIS_ERR_INVALID_INPUT for a record not starting with '@', a third line not
+ * starting with '+', an incomplete last record, mates files of different
+ * record counts, or a read longer than GIS_MAX_READ_LEN. */
+typedef struct gis_fastq gis_fastq;
+
+/* A chunk of records. With two files, mate 1 of record i is read i and mate
+ * 2 is read n_records + i (the layout of the pair step). Valid until the next
+ * gis_fastq_next or gis_fastq_close. */
+typedef struct {
+    uint32_t n_records;           /* 0: end of input */
+    uint32_t n_reads;             /* n_records, or 2 n_records with two files */
+    const char *seqs;
+    const uint64_t *offsets;      /* [n_reads] */
+    const uint32_t *lens;         /* [n_reads] */
+    const char *names;
+    const uint64_t *name_offsets; /* [n_reads + 1] */
+    uint64_t index;               /* chunk number from 0 */
+    uint64_t first_record;        /* records before this chunk */
+    uint64_t bases;
+} gis_fastq_chunk;
+
+/* path2 NULL: single-end. chunk_bytes 0: 4 MiB (the reference's 1 << 22
+ * pool buffers, step2/main.cu:4214). threads 0: up to 8 parser threads.
+ * GIS_ERR_IO if a file cannot be opened. */
+int gis_fastq_open(const char *path1, const char *path2, uint64_t chunk_bytes, uint32_t threads,
+                   gis_fastq **out);
+int gis_fastq_next(gis_fastq *f, gis_fastq_chunk *out);
+const char *gis_fastq_last_error(const gis_fastq *f);
+void gis_fastq_close(gis_fastq *f);
+/* The next chunk of f submitted to the server as one request (its reads as
+ * gis_fastq_next lays them out); id 0 and chunk->n_records 0 at the end. */
+int gis_server_submit_fastq(gis_server *s, gis_fastq *f, uint64_t *id, gis_fastq_chunk *chunk);
 
 /* ---------------- synthetic inputs (deterministic) ---------------- */
 typedef struct {
