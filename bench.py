@@ -61,14 +61,62 @@ def make_batches(G, idx, seqs, offs, lens, batch):
     mappers = []
     for i in range(nb):
         lo, hi = i * batch, min(n, (i + 1) * batch)
-        base = int(offs[lo])
-        end = int(offs[hi - 1] + lens[hi - 1])
+        base = int(offs[lo:hi].min())
+        end = int((offs[lo:hi] + lens[lo:hi]).max())
         # every batch runs on the first batch's device workspace (the batches
         # run one after another); only the reads stay resident per batch
         m = G.Mapper(idx, hi - lo, end - base, share=mappers[0] if mappers else None)
         m.upload(seqs[base:end], offs[lo:hi] - base, lens[lo:hi])
         mappers.append(m)
     return mappers
+
+
+def pair_blocks(offs, lens, batch):
+    """Adjacent mates -> per batch the pair step's layout: the batch's mate 1
+    reads, then its mate 2 reads (same pairs, same batches)."""
+    n = len(lens)
+    order = np.empty(n, dtype=np.int64)
+    for lo in range(0, n, batch):
+        hi = min(n, lo + batch)
+        k = (hi - lo) // 2
+        order[lo: lo + k] = np.arange(lo, hi, 2)
+        order[lo + k: hi] = np.arange(lo + 1, hi, 2)
+    return offs[order], lens[order]
+
+
+def pair_step(G, O, mappers, steps, mode):
+    """The pair step (reference order, types, MAPQ, joint NAM pairs) on each
+    batch's resident NAMs, timed alone; batch 0 is checked against the
+    oracle's pair step on the same NAMs, which also gives the CPU rate."""
+    C = G._native.C
+    pp = G._pair_params(joint=True)
+    nb = len(mappers)
+    ms, pairs_done = [], 0
+    for i in range(steps + 1):
+        m = mappers[i % nb]
+        m.run(mode=mode)
+        t0 = time.perf_counter()
+        m.ctx.check(m.ctx.L.gis_batch_pairs(m.h, C.byref(pp)))
+        if i:  # the first call sizes the pair buffers
+            ms.append((time.perf_counter() - t0) * 1e3)
+            pairs_done += m.n // 2
+    m = mappers[0]
+    m.run(mode=mode)
+    got = m.pairs()
+    res = m.download()
+    n_pairs = m.n // 2
+    t0 = time.perf_counter()
+    exp = O.pairs_batch(res.nam_offsets, res.nams, n_pairs)
+    cpu_s = time.perf_counter() - t0
+    same = (got.nams.tobytes() == exp["ord"].tobytes() and np.array_equal(got.types, exp["types"])
+            and got.joint.tobytes() == exp["joint"].tobytes() and np.array_equal(got.mapq, exp["mapq"])
+            and got.top_dropoff.tobytes() == exp["top_dropoff"].tobytes())
+    return {"pairs_per_s": pairs_done / (sum(ms) / 1e3), "p50_batch_ms": float(np.percentile(ms, 50)),
+            "p99_batch_ms": float(np.percentile(ms, 99)), "pairs_per_batch": n_pairs,
+            "type_counts": list(got.type_counts), "joint_entries_batch0": int(len(got.joint)),
+            "bit_exact_batch0": bool(same),
+            "cpu_oracle": {"value": n_pairs / cpu_s, "unit": "pairs/s", "cores": 1, "kind": "port",
+                           "sample": f"pair step of batch 0 ({n_pairs} pairs)"}}
 
 
 def timed(mappers, steps, warmup, mode, graph):
@@ -100,9 +148,10 @@ def kernel_times(G, mappers, k, mode):
         m = mappers[i]
         m.run(mode=mode, graph=False)
         st = m.stage_times()
-        b, q = G._native.C.c_uint64(), G._native.C.c_uint64()
-        m.ctx.check(m.ctx.L.gis_batch_lookup_bytes(m.h, G._native.C.byref(b), G._native.C.byref(q)))
-        st["_lookup_bytes"], st["_lookups"] = b.value, q.value
+        if mode == G.MODE_PLAIN:
+            b, q = G._native.C.c_uint64(), G._native.C.c_uint64()
+            m.ctx.check(m.ctx.L.gis_batch_lookup_bytes(m.h, G._native.C.byref(b), G._native.C.byref(q)))
+            st["_lookup_bytes"], st["_lookups"] = b.value, q.value
         out.append(st)
     return out
 
@@ -238,6 +287,9 @@ def run_c3(args, G):
                                         seed=2)
     log(f"reads {len(lens)} ({pairs} pairs, mean length {lens.mean():.1f}) in {time.time() - t0:.1f}s")
     batch = args.batch or 131072
+    # each batch holds whole pairs laid out for the pair step (mate 1 block,
+    # mate 2 block)
+    offs, lens = pair_blocks(offs, lens, batch)
     mappers = make_batches(G, idx, seqs, offs, lens, batch)
     steps = args.steps
 
@@ -252,6 +304,10 @@ def run_c3(args, G):
     spill_frac = float(np.mean((mappers[0].download().flags & 0x80) != 0))
     stages = kernel_times(G, mappers, steps, G.MODE_PLAIN)
     roof = lookup_roofline(stages)
+    from oracle import oracle_lib as O
+    pairs_out = pair_step(G, O, mappers, steps, G.MODE_FUSED)
+    log(f"pair step: {pairs_out['pairs_per_s'] / 1e6:.1f} M pairs/s, bit-exact batch 0: "
+        f"{pairs_out['bit_exact_batch0']}")
 
     cpu = None
     if not args.no_cpu:
@@ -287,6 +343,7 @@ def run_c3(args, G):
         "roofline": roof,
         "cpu_baseline": cpu,
         "fused_stage_ms": stage_means(fstage),
+        "pair_step": pairs_out,
         "plain_form": {"value": plain_rps, "unit": "reads/s",
                        "ms_per_step": plain_elapsed * 1e3 / steps,
                        "p50_batch_ms": float(np.percentile(plain_ms, 50)),
@@ -393,6 +450,70 @@ def run_c4(args, G):
     }
 
 
+def write_fastq(path, seqs, offs, lens, idx):
+    """FASTQ records of the given reads (name, read, '+', quality of 'I')."""
+    with open(path, "wb") as f:
+        buf = []
+        for k, i in enumerate(idx):
+            o, n = int(offs[i]), int(lens[i])
+            buf.append(b"@read%d/1 synthetic\n%s\n+\n%s\n" % (k, seqs[o: o + n].tobytes(), b"I" * n))
+            if len(buf) == 4096:
+                f.write(b"".join(buf))
+                buf = []
+        f.write(b"".join(buf))
+    return os.path.getsize(path)
+
+
+def fastq_leg(G, srv, seqs, offs, lens, n_se, n_pairs, in_flight):
+    """FASTQ ingestion into the serving queue: two mates files of n_pairs
+    records are read in 4 MiB chunks by the reader and parser threads, each
+    chunk submitted as one request, up to in_flight requests outstanding.
+    Reports the end-to-end rate (files to NAMs) and the reader's rate alone."""
+    import tempfile
+    n_pairs = min(n_pairs, (len(lens) - n_se) // 2)
+    first = n_se + 2 * np.arange(n_pairs)
+    d = tempfile.mkdtemp(prefix="gis_fastq_")
+    p1, p2 = os.path.join(d, "r_1.fq"), os.path.join(d, "r_2.fq")
+    try:
+        nbytes = write_fastq(p1, seqs, offs, lens, first) + write_fastq(p2, seqs, offs, lens, first + 1)
+        t0 = time.perf_counter()
+        recs = 0
+        c = G._native.FastqChunk()
+        with G.FastqReader(p1, p2) as r:
+            while True:
+                code = G._native.lib().gis_fastq_next(r.h, G._native.C.byref(c))
+                if code:
+                    raise G.GisError(code, r._last_error())
+                if not c.n_records:
+                    break
+                recs += c.n_records
+        parse_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        reads, chunks, pending = 0, 0, []
+        with G.FastqReader(p1, p2) as r:
+            while True:
+                rq, ch = srv.submit_fastq(r, keep=False)
+                if rq is not None:
+                    pending.append(rq)
+                    chunks += 1
+                if pending and (len(pending) >= in_flight or rq is None):
+                    res, _ = srv.wait(pending.pop(0))
+                    reads += len(res.nam_offsets) - 1
+                if rq is None and not pending:
+                    break
+        e2e_s = time.perf_counter() - t0
+    finally:
+        for f in (p1, p2):
+            if os.path.exists(f):
+                os.remove(f)
+        os.rmdir(d)
+    log(f"fastq: {recs} pairs ({nbytes / 1e6:.0f} MB) parsed in {parse_s:.2f}s; files -> NAMs "
+        f"{reads / e2e_s / 1e6:.1f} M reads/s over {chunks} chunk requests")
+    return {"pairs": int(recs), "file_bytes": int(nbytes), "chunk_bytes": 4 << 20,
+            "reader_reads_per_s": 2 * recs / parse_s, "reader_gb_per_s": nbytes / parse_s / 1e9,
+            "end_to_end_reads_per_s": reads / e2e_s, "requests": chunks, "in_flight": in_flight}
+
+
 def run_c5(args, G):
     """Serving on the 3.1 Gbp index: Poisson request arrivals at five rates
     (fractions of the saturated throughput), each request 1-8,192 reads
@@ -461,6 +582,8 @@ def run_c5(args, G):
         log(f"rate {frac:.0%}: {row['sustained_reads_per_s'] / 1e6:.1f} M reads/s sustained, "
             f"p50 {row['p50_ms']:.2f} ms, p99 {row['p99_ms']:.2f} ms over {n} requests")
         rates.append(row)
+    fastq = fastq_leg(G, srv, seqs, offs, lens, n_se, args.fastq_pairs, 2 * args.slots) \
+        if args.fastq_pairs else None
     st = srv.stats()
     srv.close()
     best = max(rates, key=lambda r: r["sustained_reads_per_s"])
@@ -489,6 +612,7 @@ def run_c5(args, G):
             "saturation_reads_per_s": sat, "parallelism": "single GPU",
         },
         "rates": rates,
+        "fastq_ingest": fastq,
         "server_stats": st,
     }
 
@@ -504,6 +628,8 @@ def main():
     ap.add_argument("--server-mode", choices=("plain", "fused"), default="plain")
     ap.add_argument("--burst", type=int, default=6000, help="c5: requests of the saturation burst")
     ap.add_argument("--rate-seconds", type=float, default=3.0, help="c5: seconds of arrivals per rate")
+    ap.add_argument("--fastq-pairs", type=int, default=400000,
+                    help="c5: pairs written to FASTQ and served from the files (0: skip)")
     ap.add_argument("--reads", type=int, default=1_000_000, help="c2: single-end reads")
     ap.add_argument("--pairs", type=int, default=0, help="c3/c4: read pairs (2M / 10M)")
     ap.add_argument("--batch", type=int, default=0, help="reads per batch (c2 65536, c3 131072, c4 262144)")

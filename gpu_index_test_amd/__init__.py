@@ -681,9 +681,11 @@ class Server:
                                             C.byref(wall)), None)
         return lat[: len(arrival_s)], wall.value
 
-    def submit_fastq(self, reader):
+    def submit_fastq(self, reader, keep=True):
         """Take the next chunk of a FastqReader and submit it as one request
-        -> (request, FastqChunk), or (None, None) at the end of the input."""
+        -> (request, FastqChunk), or (None, None) at the end of the input.
+        keep=False skips copying the chunk's reads and names to Python and
+        returns the record count in place of the FastqChunk."""
         rid = C.c_uint64()
         c = N.FastqChunk()
         code = self.ctx.L.gis_server_submit_fastq(self.h, reader.h, C.byref(rid), C.byref(c))
@@ -691,7 +693,7 @@ class Server:
             raise GisError(code, reader._last_error() if code in (7, 11) else "")
         if c.n_records == 0:
             return None, None
-        return (rid.value, c.n_reads), FastqChunk._from_c(c)
+        return (rid.value, c.n_reads), (FastqChunk._from_c(c) if keep else c.n_records)
 
     def stats(self):
         st = N.ServerStats()

@@ -7,11 +7,11 @@
 // One reader thread pulls raw bytes (plain or gzip, through zlib) in chunks
 // of about chunk_bytes and cuts them after the last complete record: records
 // are four lines, so the cut is found by counting newlines from the start of
-// the file (AVX2 compare + popcount over 64-byte blocks), never by guessing
-// at '@'. With two files the chunks of both mates are cut to the same record
+// the file (AVX2 compare over 64-byte blocks), never by guessing at '@'. Two
+// mates files are read and scanned side by side and cut to the same record
 // count. Parser threads format chunks in parallel into concatenated reads and
 // names; the consumer receives them in file order. At most `ahead` chunks are
-// read but not yet consumed.
+// read but not yet consumed, and their buffers are recycled.
 //
 // Records follow the reference's formatting: lines end at '\n' only (a '\r'
 // of CRLF files stays in the line, as in neoGetLine), the read is the second
@@ -29,6 +29,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -38,20 +39,48 @@
 
 namespace {
 
+// growable byte buffer that never zero-fills (recycled between chunks, so
+// steady state touches no fresh pages)
+struct Bytes {
+    std::unique_ptr<char[]> p;
+    size_t cap = 0, len = 0;
+    void reserve(size_t n) {
+        if (n <= cap) return;
+        const size_t c = std::max(n, cap + cap / 2);
+        std::unique_ptr<char[]> q(new char[c]);
+        if (len) std::memcpy(q.get(), p.get(), len);
+        p.swap(q);
+        cap = c;
+    }
+    void append(const char *src, size_t n) {
+        reserve(len + n);
+        std::memcpy(p.get() + len, src, n);
+        len += n;
+    }
+};
+
 struct Parsed {
     uint64_t index = 0, first_record = 0;
     uint32_t n_records = 0, n_reads = 0;
-    std::vector<char> seqs, names;
+    Bytes seqs, names;
     std::vector<uint64_t> offsets, name_offsets;
     std::vector<uint32_t> lens;
     int status = GIS_OK;
     std::string err;
+    void reset() {
+        seqs.len = names.len = 0;
+        offsets.clear();
+        name_offsets.clear();
+        lens.clear();
+        status = GIS_OK;
+        err.clear();
+    }
 };
 
 struct Raw {
     uint64_t index = 0, first_record = 0;
     uint32_t records = 0;
-    std::string part[2];
+    Bytes part[2];
 };
 
 // newline mask of 64 bytes
@@ -72,29 +101,30 @@ uint64_t count_newlines(const char *p, size_t n) {
     return c;
 }
 
-// offset just past the k-th newline (k >= 1); n if there are fewer
-size_t after_newline(const char *p, size_t n, uint64_t k) {
+// complete records of p[0, n) (every fourth newline ends one): their count,
+// with the offset just past each record's last newline in ends
+uint64_t scan_records(const char *p, size_t n, std::vector<size_t> &ends) {
+    ends.clear();
+    uint64_t lines = 0;
     size_t i = 0;
     for (; i + 64 <= n; i += 64) {
         uint64_t m = nl_mask(p + i);
-        const uint64_t c = (uint64_t)__builtin_popcountll(m);
-        if (c >= k) {
-            while (--k) m &= m - 1;
-            return i + (size_t)__builtin_ctzll(m) + 1;
+        while (m) {
+            if ((++lines & 3) == 0) ends.push_back(i + (size_t)__builtin_ctzll(m) + 1);
+            m &= m - 1;
         }
-        k -= c;
     }
     for (; i < n; i++)
-        if (p[i] == '\n' && --k == 0) return i + 1;
-    return n;
+        if (p[i] == '\n' && (++lines & 3) == 0) ends.push_back(i + 1);
+    return ends.size();
 }
 
 // The rest of a file (everything after the last cut) without the empty
-// lines ("\n" or "\r\n") that follow its last record. A line is added
-// back when the records would otherwise miss their last line: an empty
-// quality line of an empty read.
-size_t end_of_records(const std::string &s) {
-    size_t n = s.size();
+// lines ("\n" or "\r\n") that follow its last record. A line is added back
+// when the records would otherwise miss their last line: an empty quality
+// line of an empty read.
+size_t end_of_records(const char *s, size_t len) {
+    size_t n = len;
     int stripped = 0;
     for (;;) {
         if (n >= 1 && s[n - 1] == '\n' && (n == 1 || s[n - 2] == '\n')) {
@@ -106,7 +136,7 @@ size_t end_of_records(const std::string &s) {
         }
         stripped++;
     }
-    if (stripped && count_newlines(s.data(), n) % 4 == 3) n += s[n] == '\r' ? 2 : 1;
+    if (stripped && count_newlines(s, n) % 4 == 3) n += s[n] == '\r' ? 2 : 1;
     return n;
 }
 
@@ -122,76 +152,102 @@ struct gis_fastq {
     std::vector<std::thread> parsers;
     std::mutex mu;
     std::condition_variable cv;
-    std::deque<Raw> raw;
-    std::map<uint64_t, Parsed> done;
+    std::deque<std::unique_ptr<Raw>> raw;              // cut, waiting for a parser
+    std::vector<std::unique_ptr<Raw>> raw_free;        // recycled
+    std::map<uint64_t, std::unique_ptr<Parsed>> done;  // parsed, waiting for the consumer
+    std::vector<std::unique_ptr<Parsed>> parsed_free;  // recycled
     uint64_t n_cut = 0;      // chunks cut by the reader
     uint64_t next_out = 0;   // next chunk the consumer takes
     bool reader_done = false, stop = false;
     // an error of the reader, reported after the chunks cut before it
     int read_status = GIS_OK;
     std::string read_err;
-    Parsed current;
-    std::string err;  // message of the last failed call
+    std::unique_ptr<Parsed> current;  // the chunk handed out last
+    std::string err;                  // message of the last failed call
 };
 
 namespace {
 
-// append up to `want` bytes of file k to s; false at end of file
-bool read_more(gis_fastq *q, int k, std::string &s, size_t want, int &status, std::string &err) {
-    const size_t old = s.size();
-    s.resize(old + want);
+// append up to `want` bytes of file k to b; false at end of file
+bool read_more(gis_fastq *q, int k, Bytes &b, size_t want, int &status, std::string &err) {
+    b.reserve(b.len + want + 1);
     size_t got = 0;
     while (got < want) {
         const unsigned ask = (unsigned)std::min<size_t>(want - got, 1u << 30);
-        const int r = gzread(q->f[k], &s[old + got], ask);
+        const int r = gzread(q->f[k], b.p.get() + b.len + got, ask);
         if (r < 0) {
             int zerr = 0;
             const char *m = gzerror(q->f[k], &zerr);
             status = GIS_ERR_IO;
             err = q->path[k] + ": read error: " + (m ? m : "");
-            s.resize(old + got);
+            b.len += got;
             return false;
         }
         if (r == 0) break;
         got += (size_t)r;
     }
-    s.resize(old + got);
+    b.len += got;
     return got == want;
 }
 
 void reader_main(gis_fastq *q) {
-    std::string carry[2];
+    Bytes carry[2];
+    std::vector<size_t> ends[2];
     bool eof[2] = {false, false};
     uint64_t first_record = 0;
     int status = GIS_OK;
     std::string err;
     for (;;) {
+        std::unique_ptr<Raw> rc;
         {
             std::unique_lock<std::mutex> lk(q->mu);
             q->cv.wait(lk, [&] { return q->stop || q->n_cut - q->next_out < q->ahead; });
             if (q->stop) break;
-        }
-        Raw rc;
-        uint64_t recs[2] = {0, 0};
-        // read until both parts hold a complete record (a record longer than
-        // a chunk makes the read longer)
-        size_t want = q->chunk_bytes;
-        for (;;) {
-            for (int k = 0; k < q->n_files; k++) {
-                std::string &s = rc.part[k];
-                if (s.empty()) s.swap(carry[k]);
-                if (!eof[k] && s.size() < want) {
-                    if (!read_more(q, k, s, want - s.size(), status, err)) eof[k] = true;
-                    if (status) break;
-                }
-                if (eof[k]) {
-                    // a last line without '\n' ends the file
-                    if (!s.empty() && s.back() != '\n') s.push_back('\n');
-                    s.resize(end_of_records(s));
-                }
-                recs[k] = count_newlines(s.data(), s.size()) / 4;
+            if (!q->raw_free.empty()) {
+                rc = std::move(q->raw_free.back());
+                q->raw_free.pop_back();
             }
-            if (status) break;
+        }
+        if (!rc) rc.reset(new Raw());
+        uint64_t recs[2] = {0, 0};
+        for (int k = 0; k < q->n_files; k++) {
+            rc->part[k].len = 0;
+            rc->part[k].append(carry[k].p.get(), carry[k].len);
+            carry[k].len = 0;
+        }
+        // read until both parts hold a complete record (a record longer than
+        // a chunk makes the read longer); two mates files are read and
+        // scanned by two threads, as the reference reads its left and right
+        // chunks
+        size_t want = q->chunk_bytes;
+        int st[2] = {GIS_OK, GIS_OK};
+        std::string er[2];
+        auto fill = [&](int k) {
+            Bytes &b = rc->part[k];
+            if (!eof[k] && b.len < want) {
+                if (!read_more(q, k, b, want - b.len, st[k], er[k])) eof[k] = true;
+                if (st[k]) return;
+            }
+            if (eof[k]) {
+                // a last line without '\n' ends the file
+                if (b.len && b.p[b.len - 1] != '\n') b.append("\n", 1);
+                b.len = end_of_records(b.p.get(), b.len);
+            }
+            recs[k] = scan_records(b.p.get(), b.len, ends[k]);
+        };
+        for (;;) {
+            if (q->n_files == 2) {
+                std::thread t(fill, 1);
+                fill(0);
+                t.join();
+            } else {
+                fill(0);
+            }
+            if (st[0] || st[1]) {
+                status = st[0] ? st[0] : st[1];
+                err = st[0] ? er[0] : er[1];
+                break;
+            }
             const uint64_t r = q->n_files == 2 ? std::min(recs[0], recs[1]) : recs[0];
             const bool all_eof = eof[0] && (q->n_files == 1 || eof[1]);
             if (r > 0 || all_eof) break;
@@ -202,7 +258,7 @@ void reader_main(gis_fastq *q) {
         if (r == 0) {
             // end of input: whatever is left is an incomplete record or a
             // mates file with more records than the other
-            const bool left0 = !rc.part[0].empty(), left1 = q->n_files == 2 && !rc.part[1].empty();
+            const bool left0 = rc->part[0].len != 0, left1 = q->n_files == 2 && rc->part[1].len != 0;
             if (left0 || left1) {
                 status = GIS_ERR_INVALID_INPUT;
                 if (q->n_files == 2 && (recs[0] || recs[1]))
@@ -215,16 +271,17 @@ void reader_main(gis_fastq *q) {
             break;
         }
         for (int k = 0; k < q->n_files; k++) {
-            const size_t cut = after_newline(rc.part[k].data(), rc.part[k].size(), 4 * r);
-            carry[k].assign(rc.part[k], cut, std::string::npos);
-            rc.part[k].resize(cut);
+            Bytes &b = rc->part[k];
+            const size_t cut = ends[k][r - 1];
+            carry[k].append(b.p.get() + cut, b.len - cut);
+            b.len = cut;
         }
-        rc.records = (uint32_t)r;
-        rc.first_record = first_record;
+        rc->records = (uint32_t)r;
+        rc->first_record = first_record;
         first_record += r;
         {
             std::lock_guard<std::mutex> lk(q->mu);
-            rc.index = q->n_cut++;
+            rc->index = q->n_cut++;
             q->raw.push_back(std::move(rc));
         }
         q->cv.notify_all();
@@ -236,11 +293,16 @@ void reader_main(gis_fastq *q) {
     q->cv.notify_all();
 }
 
+void malformed(Parsed &out, const std::string &path, uint64_t record, const char *what) {
+    out.status = GIS_ERR_INVALID_INPUT;
+    out.err = path + ": record " + std::to_string(record) + what;
+}
+
 // one file's records of a chunk appended to out; false on malformed input
-bool format_part(const std::string &s, uint32_t records, uint64_t first_record, const std::string &path,
+bool format_part(const Bytes &b, uint32_t records, uint64_t first_record, const std::string &path,
                  Parsed &out) {
-    const char *p = s.data();
-    const char *end = p + s.size();
+    const char *p = b.p.get();
+    const char *end = p + b.len;
     for (uint32_t r = 0; r < records; r++) {
         const char *line[4];
         size_t len[4];
@@ -250,39 +312,36 @@ bool format_part(const std::string &s, uint32_t records, uint64_t first_record, 
             len[k] = (size_t)(nl - p);
             p = nl + 1;
         }
-        const std::string where = path + ": record " + std::to_string(first_record + r + 1);
         if (len[0] == 0 || line[0][0] != '@') {
-            out.status = GIS_ERR_INVALID_INPUT;
-            out.err = where + " does not start with '@'";
+            malformed(out, path, first_record + r + 1, " does not start with '@'");
             return false;
         }
         if (len[2] == 0 || line[2][0] != '+') {
-            out.status = GIS_ERR_INVALID_INPUT;
-            out.err = where + ": third line does not start with '+'";
+            malformed(out, path, first_record + r + 1, ": third line does not start with '+'");
             return false;
         }
         if (len[1] > GIS_MAX_READ_LEN) {
-            out.status = GIS_ERR_INVALID_INPUT;
-            out.err = where + ": read longer than " + std::to_string(GIS_MAX_READ_LEN) + " bases";
+            malformed(out, path, first_record + r + 1, ": read longer than 65535 bases");
             return false;
         }
-        out.offsets.push_back(out.seqs.size());
+        out.offsets.push_back(out.seqs.len);
         out.lens.push_back((uint32_t)len[1]);
-        out.seqs.insert(out.seqs.end(), line[1], line[1] + len[1]);
+        out.seqs.append(line[1], len[1]);
         // ConvertNeo2KSeq: drop a leading '@', stop at the first space
         const char *nm = line[0] + 1;
         size_t nl = len[0] - 1;
         const void *sp = std::memchr(line[0], ' ', len[0]);
         if (sp) nl = (size_t)(static_cast<const char *>(sp) - nm);
-        out.names.insert(out.names.end(), nm, nm + nl);
-        out.name_offsets.push_back(out.names.size());
+        out.names.append(nm, nl);
+        out.name_offsets.push_back(out.names.len);
     }
     return true;
 }
 
 void parser_main(gis_fastq *q) {
     for (;;) {
-        Raw rc;
+        std::unique_ptr<Raw> rc;
+        std::unique_ptr<Parsed> out;
         {
             std::unique_lock<std::mutex> lk(q->mu);
             q->cv.wait(lk, [&] { return q->stop || !q->raw.empty() || q->reader_done; });
@@ -293,23 +352,28 @@ void parser_main(gis_fastq *q) {
             }
             rc = std::move(q->raw.front());
             q->raw.pop_front();
+            if (!q->parsed_free.empty()) {
+                out = std::move(q->parsed_free.back());
+                q->parsed_free.pop_back();
+            }
         }
-        Parsed out;
-        out.index = rc.index;
-        out.first_record = rc.first_record;
-        out.n_records = rc.records;
-        out.n_reads = rc.records * (uint32_t)q->n_files;
-        size_t bytes = rc.part[0].size() + rc.part[1].size();
-        out.seqs.reserve(bytes / 2);
-        out.offsets.reserve(out.n_reads);
-        out.lens.reserve(out.n_reads);
-        out.name_offsets.reserve(out.n_reads + 1);
-        out.name_offsets.push_back(0);
+        if (!out) out.reset(new Parsed());
+        out->reset();
+        out->index = rc->index;
+        out->first_record = rc->first_record;
+        out->n_records = rc->records;
+        out->n_reads = rc->records * (uint32_t)q->n_files;
+        out->seqs.reserve(rc->part[0].len + rc->part[1].len);
+        out->offsets.reserve(out->n_reads);
+        out->lens.reserve(out->n_reads);
+        out->name_offsets.reserve(out->n_reads + 1);
+        out->name_offsets.push_back(0);
         for (int k = 0; k < q->n_files; k++)
-            if (!format_part(rc.part[k], rc.records, rc.first_record, q->path[k], out)) break;
+            if (!format_part(rc->part[k], rc->records, rc->first_record, q->path[k], *out)) break;
         {
             std::lock_guard<std::mutex> lk(q->mu);
-            q->done.emplace(out.index, std::move(out));
+            q->done.emplace(out->index, std::move(out));
+            q->raw_free.push_back(std::move(rc));
         }
         q->cv.notify_all();
     }
@@ -357,7 +421,7 @@ extern "C" int gis_fastq_open(const char *path1, const char *path2, uint64_t chu
 extern "C" int gis_fastq_next(gis_fastq *q, gis_fastq_chunk *out) {
     if (!q || !out) return GIS_ERR_BAD_PARAMETER;
     std::memset(out, 0, sizeof *out);
-    Parsed next;
+    std::unique_ptr<Parsed> next;
     {
         std::unique_lock<std::mutex> lk(q->mu);
         q->cv.wait(lk, [&] {
@@ -376,24 +440,27 @@ extern "C" int gis_fastq_next(gis_fastq *q, gis_fastq_chunk *out) {
         next = std::move(it->second);
         q->done.erase(it);
         q->next_out++;
+        if (q->current) q->parsed_free.push_back(std::move(q->current));
     }
     q->cv.notify_all();
-    if (next.status) {
-        q->err = next.err;
-        return next.status;
+    if (next->status) {
+        q->err = next->err;
+        std::lock_guard<std::mutex> lk(q->mu);
+        q->parsed_free.push_back(std::move(next));
+        return GIS_ERR_INVALID_INPUT;
     }
     q->current = std::move(next);
-    Parsed &c = q->current;
+    Parsed &c = *q->current;
     out->n_records = c.n_records;
     out->n_reads = c.n_reads;
-    out->seqs = c.seqs.data();
+    out->seqs = c.seqs.p.get();
     out->offsets = c.offsets.data();
     out->lens = c.lens.data();
-    out->names = c.names.data();
+    out->names = c.names.p.get();
     out->name_offsets = c.name_offsets.data();
     out->index = c.index;
     out->first_record = c.first_record;
-    out->bases = c.seqs.size();
+    out->bases = c.seqs.len;
     return GIS_OK;
 }
 
