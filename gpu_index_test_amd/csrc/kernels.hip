@@ -766,13 +766,56 @@ __device__ __forceinline__ bool nam_accepts(int o_qp, int o_qe, int o_rp, int o_
     return near && (ext || con);
 }
 
+// The wide list of one merge: entries below near_cap in `near` (LDS when the
+// caller has it), the rest at the same indices of `far`. near_cap is a
+// multiple of 64, so the 64 entries of a chunk are all on one side.
+// The LDS part is typed as LDS so that its accesses compile to LDS
+// instructions rather than flat ones.
+typedef __attribute__((address_space(3))) WideNam LdsWideNam;
+
+struct WideList {
+    LdsWideNam *near;
+    WideNam *far;
+    uint32_t near_cap;
+};
+
+// a wide list wholly in `p` (LDS or global)
+__device__ __forceinline__ WideList wide_in(WideNam *p) { return WideList{nullptr, p, 0}; }
+
+__device__ __forceinline__ WideNam wget(const WideList &L, uint32_t e) {
+    WideNam w;
+    if (e < L.near_cap) {
+        const LdsWideNam &x = L.near[e];
+        w = WideNam{x.qs, x.qe, x.qp, x.rs, x.re, x.rp, x.n, x.cid};
+    } else {
+        w = L.far[e];
+    }
+    return w;
+}
+
+__device__ __forceinline__ void wput(const WideList &L, uint32_t e, const WideNam &w) {
+    if (e < L.near_cap) {
+        LdsWideNam &x = L.near[e];
+        x.qs = w.qs;
+        x.qe = w.qe;
+        x.qp = w.qp;
+        x.rs = w.rs;
+        x.re = w.re;
+        x.rp = w.rp;
+        x.n = w.n;
+        x.cid = w.cid;
+    } else {
+        L.far[e] = w;
+    }
+}
+
 // write the wide list's NAMs of the group that ends (all of them are final)
-__device__ __forceinline__ void wide_flush(const WideNam *wide, uint32_t nw, gis_nam *out, uint32_t base,
+__device__ __forceinline__ void wide_flush(const WideList &wide, uint32_t nw, gis_nam *out, uint32_t base,
                                            uint32_t grp, uint32_t cap) {
     for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
         const uint32_t e = c0 + lane_id();
         if (e < nw) {
-            const WideNam w = wide[e];
+            const WideNam w = wget(wide, e);
             if (base + w.cid < cap) lane_nam_store(out, base + w.cid, grp, from_wide(w));
         }
     }
@@ -819,16 +862,53 @@ __device__ __forceinline__ const uint64_t *group_order(uint32_t nh, const DevHit
     return order;
 }
 
+// Reference-interval bounds of one 64-entry chunk of the wide list: every
+// entry's previous hit start is at least lo and its ref end at most hi. Starts only grow and ends are raised with the entry, so the bounds
+// stay valid (if loose) until the list is compacted and they are recomputed.
+struct ChunkBox {
+    int lo, hi;
+};
+
+__device__ __forceinline__ int wave_min_i(int v) {
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+// bounds of the wide list's chunks c0 .. (all of them when c0 == 0)
+__device__ __forceinline__ void boxes_rebuild(const WideList &wide, uint32_t nw, ChunkBox *box) {
+    for (uint32_t c = 0; c * 64 < nw; c++) {
+        const uint32_t e = c * 64 + lane_id();
+        int lo = 0x7fffffff, hi = (int)0x80000000;
+        if (e < nw) {
+            const WideNam w = wget(wide, e);
+            lo = w.rp;
+            hi = w.re;
+        }
+        lo = wave_min_i(lo);
+        hi = wave_max_i(hi);
+        if (lane_id() == 0) box[c] = ChunkBox{lo, hi};
+    }
+}
+
 // Merge hits order[xb .. xe) (hit indices in grouped order; order nullptr:
 // the hits themselves in order), or, with pre, the xe <= 64 hits already in
 // the chunk registers. NAM c of the k-th group of the range is written at
 // out[(NAMs of the range's earlier groups) + c]; entries at index >= cap are
 // not written. Returns the NAM count, or MERGE_OVERFLOW when a group needs
 // more than 64 open NAMs and there is no wide list.
+// box (may be nullptr): bounds of up to box_cap chunks of the wide list. A
+// hit can only be accepted by an entry with previous hit start < ref start <=
+// ref end, so the scan for the first accepting entry visits only the chunks
+// whose bounds admit the hit, in order: the same entry as a full scan.
 __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64_t *order, uint32_t xb,
                                                 uint32_t xe, bool pre, uint32_t c_rsd, uint32_t c_rs,
-                                                uint32_t c_re, uint32_t c_q, WideNam *wide, gis_nam *out,
-                                                uint32_t cap) {
+                                                uint32_t c_re, uint32_t c_q, const WideList wide, gis_nam *out,
+                                                uint32_t cap, ChunkBox *box = nullptr, uint32_t box_cap = 0) {
     const uint32_t lane = lane_id();
     LaneNam o{0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t n_open = 0, grp = 0xffffffffu, base = 0, gcnt = 0;
@@ -866,19 +946,26 @@ __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64
                 grp = rsd;
             }
             if (!wide_mode) {
-                bool ext;
+                bool ext = false;
                 const bool acc = lane < n_open && nam_accepts(o.qp, o.qe, o.rp, o.re, qs, qe, rs, re, ext);
                 const uint64_t bal = __ballot(acc);
-                if (bal) {
-                    if (lane == (uint32_t)__builtin_ctzll(bal)) {
-                        if (ext) {
-                            o.qe = qe;
-                            o.re = re;
-                        }
-                        o.qp = qs;
-                        o.rp = rs;
-                        o.n++;
-                    }
+                if (bal || n_open < 64) {
+                    // the first accepting NAM takes the hit, or, with none, the
+                    // hit opens NAM n_open; predicated updates, no branches
+                    const bool fresh = bal == 0;
+                    const uint32_t win = fresh ? n_open : (uint32_t)__builtin_ctzll(bal);
+                    const bool me = lane == win;
+                    const bool end = me && (fresh || ext);
+                    o.qs = me && fresh ? qs : o.qs;
+                    o.rs = me && fresh ? rs : o.rs;
+                    o.qe = end ? qe : o.qe;
+                    o.re = end ? re : o.re;
+                    o.qp = me ? qs : o.qp;
+                    o.rp = me ? rs : o.rp;
+                    o.n = me ? (fresh ? 1 : o.n + 1) : o.n;
+                    o.cid = me && fresh ? gcnt : o.cid;
+                    n_open += fresh ? 1u : 0u;
+                    gcnt += fresh ? 1u : 0u;
                     continue;
                 }
                 if (n_open == 64) {
@@ -886,13 +973,17 @@ __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64
                     const uint64_t km = __ballot(keep);
                     const uint32_t nk = (uint32_t)__popcll(km);
                     if (nk == 64) {
-                        if (!wide) return MERGE_OVERFLOW;
+                        if (!wide.near && !wide.far) return MERGE_OVERFLOW;
                         // the group continues on the wide list, in creation order
-                        wide[lane] = WideNam{o.qs, o.qe, o.qp, o.rs, o.re, o.rp, (uint32_t)o.n, o.cid};
+                        wput(wide, lane, WideNam{o.qs, o.qe, o.qp, o.rs, o.re, o.rp, (uint32_t)o.n, o.cid});
                         nw = 64;
                         wide_mark = 64;
                         wide_mode = true;
                         n_open = 0;
+                        if (box && box_cap) {
+                            const int lo = wave_min_i(o.rp), hi = wave_max_i(o.re);
+                            if (lane == 0) box[0] = ChunkBox{lo, hi};
+                        }
                     } else {
                         if (!keep && base + o.cid < cap) lane_nam_store(out, base + o.cid, grp, o);
                         const uint32_t dst = keep ? mbcnt(km) : nk + mbcnt(~km);
@@ -917,27 +1008,47 @@ __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64
             } else {
                 // wide form: the first accepting entry of the list
                 bool done = false;
-                for (uint32_t c0 = 0; c0 < nw && !done; c0 += 64) {
-                    const uint32_t e = c0 + lane;
-                    WideNam w{};
-                    bool ext = false, acc = false;
-                    if (e < nw) {
-                        w = wide[e];
-                        acc = nam_accepts(w.qp, w.qe, w.rp, w.re, qs, qe, rs, re, ext);
-                    }
-                    const uint64_t bal = __ballot(acc);
-                    if (bal) {
-                        if (lane == (uint32_t)__builtin_ctzll(bal)) {
-                            if (ext) {
-                                w.qe = qe;
-                                w.re = re;
-                            }
-                            w.qp = qs;
-                            w.rp = rs;
-                            w.n++;
-                            wide[e] = w;
+                const uint32_t nch = (nw + 63) >> 6;
+                const bool boxed = box && nch <= box_cap;
+                for (uint32_t cc = 0; cc < nch && !done; cc += 64) {
+                    // chunks whose bounds admit the hit (all of them without bounds)
+                    uint64_t cm;
+                    if (boxed) {
+                        const uint32_t c = cc + lane;
+                        bool cand = false;
+                        if (c < nch) {
+                            const ChunkBox bx = box[c];
+                            cand = bx.lo < rs && rs <= bx.hi;
                         }
-                        done = true;
+                        cm = __ballot(cand);
+                    } else {
+                        cm = nch - cc >= 64 ? ~0ull : ((1ull << (nch - cc)) - 1);
+                    }
+                    while (cm && !done) {
+                        const uint32_t c = cc + (uint32_t)__builtin_ctzll(cm);
+                        cm &= cm - 1;
+                        const uint32_t e = c * 64 + lane;
+                        WideNam w{};
+                        bool ext = false, acc = false;
+                        if (e < nw) {
+                            w = wget(wide, e);
+                            acc = nam_accepts(w.qp, w.qe, w.rp, w.re, qs, qe, rs, re, ext);
+                        }
+                        const uint64_t bal = __ballot(acc);
+                        if (bal) {
+                            if (lane == (uint32_t)__builtin_ctzll(bal)) {
+                                if (ext) {
+                                    w.qe = qe;
+                                    w.re = re;
+                                    if (boxed && box[c].hi < re) box[c].hi = re;
+                                }
+                                w.qp = qs;
+                                w.rp = rs;
+                                w.n++;
+                                wput(wide, e, w);
+                            }
+                            done = true;
+                        }
                     }
                 }
                 if (done) {
@@ -952,21 +1063,34 @@ __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64
                         WideNam w{};
                         bool keep = false;
                         if (e < nw) {
-                            w = wide[e];
+                            w = wget(wide, e);
                             keep = w.qe >= qs;
                             if (!keep && base + w.cid < cap) lane_nam_store(out, base + w.cid, grp, from_wide(w));
                         }
                         const uint64_t kb = __ballot(keep);
                         wave_sync_mem();
-                        if (keep) wide[kept + mbcnt(kb)] = w;
+                        if (keep) wput(wide, kept + mbcnt(kb), w);
                         kept += (uint32_t)__popcll(kb);
                         wave_sync_mem();
                     }
                     nw = kept;
                     wide_mark = kept > 64 ? kept : 64;
+                    if (box && ((nw + 63) >> 6) <= box_cap) boxes_rebuild(wide, nw, box);
                 }
             }
-            if (lane == 0) wide[nw] = WideNam{qs, qe, qs, rs, re, rs, 1u, gcnt};
+            if (lane == 0) {
+                wput(wide, nw, WideNam{qs, qe, qs, rs, re, rs, 1u, gcnt});
+                // the new entry's chunk bounds (only while every chunk has them)
+                const uint32_t c = nw >> 6;
+                if (box && c < box_cap && ((nw + 63) >> 6) <= box_cap) {
+                    if ((nw & 63) == 0) {
+                        box[c] = ChunkBox{rs, re};
+                    } else {
+                        if (box[c].lo > rs) box[c].lo = rs;
+                        if (box[c].hi < re) box[c].hi = re;
+                    }
+                }
+            }
             nw++;
             gcnt++;
             wave_sync_mem();
@@ -988,7 +1112,7 @@ __device__ __forceinline__ uint32_t merge_wave(uint32_t nh, const DevHit *hits, 
                                                uint32_t cap) {
     const uint32_t lane = lane_id();
     if (nh > 64) return merge_range(hits, group_order(nh, hits, keys, keys_alt), 0, nh, false, 0, 0, 0, 0,
-                                    wide, out, cap);
+                                    wide_in(wide), out, cap);
     DevHit h{};
     if (lane < nh) h = hits[lane];
     const uint32_t rsd = lane < nh ? h.ref_strand : 0xffffffffu;
@@ -1005,7 +1129,7 @@ __device__ __forceinline__ uint32_t merge_wave(uint32_t nh, const DevHit *hits, 
         c_re = (uint32_t)__shfl((int)c_re, src, 64);
         c_q = (uint32_t)__shfl((int)c_q, src, 64);
     }
-    return merge_range(hits, nullptr, 0, nh, true, c_rsd, c_rs, c_re, c_q, wide, out, cap);
+    return merge_range(hits, nullptr, 0, nh, true, c_rsd, c_rs, c_re, c_q, wide_in(wide), out, cap);
 }
 
 struct alignas(8) MergeLds {
@@ -1020,10 +1144,15 @@ constexpr uint32_t MERGE_DEFER = 0xfffffffeu;
 // then k_merge_split_finish), for reads with more than SPLIT_MIN hits
 constexpr uint32_t MERGE_SPLIT = 0xfffffffdu;
 constexpr uint32_t SPLIT_MIN = 512;
+// hits of one k_merge_groups work item: consecutive small groups of a split
+// read merge on one wave
+constexpr uint32_t ITEM_HITS = 384;
 // wide lists in LDS: per wave of k_merge for reads of at most this many
 // hits, and per group of k_merge_groups (one wave per block)
 constexpr int MERGE_WIDE_LDS = 256;
 constexpr int GROUP_WIDE_LDS = 1024;
+// chunk bounds of a k_merge_groups wide list (lists up to 32,768 entries)
+constexpr uint32_t GROUP_BOXES = 512;
 
 __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int rescue, int only_deferred,
                                                     int split) {
@@ -1051,7 +1180,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
         uint64_t *gkeys = b.merge_scratch ? (rescue ? b.rkeys : b.keys) + 2 * hb : nullptr;
         WideNam *gwide = b.merge_scratch ? (rescue ? b.rnam_ws : b.nam_ws) + hb : nullptr;
         if (split && nh > SPLIT_MIN && b.merge_scratch) {
-            // group the hits; with several groups, each merges on its own wave
+            // group the hits; each group merges on its own wave
             const uint64_t *ord = group_order(nh, ghits, gkeys, gkeys + nh);
             const uint32_t code = ord == nullptr ? 0u : (ord == gkeys ? 1u : 2u);
             uint64_t *slots = code == 1u ? gkeys + nh : gkeys;
@@ -1068,27 +1197,40 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
                 G += (uint32_t)__popcll(bal);
             }
             wave_sync_mem();
-            if (G >= 2) {
+            // work items: runs of consecutive groups of at least ITEM_HITS
+            // hits (a larger group is an item of its own); each item's start
+            // goes to slots[item], which later holds its NAM count
+            uint32_t K = 0;
+            if (lane == 0) {
+                uint32_t cur = 0;
+                for (uint32_t g = 1; g < G; g++) {
+                    const uint32_t st = (uint32_t)(slots[g] >> 32);
+                    if (st - cur >= ITEM_HITS) {
+                        slots[K++] = (uint64_t)cur << 32;  // K < g: heads not read yet stay intact
+                        cur = st;
+                    }
+                }
+                slots[K++] = (uint64_t)cur << 32;
+            }
+            K = (uint32_t)__builtin_amdgcn_readfirstlane((int)K);
+            wave_sync_mem();
+            {
                 uint32_t at = 0;
-                if (lane == 0) at = atomicAdd(&b.ctl->n_gwork[rescue ? 1 : 0], G);
+                if (lane == 0) at = atomicAdd(&b.ctl->n_gwork[rescue ? 1 : 0], K);
                 at = (uint32_t)__builtin_amdgcn_readfirstlane((int)at);
-                for (uint32_t g = lane; g < G; g += 64) {
+                for (uint32_t g = lane; g < K; g += 64) {
                     const uint32_t gs = (uint32_t)(slots[g] >> 32);
-                    const uint32_t ge = g + 1 < G ? (uint32_t)(slots[g + 1] >> 32) : nh;
+                    const uint32_t ge = g + 1 < K ? (uint32_t)(slots[g + 1] >> 32) : nh;
                     b.gwork[at + g] = make_uint4(r, gs, ge, g | (code << 30));
                 }
                 if (lane == 0) {
                     const uint32_t si = atomicAdd(&b.ctl->n_split[rescue ? 1 : 0], 1u);
-                    b.split_list[si] = make_uint4(r, G, code, 0u);
+                    b.split_list[si] = make_uint4(r, K, code, 0u);
                     b.nam_cnt[r] = MERGE_SPLIT;
                 }
                 wave_sync_mem();
                 continue;
             }
-            const uint32_t t1 = merge_range(ghits, ord, 0, nh, false, 0, 0, 0, 0, gwide, fin, nh);
-            if (lane == 0) b.nam_cnt[r] = t1;
-            wave_sync_mem();
-            continue;
         }
         uint32_t total = MERGE_OVERFLOW;
         if (nh <= (uint32_t)MERGE_WIDE_LDS) gwide = lwide[wid];  // NAMs <= hits fit
@@ -1109,11 +1251,12 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
     }
 }
 
-// one wave per group work item: the group's hits (in grouped order) merge
-// into NAMs at the group's own hit-indexed region of the read's output; the
-// NAM count goes to the read's group slot
+// one wave per work item (a run of consecutive groups of a split read): its
+// hits (in grouped order) merge into NAMs at the item's own hit-indexed region
+// of the read's output; the NAM count goes to the read's item slot
 __global__ __launch_bounds__(64) void k_merge_groups(DevBatch b, int rescue) {
     __shared__ WideNam lwide[GROUP_WIDE_LDS];
+    __shared__ ChunkBox lbox[GROUP_BOXES];
     const uint32_t lane = lane_id();
     const unsigned int cnt = b.ctl->n_gwork[rescue ? 1 : 0];
     for (uint32_t w = blockIdx.x; w < cnt; w += gridDim.x) {
@@ -1125,9 +1268,12 @@ __global__ __launch_bounds__(64) void k_merge_groups(DevBatch b, int rescue) {
         const uint64_t *ord = code == 0u ? nullptr : (code == 1u ? gkeys : gkeys + nh);
         uint64_t *slots = code == 1u ? gkeys + nh : gkeys;
         const DevHit *ghits = (rescue ? b.rhits : b.hits) + hb;
-        WideNam *gwide = ge - gs <= (uint32_t)GROUP_WIDE_LDS ? lwide : (rescue ? b.rnam_ws : b.nam_ws) + hb + gs;
+        // the first GROUP_WIDE_LDS wide entries in LDS, the rest in the
+        // group's global region (as many entries as hits)
+        WideNam *far = (rescue ? b.rnam_ws : b.nam_ws) + hb + gs;
         gis_nam *out = (rescue ? b.rnam_fin : b.nam_fin) + hb + gs;
-        const uint32_t c = merge_range(ghits, ord, gs, ge, false, 0, 0, 0, 0, gwide, out, ge - gs);
+        const WideList wl{(LdsWideNam *)lwide, far, (uint32_t)GROUP_WIDE_LDS};
+        const uint32_t c = merge_range(ghits, ord, gs, ge, false, 0, 0, 0, 0, wl, out, ge - gs, lbox, GROUP_BOXES);
         if (lane == 0) slots[g] = ((uint64_t)gs << 32) | c;
         wave_sync_mem();
     }

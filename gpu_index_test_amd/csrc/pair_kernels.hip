@@ -12,10 +12,11 @@
 //                 counts, the bound on its joint entries.
 //  k_type_scatter pair ids grouped by type in id order (the host loop of
 //                 step2/main.cu:4641-4646) from the scanned block counts.
-//  k_joint        one lane per pair with NAMs on both mates:
+//  k_joint_wave   one wave per pair with NAMs on both mates:
 //                 get_best_scoring_nam_pairs (step2/main.cu:1719-1830),
 //                 including its quicksort, whose order of equal entries is
-//                 part of the result.
+//                 part of the result; k_joint (one lane per pair) takes the
+//                 pairs whose entries exceed the wave kernel's LDS.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -337,6 +338,7 @@ __device__ void lomuto_sort(gis_nam_pair *E, int n, const gis_nam *A, const gis_
 }
 
 constexpr int FLAG_WORDS = PAIR_MAX_TRIES / 32;
+constexpr uint32_t JOINT_LDS_CAP = 128;  // entries a pair may have in k_joint_wave
 constexpr int FLAG_STRIDE = 2 * FLAG_WORDS + 1;  // odd stride: lanes fall on different banks
 
 // get_best_scoring_nam_pairs, step2/main.cu:1719-1830
@@ -346,10 +348,8 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_joint(PairDev P) {
     const uint32_t n = P.n_pairs;
     if (p >= n) return;
     const uint32_t cap = P.jbound[p];
-    if (!cap) {
-        P.jcnt[p] = 0;
-        return;
-    }
+    // bounds within JOINT_LDS_CAP (and pairs without entries) are k_joint_wave's
+    if (cap <= JOINT_LDS_CAP) return;
     uint32_t *f1 = flags + threadIdx.x * FLAG_STRIDE, *f2 = f1 + FLAG_WORDS;
     for (int w = 0; w < FLAG_WORDS; w++) f1[w] = f2[w] = 0;
     const uint64_t o1 = P.off[p], o2 = P.off[n + p];
@@ -407,6 +407,250 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_joint(PairDev P) {
     P.jcnt[p] = size;
 }
 
+// ---------------------------------------------------------------------------
+// get_best_scoring_nam_pairs with one wave per pair (pairs whose entry bound
+// fits JOINT_LDS_CAP; max_tries up to 31 always does)
+//
+// The reference's scan over (i, j) is sequential: at each j it stops when
+// the joint hits fall below half the best pushed so far or more than
+// 2 max_tries entries are pushed, then pushes (i, j) if the NAMs form a
+// proper pair. Over 64 lanes j at once, the best and the count before each
+// lane are the running values plus an exclusive prefix max / count over the
+// proper lanes below it; the first lane whose condition holds is where the
+// reference stops, and the proper lanes below it are exactly its pushes.
+// The entries are then sorted by ranking keys in LDS. Without two equal keys
+// the sorted order is unique and equals the reference quicksort's; with
+// equal keys one lane replays the quicksort on the entry indices.
+
+constexpr int JOINT_WPB = 4;
+
+// the NamPair comparator's fields of one entry
+struct EntryKey {
+    int32_t sc;
+    float s1;
+    int32_t qe1, qs1, re1, rs1;
+    uint32_t rc1;
+    float s2;
+    int32_t qe2, qs2, re2, rs2;
+    uint32_t rc2;
+};
+
+__device__ __forceinline__ EntryKey entry_key(const gis_nam_pair &e, const gis_nam *A, const gis_nam *B) {
+    const CmpNam a = cmp_nam(A, e.i1), b = cmp_nam(B, e.i2);
+    return EntryKey{e.score, a.score, a.qe, a.qs, a.re, a.rs, a.is_rc, b.score, b.qe, b.qs, b.re, b.rs, b.is_rc};
+}
+
+// entry_less on materialised keys
+__device__ __forceinline__ bool ekey_less(const EntryKey &x, const EntryKey &y) {
+    if (x.sc != y.sc) return x.sc > y.sc;
+    if (x.s1 != y.s1) return x.s1 > y.s1;
+    if (x.rc1 != y.rc1) return !x.rc1;
+    if (x.qe1 != y.qe1) return x.qe1 < y.qe1;
+    if (x.qs1 != y.qs1) return x.qs1 < y.qs1;
+    if (x.re1 != y.re1) return x.re1 < y.re1;
+    if (x.rs1 != y.rs1) return x.rs1 < y.rs1;
+    if (x.s2 != y.s2) return x.s2 > y.s2;
+    if (x.rc2 != y.rc2) return !x.rc2;
+    if (x.qe2 != y.qe2) return x.qe2 < y.qe2;
+    if (x.qs2 != y.qs2) return x.qs2 < y.qs2;
+    if (x.re2 != y.re2) return x.re2 < y.re2;
+    return x.rs2 < y.rs2;
+}
+
+__device__ __forceinline__ bool ekey_eq(const EntryKey &x, const EntryKey &y) {
+    return x.sc == y.sc && x.s1 == y.s1 && x.rc1 == y.rc1 && x.qe1 == y.qe1 && x.qs1 == y.qs1 &&
+           x.re1 == y.re1 && x.rs1 == y.rs1 && x.s2 == y.s2 && x.rc2 == y.rc2 && x.qe2 == y.qe2 &&
+           x.qs2 == y.qs2 && x.re2 == y.re2 && x.rs2 == y.rs2;
+}
+
+// quick_sort_iterative (step2/main.cu:317-359) on entry indices, smaller
+// part first as in lomuto_sort
+__device__ void lomuto_idx(uint8_t *I, int n, const EntryKey *K) {
+    int st_lo[16], st_hi[16];
+    int sp = 0;
+    int lo = 0, hi = n - 1;
+    for (;;) {
+        while (lo < hi) {
+            const uint8_t pv = I[hi];
+            int i = lo - 1;
+            for (int j = lo; j < hi; j++) {
+                const uint8_t ej = I[j];
+                if (ekey_less(K[ej], K[pv])) {
+                    ++i;
+                    I[j] = I[i];
+                    I[i] = ej;
+                }
+            }
+            const int p = i + 1;
+            I[hi] = I[p];
+            I[p] = pv;
+            const int lsz = p - lo, rsz = hi - p;
+            if (lsz <= rsz) {
+                if (rsz >= 2) {
+                    st_lo[sp] = p + 1;
+                    st_hi[sp] = hi;
+                    sp++;
+                }
+                hi = p - 1;
+            } else {
+                if (lsz >= 2) {
+                    st_lo[sp] = lo;
+                    st_hi[sp] = p - 1;
+                    sp++;
+                }
+                lo = p + 1;
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        lo = st_lo[sp];
+        hi = st_hi[sp];
+    }
+}
+
+__device__ __forceinline__ int wave_excl_max(int v) {
+    const uint32_t lane = lane_id();
+    int x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x = max(x, y);
+    }
+    const int ex = __shfl_up(x, 1, 64);
+    return lane == 0 ? 0 : ex;
+}
+
+__global__ __launch_bounds__(64 * JOINT_WPB) void k_joint_wave(PairDev P) {
+    __shared__ gis_nam_pair lE[JOINT_WPB][JOINT_LDS_CAP];
+    __shared__ EntryKey lK[JOINT_WPB][JOINT_LDS_CAP];
+    __shared__ uint32_t lF[JOINT_WPB][2 * FLAG_WORDS];
+    __shared__ uint8_t lI[JOINT_WPB][JOINT_LDS_CAP];
+    const uint32_t wid = threadIdx.x >> 6, lane = lane_id();
+    gis_nam_pair *E = lE[wid];
+    EntryKey *K = lK[wid];
+    uint32_t *f1 = lF[wid], *f2 = f1 + FLAG_WORDS;
+    uint8_t *I = lI[wid];
+    const uint32_t n = P.n_pairs;
+    const int mt = P.max_tries;
+    const float lim = __builtin_fmaf(10.0f, P.sigma, P.mu);
+    for (uint32_t p = blockIdx.x * JOINT_WPB + wid; p < n; p += gridDim.x * JOINT_WPB) {
+        const uint32_t cap = P.jbound[p];
+        if (cap == 0 || cap > JOINT_LDS_CAP) {
+            // no entries, or a bound beyond LDS: k_joint takes the pair
+            if (cap == 0 && lane == 0) P.jcnt[p] = 0;
+            continue;
+        }
+        if (lane < 2 * FLAG_WORDS) f1[lane] = 0;
+        wave_sync();
+        const uint64_t o1 = P.off[p], o2 = P.off[n + p];
+        const uint32_t c1 = (uint32_t)(P.off[p + 1] - o1), c2 = (uint32_t)(P.off[n + p + 1] - o2);
+        const gis_nam *A = P.ord + o1, *B = P.ord + o2;
+        // the first 64 NAMs of each list stay in registers (lane i holds
+        // NAM i); rows and columns beyond them are loaded as they come
+        PairNam ra{}, rb{};
+        if (lane < c1) ra = pair_nam(A[lane]);
+        if (lane < c2) rb = pair_nam(B[lane]);
+        uint32_t size = 0;
+        int best = 0;
+        for (uint32_t i = 0; i < c1; i++) {
+            PairNam a;
+            if (i < 64) {
+                a.ref_id = __builtin_amdgcn_readlane(ra.ref_id, (int)i);
+                a.rs = __builtin_amdgcn_readlane(ra.rs, (int)i);
+                a.qs = __builtin_amdgcn_readlane(ra.qs, (int)i);
+                a.nh = __builtin_amdgcn_readlane(ra.nh, (int)i);
+                a.is_rc = (uint8_t)__builtin_amdgcn_readlane((int)ra.is_rc, (int)i);
+            } else {
+                a = pair_nam(A[i]);
+            }
+            bool stopped = false;
+            for (uint32_t jc = 0; jc < c2 && !stopped; jc += 64) {
+                const uint32_t j = jc + lane;
+                const bool valid = j < c2;
+                PairNam b = rb;
+                if (jc && valid) b = pair_nam(B[j]);
+                const int joint = a.nh + b.nh;
+                const bool prop = valid && proper_pair(a, b, lim);
+                const uint64_t pm = __ballot(prop);
+                const uint32_t before = size + mbcnt(pm);
+                const int bb = max(best, wave_excl_max(prop ? joint : 0));
+                // joint < 0.5 * best in double is 2 joint < best in integers
+                const bool stop = valid && (2 * joint < bb || before > (uint32_t)(2 * mt));
+                const uint64_t sm = __ballot(stop);
+                const uint32_t first = sm ? (uint32_t)__builtin_ctzll(sm) : 64u;
+                const bool push = prop && lane < first;
+                if (push) {
+                    E[before] = gis_nam_pair{joint, (int)i, (int)j};
+                    if (i < (uint32_t)mt) atomicOr(&f1[i >> 5], 1u << (i & 31));
+                    if (j < (uint32_t)mt) atomicOr(&f2[j >> 5], 1u << (j & 31));
+                }
+                const uint64_t qm = __ballot(push);
+                size += (uint32_t)__popcll(qm);
+                best = max(best, (int)wave_max(push ? (uint32_t)joint : 0u));
+                stopped = first < 64;
+            }
+            if (size > (uint32_t)(2 * mt)) break;
+        }
+        wave_sync();
+        // NAMs of each mate in no pushed pair, while their hits reach half
+        // the best joint hits (the list's first NAM's hits without one)
+        for (int m = 0; m < 2; m++) {
+            const gis_nam *L = m ? B : A;
+            const uint32_t c = m ? c2 : c1;
+            const uint32_t *fl = m ? f2 : f1;
+            const int bestm = best > 0 ? best : __builtin_amdgcn_readfirstlane(m ? rb.nh : ra.nh);
+            const uint32_t lm = min(c, (uint32_t)mt);
+            for (uint32_t ic = 0; ic < lm; ic += 64) {
+                const uint32_t i = ic + lane;
+                const bool valid = i < lm;
+                const int nh = !valid ? 0 : (ic == 0 ? (m ? rb.nh : ra.nh) : L[i].n_hits);
+                const bool stop = valid && nh < bestm / 2;
+                const uint64_t sm = __ballot(stop);
+                const uint32_t first = sm ? (uint32_t)__builtin_ctzll(sm) : 64u;
+                const bool push = valid && lane < first && !((fl[i >> 5] >> (i & 31)) & 1u);
+                const uint64_t qm = __ballot(push);
+                if (push) E[size + mbcnt(qm)] = m ? gis_nam_pair{nh, -1, (int)i} : gis_nam_pair{nh, (int)i, -1};
+                size += (uint32_t)__popcll(qm);
+                if (first < 64) break;
+            }
+        }
+        wave_sync();
+        const uint32_t ne = size;
+        for (uint32_t e = lane; e < ne; e += 64) K[e] = entry_key(E[e], A, B);
+        wave_sync();
+        uint32_t rank[2] = {0, 0};
+        bool tie = false;
+        for (int k = 0; k < 2; k++) {
+            const uint32_t e = lane + 64 * k;
+            if (e < ne) {
+                const EntryKey me = K[e];
+                uint32_t r = 0;
+                for (uint32_t f = 0; f < ne; f++) {
+                    const EntryKey o = K[f];
+                    if (ekey_less(o, me)) r++;
+                    else if (f != e && ekey_eq(o, me)) tie = true;
+                }
+                rank[k] = r;
+            }
+        }
+        gis_nam_pair *out = P.jtmp + P.jboff[p];
+        if (!__any(tie)) {
+            for (int k = 0; k < 2; k++) {
+                const uint32_t e = lane + 64 * k;
+                if (e < ne) out[rank[k]] = E[e];
+            }
+        } else {
+            if (lane == 0) {
+                for (uint32_t e = 0; e < ne; e++) I[e] = (uint8_t)e;
+                lomuto_idx(I, (int)ne, K);
+            }
+            wave_sync();
+            for (uint32_t e = lane; e < ne; e += 64) out[e] = E[I[e]];
+        }
+        if (lane == 0) P.jcnt[p] = ne;
+        wave_sync();
+    }
+}
+
 __global__ __launch_bounds__(PAIR_BLOCK) void k_joint_compact(PairDev P) {
     const uint32_t p = blockIdx.x * PAIR_BLOCK + threadIdx.x;
     if (p >= P.n_pairs) return;
@@ -434,7 +678,14 @@ void launch_pairs_classify(const PairDev &P, uint64_t *scan_tmp, hipStream_t st)
 
 void launch_pairs_joint(const PairDev &P, uint64_t *scan_tmp, hipStream_t st) {
     const unsigned nb = (unsigned)pair_blocks(P.n_pairs);
-    if (nb) hipLaunchKernelGGL(k_joint, dim3(nb), dim3(PAIR_BLOCK), 0, st, P);
+    if (P.n_pairs) {
+        const unsigned g =
+            (unsigned)std::min<uint64_t>(((uint64_t)P.n_pairs + JOINT_WPB - 1) / JOINT_WPB, 65536);
+        hipLaunchKernelGGL(k_joint_wave, dim3(g), dim3(64 * JOINT_WPB), 0, st, P);
+    }
+    // bounds beyond LDS need max_tries above 31
+    if (nb && 4u * (uint32_t)P.max_tries + 1 > JOINT_LDS_CAP)
+        hipLaunchKernelGGL(k_joint, dim3(nb), dim3(PAIR_BLOCK), 0, st, P);
     launch_scan(P.jcnt, const_cast<uint64_t *>(P.joff), P.n_pairs, scan_tmp, st);
     if (nb) hipLaunchKernelGGL(k_joint_compact, dim3(nb), dim3(PAIR_BLOCK), 0, st, P);
 }
