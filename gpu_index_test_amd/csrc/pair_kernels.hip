@@ -2,7 +2,8 @@
 //
 //  k_pair_order   one wave per NAM list: each NAM's position in the reference
 //                 order is the number of NAMs of the list with a smaller key,
-//                 counted against 64-key tiles in LDS. Keys are unique (each
+//                 counted against the other lanes' keys (lists of up to 64)
+//                 or against 64-key tiles in LDS. Keys are unique (each
 //                 NAM's (is_rc, ref_id, query_start, ref_start) is its own
 //                 first hit), so counting ranks gives the one sorted order the
 //                 reference's quicksort reaches. Lists in canonical order are
@@ -81,6 +82,36 @@ __global__ __launch_bounds__(64 * ORDER_WPB) void k_pair_order(PairDev P) {
         const bool mate2 = r >= P.n_pairs;
         const gis_nam *in = P.in + o;
         gis_nam *out = P.ord + o;
+        if (cnt <= 64) {
+            // the whole list in the lanes: each lane ranks its NAM against
+            // every other one read straight from the lanes
+            const bool valid = lane < cnt;
+            gis_nam mine{};
+            NamKey ki{};
+            if (valid) {
+                mine = in[lane];
+                ki = nam_key(mine, mate2);
+            }
+            const uint32_t glo = (uint32_t)ki.g, ghi = (uint32_t)(ki.g >> 32);
+            uint32_t rank = 0;
+            bool dup = false;
+            for (uint32_t q = 0; q < cnt; q++) {
+                NamKey kj;
+                kj.g = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)ghi, (int)q) << 32 |
+                       (uint32_t)__builtin_amdgcn_readlane((int)glo, (int)q);
+                kj.d = __builtin_amdgcn_readlane(ki.d, (int)q);
+                kj.nh = __builtin_amdgcn_readlane(ki.nh, (int)q);
+                kj.qe = __builtin_amdgcn_readlane(ki.qe, (int)q);
+                kj.qs = __builtin_amdgcn_readlane(ki.qs, (int)q);
+                kj.re = __builtin_amdgcn_readlane(ki.re, (int)q);
+                kj.rs = __builtin_amdgcn_readlane(ki.rs, (int)q);
+                rank += key_less(kj, ki) ? 1u : 0u;
+                dup |= q != lane && key_eq(kj, ki);
+            }
+            if (valid) out[rank] = mine;
+            if (__ballot(valid && dup) != 0 && lane == 0) atomicOr(P.err, PAIR_ERR_DUPLICATE);
+            continue;
+        }
         for (uint32_t i0 = 0; i0 < cnt; i0 += 64) {
             const uint32_t i = i0 + lane;
             const bool valid = i < cnt;
@@ -414,10 +445,10 @@ __global__ __launch_bounds__(PAIR_BLOCK) void k_joint(PairDev P) {
 // The reference's scan over (i, j) is sequential: at each j it stops when
 // the joint hits fall below half the best pushed so far or more than
 // 2 max_tries entries are pushed, then pushes (i, j) if the NAMs form a
-// proper pair. Over 64 lanes j at once, the best and the count before each
-// lane are the running values plus an exclusive prefix max / count over the
-// proper lanes below it; the first lane whose condition holds is where the
-// reference stops, and the proper lanes below it are exactly its pushes.
+// proper pair. Over 64 lanes j at once, the running best and count only
+// change at proper lanes, so the row splits into segments ending at each
+// proper lane: one ballot tests a segment's lanes against the running
+// values, and if none stops, the proper lane ending it pushes.
 // The entries are then sorted by ranking keys in LDS. Without two equal keys
 // the sorted order is unique and equals the reference quicksort's; with
 // equal keys one lane replays the quicksort on the entry indices.
@@ -508,17 +539,6 @@ __device__ void lomuto_idx(uint8_t *I, int n, const EntryKey *K) {
     }
 }
 
-__device__ __forceinline__ int wave_excl_max(int v) {
-    const uint32_t lane = lane_id();
-    int x = v;
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
-        if (lane >= (uint32_t)d) x = max(x, y);
-    }
-    const int ex = __shfl_up(x, 1, 64);
-    return lane == 0 ? 0 : ex;
-}
-
 __global__ __launch_bounds__(64 * JOINT_WPB) void k_joint_wave(PairDev P) {
     __shared__ gis_nam_pair lE[JOINT_WPB][JOINT_LDS_CAP];
     __shared__ EntryKey lK[JOINT_WPB][JOINT_LDS_CAP];
@@ -570,23 +590,35 @@ __global__ __launch_bounds__(64 * JOINT_WPB) void k_joint_wave(PairDev P) {
                 if (jc && valid) b = pair_nam(B[j]);
                 const int joint = a.nh + b.nh;
                 const bool prop = valid && proper_pair(a, b, lim);
-                const uint64_t pm = __ballot(prop);
-                const uint32_t before = size + mbcnt(pm);
-                const int bb = max(best, wave_excl_max(prop ? joint : 0));
-                // joint < 0.5 * best in double is 2 joint < best in integers
-                const bool stop = valid && (2 * joint < bb || before > (uint32_t)(2 * mt));
-                const uint64_t sm = __ballot(stop);
-                const uint32_t first = sm ? (uint32_t)__builtin_ctzll(sm) : 64u;
-                const bool push = prop && lane < first;
-                if (push) {
-                    E[before] = gis_nam_pair{joint, (int)i, (int)j};
-                    if (i < (uint32_t)mt) atomicOr(&f1[i >> 5], 1u << (i & 31));
-                    if (j < (uint32_t)mt) atomicOr(&f2[j >> 5], 1u << (j & 31));
+                // the lanes in order, one segment per proper lane: the stop
+                // test (joint < 0.5 * best in double is 2 joint < best in
+                // integers) runs on the segment's lanes with the running
+                // values, then the proper lane that ends it pushes
+                uint64_t rem = __ballot(prop);
+                uint32_t from = 0;
+                for (;;) {
+                    const uint32_t pl = rem ? (uint32_t)__builtin_ctzll(rem) : 64u;
+                    const uint32_t to = pl < 64 ? pl : 63u;
+                    const uint64_t seg = (to == 63 ? ~0ull : ((1ull << (to + 1)) - 1)) & (~0ull << from);
+                    const bool stop = valid && (2 * joint < best || size > (uint32_t)(2 * mt));
+                    const uint64_t sm = __ballot(stop) & seg;
+                    if (sm) {
+                        stopped = true;
+                        break;
+                    }
+                    if (pl == 64) break;
+                    const int jp = __builtin_amdgcn_readlane(joint, (int)pl);
+                    if (lane == pl) {
+                        E[size] = gis_nam_pair{jp, (int)i, (int)j};
+                        if (i < (uint32_t)mt) f1[i >> 5] |= 1u << (i & 31);
+                        if (j < (uint32_t)mt) f2[j >> 5] |= 1u << (j & 31);
+                    }
+                    size++;
+                    best = max(best, jp);
+                    rem &= rem - 1;
+                    from = pl + 1;
+                    if (from >= 64) break;
                 }
-                const uint64_t qm = __ballot(push);
-                size += (uint32_t)__popcll(qm);
-                best = max(best, (int)wave_max(push ? (uint32_t)joint : 0u));
-                stopped = first < 64;
             }
             if (size > (uint32_t)(2 * mt)) break;
         }
