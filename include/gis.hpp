@@ -2,7 +2,8 @@
 // IndexParameters (step2/indexparameters.hpp:72-95), References
 // (step2/refs.hpp:53-91), StrobemerIndex (step2/index.hpp:37-183),
 // randstrobes_query (step2/randstrobes.hpp:107), find_nams and
-// find_nams_rescue (step2/nam.hpp:11-20). Header-only; link against
+// find_nams_rescue (step2/nam.hpp:11-20), the pair step after them and the
+// FASTQ reader in front of them. Header-only; link against
 // libgis.so. Errors surface as the reference's exception types
 // (step2/exceptions.hpp:7-25), all derived from gis::Error.
 //
@@ -388,12 +389,98 @@ class Batch {
         return r;
     }
 
+    // The pair step on the last run's NAMs (reads p and n + p are the mates of
+    // pair p): gpu_sort_nams, gpu_pre_cal_type with its per-type lists,
+    // top_dropoff, get_mapq and get_best_scoring_nam_pairs
+    // (step2/main.cu:870-910, 1719-1830, 3449-3472, 3552-3581, 4636-4646).
+    struct Pairs {
+        BatchNams lists;                      // NAM lists in the reference order
+        std::vector<uint8_t> types;           // 0 .. 4 per pair
+        std::vector<std::vector<uint32_t>> by_type;  // the five types[] lists
+        std::vector<float> top_dropoff;       // per list
+        std::vector<uint8_t> mapq;            // per list
+        std::vector<uint64_t> joint_offsets;  // n_pairs + 1
+        std::vector<gis_nam_pair> joint;      // NamPair{score, i1, i2}
+    };
+    Pairs pairs(const gis_pair_params *pp = nullptr) {
+        gis_pair_params d;
+        gis_pair_params_default(&d);
+        d.flags = GIS_PAIRS_JOINT;
+        check(gis_batch_pairs(b_, pp ? pp : &d), index_.context());
+        const size_t n = lens_.size() / 2;
+        uint64_t total = 0;
+        check(gis_batch_nam_count(b_, &total), index_.context());
+        Pairs r;
+        r.lists.offsets.resize(2 * n + 1);
+        r.lists.nams.resize(total ? total : 1);
+        r.types.resize(n ? n : 1);
+        std::vector<uint32_t> ids(n ? n : 1);
+        r.top_dropoff.resize(n ? 2 * n : 1);
+        r.mapq.resize(n ? 2 * n : 1);
+        r.joint_offsets.resize(n + 1);
+        gis_pair_out o{};
+        o.nam_offsets = r.lists.offsets.data();
+        o.nams = r.lists.nams.data();
+        o.nam_capacity = r.lists.nams.size();
+        o.types = r.types.data();
+        o.type_ids = ids.data();
+        o.top_dropoff = r.top_dropoff.data();
+        o.mapq = r.mapq.data();
+        o.joint_offsets = r.joint_offsets.data();
+        // first call sizes the joint entries
+        int rc = gis_batch_pairs_download(b_, &o);
+        if (rc == GIS_ERR_CAPACITY || o.n_joint) {
+            r.joint.resize(o.n_joint ? o.n_joint : 1);
+            o.joint = r.joint.data();
+            o.joint_capacity = r.joint.size();
+            rc = gis_batch_pairs_download(b_, &o);
+        }
+        check(rc, index_.context());
+        r.lists.nams.resize(o.n_nams);
+        r.joint.resize(o.n_joint);
+        r.types.resize(n);
+        r.top_dropoff.resize(2 * n);
+        r.mapq.resize(2 * n);
+        size_t at = 0;
+        for (int t = 0; t < 5; t++) {
+            r.by_type.emplace_back(ids.begin() + at, ids.begin() + at + o.type_counts[t]);
+            at += o.type_counts[t];
+        }
+        return r;
+    }
+
   private:
     const StrobemerIndex &index_;
     gis_batch *b_ = nullptr;
     std::string seqs_;
     std::vector<uint64_t> offs_;
     std::vector<uint32_t> lens_;
+};
+
+// FASTQ records in chunks (one file, or two mates files: mate 1 of record i
+// is read i and mate 2 read n_records + i of a chunk), read and formatted
+// ahead by a reader thread and parser threads (producer_pe_fastq_task and
+// chunkFormat, step2/main.cu:65-84, 4212-4240).
+class FastqReader {
+  public:
+    explicit FastqReader(const std::string &path1, const std::string &path2 = "", uint64_t chunk_bytes = 0) {
+        const int rc = gis_fastq_open(path1.c_str(), path2.empty() ? nullptr : path2.c_str(), chunk_bytes, 0, &f_);
+        if (rc) throw Error(rc, "cannot open " + path1 + (path2.empty() ? "" : " / " + path2));
+    }
+    ~FastqReader() {
+        if (f_) gis_fastq_close(f_);
+    }
+    FastqReader(const FastqReader &) = delete;
+    FastqReader &operator=(const FastqReader &) = delete;
+    // the next chunk (valid until the next call); false at the end of the input
+    bool next(gis_fastq_chunk &chunk) {
+        const int rc = gis_fastq_next(f_, &chunk);
+        if (rc) throw Error(rc, gis_fastq_last_error(f_));
+        return chunk.n_records != 0;
+    }
+
+  private:
+    gis_fastq *f_ = nullptr;
 };
 
 }  // namespace gis

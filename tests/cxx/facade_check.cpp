@@ -1,11 +1,15 @@
 // Drives the C++ façade (include/gis.hpp) the way a reference caller would:
 // StrobemerIndex::populate, then per read randstrobes_query -> find_nams ->
 // the caller's rescue decision -> find_nams_rescue (step2/main.cu:3209-3216),
-// and the same reads again through one device-resident Batch.
+// and the same reads again through one device-resident Batch, then (for an
+// even read count, read p and n + p as the mates of pair p) the pair step.
 //
 // usage: facade_check <refs.fa> <reads.txt> <out_prefix>
 // writes <out_prefix>.single and <out_prefix>.batch, each:
 //   u64 n_reads, u64 offsets[n_reads + 1], gis_nam nams[offsets[n_reads]]
+// and, for an even read count, <out_prefix>.pairs (the same layout, lists in
+// the reference order) and <out_prefix>.types:
+//   u64 n_pairs, u8 types[n_pairs], u64 n_joint, gis_nam_pair joint[n_joint]
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -69,6 +73,20 @@ int main(int argc, char **argv) {
         batch.run();
         const gis::BatchNams res = batch.download();
         write_out(std::string(argv[3]) + ".batch", res.offsets, res.nams);
+        if (reads.size() % 2 == 0) {
+            // reads p and n + p as the mates of pair p: the pair step
+            stage = "pairs";
+            const auto pr = batch.pairs();
+            write_out(std::string(argv[3]) + ".pairs", pr.lists.offsets, pr.lists.nams);
+            FILE *f = std::fopen((std::string(argv[3]) + ".types").c_str(), "wb");
+            if (!f) throw std::runtime_error("cannot write the pair types");
+            const uint64_t n = pr.types.size(), nj = pr.joint.size();
+            std::fwrite(&n, 8, 1, f);
+            std::fwrite(pr.types.data(), 1, n, f);
+            std::fwrite(&nj, 8, 1, f);
+            if (nj) std::fwrite(pr.joint.data(), sizeof(gis_nam_pair), nj, f);
+            std::fclose(f);
+        }
         std::cout << "facade_check: " << reads.size() << " reads, " << all.size() << " NAMs\n";
     } catch (const gis::Error &e) {
         std::cerr << "gis error " << e.code() << " in " << stage << ": " << e.what() << "\n";

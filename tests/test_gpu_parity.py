@@ -327,13 +327,14 @@ def test_graph_replays_match_eager(setup, mode):
 
 def test_cxx_facade_matches_oracle(setup, tmp_path):
     """A C++ caller of include/gis.hpp (per-read randstrobes_query ->
-    find_nams -> rescue decision -> find_nams_rescue, and one Batch) gets
-    the oracle's NAMs."""
+    find_nams -> rescue decision -> find_nams_rescue, one Batch, the pair
+    step) gets the oracle's NAMs and pair results."""
     import os
     import subprocess
     g, lens, p, idx, op, oidx = setup
     reads = _read_set(g, lens, 150, seed=21)
     reads = [r for r in reads if len(r) > 0 and b"\n" not in r]
+    reads = reads[: len(reads) // 2 * 2]  # whole pairs for the pair step
     fa = tmp_path / "refs.fa"
     with open(fa, "wb") as fh:
         pos = 0
@@ -358,6 +359,16 @@ def test_cxx_facade_matches_oracle(setup, tmp_path):
         assert n == len(reads)
         assert np.array_equal(offs, eo), suffix
         assert nams.tobytes() == en.tobytes(), suffix
+    ep = O.pairs_batch(eo, en, len(reads) // 2)
+    raw = open(out + ".pairs", "rb").read()
+    n = int(np.frombuffer(raw[:8], dtype=np.uint64)[0])
+    assert np.array_equal(np.frombuffer(raw[8: 8 + 8 * (n + 1)], dtype=np.uint64), eo)
+    assert raw[8 + 8 * (n + 1):] == ep["ord"].tobytes()
+    raw = open(out + ".types", "rb").read()
+    npairs = int(np.frombuffer(raw[:8], dtype=np.uint64)[0])
+    assert raw[8: 8 + npairs] == ep["types"].tobytes()
+    nj = int(np.frombuffer(raw[8 + npairs: 16 + npairs], dtype=np.uint64)[0])
+    assert raw[16 + npairs:] == ep["joint"].tobytes() and nj == len(ep["joint"])
 
 
 FUSED_CONFIGS = [dict(waves_per_block=w, length_split=sp, queue=q, chunk=ch, blocks_per_cu=bpc)
