@@ -169,7 +169,7 @@ void ensure_read_arrays(gis_batch *b, uint32_t n) {
     ensure<DevCtl>(b->ws->ctl, 1);
     ensure<uint32_t>(b->ws->spill_ids, n);
     ensure<uint4>(b->ws->split_list, n);
-    ensure<uint32_t>(b->ws->sel_ids, 2ull * n + 2);
+    ensure<uint32_t>(b->ws->sel_ids, 3ull * n + 3);
     if (!b->ws->h_ctl) HIP_CHECK(hipHostMalloc((void **)&b->ws->h_ctl, sizeof(DevCtl)));
     if (!b->ws->h_total) HIP_CHECK(hipHostMalloc((void **)&b->ws->h_total, 3 * sizeof(uint64_t)));
 }
@@ -317,7 +317,6 @@ void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_
     if (stop != STOP_SEED) {
         launch_lookup(d, set, ix, rescue_mode, st);
         mark(b, "lookup");
-        launch_scan(d.hit_total, d.hit_off, n, tmp, st);
         launch_hits(d, set, ix, st);
         mark(b, "hits");
         launch_rescue_count(d, set, ix, rcut, hard, st);
@@ -354,10 +353,13 @@ void prepare_workspace(gis_batch *b) {
     // initial pools: about the hits of typical reads (~0.16 per base), grown
     // on demand to what a run's scans report.
     // GIS_TEST_SMALL_POOLS starts them tiny so tests exercise the overflow path.
+    // the hit array holds a HIT_REGION-hit region per read, then the pool of
+    // the reads with more hits
+    const uint64_t regions = (uint64_t)HIT_REGION * n;
     if (const char *e = std::getenv("GIS_TEST_SMALL_POOLS"); e && *e == '1')
-        ensure_pools(b, 64, 64, 16);
+        ensure_pools(b, regions + 64, 64, 16);
     else
-        ensure_pools(b, 24ull * n + b->span / 16 + 4096, 4ull * n + 4096, 12ull * n + 1024);
+        ensure_pools(b, regions + 4ull * n + b->span / 64 + 4096, 4ull * n + 4096, 12ull * n + 1024);
 }
 
 RunOpts run_opts(const gis_batch *b, const gis_map_options &o, int rescue_mode, int stop) {
@@ -607,15 +609,19 @@ static int copy_hits_out(gis_ctx *ctx, gis_batch *b, uint64_t *out_offsets, gis_
     const uint32_t n = b->n_reads;
     std::vector<uint8_t> flags(n);
     std::vector<uint64_t> ho(n + 1), rho(n + 1);
+    std::vector<uint32_t> hcount(n);
     if (n) HIP_CHECK(hipMemcpy(flags.data(), b->ws->flags.p, n, hipMemcpyDeviceToHost));
+    if (n) HIP_CHECK(hipMemcpy(hcount.data(), b->ws->hit_total.p, 4ull * n, hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(ho.data(), b->ws->hit_off.p, 8ull * (n + 1), hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(rho.data(), b->ws->rhit_off.p, 8ull * (n + 1), hipMemcpyDeviceToHost));
     std::vector<DevHit> h(ho[n]), rh(rho[n]);
     if (ho[n]) HIP_CHECK(hipMemcpy(h.data(), b->ws->hits.p, sizeof(DevHit) * ho[n], hipMemcpyDeviceToHost));
     if (rho[n]) HIP_CHECK(hipMemcpy(rh.data(), b->ws->rhits.p, sizeof(DevHit) * rho[n], hipMemcpyDeviceToHost));
     uint64_t total = 0;
+    // a read's hits: its region or pool place (hit_off) and count (hit_total);
+    // rescued reads: the scan-placed rescue array
     for (uint32_t r = 0; r < n; r++)
-        total += (flags[r] & GIS_FLAG_RESCUED) ? rho[r + 1] - rho[r] : ho[r + 1] - ho[r];
+        total += (flags[r] & GIS_FLAG_RESCUED) ? rho[r + 1] - rho[r] : hcount[r];
     if (n_out) *n_out = total;
     if (total > capacity || (total && !out)) return fail(ctx, GIS_ERR_CAPACITY, "hit capacity too small");
     uint64_t o = 0;
@@ -624,7 +630,7 @@ static int copy_hits_out(gis_ctx *ctx, gis_batch *b, uint64_t *out_offsets, gis_
         if (out_offsets) out_offsets[r] = o;
         const bool resc = (flags[r] & GIS_FLAG_RESCUED) != 0;
         const std::vector<DevHit> &src = resc ? rh : h;
-        const uint64_t lo = resc ? rho[r] : ho[r], hi = resc ? rho[r + 1] : ho[r + 1];
+        const uint64_t lo = resc ? rho[r] : ho[r], hi = resc ? rho[r + 1] : ho[r] + hcount[r];
         // group by (strand, ref) keeping the emission order inside a group
         idx.clear();
         for (uint64_t i = lo; i < hi; i++) idx.push_back(((uint64_t)src[i].ref_strand << 32) | (i - lo));

@@ -46,7 +46,8 @@ struct DevCtl {
     unsigned int q_head[2]; // dynamic work-queue heads of the fused kernel's two bins
     unsigned int n_gwork[2];  // group work items of split reads (normal, rescue merge)
     unsigned int n_split[2];  // split reads (normal, rescue merge)
-    unsigned int n_sel[2];    // selected reads: rescued, deferred merges
+    unsigned int n_sel[3];    // selected reads: rescued, deferred merges, pooled hits
+    unsigned long long hit_pool;  // hits placed in the pool after the per-read regions
 };
 
 enum : unsigned { OVF_HITS = 1, OVF_RHITS = 2, OVF_MERGE = 4, OVF_OUT = 8 };
@@ -92,7 +93,9 @@ struct DevBatch {
     float *frac;
     uint8_t *flags;
     uint32_t *nam_cnt;
-    uint64_t *hit_off;        // [n+1] scan of hit_total: the read's hits start here
+    uint64_t *hit_off;        // [n+1] the read's hits start here: its own region r * HIT_REGION, or
+                              // (more hits than that) a place in the pool after the regions;
+                              // hit_off[n] = end of the pool
     uint64_t *rhit_off;       // [n+1] scan of rhit_total
     // hit arrays (capacity hit_cap / rhit_cap hits); merge scratch per read r
     // with h hits at hit offset o: keys[2o .. 2o+2h), nam_ws[o .. o+h)
@@ -108,7 +111,7 @@ struct DevBatch {
     uint4 *gwork;
     uint4 *split_list;
     // ids of the reads of the sparse passes: rescued at [0, n), deferred
-    // merges at [n, 2n)
+    // merges at [n, 2n), reads whose hits go to the pool at [2n, 3n)
     uint32_t *sel_ids;
     gis_nam *nam_fin, *rnam_fin;
     // fused kernel: NAM slots of F_NAMCAP per read
@@ -129,6 +132,9 @@ constexpr int SEED_WPB = 4;    // waves per block of the seeding kernel
 constexpr int WPB = 4;         // waves per block of the per-read kernels
 constexpr int MERGE_LDS = 128; // hits a read may have for the LDS merge
 constexpr int F_NAMCAP = 32;   // NAMs a read may have in the fused kernel
+// hits of a read written by the lookup itself into the read's own region of
+// the hit array; reads with more go to the pool and a second pass
+constexpr uint32_t HIT_REGION = 48;
 
 // launch wrappers (kernels.hip)
 void launch_reset_ctl(DevCtl *ctl, hipStream_t st);

@@ -459,13 +459,18 @@ __global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, ReadSet set, De
         const uint64_t base = b.rs_off[r];
         const uint32_t half = b.rs_nfwd[r];
         uint32_t found = 0, good = 0, hits = 0, bytes = 0;
+        DevHit *region = b.hits + (uint64_t)r * HIT_REGION;
+        // the regions must lie inside the hit array (else every read pools)
+        const bool region_ok = (uint64_t)(r + 1) * HIT_REGION <= b.hit_cap;
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const uint32_t i = i0 + lane;
             uint32_t f = 0, g = 0, hc = 0, by = 0;
+            uint64_t pos = GIS_NOT_FOUND, key = 0;
+            int qs = 0, qe = 0;
             if (i < n) {
-                const uint64_t key = b.rs_hash[base + i];
+                key = b.rs_hash[base + i];
                 uint32_t probes = 0, records = 0;
-                const uint64_t pos = dev_find(ix, key, &probes);
+                pos = dev_find(ix, key, &probes);
                 // query in (16) + result out (12) + bucket bounds (16) + search records
                 by = 44 + 16 * probes;
                 if (pos != GIS_NOT_FOUND) {
@@ -473,8 +478,9 @@ __global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, ReadSet set, De
                     by += pos + ix.filter_cutoff < ix.n ? 8 : 0;
                     if (!dev_filtered(ix, pos, key)) {
                         g = 1;
-                        hc = walk_run(ix, pos, key, (int)b.rs_start[base + i], (int)b.rs_end[base + i],
-                                      i >= half ? 0x80000000u : 0u, nullptr, &records);
+                        qs = (int)b.rs_start[base + i];
+                        qe = (int)b.rs_end[base + i];
+                        hc = walk_run(ix, pos, key, qs, qe, i >= half ? 0x80000000u : 0u, nullptr, &records);
                         by += 16 * records;
                     }
                 }
@@ -483,7 +489,15 @@ __global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, ReadSet set, De
             }
             found += wave_sum(f);
             good += wave_sum(g);
-            hits += wave_sum(hc);
+            const uint32_t chunk_hits = wave_sum(hc);
+            // while the read's hits fit its region, write them there in
+            // seed order (the run records are still in cache): 16 per hit
+            if (region_ok && hits + chunk_hits <= HIT_REGION) {
+                const uint32_t ex = wave_excl_scan(hc);
+                if (hc > 0) walk_run(ix, pos, key, qs, qe, i >= half ? 0x80000000u : 0u, region + hits + ex);
+                by += 16 * hc;
+            }
+            hits += chunk_hits;
             bytes += wave_sum(by);
         }
         if (lane == 0) {
@@ -497,11 +511,26 @@ __global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, ReadSet set, De
                                    (n == 0 ? GIS_FLAG_NO_RANDSTROBES : 0));
             b.hit_total[r] = rescue ? 0u : hits;  // rescued reads discard these hits
             b.rhit_total[r] = 0;
+            // hits beyond the read's region: a place in the pool, and k_hits
+            // writes them
+            uint64_t off = (uint64_t)r * HIT_REGION;
+            if (!rescue && hits > 0 && (hits > HIT_REGION || !region_ok)) {
+                off = (uint64_t)b.n_reads * HIT_REGION + atomicAdd(&b.ctl->hit_pool, (unsigned long long)hits);
+                b.sel_ids[2ull * b.n_reads + atomicAdd(&b.ctl->n_sel[2], 1u)] = r;
+            }
+            b.hit_off[r] = off;
         }
     }
 }
 
-// write the hits of non-rescued reads into their region (same walk as k_lookup)
+// end of the hit pool, for the capacity checks of the kernels that follow
+__global__ void k_pool_total(DevBatch b) {
+    if (threadIdx.x == 0)
+        b.hit_off[b.n_reads] = (uint64_t)b.n_reads * HIT_REGION + b.ctl->hit_pool;
+}
+
+// write the hits of the reads whose hits did not fit their region into the
+// pool (same walk as k_lookup); runs over the pooled reads' list
 // capacity check of a hit array against the scan total (wave-uniform)
 __device__ __forceinline__ bool over_capacity(const uint64_t *off, uint32_t n, uint64_t cap,
                                               DevCtl *ctl, unsigned bit) {
@@ -1625,14 +1654,14 @@ __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
 // Ids of the set's reads that a sparse pass visits, in set order within a
 // block: SEL_RESCUED (rescued reads) or SEL_DEFERRED (normal reads the
 // four-per-wave merge handed on). One atomic per block places a block's ids.
-enum { SEL_RESCUED = 0, SEL_DEFERRED = 1 };
+enum { SEL_RESCUED = 0, SEL_DEFERRED = 1, SEL_POOLED = 2 };
 
 __global__ __launch_bounds__(256) void k_select(DevBatch b, ReadSet set, int kind) {
     __shared__ uint32_t wcount[4];
     __shared__ uint32_t block_base;
     const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
     const uint32_t cnt = set_count(set);
-    uint32_t *out = b.sel_ids + (kind ? b.n_reads : 0u);
+    uint32_t *out = b.sel_ids + (uint64_t)kind * b.n_reads;
     for (uint32_t base = blockIdx.x * 256u; base < cnt; base += gridDim.x * 256u) {
         const uint32_t i = base + threadIdx.x;
         uint32_t r = 0;
@@ -1665,7 +1694,8 @@ __global__ void k_reset_ctl(DevCtl *ctl) {
         ctl->q_head[1] = 0;
         ctl->n_gwork[0] = ctl->n_gwork[1] = 0;
         ctl->n_split[0] = ctl->n_split[1] = 0;
-        ctl->n_sel[0] = ctl->n_sel[1] = 0;
+        ctl->n_sel[0] = ctl->n_sel[1] = ctl->n_sel[2] = 0;
+        ctl->hit_pool = 0;
     }
 }
 
@@ -1761,7 +1791,7 @@ static inline unsigned grid_for_set(const DevBatch &b, const ReadSet &set, unsig
 
 // the reads a sparse pass visits: the ids k_select wrote
 static inline ReadSet selected(const DevBatch &b, int kind) {
-    return ReadSet{b.sel_ids + (kind ? b.n_reads : 0u), &b.ctl->n_sel[kind], b.n_reads};
+    return ReadSet{b.sel_ids + (uint64_t)kind * b.n_reads, &b.ctl->n_sel[kind], b.n_reads};
 }
 
 // blocks for a per-read kernel over a selected list (its length is on the
@@ -1797,8 +1827,10 @@ void launch_lookup(const DevBatch &b, const ReadSet &set, const DevIndex &ix, in
 }
 
 void launch_hits(const DevBatch &b, const ReadSet &set, const DevIndex &ix, hipStream_t st) {
+    (void)set;
+    hipLaunchKernelGGL(k_pool_total, dim3(1), dim3(64), 0, st, b);
     if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_hits, dim3(grid_for_set(b, set, WPB)), dim3(64 * WPB), 0, st, b, set, ix);
+    hipLaunchKernelGGL(k_hits, dim3(grid_selected(b)), dim3(64 * WPB), 0, st, b, selected(b, SEL_POOLED), ix);
 }
 
 void launch_rescue_count(const DevBatch &b, const ReadSet &set, const DevIndex &ix,

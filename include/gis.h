@@ -261,7 +261,8 @@ int gis_batch_download(gis_batch *b, gis_nam_out *out);
 int gis_batch_stage_times(gis_batch *b, double *ms, int cap, int *n, char *names, int names_cap);
 /* Algorithmic bytes of the lookup kernel in the last run: per randstrobe 16
  * (query) + 12 (results) + 16 (bucket bounds) + 16 per table record searched
- * + 8 (filter probe) + 16 per run record walked; n_lookups = randstrobes. */
+ * + 8 (filter probe) + 16 per run record walked + 16 per hit it writes
+ * (the hits of reads with at most 48 hits); n_lookups = randstrobes. */
 int gis_batch_lookup_bytes(gis_batch *b, uint64_t *bytes, uint64_t *n_lookups);
 /* total NAMs of the last run (available after gis_batch_run, before download) */
 int gis_batch_nam_count(gis_batch *b, uint64_t *n_nams);
