@@ -52,7 +52,6 @@ static void workspace_release(Workspace *w) {
                      &w->ctl, &w->spill_ids, &w->gwork, &w->split_list, &w->sel_ids};
     for (DevBuf *x : all) release(*x);
     if (w->h_ctl) (void)hipHostFree(w->h_ctl);
-    if (w->h_total) (void)hipHostFree(w->h_total);
     pairwork_free(w->pairs);
     delete w;
 }
@@ -171,7 +170,6 @@ void ensure_read_arrays(gis_batch *b, uint32_t n) {
     ensure<uint4>(b->ws->split_list, n);
     ensure<uint32_t>(b->ws->sel_ids, 3ull * n + 3);
     if (!b->ws->h_ctl) HIP_CHECK(hipHostMalloc((void **)&b->ws->h_ctl, sizeof(DevCtl)));
-    if (!b->ws->h_total) HIP_CHECK(hipHostMalloc((void **)&b->ws->h_total, 3 * sizeof(uint64_t)));
 }
 
 void ensure_slot_arrays(gis_batch *b, uint64_t slots) {
@@ -323,8 +321,6 @@ void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_
         launch_scan(d.rhit_total, d.rhit_off, n, tmp, st);
         launch_rescue_hits(d, set, ix, st);
         mark(b, "rescue");
-        HIP_CHECK(hipMemcpyAsync(b->ws->h_total, d.hit_off + n, 8, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(b->ws->h_total + 1, d.rhit_off + n, 8, hipMemcpyDeviceToHost, st));
     }
     if (stop == STOP_NONE) {
         launch_merge(d, set, 0, st);
@@ -334,7 +330,6 @@ void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_
         launch_scan(d.nam_cnt, d.nam_off, n, tmp, st);
         launch_compact(d, fused ? 1 : 0, st);
         mark(b, "output");
-        HIP_CHECK(hipMemcpyAsync(b->ws->h_total + 2, d.nam_off + n, 8, hipMemcpyDeviceToHost, st));
     }
     HIP_CHECK(hipMemcpyAsync(b->ws->h_ctl, d.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, st));
 }
@@ -440,7 +435,7 @@ static unsigned finish_once(gis_batch *b, const RunOpts &r) {
     HIP_CHECK(hipStreamSynchronize(b->stream));
     const bool fused = r.mode == GIS_MODE_FUSED && r.stop == STOP_NONE;
     b->ws->n_spill = fused ? b->ws->h_ctl->n_spill : 0;
-    b->ws->tot_nams = r.stop == STOP_NONE ? b->ws->h_total[2] : 0;
+    b->ws->tot_nams = r.stop == STOP_NONE ? b->ws->h_ctl->tot[2] : 0;
     return b->ws->h_ctl->overflow;
 }
 
@@ -452,8 +447,8 @@ void batch_complete(gis_batch *b, const RunOpts &r) {
         // not final: grow what overflowed now and size the output next time
         if (ovf & (OVF_HITS | OVF_RHITS | OVF_MERGE)) ovf &= ~(unsigned)OVF_OUT;
         if (ovf & OVF_MERGE) b->ws->merge_scratch = true;
-        ensure_pools(b, (ovf & OVF_HITS) ? b->ws->h_total[0] + b->ws->h_total[0] / 8 + 4096 : 0,
-                     (ovf & OVF_RHITS) ? b->ws->h_total[1] + b->ws->h_total[1] / 8 + 4096 : 0,
+        ensure_pools(b, (ovf & OVF_HITS) ? b->ws->h_ctl->tot[0] + b->ws->h_ctl->tot[0] / 8 + 4096 : 0,
+                     (ovf & OVF_RHITS) ? b->ws->h_ctl->tot[1] + b->ws->h_ctl->tot[1] / 8 + 4096 : 0,
                      (ovf & OVF_OUT) ? b->ws->tot_nams + b->ws->tot_nams / 8 + 1024 : 0);
         batch_launch(b, r);
         ovf = finish_once(b, r);
@@ -724,7 +719,7 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
     ensure_read_arrays(b, n);
     ensure_slot_arrays(b, total);
     b->ws->owner = b;
-    ensure_pools(b, 64ull * n + 4096, 16ull * n + 4096, 16ull * n + 1024);
+    ensure_pools(b, (uint64_t)HIT_REGION * n + 16ull * n + 4096, 16ull * n + 4096, 16ull * n + 1024);
     b->tot_rs_cap = total;
     hipStream_t st = ctx->stream;
     if (total) {
@@ -746,7 +741,6 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
         const DevBatch d = make_devbatch(b);
         launch_reset_ctl(d.ctl, st);
         launch_lookup(d, set, ix, rescue ? 2 : 1, st);
-        launch_scan(d.hit_total, d.hit_off, n, tmp, st);
         launch_hits(d, set, ix, st);
         launch_rescue_count(d, set, ix, rescue_cutoff, hard_cutoff, st);
         launch_scan(d.rhit_total, d.rhit_off, n, tmp, st);
@@ -755,20 +749,17 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
         launch_merge(d, set, 1, st);
         launch_scan(d.nam_cnt, d.nam_off, n, tmp, st);
         launch_compact(d, 0, st);
-        HIP_CHECK(hipMemcpyAsync(b->ws->h_total, d.hit_off + n, 8, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(b->ws->h_total + 1, d.rhit_off + n, 8, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(hipMemcpyAsync(b->ws->h_total + 2, d.nam_off + n, 8, hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipMemcpyAsync(b->ws->h_ctl, d.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, st));
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(st));
-        b->ws->tot_nams = b->ws->h_total[2];
+        b->ws->tot_nams = b->ws->h_ctl->tot[2];
         unsigned ovf = b->ws->h_ctl->overflow;
         if (!ovf) break;
         if (attempt == 5) throw HipError{hipErrorOutOfMemory, "batch arrays still too small after regrowth"};
         if (ovf & (OVF_HITS | OVF_RHITS | OVF_MERGE)) ovf &= ~(unsigned)OVF_OUT;
         if (ovf & OVF_MERGE) b->ws->merge_scratch = true;
-        ensure_pools(b, (ovf & OVF_HITS) ? b->ws->h_total[0] + 4096 : 0,
-                     (ovf & OVF_RHITS) ? b->ws->h_total[1] + 4096 : 0,
+        ensure_pools(b, (ovf & OVF_HITS) ? b->ws->h_ctl->tot[0] + 4096 : 0,
+                     (ovf & OVF_RHITS) ? b->ws->h_ctl->tot[1] + 4096 : 0,
                      (ovf & OVF_OUT) ? b->ws->tot_nams + 1024 : 0);
     }
     b->ran = true;

@@ -29,7 +29,6 @@ struct Workspace {
     const gis_batch *owner = nullptr;  // the batch whose run the results belong to
     // pinned landing area for the end-of-run readback
     DevCtl *h_ctl = nullptr;
-    uint64_t *h_total = nullptr;  // [3]: hits, rescue hits, NAMs
     int refs = 0;
     PairWork *pairs = nullptr;    // the pair step on this workspace's NAMs (pairs.cpp)
 };

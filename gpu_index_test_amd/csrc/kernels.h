@@ -48,6 +48,9 @@ struct DevCtl {
     unsigned int n_split[2];  // split reads (normal, rescue merge)
     unsigned int n_sel[3];    // selected reads: rescued, deferred merges, pooled hits
     unsigned long long hit_pool;  // hits placed in the pool after the per-read regions
+    // totals the host reads with this block: end of the hit pool, rescue
+    // hits, NAMs
+    unsigned long long tot[3];
 };
 
 enum : unsigned { OVF_HITS = 1, OVF_RHITS = 2, OVF_MERGE = 4, OVF_OUT = 8 };

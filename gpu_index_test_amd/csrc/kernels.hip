@@ -525,8 +525,11 @@ __global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, ReadSet set, De
 
 // end of the hit pool, for the capacity checks of the kernels that follow
 __global__ void k_pool_total(DevBatch b) {
-    if (threadIdx.x == 0)
-        b.hit_off[b.n_reads] = (uint64_t)b.n_reads * HIT_REGION + b.ctl->hit_pool;
+    if (threadIdx.x == 0) {
+        const uint64_t end = (uint64_t)b.n_reads * HIT_REGION + b.ctl->hit_pool;
+        b.hit_off[b.n_reads] = end;
+        b.ctl->tot[0] = end;
+    }
 }
 
 // write the hits of the reads whose hits did not fit their region into the
@@ -631,6 +634,7 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_count(DevBatch b, ReadSet s
 
 __global__ __launch_bounds__(64 * WPB) void k_rescue_hits(DevBatch b, ReadSet set, DevIndex ix) {
     const uint32_t lane = lane_id();
+    if (blockIdx.x == 0 && threadIdx.x == 0) b.ctl->tot[1] = b.rhit_off[b.n_reads];
     if (over_capacity(b.rhit_off, b.n_reads, b.rhit_cap, b.ctl, OVF_RHITS)) return;
     const uint32_t cnt = set_count(set);
     for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
@@ -1630,6 +1634,7 @@ __global__ __launch_bounds__(64 * FWPB) void k_fused(DevBatch b, DevIndex ix, Se
 
 __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
     const uint32_t lane = lane_id();
+    if (blockIdx.x == 0 && threadIdx.x == 0) b.ctl->tot[2] = b.nam_off[b.n_reads];
     // a hit array overflowed: the merges did not run and the counts are stale
     if (b.ctl->overflow & (OVF_HITS | OVF_RHITS)) return;
     if (b.nam_off[b.n_reads] > b.out_cap) {
@@ -1696,6 +1701,7 @@ __global__ void k_reset_ctl(DevCtl *ctl) {
         ctl->n_split[0] = ctl->n_split[1] = 0;
         ctl->n_sel[0] = ctl->n_sel[1] = ctl->n_sel[2] = 0;
         ctl->hit_pool = 0;
+        ctl->tot[0] = ctl->tot[1] = ctl->tot[2] = 0;
     }
 }
 

@@ -202,6 +202,16 @@ def test_find_nams_qrs_parity(setup):
         rn = G.find_nams_rescue(qrs, idx, rcut)
         ern = O.find_nams_rescue(oidx, qrs, rcut, 1000, canonical=True)
         assert rn.tobytes() == ern.tobytes()
+    # many reads' query randstrobes in one call, both decisions
+    qs = [O.randstrobes_query(seq, op) for seq in reads]
+    for rescue in (0, 1):
+        res = G._qrs_call(idx, qs, rescue, rcut, 1000)
+        for r, q in enumerate(qs):
+            if rescue:
+                exp = O.find_nams_rescue(oidx, q, rcut, 1000, canonical=True)
+            else:
+                _, exp, _ = O.find_nams(oidx, q, canonical=True)
+            assert res.read_nams(r).tobytes() == exp.tobytes(), (rescue, r)
 
 
 def test_sti_roundtrip(setup, tmp_path):
