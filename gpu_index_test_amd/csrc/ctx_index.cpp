@@ -24,7 +24,7 @@ void upload_index(gis_ctx *ctx, const HostIndex &h) {
     // drop the previous table first so two tables never coexist in HBM
     release(ix.rs);
     release(ix.starts);
-    ensure<gis_ref_randstrobe>(ix.rs, h.rs.size());
+    ensure<gis_ref_randstrobe>(ix.rs, h.rs.size() + 1);  // + the probe pad record
     ensure<uint64_t>(ix.starts, h.starts.size());
     if (!h.rs.empty())
         HIP_CHECK(hipMemcpy(ix.rs.p, h.rs.data(), h.rs.size() * sizeof(gis_ref_randstrobe),
@@ -130,7 +130,7 @@ extern "C" int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint
     release(ix.rs);
     release(ix.starts);
     const uint64_t n = recs.size();
-    ensure<gis_ref_randstrobe>(ix.rs, n);
+    ensure<gis_ref_randstrobe>(ix.rs, n + 1);  // + the probe pad record
     ensure<uint64_t>(ix.starts, (1ull << bits) + 1);
     if (n) HIP_CHECK(hipMemcpy(ix.rs.p, recs.data(), n * sizeof(gis_ref_randstrobe), hipMemcpyHostToDevice));
     std::vector<gis_ref_randstrobe>().swap(recs);
@@ -217,7 +217,7 @@ extern "C" int gis_index_read_sti(gis_ctx *ctx, const char *path, const gis_para
         ix.loaded = false;
         release(ix.rs);
         release(ix.starts);
-        ensure<gis_ref_randstrobe>(ix.rs, m.n);
+        ensure<gis_ref_randstrobe>(ix.rs, m.n + 1);  // + the probe pad record
         ensure<uint64_t>(ix.starts, (1ull << m.bits) + 1);
         const uint64_t chunk = 256ull << 20;
         const char *src = reinterpret_cast<const char *>(m.rs);

@@ -11,6 +11,8 @@ namespace gis {
 // Resident index: RefRandstrobe table (16 B records: u64 hash, u32 position,
 // u32 packed) sorted by (hash, position), and bucket starts of 2^bits + 1
 // entries (step2/index.hpp:168-182).
+// rs holds n records and one pad record after them (never compared: the
+// lookup's unconditional probe loads may read it)
 struct DevIndex {
     const uint4 *rs;
     const uint64_t *starts;

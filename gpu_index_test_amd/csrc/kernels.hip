@@ -444,10 +444,146 @@ __device__ __forceinline__ uint32_t walk_run(const DevIndex &ix, uint64_t pos, u
 }
 
 // ---------------------------------------------------------------------------
+// Probe of one query randstrobe for the per-stage lookup. The result is the
+// reference's (StrobemerIndex::find + is_filtered + the run walk of
+// add_to_hits_per_ref), reached with fewer dependent memory steps: buckets of
+// at most PROBE_SMALL records (most of them: about 2.4 records per bucket)
+// are read whole with independent loads once their bounds are known; the
+// first equal record, the run length (equal hashes never cross a bucket),
+// the filter test and the kept hits then follow from registers. Larger
+// buckets take the reference's search and the separate filter probe.
+
+constexpr uint32_t PROBE_SMALL = 4;
+
+// a table record as the probe reads it (gis_ref_randstrobe)
+struct alignas(16) IdxRec {
+    uint64_t hash;
+    uint32_t position;
+    uint32_t packed;  // ref_index << 8 | strobe2 offset
+};
+
+struct Probe {
+    uint64_t key, s, e, pos;
+    IdxRec r[PROBE_SMALL];  // records s, s + 1, ... of a small bucket
+    uint32_t probes;        // records the reference's search reads (byte model)
+    uint32_t records;       // records its run walk reads (byte model)
+    uint32_t hc;            // kept hits (0 when absent or filtered)
+    uint32_t keep;          // small bucket: bit j = record s + j is a kept hit
+    bool found, filtered;
+};
+
+__device__ __forceinline__ bool probe_small(const Probe &p) { return p.e - p.s <= PROBE_SMALL; }
+
+// bucket bounds (StrobemerIndex::find, step2/index.hpp:57-62)
+__device__ __forceinline__ void probe_bounds(const DevIndex &ix, Probe &p) {
+    const uint32_t bk = (uint32_t)(p.key >> (64 - ix.bits));
+    p.s = ix.starts[bk];
+    p.e = ix.starts[bk + 1];
+}
+
+// The records of a small bucket. The loads are unconditional and clamped to
+// the bucket's last record (its start, for an empty bucket; at most the pad
+// record after the table) so that they issue back to back: a load under a
+// divergent branch is followed by a wait for it. Records beyond the bucket
+// are read but never compared.
+__device__ __forceinline__ void probe_load(const DevIndex &ix, Probe &p) {
+    const IdxRec *recs = reinterpret_cast<const IdxRec *>(ix.rs);
+    const uint64_t last = p.e > p.s ? p.e - 1 : p.s;
+#pragma unroll
+    for (uint32_t j = 0; j < PROBE_SMALL; j++) {
+        const uint64_t x = p.s + j;
+        p.r[j] = recs[x < last ? x : last];
+    }
+}
+
+// Search, filter and run walk. Small buckets: the reference's linear scan
+// stops at the first record >= key, which is record s + (records below key);
+// the run is the records equal to key; hash[pos + filter_cutoff] == key
+// exactly when the run is longer than filter_cutoff (past the bucket the
+// hashes differ, past the table the reference reads ~0).
+__device__ __forceinline__ void probe_finish(const DevIndex &ix, Probe &p, int qs, int qe) {
+    p.hc = 0;
+    p.keep = 0;
+    p.records = 0;
+    p.filtered = false;
+    if (probe_small(p)) {
+        const uint32_t sz = (uint32_t)(p.e - p.s);
+        uint32_t lt = 0, eq = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < PROBE_SMALL; j++) {
+            const uint64_t h = p.r[j].hash;
+            if (j < sz) {
+                lt += h < p.key ? 1u : 0u;
+                eq += h == p.key ? 1u : 0u;
+            }
+        }
+        p.probes = lt < sz ? lt + 1 : sz;
+        p.found = eq > 0;
+        p.pos = p.found ? p.s + lt : GIS_NOT_FOUND;
+        if (!p.found) return;
+        p.filtered = eq > ix.filter_cutoff ||
+                     (p.key == UINT64_MAX && p.pos + ix.filter_cutoff >= ix.n);
+        if (p.filtered) return;
+        p.records = eq + (p.pos + eq < ix.n ? 1u : 0u);
+        const int qspan = qe - qs;
+        int min_diff = 0x7fffffff;
+#pragma unroll
+        for (uint32_t j = 0; j < PROBE_SMALL; j++) {
+            if (j >= lt && j < lt + eq) {
+                int d = qspan - ((int)(p.r[j].packed & 0xffu) + ix.k);
+                d = d < 0 ? -d : d;
+                if (d <= min_diff) {
+                    p.keep |= 1u << j;
+                    p.hc++;
+                    min_diff = d;
+                }
+            }
+        }
+        return;
+    }
+    p.pos = dev_find(ix, p.key, &p.probes);
+    p.found = p.pos != GIS_NOT_FOUND;
+    if (!p.found) return;
+    p.filtered = dev_filtered(ix, p.pos, p.key);
+    if (p.filtered) return;
+    p.hc = walk_run(ix, p.pos, p.key, qs, qe, 0u, nullptr, &p.records);
+}
+
+// the kept hits of a probe, in run order
+__device__ __forceinline__ void probe_emit(const DevIndex &ix, const Probe &p, int qs, int qe,
+                                           uint32_t strand_bit, DevHit *out) {
+    if (!probe_small(p)) {
+        walk_run(ix, p.pos, p.key, qs, qe, strand_bit, out);
+        return;
+    }
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < PROBE_SMALL; j++) {
+        if ((p.keep >> j) & 1u) {
+            DevHit hh;
+            hh.ref_strand = strand_bit | (p.r[j].packed >> 8);
+            hh.ref_start = (int32_t)p.r[j].position;
+            hh.ref_end = (int32_t)p.r[j].position + (int)(p.r[j].packed & 0xffu) + ix.k;
+            hh.query_start = (uint16_t)qs;
+            hh.query_end = (uint16_t)qe;
+            out[c++] = hh;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // per-stage: lookup + filter + hit count per randstrobe; nonrepetitive
 // fraction and the rescue decision per read (find_nams, step2/nam.cu:553-732,
-// and the caller's test, step2/main.cu:3209-3216); the read's hit region is
-// taken from the hit pool.
+// and the caller's test, step2/main.cu:3209-3216). Lanes probe
+// LOOKUP_ITEMS randstrobes of their read per pass (i, i + 64, ... of a
+// 64 * LOOKUP_ITEMS-wide chunk). One is fastest on gfx950: two raise the
+// kernel from 40 to 52 VGPRs and its time by a fifth at the 150 bp workload,
+// where a read's ~42 randstrobes fill one 64-wide pass anyway. While the
+// read's hits fit its region of the hit array they are written there in seed
+// order; a read with more hits takes a place in the pool, and k_hits writes
+// them.
+
+constexpr uint32_t LOOKUP_ITEMS = 1;
 
 __global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, ReadSet set, DevIndex ix,
                                                      int rescue_mode, int keep_spill) {
@@ -462,40 +598,64 @@ __global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, ReadSet set, De
         DevHit *region = b.hits + (uint64_t)r * HIT_REGION;
         // the regions must lie inside the hit array (else every read pools)
         const bool region_ok = (uint64_t)(r + 1) * HIT_REGION <= b.hit_cap;
-        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
-            const uint32_t i = i0 + lane;
-            uint32_t f = 0, g = 0, hc = 0, by = 0;
-            uint64_t pos = GIS_NOT_FOUND, key = 0;
-            int qs = 0, qe = 0;
-            if (i < n) {
-                key = b.rs_hash[base + i];
-                uint32_t probes = 0, records = 0;
-                pos = dev_find(ix, key, &probes);
-                // query in (16) + result out (12) + bucket bounds (16) + search records
-                by = 44 + 16 * probes;
-                if (pos != GIS_NOT_FOUND) {
-                    f = 1;
-                    by += pos + ix.filter_cutoff < ix.n ? 8 : 0;
-                    if (!dev_filtered(ix, pos, key)) {
-                        g = 1;
-                        qs = (int)b.rs_start[base + i];
-                        qe = (int)b.rs_end[base + i];
-                        hc = walk_run(ix, pos, key, qs, qe, i >= half ? 0x80000000u : 0u, nullptr, &records);
-                        by += 16 * records;
-                    }
-                }
-                b.pos[base + i] = pos;
-                b.hcnt[base + i] = hc;
+        for (uint32_t i0 = 0; i0 < n; i0 += 64 * LOOKUP_ITEMS) {
+            Probe P[LOOKUP_ITEMS];
+            int qs[LOOKUP_ITEMS], qe[LOOKUP_ITEMS];
+            bool act[LOOKUP_ITEMS];
+            // straight-line loads (lanes past the end repeat the last item)
+#pragma unroll
+            for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) {
+                const uint32_t i = i0 + 64 * u + lane;
+                act[u] = i < n;
+                const uint64_t x = base + (act[u] ? i : n - 1);
+                P[u].key = b.rs_hash[x];
+                qs[u] = (int)b.rs_start[x];
+                qe[u] = (int)b.rs_end[x];
             }
-            found += wave_sum(f);
-            good += wave_sum(g);
-            const uint32_t chunk_hits = wave_sum(hc);
-            // while the read's hits fit its region, write them there in
-            // seed order (the run records are still in cache): 16 per hit
+            // each stage's loads issue together; the scheduler may not hoist
+            // their uses between them
+#pragma unroll
+            for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) probe_bounds(ix, P[u]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) probe_load(ix, P[u]);
+            __builtin_amdgcn_sched_barrier(0);
+            uint32_t by = 0, hc_all = 0, f_all = 0, g_all = 0;
+#pragma unroll
+            for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) {
+                const uint32_t i = i0 + 64 * u + lane;
+                if (act[u]) {
+                    probe_finish(ix, P[u], qs[u], qe[u]);
+                    // query in (16) + result out (12) + bucket bounds (16) + search
+                    // records, the filter probe, the run walk
+                    by += 44 + 16 * P[u].probes;
+                    if (P[u].found) by += P[u].pos + ix.filter_cutoff < ix.n ? 8 : 0;
+                    by += 16 * P[u].records;
+                    b.pos[base + i] = P[u].pos;
+                    b.hcnt[base + i] = P[u].hc;
+                    f_all += P[u].found ? 1u : 0u;
+                    g_all += P[u].found && !P[u].filtered ? 1u : 0u;
+                    hc_all += P[u].hc;
+                } else {
+                    P[u].hc = 0;
+                }
+            }
+            found += wave_sum(f_all);
+            good += wave_sum(g_all);
+            const uint32_t chunk_hits = wave_sum(hc_all);
+            // while the read's hits fit its region, write them there in seed
+            // order: 16 per hit
             if (region_ok && hits + chunk_hits <= HIT_REGION) {
-                const uint32_t ex = wave_excl_scan(hc);
-                if (hc > 0) walk_run(ix, pos, key, qs, qe, i >= half ? 0x80000000u : 0u, region + hits + ex);
-                by += 16 * hc;
+                uint32_t at = hits;
+#pragma unroll
+                for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) {
+                    const uint32_t i = i0 + 64 * u + lane;
+                    const uint32_t ex = wave_excl_scan(P[u].hc);
+                    if (P[u].hc > 0)
+                        probe_emit(ix, P[u], qs[u], qe[u], i >= half ? 0x80000000u : 0u, region + at + ex);
+                    at += wave_sum(P[u].hc);
+                }
+                by += 16 * hc_all;
             }
             hits += chunk_hits;
             bytes += wave_sum(by);
