@@ -55,7 +55,13 @@ def build_index(G, genome, clens, threads):
     return idx, info
 
 
-def make_batches(G, idx, seqs, offs, lens, batch):
+def make_batches(G, idx, seqs, offs, lens, batch, lanes=1):
+    """Resident batches; batch i runs on lane i % lanes. With one lane every
+    batch runs on the first batch's device workspace and the context's stream
+    (the batches run one after another); with more, each lane's first batch
+    has a workspace and stream of its own that the lane's other batches share,
+    so that runs on different lanes overlap. Only the reads stay resident per
+    batch."""
     n = len(lens)
     nb = math.ceil(n / batch)
     mappers = []
@@ -63,9 +69,9 @@ def make_batches(G, idx, seqs, offs, lens, batch):
         lo, hi = i * batch, min(n, (i + 1) * batch)
         base = int(offs[lo:hi].min())
         end = int((offs[lo:hi] + lens[lo:hi]).max())
-        # every batch runs on the first batch's device workspace (the batches
-        # run one after another); only the reads stay resident per batch
-        m = G.Mapper(idx, hi - lo, end - base, share=mappers[0] if mappers else None)
+        first = mappers[i % lanes] if i >= lanes else None
+        m = G.Mapper(idx, hi - lo, end - base, share=first if lanes > 1 else (mappers[0] if mappers else None),
+                     own_stream=lanes > 1 and first is None)
         m.upload(seqs[base:end], offs[lo:hi] - base, lens[lo:hi])
         mappers.append(m)
     return mappers
@@ -135,6 +141,47 @@ def timed(mappers, steps, warmup, mode, graph):
         m.run(mode=mode, graph=graph)  # ends with a stream synchronisation
         step_ms.append((time.perf_counter() - ts) * 1e3)
         reads_done += m.n
+    elapsed = time.perf_counter() - t_start
+    return reads_done / elapsed, elapsed, step_ms
+
+
+def timed_lanes(mappers, lanes, steps, warmup, mode, graph):
+    """K timed batch runs with up to `lanes` in flight, after an untimed pass
+    over every batch and W warm-up runs: step i runs batch i % nb on lane
+    (i % nb) % lanes, whose previous run is waited for before the launch.
+    A batch's latency runs from its launch to the return of its wait."""
+    nb = len(mappers)
+    for m in mappers:
+        m.run(mode=mode, graph=graph)
+    pending = [None] * lanes
+    step_ms = []
+
+    def drain(lane, record):
+        if pending[lane] is not None:
+            m, t0 = pending[lane]
+            m.wait()
+            if record:
+                step_ms.append((time.perf_counter() - t0) * 1e3)
+            pending[lane] = None
+
+    def go(i, record):
+        b = i % nb
+        lane = b % lanes
+        drain(lane, record)
+        pending[lane] = (mappers[b], time.perf_counter())
+        mappers[b].launch(mode=mode, graph=graph)
+        return mappers[b].n
+
+    for i in range(warmup):
+        go(i, False)
+    for lane in range(lanes):
+        drain(lane, False)
+    reads_done = 0
+    t_start = time.perf_counter()
+    for i in range(steps):
+        reads_done += go(i, True)
+    for lane in range(lanes):
+        drain(lane, True)
     elapsed = time.perf_counter() - t_start
     return reads_done / elapsed, elapsed, step_ms
 
@@ -220,7 +267,8 @@ def run_c2(args, G):
                                         sub_rate=0.005, indel_rate=0.0005, seed=2)
     log(f"reads {args.reads} in {time.time() - t0:.1f}s")
     batch = args.batch or 65536
-    mappers = make_batches(G, idx, seqs, offs, lens, batch)
+    lanes = max(1, args.lanes)
+    mappers = make_batches(G, idx, seqs, offs, lens, batch, lanes=lanes)
 
     # the fused form (one wave-per-read kernel, spills to the per-stage
     # kernels), reported beside the plain per-stage drop-in
@@ -229,8 +277,13 @@ def run_c2(args, G):
     log(f"fused form: {fused_rps / 1e6:.1f} M reads/s (graph), {fused_eager_rps / 1e6:.1f} M reads/s (eager)")
     # the plain per-stage drop-in: the configuration's form and the value
     plain_eager_rps, _, _ = timed(mappers, args.steps, args.warmup, G.MODE_PLAIN, False)
-    reads_per_s, elapsed, step_ms = timed(mappers, args.steps, args.warmup, G.MODE_PLAIN, True)
-    log(f"plain form: {reads_per_s / 1e6:.1f} M reads/s (graph), {plain_eager_rps / 1e6:.1f} M reads/s (eager)")
+    serial_rps, serial_elapsed, serial_ms = timed(mappers, args.steps, args.warmup, G.MODE_PLAIN, True)
+    log(f"plain form, one batch at a time: {serial_rps / 1e6:.1f} M reads/s (graph), "
+        f"{plain_eager_rps / 1e6:.1f} M reads/s (eager)")
+    reads_per_s, elapsed, step_ms = serial_rps, serial_elapsed, serial_ms
+    if lanes > 1:
+        reads_per_s, elapsed, step_ms = timed_lanes(mappers, lanes, args.steps, args.warmup, G.MODE_PLAIN, True)
+        log(f"plain form, {lanes} batches in flight: {reads_per_s / 1e6:.1f} M reads/s (graph)")
     stages = kernel_times(G, mappers, args.steps, G.MODE_PLAIN)
     roof = lookup_roofline(stages)
     log("stage ms (mean): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_means(stages).items()))
@@ -260,13 +313,17 @@ def run_c2(args, G):
             "workload": "plain GPU drop-in: 100 Mbp synthetic genome (8 contigs, GC 41%, 5% "
                         "repeats), 1M SE 150 bp reads, 65,536-read batches, one launch per stage",
             "genome_bp": args.genome, "reads": args.reads, "batch_reads": batch,
-            "batches": len(mappers), "index_randstrobes": int(info.n_randstrobes),
+            "batches": len(mappers), "batches_in_flight": lanes,
+            "index_randstrobes": int(info.n_randstrobes),
             "bits": int(info.bits), "filter_cutoff": int(info.filter_cutoff),
             "parallelism": "single GPU",
         },
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_eager_value": plain_eager_rps,
+        "one_batch_at_a_time": {"value": serial_rps, "ms_per_step": serial_elapsed * 1e3 / args.steps,
+                                "p50_batch_ms": float(np.percentile(serial_ms, 50)),
+                                "p99_batch_ms": float(np.percentile(serial_ms, 99))},
         "stage_ms": stage_means(stages),
         "fused_form": {"value": fused_rps, "unit": "reads/s", "eager_value": fused_eager_rps,
                        "ms_per_step": fused_elapsed * 1e3 / args.steps,
@@ -633,6 +690,8 @@ def main():
     ap.add_argument("--reads", type=int, default=1_000_000, help="c2: single-end reads")
     ap.add_argument("--pairs", type=int, default=0, help="c3/c4: read pairs (2M / 10M)")
     ap.add_argument("--batch", type=int, default=0, help="reads per batch (c2 65536, c3 131072, c4 262144)")
+    ap.add_argument("--lanes", type=int, default=2,
+                    help="c2: batches in flight, each lane on its own stream and workspace (1: one at a time)")
     ap.add_argument("--genome", type=int, default=0, help="genome bp (c2/c3 100M, c4 3.1G)")
     ap.add_argument("--cpu-reads", type=int, default=1_000_000,
                     help="bounded sample of the workload timed on the CPU oracle")

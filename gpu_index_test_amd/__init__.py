@@ -553,13 +553,17 @@ def pairs_from_nams(ctx, nam_offsets, nams, n_pairs, dropoff_threshold=0.5, max_
 class Mapper:
     """Device-resident batches: upload reads, run the path, download NAMs."""
 
-    def __init__(self, index, max_reads, max_bases, share=None):
-        """share: another Mapper whose device workspace this one runs on
-        (batches run one at a time; download before running another)."""
+    def __init__(self, index, max_reads, max_bases, share=None, own_stream=False):
+        """share: another Mapper whose device workspace (and stream) this one
+        runs on (batches run one at a time; download before running another).
+        own_stream: without share, a workspace on a stream of its own, so that
+        launch() of this mapper overlaps launches of mappers on other streams."""
         index._check_owner()
         self.index, self.ctx = index, index.ctx
         h = C.c_void_p()
-        if share is None:
+        if share is None and own_stream:
+            self.ctx.check(self.ctx.L.gis_batch_create_async(self.ctx.h, max_reads, max_bases, C.byref(h)))
+        elif share is None:
             self.ctx.check(self.ctx.L.gis_batch_create(self.ctx.h, max_reads, max_bases, C.byref(h)))
         else:
             self._share = share
@@ -578,6 +582,15 @@ class Mapper:
         self.index._check_owner()
         opt = _map_options(rescue_level, hard_cutoff, mode, graph)
         self.ctx.check(self.ctx.L.gis_batch_run(self.h, C.byref(opt)))
+
+    def launch(self, rescue_level=2, hard_cutoff=1000, mode=N.MODE_PLAIN, graph=True):
+        """Enqueue the run() and return; wait() completes it."""
+        self.index._check_owner()
+        opt = _map_options(rescue_level, hard_cutoff, mode, graph)
+        self.ctx.check(self.ctx.L.gis_batch_launch(self.h, C.byref(opt)))
+
+    def wait(self):
+        self.ctx.check(self.ctx.L.gis_batch_wait(self.h))
 
     def nam_count(self):
         v = C.c_uint64()

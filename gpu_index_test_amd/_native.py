@@ -158,6 +158,9 @@ PROTOTYPES = {
     "gis_batch_destroy": (None, [_P]),
     "gis_batch_upload": (C.c_int, [_P, _P, _P, _P, _U32]),
     "gis_batch_run": (C.c_int, [_P, C.POINTER(MapOptions)]),
+    "gis_batch_create_async": (C.c_int, [_P, _U32, _U64, C.POINTER(_P)]),
+    "gis_batch_launch": (C.c_int, [_P, C.POINTER(MapOptions)]),
+    "gis_batch_wait": (C.c_int, [_P]),
     "gis_batch_download": (C.c_int, [_P, C.POINTER(NamOut)]),
     "gis_batch_stage_times": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_int), C.c_char_p, C.c_int]),
     "gis_batch_lookup_bytes": (C.c_int, [_P, C.POINTER(_U64), C.POINTER(_U64)]),

@@ -34,12 +34,14 @@ gis_batch *batch_new(gis_ctx *ctx, Workspace *share) {
 
 gis_batch *batch_new_stream(gis_ctx *ctx) {
     gis_batch *b = batch_new(ctx, nullptr);
-    // default priority, like the context's stream
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    // default priority, like the context's stream; the workspace owns it, so
+    // batches sharing the workspace keep it while any of them lives
+    if (hipStreamCreateWithFlags(&b->ws->stream, hipStreamNonBlocking) != hipSuccess) {
+        b->ws->stream = nullptr;
         batch_free(b);
         throw HipError{hipErrorOutOfMemory, "hipStreamCreateWithFlags"};
     }
-    b->own_stream = true;
+    b->stream = b->ws->stream;
     return b;
 }
 
@@ -53,6 +55,7 @@ static void workspace_release(Workspace *w) {
     for (DevBuf *x : all) release(*x);
     if (w->h_ctl) (void)hipHostFree(w->h_ctl);
     pairwork_free(w->pairs);
+    if (w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
 }
 
@@ -68,9 +71,9 @@ void batch_free(gis_batch *b) {
     }
     if (b->ws) {
         if (b->ws->owner == b) b->ws->owner = nullptr;
+        if (b->ws->in_flight == b) b->ws->in_flight = nullptr;
         if (--b->ws->refs == 0) workspace_release(b->ws);
     }
-    if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
@@ -769,6 +772,8 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
 
 // ---------------- device-resident batches ----------------
 
+static const char *const IN_FLIGHT_MSG = "a launched run on this batch's workspace was not waited for";
+
 extern "C" int gis_batch_create(gis_ctx *ctx, uint32_t max_reads, uint64_t max_bases, gis_batch **out) {
     API_BEGIN(ctx)
     if (!out) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null output");
@@ -776,6 +781,23 @@ extern "C" int gis_batch_create(gis_ctx *ctx, uint32_t max_reads, uint64_t max_b
     ensure_input_arrays(b, max_reads);
     ensure_read_arrays(b, max_reads);
     ensure<char>(b->seqs, max_bases);
+    *out = b;
+    return GIS_OK;
+    API_END(ctx)
+}
+
+extern "C" int gis_batch_create_async(gis_ctx *ctx, uint32_t max_reads, uint64_t max_bases, gis_batch **out) {
+    API_BEGIN(ctx)
+    if (!out) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null output");
+    gis_batch *b = batch_new_stream(ctx);
+    try {
+        ensure_input_arrays(b, max_reads);
+        ensure_read_arrays(b, max_reads);
+        ensure<char>(b->seqs, max_bases);
+    } catch (...) {
+        batch_free(b);
+        throw;
+    }
     *out = b;
     return GIS_OK;
     API_END(ctx)
@@ -809,6 +831,7 @@ extern "C" int gis_batch_upload(gis_batch *b, const char *seqs, const uint64_t *
     if (!b) return GIS_ERR_BAD_PARAMETER;
     gis_ctx *ctx = b->ctx;
     API_BEGIN(ctx)
+    if (b->ws->in_flight) return fail(ctx, GIS_ERR_STATE, IN_FLIGHT_MSG);
     return batch_upload(b, seqs, offsets, lens, n_reads, false);
     API_END(ctx)
 }
@@ -818,7 +841,33 @@ extern "C" int gis_batch_run(gis_batch *b, const gis_map_options *opt) {
     gis_ctx *ctx = b->ctx;
     API_BEGIN(ctx)
     if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
+    if (b->ws->in_flight) return fail(ctx, GIS_ERR_STATE, IN_FLIGHT_MSG);
     batch_run(b, opts_or_default(opt), 0, STOP_NONE);
+    return GIS_OK;
+    API_END(ctx)
+}
+
+extern "C" int gis_batch_launch(gis_batch *b, const gis_map_options *opt) {
+    if (!b) return GIS_ERR_BAD_PARAMETER;
+    gis_ctx *ctx = b->ctx;
+    API_BEGIN(ctx)
+    if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
+    if (b->ws->in_flight) return fail(ctx, GIS_ERR_STATE, IN_FLIGHT_MSG);
+    b->pending_opt = opts_or_default(opt);
+    batch_launch(b, run_opts(b, b->pending_opt, 0, STOP_NONE));
+    b->ws->in_flight = b;
+    return GIS_OK;
+    API_END(ctx)
+}
+
+extern "C" int gis_batch_wait(gis_batch *b) {
+    if (!b) return GIS_ERR_BAD_PARAMETER;
+    gis_ctx *ctx = b->ctx;
+    API_BEGIN(ctx)
+    if (b->ws->in_flight != b) return fail(ctx, GIS_ERR_STATE, "no launched run of this batch to wait for");
+    // released first: after an error the workspace takes runs again
+    b->ws->in_flight = nullptr;
+    batch_complete(b, run_opts(b, b->pending_opt, 0, STOP_NONE));
     return GIS_OK;
     API_END(ctx)
 }
@@ -833,7 +882,7 @@ extern "C" int gis_batch_download(gis_batch *b, gis_nam_out *out) {
 
 extern "C" int gis_batch_nam_count(gis_batch *b, uint64_t *n_nams) {
     if (!b || !n_nams) return GIS_ERR_BAD_PARAMETER;
-    if (b->ws->owner != b) return GIS_ERR_STATE;
+    if (b->ws->owner != b || b->ws->in_flight) return GIS_ERR_STATE;
     *n_nams = b->ws->tot_nams;
     return GIS_OK;
 }
@@ -842,6 +891,7 @@ extern "C" int gis_batch_stage_times(gis_batch *b, double *ms, int cap, int *n, 
     if (!b) return GIS_ERR_BAD_PARAMETER;
     gis_ctx *ctx = b->ctx;
     API_BEGIN(ctx)
+    if (b->ws->in_flight) return fail(ctx, GIS_ERR_STATE, IN_FLIGHT_MSG);
     if (b->last_graph)
         return fail(ctx, GIS_ERR_STATE, "stage times need a run with GIS_RUN_NO_GRAPH (graph replays carry no events)");
     std::string nm;
@@ -892,6 +942,7 @@ extern "C" int gis_batch_autotune(gis_batch *b, const gis_map_options *opt, gis_
     API_BEGIN(ctx)
     if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
     if (b->n_reads == 0) return fail(ctx, GIS_ERR_STATE, "no reads uploaded");
+    if (b->ws->in_flight) return fail(ctx, GIS_ERR_STATE, IN_FLIGHT_MSG);
     gis_map_options o = opts_or_default(opt);
     o.mode = GIS_MODE_FUSED;
     o.flags |= GIS_RUN_NO_GRAPH;

@@ -27,6 +27,11 @@ struct Workspace {
     uint64_t tot_nams = 0;
     uint32_t n_spill = 0;
     const gis_batch *owner = nullptr;  // the batch whose run the results belong to
+    // the batch whose run was launched (gis_batch_launch) and not yet waited
+    // for: the workspace takes no other run meanwhile
+    const gis_batch *in_flight = nullptr;
+    // a stream created for this workspace (destroyed with it), else null
+    hipStream_t stream = nullptr;
     // pinned landing area for the end-of-run readback
     DevCtl *h_ctl = nullptr;
     int refs = 0;
@@ -66,9 +71,11 @@ struct gis_batch {
     };
     std::vector<GraphEntry> graphs;
     uint64_t graph_clock = 0;
-    // the stream the batch runs on (the context's, or its own for a serving slot)
+    // the stream the batch runs on: the context's, or its workspace's own
+    // (a serving slot, gis_batch_create_async and the batches sharing it)
     hipStream_t stream = nullptr;
-    bool own_stream = false;
+    // the options of the launched run gis_batch_wait completes
+    gis_map_options pending_opt{};
 };
 
 namespace gis {

@@ -254,6 +254,20 @@ int gis_batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets,
                      const uint32_t *lens, uint32_t n_reads);
 /* Run the whole path on the uploaded reads; stays on the device. */
 int gis_batch_run(gis_batch *b, const gis_map_options *opt);
+/* Overlapped batches. The reference's GPU driver maps one batch after another
+ * with a device-wide synchronisation after each stage
+ * (step2/main.cu:4396-4729); here a run can be enqueued and completed apart,
+ * so that runs of batches on different streams overlap on the device.
+ * gis_batch_create_async makes a batch with its own stream and workspace
+ * (gis_batch_create_shared batches of it share both). gis_batch_launch
+ * enqueues the run gis_batch_run would do and returns; gis_batch_wait
+ * completes it (regrowth reruns included) and leaves the batch as
+ * gis_batch_run would. Until then the workspace takes no other run, upload,
+ * autotune or stage-time query (GIS_ERR_STATE). One host thread drives a
+ * context's launches and waits (calls on a context are serialised). */
+int gis_batch_create_async(gis_ctx *ctx, uint32_t max_reads, uint64_t max_bases, gis_batch **out);
+int gis_batch_launch(gis_batch *b, const gis_map_options *opt);
+int gis_batch_wait(gis_batch *b);
 /* Copy results out (canonical NAM order). */
 int gis_batch_download(gis_batch *b, gis_nam_out *out);
 /* Per-stage kernel times (ms) of the last run, measured with HIP events on the

@@ -342,11 +342,16 @@ struct BatchNams {
 };
 
 // A device-resident batch: upload reads once, run the path (seeding through
-// NAMs, with the caller's rescue decision), download NAMs.
+// NAMs, with the caller's rescue decision), download NAMs. With own_stream the
+// batch runs on a stream of its own, and launch() / wait() overlap its run
+// with runs of batches on other streams.
 class Batch {
   public:
-    Batch(const StrobemerIndex &index, uint32_t max_reads, uint64_t max_bases) : index_(index) {
-        check(gis_batch_create(index.context(), max_reads, max_bases, &b_), index.context());
+    Batch(const StrobemerIndex &index, uint32_t max_reads, uint64_t max_bases, bool own_stream = false)
+        : index_(index) {
+        check(own_stream ? gis_batch_create_async(index.context(), max_reads, max_bases, &b_)
+                         : gis_batch_create(index.context(), max_reads, max_bases, &b_),
+              index.context());
     }
     ~Batch() {
         if (b_) gis_batch_destroy(b_);
@@ -371,6 +376,12 @@ class Batch {
         gis_map_options_default(&o);
         check(gis_batch_run(b_, opt ? opt : &o), index_.context());
     }
+    void launch(const gis_map_options *opt = nullptr) {
+        gis_map_options o;
+        gis_map_options_default(&o);
+        check(gis_batch_launch(b_, opt ? opt : &o), index_.context());
+    }
+    void wait() { check(gis_batch_wait(b_), index_.context()); }
     BatchNams download() {
         uint64_t total = 0;
         check(gis_batch_nam_count(b_, &total), index_.context());

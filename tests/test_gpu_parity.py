@@ -523,6 +523,59 @@ def test_shared_workspace_batches(setup):
         a.close()
 
 
+def test_overlapped_batches(setup):
+    """Batches on two lanes (a stream and workspace each, a second batch
+    sharing each lane) launched so that runs of both lanes are in flight at
+    once give the oracle's NAMs, graph-replayed and eager, in both forms; a
+    lane's workspace refuses a second launch, a run, an upload and a wait of
+    another batch while its run is in flight."""
+    g, lens, p, idx, op, oidx = setup
+    sets = [_read_set(g, lens, 150, seed=s) for s in (111, 112, 113, 114)]
+    packs = [G.pack_reads(r) for r in sets]
+    exp = [O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8) for s, o, l in packs]
+    size = (max(len(pk[2]) for pk in packs), max(len(pk[0]) for pk in packs))
+    a0 = G.Mapper(idx, *size, own_stream=True)
+    b0 = G.Mapper(idx, *size, own_stream=True)
+    a1 = G.Mapper(idx, *size, share=a0)
+    b1 = G.Mapper(idx, *size, share=b0)
+    ms = [a0, b0, a1, b1]
+
+    def check(i):
+        res = ms[i].download()
+        e = exp[i]
+        _compare_nams((res.nam_offsets, res.nams), (e[0], e[1]), res.nonrepetitive_fraction, e[2],
+                      res.flags, e[3])
+
+    try:
+        for m, pk in zip(ms, packs):
+            m.upload(*pk)
+        for mode in (G.MODE_PLAIN, G.MODE_FUSED):
+            for graph in (True, False, True):
+                a0.launch(mode=mode, graph=graph)
+                b0.launch(mode=mode, graph=graph)
+                a0.wait()
+                check(0)
+                a1.launch(mode=mode, graph=graph)
+                b0.wait()
+                check(1)
+                b1.launch(mode=mode, graph=graph)
+                a1.wait()
+                check(2)
+                b1.wait()
+                check(3)
+        a0.launch()
+        for bad in (a1.launch, a0.launch, a0.run, a1.wait, lambda: a1.upload(*packs[2])):
+            with pytest.raises(G.GisError):
+                bad()
+        a0.wait()
+        check(0)
+        with pytest.raises(G.GisError):
+            a0.wait()
+    finally:
+        for m in reversed(ms):
+            m.close()
+
+
 def test_sti_rejects_damaged_files(setup, tmp_path):
     """A .sti whose bucket starts leave the table, or that is cut short, is
     refused on the host before anything reaches the GPU."""
