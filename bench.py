@@ -186,6 +186,20 @@ def timed_lanes(mappers, lanes, steps, warmup, mode, graph):
     return reads_done / elapsed, elapsed, step_ms
 
 
+def timed_forms(mappers, lanes, steps, warmup, mode):
+    """Graph-replayed runs one batch at a time and with `lanes` in flight."""
+    one = timed(mappers, steps, warmup, mode, True)
+    many = timed_lanes(mappers, lanes, steps, warmup, mode, True) if lanes > 1 else one
+    return one, many
+
+
+def form_json(t, steps, **extra):
+    rps, elapsed, ms = t
+    return {"value": rps, "unit": "reads/s", "ms_per_step": elapsed * 1e3 / steps,
+            "p50_batch_ms": float(np.percentile(ms, 50)), "p99_batch_ms": float(np.percentile(ms, 99)),
+            **extra}
+
+
 def kernel_times(G, mappers, k, mode):
     """The same runs enqueued launch by launch, with HIP events between the
     kernels on the batch's stream (graph replays carry no events); with each
@@ -347,15 +361,19 @@ def run_c3(args, G):
     # each batch holds whole pairs laid out for the pair step (mate 1 block,
     # mate 2 block)
     offs, lens = pair_blocks(offs, lens, batch)
-    mappers = make_batches(G, idx, seqs, offs, lens, batch)
+    lanes = max(1, args.lanes)
+    mappers = make_batches(G, idx, seqs, offs, lens, batch, lanes=lanes)
     steps = args.steps
 
     best, best_ms, n_tried = mappers[0].autotune()
     log(f"autotune: {n_tried} configurations, best {best} at {best_ms:.3f} ms per batch")
-    fused_rps, fused_elapsed, fused_ms = timed(mappers, steps, args.warmup, G.MODE_FUSED, True)
-    log(f"fused (autotuned): {fused_rps / 1e6:.1f} M reads/s")
-    plain_rps, plain_elapsed, plain_ms = timed(mappers, steps, args.warmup, G.MODE_PLAIN, True)
-    log(f"plain per-stage: {plain_rps / 1e6:.1f} M reads/s")
+    fused_one, fused_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_FUSED)
+    log(f"fused (autotuned): {fused_one[0] / 1e6:.1f} M reads/s one batch at a time, "
+        f"{fused_many[0] / 1e6:.1f} M with {lanes} in flight")
+    plain_one, plain_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_PLAIN)
+    log(f"plain per-stage: {plain_one[0] / 1e6:.1f} M reads/s one batch at a time, "
+        f"{plain_many[0] / 1e6:.1f} M with {lanes} in flight")
+    fused_rps, fused_elapsed, fused_ms = fused_many
     fstage = kernel_times(G, mappers, steps, G.MODE_FUSED)
     mappers[0].run(mode=G.MODE_FUSED)
     spill_frac = float(np.mean((mappers[0].download().flags & 0x80) != 0))
@@ -393,19 +411,17 @@ def run_c3(args, G):
                         "50/100/150/250/300 bp (0.1/0.2/0.4/0.2/0.1), insert N(400, 50), "
                         "131,072-read batches, persistent fused kernel (autotuned)",
             "genome_bp": args.genome, "pairs": pairs, "reads": int(len(lens)),
-            "batch_reads": batch, "batches": len(mappers), "fused_config": best,
+            "batch_reads": batch, "batches": len(mappers), "batches_in_flight": lanes, "fused_config": best,
             "autotune_candidates": n_tried, "spilled_fraction": spill_frac,
             "index_randstrobes": int(info.n_randstrobes), "parallelism": "single GPU",
         },
         "roofline": roof,
         "cpu_baseline": cpu,
         "fused_stage_ms": stage_means(fstage),
+        "fused_one_batch_at_a_time": form_json(fused_one, steps),
         "pair_step": pairs_out,
-        "plain_form": {"value": plain_rps, "unit": "reads/s",
-                       "ms_per_step": plain_elapsed * 1e3 / steps,
-                       "p50_batch_ms": float(np.percentile(plain_ms, 50)),
-                       "p99_batch_ms": float(np.percentile(plain_ms, 99)),
-                       "stage_ms": stage_means(stages)},
+        "plain_form": form_json(plain_many, steps, stage_ms=stage_means(stages),
+                                one_batch_at_a_time=form_json(plain_one, steps)),
     }
 
 
@@ -436,17 +452,21 @@ def run_c4(args, G):
     log(f"reads {len(lens)} ({pairs} pairs) in {time.time() - t0:.1f}s")
     batch = args.batch or 262144
     t0 = time.time()
-    mappers = make_batches(G, idx, seqs, offs, lens, batch)
+    lanes = max(1, args.lanes)
+    mappers = make_batches(G, idx, seqs, offs, lens, batch, lanes=lanes)
     log(f"{len(mappers)} batches resident in {time.time() - t0:.1f}s")
     steps = args.steps
     best, best_ms, n_tried = mappers[0].autotune()
     log(f"autotune: {n_tried} configurations, best {best} at {best_ms:.3f} ms per batch")
-    fused_rps, fused_elapsed, fused_ms = timed(mappers, steps, args.warmup, G.MODE_FUSED, True)
-    log(f"fused (autotuned): {fused_rps / 1e6:.1f} M reads/s")
+    fused_one, fused_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_FUSED)
+    log(f"fused (autotuned): {fused_one[0] / 1e6:.1f} M reads/s one batch at a time, "
+        f"{fused_many[0] / 1e6:.1f} M with {lanes} in flight")
+    fused_rps, fused_elapsed, fused_ms = fused_many
     mappers[0].run(mode=G.MODE_FUSED)
     fres = mappers[0].download()
-    plain_rps, plain_elapsed, plain_ms = timed(mappers, steps, args.warmup, G.MODE_PLAIN, True)
-    log(f"plain per-stage: {plain_rps / 1e6:.1f} M reads/s")
+    plain_one, plain_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_PLAIN)
+    log(f"plain per-stage: {plain_one[0] / 1e6:.1f} M reads/s one batch at a time, "
+        f"{plain_many[0] / 1e6:.1f} M with {lanes} in flight")
     mappers[0].run(mode=G.MODE_PLAIN)
     pres = mappers[0].download()
     same = bool(np.array_equal(fres.nam_offsets, pres.nam_offsets) and fres.nams.tobytes() == pres.nams.tobytes())
@@ -491,19 +511,17 @@ def run_c4(args, G):
                         "45% repeats, 2% tandem repeats), 10M pairs of 150 bp, 262,144-read batches, "
                         "batched fused form (autotuned) vs per-stage launches",
             "genome_bp": genome_bp, "pairs": pairs, "reads": int(len(lens)), "batch_reads": batch,
-            "batches": len(mappers), "fused_config": best, "autotune_candidates": n_tried,
-            "index_randstrobes": int(info.n_randstrobes), "bits": int(info.bits),
+            "batches": len(mappers), "batches_in_flight": lanes, "fused_config": best,
+            "autotune_candidates": n_tried, "index_randstrobes": int(info.n_randstrobes), "bits": int(info.bits),
             "filter_cutoff": int(info.filter_cutoff), "index_gb": info.device_bytes / 1e9,
             "index_build_s": build_s, "parallelism": "single GPU",
         },
         "parity_sample": {"reads": n_par, "bit_exact_vs_oracle": exact, "fused_equals_plain": same},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "plain_form": {"value": plain_rps, "unit": "reads/s",
-                       "ms_per_step": plain_elapsed * 1e3 / steps,
-                       "p50_batch_ms": float(np.percentile(plain_ms, 50)),
-                       "p99_batch_ms": float(np.percentile(plain_ms, 99)),
-                       "stage_ms": stage_means(stages)},
+        "fused_one_batch_at_a_time": form_json(fused_one, steps),
+        "plain_form": form_json(plain_many, steps, stage_ms=stage_means(stages),
+                                one_batch_at_a_time=form_json(plain_one, steps)),
     }
 
 
@@ -691,7 +709,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=0, help="c3/c4: read pairs (2M / 10M)")
     ap.add_argument("--batch", type=int, default=0, help="reads per batch (c2 65536, c3 131072, c4 262144)")
     ap.add_argument("--lanes", type=int, default=2,
-                    help="c2: batches in flight, each lane on its own stream and workspace (1: one at a time)")
+                    help="c2-c4: batches in flight, each lane on its own stream and workspace (1: one at a time)")
     ap.add_argument("--genome", type=int, default=0, help="genome bp (c2/c3 100M, c4 3.1G)")
     ap.add_argument("--cpu-reads", type=int, default=1_000_000,
                     help="bounded sample of the workload timed on the CPU oracle")
