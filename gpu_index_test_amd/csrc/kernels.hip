@@ -249,7 +249,53 @@ __device__ __forceinline__ RsPair randstrobe_pair(const SeedBufs &B, uint32_t n,
     return o;
 }
 
-// randstrobes_query order: forward (strobe1 ascending) then reverse complement
+// Randstrobe number j of a read's query list in randstrobes_query order
+// (step2/randstrobes.cu:195-241): j < n_fwd is the forward randstrobe of
+// strobe1 = syncmer j, j >= n_fwd the reverse-complement one of strobe1 =
+// syncmer n - 1 - (j - n_fwd). The two directions share one code path (lanes
+// of both run together without divergence): the reverse-complement pass walks
+// the syncmer list backwards, and its positions are mirrored, L - k - pos.
+struct Rs1 {
+    uint64_t h;
+    uint32_t s, e;
+};
+
+__device__ __forceinline__ Rs1 randstrobe_one(const SeedBufs &B, uint32_t n, uint32_t n_fwd, uint32_t j, int L,
+                                              const SeedParams &P) {
+    const bool rc = j >= n_fwd;
+    const uint32_t i = rc ? j - n_fwd : j;
+    const uint32_t last = n - 1;
+    const uint32_t k = (uint32_t)P.k;
+    const uint32_t mirror = (uint32_t)L - k;  // a syncmer starts at most at L - k
+    const uint32_t a = rc ? last - i : i;
+    const uint32_t p1 = rc ? mirror - B.sy_pos[a] : B.sy_pos[a];
+    const uint64_t h1 = B.sy_hash[a];
+    const uint64_t maxp = (uint64_t)p1 + P.max_dist_u;
+    uint32_t best = a, pbest = p1;
+    uint64_t minv = UINT64_MAX;
+#pragma unroll 1
+    for (uint32_t d = P.w_min; d <= P.w_max; d++) {
+        if (i + d <= last) {
+            const uint32_t x = rc ? a - d : a + d;
+            const uint64_t hx = B.sy_hash[x];
+            const uint32_t px = rc ? mirror - B.sy_pos[x] : B.sy_pos[x];
+            const uint64_t c = (uint64_t)__popcll((h1 ^ hx) & P.q);
+            if ((uint64_t)px <= maxp && c < minv) {
+                minv = c;
+                best = x;
+                pbest = px;
+            }
+        }
+    }
+    Rs1 o;
+    o.h = h1 + B.sy_hash[best];
+    o.s = p1;
+    o.e = pbest + k;
+    return o;
+}
+
+// randstrobes_query order: forward (strobe1 ascending) then reverse complement,
+// one randstrobe per lane
 __device__ __forceinline__ uint32_t seed_one(const char *__restrict__ seq, int L, const SeedParams &P,
                                              const SeedBufs &B, uint64_t *out_sy_hash,
                                              uint64_t *out_sy_pos, uint64_t *rs_hash,
@@ -257,20 +303,18 @@ __device__ __forceinline__ uint32_t seed_one(const char *__restrict__ seq, int L
                                              uint32_t *n_sync_out) {
     const uint32_t n = seed_syncmers(seq, L, P, B, 0xffffffffu, out_sy_hash, out_sy_pos);
     const uint32_t n_fwd = ((uint32_t)L >= P.w_max && n > P.w_min) ? n - P.w_min : 0;
-    for (uint32_t i0 = 0; i0 < n_fwd; i0 += 64) {
-        const uint32_t i = i0 + lane_id();
-        if (i < n_fwd) {
-            const RsPair o = randstrobe_pair(B, n, i, L, P);
-            rs_hash[i] = o.fh;
-            rs_start[i] = o.fs;
-            rs_end[i] = o.fe;
-            rs_hash[n_fwd + i] = o.rh;
-            rs_start[n_fwd + i] = o.rs;
-            rs_end[n_fwd + i] = o.re;
+    const uint32_t nrs = 2 * n_fwd;
+    for (uint32_t j0 = 0; j0 < nrs; j0 += 64) {
+        const uint32_t j = j0 + lane_id();
+        if (j < nrs) {
+            const Rs1 o = randstrobe_one(B, n, n_fwd, j, L, P);
+            rs_hash[j] = o.h;
+            rs_start[j] = o.s;
+            rs_end[j] = o.e;
         }
     }
     *n_sync_out = n;
-    return 2 * n_fwd;
+    return nrs;
 }
 
 struct alignas(8) SeedLds {
