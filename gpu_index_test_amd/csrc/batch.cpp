@@ -719,6 +719,9 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
     b->want_sy = false;
     b->any_long = false;
     b->h_rs_off = off;
+    // the per-read randstrobe offsets are the one input array the stages
+    // after seeding read (a fresh context's scratch batch has none yet)
+    ensure<uint64_t>(b->rs_off, (size_t)n + 1);
     ensure_read_arrays(b, n);
     ensure_slot_arrays(b, total);
     b->ws->owner = b;

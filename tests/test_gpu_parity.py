@@ -8,7 +8,8 @@ import pytest
 
 import gpu_index_test_amd as G
 from oracle import oracle_lib as O
-from tests.helpers import length_reads, mutate, n_reads, rand_seq, revcomp, tie_heavy_reads
+from tests.helpers import (HAND_NAMS, hand_case, length_reads, mutate, n_reads, rand_seq, revcomp,
+                           tie_heavy_reads, tiny_index_arrays)
 
 pytestmark = pytest.mark.gpu
 
@@ -212,6 +213,24 @@ def test_find_nams_qrs_parity(setup):
             else:
                 _, exp, _ = O.find_nams(oidx, q, canonical=True)
             assert res.read_nams(r).tobytes() == exp.tobytes(), (rescue, r)
+
+
+def test_find_nams_composition_worked_example(ctx):
+    """The find_nams case worked by hand from the reference's text (the
+    running min_diff filter of a run, a filtered and an absent seed, both
+    strands, the merge and its scores; tests/helpers.py) on the GPU path: the
+    table uploaded as is, the query randstrobes through gis_find_nams_qrs."""
+    ents, q = hand_case()
+    arr, starts = tiny_index_arrays(ents, 8)
+    idx = G.StrobemerIndex(None, G.IndexParameters.from_read_length(150))
+    idx.upload(arr, starts, 8, 5)
+    fr, nams = G.find_nams(q, idx)
+    assert np.float32(fr) == np.float32(3) / np.float32(4)
+    got = [(int(n["is_rc"]), int(n["query_start"]), int(n["query_end"]), int(n["ref_start"]), int(n["ref_end"]),
+            int(n["n_hits"]), float(n["score"])) for n in nams]
+    assert got == HAND_NAMS
+    assert [int(n["nam_id"]) for n in nams] == list(range(len(HAND_NAMS)))
+    idx.ctx.close()
 
 
 def test_sti_roundtrip(setup, tmp_path):

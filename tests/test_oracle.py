@@ -11,7 +11,7 @@ import xxhash
 
 from oracle import oracle_lib as O
 from tests import spec
-from tests.helpers import n_reads, rand_seq, revcomp, tie_heavy_reads
+from tests.helpers import HAND_NAMS, hand_case, n_reads, rand_seq, revcomp, tie_heavy_reads, tiny_index_arrays
 
 P150 = None
 
@@ -415,16 +415,8 @@ def test_error_free_reads_find_their_origin():
 
 
 def _tiny_index(entries, filter_cutoff, bits=8):
-    """Index from (hash, position, packed) entries with a leading dummy entry
-    (the first hash of a table is never findable)."""
-    ents = sorted([(1, 0, 0)] + list(entries))
-    arr = np.array(ents, dtype=O.REF_RANDSTROBE)
-    h = arr["hash"]
-    first_run = int(np.searchsorted(h, h[0], side="right"))
-    top = (h >> np.uint64(64 - bits)).astype(np.int64)
-    starts = np.searchsorted(top[first_run:], np.arange((1 << bits) + 1)) + first_run
-    starts[-1] = len(h)
-    return O.Index.wrap(arr, starts.astype(np.uint64), bits, filter_cutoff, p150())
+    arr, starts = tiny_index_arrays(entries, bits)
+    return O.Index.wrap(arr, starts, bits, filter_cutoff, p150())
 
 
 def _qrs(hashes, strand=0):
@@ -434,6 +426,18 @@ def _qrs(hashes, strand=0):
     q["end"] = q["start"] + 30
     q["is_reverse"] = strand
     return q
+
+
+def test_find_nams_composition_worked_example():
+    ents, q = hand_case()
+    idx = _tiny_index(ents, filter_cutoff=5, bits=8)
+    frac, nams, nh = O.find_nams(idx, q, canonical=True)
+    assert np.float32(frac) == np.float32(3) / np.float32(4)
+    got = [(int(n["is_rc"]), int(n["query_start"]), int(n["query_end"]), int(n["ref_start"]), int(n["ref_end"]),
+            int(n["n_hits"]), float(n["score"])) for n in nams]
+    assert got == HAND_NAMS
+    assert [int(n["nam_id"]) for n in nams] == list(range(5))
+    assert all(int(n["ref_id"]) == 0 for n in nams)
 
 
 @pytest.mark.parametrize("n_good,n_rep,rescued", [(7, 3, True), (8, 2, False), (6, 4, True),
