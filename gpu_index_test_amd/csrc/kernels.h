@@ -138,8 +138,10 @@ constexpr int WPB = 4;         // waves per block of the per-read kernels
 constexpr int MERGE_LDS = 128; // hits a read may have for the LDS merge
 constexpr int F_NAMCAP = 32;   // NAMs a read may have in the fused kernel
 // hits of a read written by the lookup itself into the read's own region of
-// the hit array; reads with more go to the pool and a second pass
-constexpr uint32_t HIT_REGION = 48;
+// the hit array; reads with more go to the pool and a second pass. 128 holds
+// the hits of non-repetitive reads up to 300 bp (~100 randstrobes), so that
+// only repetitive reads take the pool's global counters (one atomic each)
+constexpr uint32_t HIT_REGION = 128;
 
 // launch wrappers (kernels.hip)
 void launch_reset_ctl(DevCtl *ctl, hipStream_t st);

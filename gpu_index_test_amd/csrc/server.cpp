@@ -99,9 +99,10 @@ struct Slot {
 };
 
 // device bytes a request holds while in flight: randstrobe slots and
-// lookup results (~64 B per base), hits, NAMs and per-read arrays (~2 KB
-// per read)
-uint64_t estimate_bytes(uint64_t reads, uint64_t bases) { return 64 * bases + 2048 * reads + 4096; }
+// lookup results (~64 B per base); per read its 128-hit region of the hit
+// array (2 KB), the merge's NAM staging beside it (5.6 KB), merge keys and
+// per-read arrays (~10 KB per read)
+uint64_t estimate_bytes(uint64_t reads, uint64_t bases) { return 64 * bases + 10240 * reads + 4096; }
 
 // padded step sizes, so that a slot replays one of a few captured graphs
 uint32_t padded(uint32_t n) {
