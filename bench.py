@@ -662,6 +662,15 @@ def run_c5(args, G):
     st = srv.stats()
     srv.close()
     best = max(rates, key=lambda r: r["sustained_reads_per_s"])
+    cpu = None
+    if not args.no_cpu:
+        # the oracle over the same (GPU-built) table, on the pool's first
+        # reads (single-end, the 50-300 bp mixture)
+        from oracle import oracle_lib as O
+        oidx = O.Index.wrap(idx.randstrobes, idx.randstrobe_start_indices, info.bits, info.filter_cutoff,
+                            O.params_from_read_length(150))
+        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C5")
+        oidx.free()
     return {
         "metric": "served reads/sec to NAMs, 3.1 Gbp index, Poisson request arrivals (1 GPU, "
                   "bit-exact); p50/p99 request latency",
@@ -686,6 +695,7 @@ def run_c5(args, G):
             "max_step_reads": args.step_reads, "server_mode": args.server_mode,
             "saturation_reads_per_s": sat, "parallelism": "single GPU",
         },
+        "cpu_baseline": cpu,
         "rates": rates,
         "fastq_ingest": fastq,
         "server_stats": st,
@@ -701,7 +711,7 @@ def main():
     ap.add_argument("--slots", type=int, default=8, help="c5: steps in flight")
     ap.add_argument("--step-reads", type=int, default=65536, help="c5: reads per step")
     ap.add_argument("--server-mode", choices=("plain", "fused"), default="plain")
-    ap.add_argument("--burst", type=int, default=6000, help="c5: requests of the saturation burst")
+    ap.add_argument("--burst", type=int, default=24000, help="c5: requests of the saturation burst")
     ap.add_argument("--rate-seconds", type=float, default=3.0, help="c5: seconds of arrivals per rate")
     ap.add_argument("--fastq-pairs", type=int, default=400000,
                     help="c5: pairs written to FASTQ and served from the files (0: skip)")
