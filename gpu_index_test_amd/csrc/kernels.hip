@@ -1640,7 +1640,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge4(DevBatch b, ReadSet set, in
             const bool acc = valid && j < n_open && rsd_o == h.ref_strand &&
                              nam_accepts(o.qp, o.qe, o.rp, o.re, qs, qe, rs, re, ext);
             const uint64_t bal = __ballot(acc);
-            const uint32_t mine = (uint32_t)(bal >> (seg * SEG)) & 0xffffu;
+            const uint32_t mine = (uint32_t)(bal >> (seg * SEG)) & ((1u << SEG) - 1u);
             if (mine && j == (uint32_t)__builtin_ctz(mine)) {
                 if (ext) {
                     o.qe = qe;
