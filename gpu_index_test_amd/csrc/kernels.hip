@@ -82,6 +82,32 @@ __device__ __forceinline__ uint64_t canonical_code(const uint64_t *pk, int j, in
     return f < r ? f : r;
 }
 
+// anchor_state (gis_core.h) for a window of W s-mers known at compile time:
+// the loads issue together and the minimum, its first index and the count
+// of equal values come from straight-line code. The minimum's first index
+// and the number of values equal to it are what the running scan of
+// anchor_state finds (values before the first minimum are larger).
+template <int W>
+__device__ __forceinline__ uint8_t anchor_state_w(const uint64_t *h, int p, bool seg_first) {
+    uint64_t v[W];
+#pragma unroll
+    for (int o = 0; o < W; o++) v[o] = h[p + o];
+    uint64_t minv = v[0];
+    int first = 0;
+#pragma unroll
+    for (int o = 1; o < W; o++) {
+        const bool lt = v[o] < minv;
+        minv = lt ? v[o] : minv;
+        first = lt ? o : first;
+    }
+    int eq = 0;
+#pragma unroll
+    for (int o = 0; o < W; o++) eq += v[o] == minv ? 1 : 0;
+    if (seg_first) return minv == UINT64_MAX ? ST_NONE : (uint8_t)first;
+    if (eq == 1 && minv != UINT64_MAX) return (uint8_t)first;
+    return ST_UNKNOWN;
+}
+
 // Canonical syncmers of one read (SyncmerIterator, step2/randstrobes.cu:18-79),
 // the whole wave working on the same read. The first `cap` syncmers go to
 // B.sy_hash/B.sy_pos (and out_sy_* when given); returns the total count.
@@ -135,7 +161,8 @@ __device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, 
                 st = ST_INVALID;
             } else {
                 const bool seg_first = (p == 0) || ((B.nmask[(p - 1) >> 6] >> ((p - 1) & 63)) & 1);
-                st = anchor_state(B.sh, p, w, seg_first);
+                // w = k - s + 1 is 5 in every read-length profile up to 375 bp
+                st = w == 5 ? anchor_state_w<5>(B.sh, p, seg_first) : anchor_state(B.sh, p, w, seg_first);
             }
             B.st[p] = st;
         }
@@ -270,17 +297,23 @@ __device__ __forceinline__ Rs1 randstrobe_one(const SeedBufs &B, uint32_t n, uin
     const uint32_t a = rc ? last - i : i;
     const uint32_t p1 = rc ? mirror - B.sy_pos[a] : B.sy_pos[a];
     const uint64_t h1 = B.sy_hash[a];
-    const uint64_t maxp = (uint64_t)p1 + P.max_dist_u;
-    uint32_t best = a, pbest = p1;
-    uint64_t minv = UINT64_MAX;
+    // 32-bit arithmetic in the candidate loop: positions are below 2^32, so
+    // the limit clamps there; the cost is at most 64; the high halves of the
+    // hashes only count when q has high bits (q = 2^c - 1, c = 8 by default)
+    const uint64_t maxp64 = (uint64_t)p1 + P.max_dist_u;
+    const uint32_t maxp = maxp64 > 0xffffffffull ? 0xffffffffu : (uint32_t)maxp64;
+    const uint32_t qlo = (uint32_t)P.q, qhi = (uint32_t)(P.q >> 32);
+    const uint32_t h1lo = (uint32_t)h1, h1hi = (uint32_t)(h1 >> 32);
+    const uint32_t *hw = reinterpret_cast<const uint32_t *>(B.sy_hash);
+    uint32_t best = a, pbest = p1, minv = 0xffffffffu;
 #pragma unroll 1
     for (uint32_t d = P.w_min; d <= P.w_max; d++) {
         if (i + d <= last) {
             const uint32_t x = rc ? a - d : a + d;
-            const uint64_t hx = B.sy_hash[x];
+            uint32_t c = (uint32_t)__popc((h1lo ^ hw[2 * x]) & qlo);
+            if (qhi) c += (uint32_t)__popc((h1hi ^ hw[2 * x + 1]) & qhi);
             const uint32_t px = rc ? mirror - B.sy_pos[x] : B.sy_pos[x];
-            const uint64_t c = (uint64_t)__popcll((h1 ^ hx) & P.q);
-            if ((uint64_t)px <= maxp && c < minv) {
+            if (px <= maxp && c < minv) {
                 minv = c;
                 best = x;
                 pbest = px;
