@@ -1869,6 +1869,8 @@ __global__ __launch_bounds__(64 * FWPB) void k_fused(DevBatch b, DevIndex ix, Se
 // ---------------------------------------------------------------------------
 // dense output: NAMs of read r to nams_out[nam_off[r] ..]
 
+constexpr uint32_t COMPACT_SEG = 16;
+
 __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
     const uint32_t lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0) b.ctl->tot[2] = b.nam_off[b.n_reads];
@@ -1878,7 +1880,14 @@ __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&b.ctl->overflow, OVF_OUT);
         return;
     }
-    for (uint32_t r = blockIdx.x * WPB + (threadIdx.x >> 6); r < b.n_reads; r += gridDim.x * WPB) {
+    // COMPACT_SEG lanes per read, so that a wave's reads start their chains
+    // of dependent loads (count, flags, offsets, NAMs) together
+    const uint32_t seg = lane / COMPACT_SEG, j = lane % COMPACT_SEG;
+    const uint32_t per_wave = 64 / COMPACT_SEG;
+    for (uint32_t r0 = (blockIdx.x * WPB + (threadIdx.x >> 6)) * per_wave; r0 < b.n_reads;
+         r0 += gridDim.x * WPB * per_wave) {
+        const uint32_t r = r0 + seg;
+        if (r >= b.n_reads) continue;
         const uint32_t c = b.nam_cnt[r];
         if (c == 0) continue;
         const uint8_t fl = b.flags[r];
@@ -1889,7 +1898,7 @@ __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
         const uint32_t *src = reinterpret_cast<const uint32_t *>(srcn);
         uint32_t *dst = reinterpret_cast<uint32_t *>(b.nams_out + b.nam_off[r]);
         const uint32_t words = c * (uint32_t)(sizeof(gis_nam) / 4);
-        for (uint32_t i = lane; i < words; i += 64) dst[i] = src[i];
+        for (uint32_t i = j; i < words; i += COMPACT_SEG) dst[i] = src[i];
     }
 }
 
@@ -2143,7 +2152,8 @@ bool launch_fused(const DevBatch &b, const DevIndex &ix, const SeedParams &sp, u
 
 void launch_compact(const DevBatch &b, int fused, hipStream_t st) {
     if (b.n_reads == 0) return;
-    hipLaunchKernelGGL(k_compact, dim3(grid_for(b.n_reads, WPB)), dim3(64 * WPB), 0, st, b, fused);
+    hipLaunchKernelGGL(k_compact, dim3(grid_for(b.n_reads, WPB * (64 / COMPACT_SEG))), dim3(64 * WPB), 0, st, b,
+                       fused);
 }
 
 void launch_find_flat(const DevIndex &ix, const uint64_t *hashes, uint64_t n, uint64_t *positions,
