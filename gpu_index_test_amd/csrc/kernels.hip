@@ -2026,6 +2026,30 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_add(uint64_t *out, uint64_t
         if (base + j < n) out[base + j] += add;
 }
 
+// k_scan_sums and k_scan_add in one launch for up to SCAN_FEW_TILES tiles:
+// each block adds the sums of the tiles before its own (one wave reads them),
+// and the last block writes the total
+constexpr unsigned SCAN_FEW_TILES = 256;
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_add_few(uint64_t *out, uint64_t n, const uint64_t *tile_sums,
+                                                             uint32_t ntiles) {
+    __shared__ uint64_t add_s;
+    if (threadIdx.x < 64) {
+        uint64_t v = 0;
+        for (uint32_t t = threadIdx.x; t < blockIdx.x; t += 64) v += tile_sums[t];
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+        if (threadIdx.x == 0) {
+            add_s = v;
+            if (blockIdx.x == ntiles - 1) out[n] = v + tile_sums[blockIdx.x];
+        }
+    }
+    __syncthreads();
+    const uint64_t add = add_s;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    for (int j = 0; j < SCAN_ITEMS; j++)
+        if (base + j < n) out[base + j] += add;
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers
 
@@ -2166,6 +2190,11 @@ size_t scan_tmp_words(uint64_t n) { return (size_t)grid_for(n ? n : 1, SCAN_TILE
 
 void launch_scan(const uint32_t *in, uint64_t *out, uint64_t n, uint64_t *tmp, hipStream_t st) {
     const unsigned tiles = grid_for(n ? n : 1, SCAN_TILE);
+    if (n > 0 && tiles <= SCAN_FEW_TILES) {
+        hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(SCAN_BLOCK), 0, st, in, out, n, tmp);
+        hipLaunchKernelGGL(k_scan_add_few, dim3(tiles), dim3(SCAN_BLOCK), 0, st, out, n, tmp, (uint32_t)tiles);
+        return;
+    }
     if (n > 0)
         hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(SCAN_BLOCK), 0, st, in, out, n, tmp);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_BLOCK), 0, st, tmp, (uint64_t)(n ? tiles : 0), out, n);
