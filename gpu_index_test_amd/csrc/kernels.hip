@@ -1872,6 +1872,7 @@ __global__ __launch_bounds__(64 * FWPB) __attribute__((amdgpu_waves_per_eu(5))) 
 // dense output: NAMs of read r to nams_out[nam_off[r] ..]
 
 constexpr uint32_t COMPACT_SEG = 16;
+constexpr uint32_t COMPACT_SEG_WORDS = 8 * COMPACT_SEG;  // 11 NAMs of 11 words
 
 __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
     const uint32_t lane = lane_id();
@@ -1883,24 +1884,43 @@ __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
         return;
     }
     // COMPACT_SEG lanes per read, so that a wave's reads start their chains
-    // of dependent loads (count, flags, offsets, NAMs) together
+    // of dependent loads (count, flags, offsets, NAMs) together; reads with
+    // more than COMPACT_SEG_WORDS words of NAMs (repeats) are copied by the
+    // whole wave afterwards
     const uint32_t seg = lane / COMPACT_SEG, j = lane % COMPACT_SEG;
     const uint32_t per_wave = 64 / COMPACT_SEG;
     for (uint32_t r0 = (blockIdx.x * WPB + (threadIdx.x >> 6)) * per_wave; r0 < b.n_reads;
          r0 += gridDim.x * WPB * per_wave) {
         const uint32_t r = r0 + seg;
-        if (r >= b.n_reads) continue;
-        const uint32_t c = b.nam_cnt[r];
-        if (c == 0) continue;
-        const uint8_t fl = b.flags[r];
-        const gis_nam *srcn;
-        if (fused && !(fl & FLAG_SPILL)) srcn = b.nam_slots + (uint64_t)r * F_NAMCAP;
-        else if (fl & GIS_FLAG_RESCUED) srcn = b.rnam_fin + b.rhit_off[r];
-        else srcn = b.nam_fin + b.hit_off[r];
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(srcn);
-        uint32_t *dst = reinterpret_cast<uint32_t *>(b.nams_out + b.nam_off[r]);
+        const uint32_t c = r < b.n_reads ? b.nam_cnt[r] : 0u;
+        const uint32_t *src = nullptr;
+        uint32_t *dst = nullptr;
+        if (c > 0) {
+            const uint8_t fl = b.flags[r];
+            const gis_nam *srcn;
+            if (fused && !(fl & FLAG_SPILL)) srcn = b.nam_slots + (uint64_t)r * F_NAMCAP;
+            else if (fl & GIS_FLAG_RESCUED) srcn = b.rnam_fin + b.rhit_off[r];
+            else srcn = b.nam_fin + b.hit_off[r];
+            src = reinterpret_cast<const uint32_t *>(srcn);
+            dst = reinterpret_cast<uint32_t *>(b.nams_out + b.nam_off[r]);
+        }
         const uint32_t words = c * (uint32_t)(sizeof(gis_nam) / 4);
-        for (uint32_t i = j; i < words; i += COMPACT_SEG) dst[i] = src[i];
+        const bool big = words > COMPACT_SEG_WORDS;
+        if (!big)
+            for (uint32_t i = j; i < words; i += COMPACT_SEG) dst[i] = src[i];
+        uint64_t bigs = __ballot(big && j == 0);
+        while (bigs) {
+            const uint32_t l = (uint32_t)__builtin_ctzll(bigs);
+            bigs &= bigs - 1;
+            const uint32_t wl = readlane_u32(words, l);
+            const uint32_t *sl = reinterpret_cast<const uint32_t *>(
+                (uint64_t)readlane_u32((uint32_t)(uintptr_t)src, l) |
+                ((uint64_t)readlane_u32((uint32_t)((uintptr_t)src >> 32), l) << 32));
+            uint32_t *dl = reinterpret_cast<uint32_t *>(
+                (uint64_t)readlane_u32((uint32_t)(uintptr_t)dst, l) |
+                ((uint64_t)readlane_u32((uint32_t)((uintptr_t)dst >> 32), l) << 32));
+            for (uint32_t i = lane; i < wl; i += 64) dl[i] = sl[i];
+        }
     }
 }
 
