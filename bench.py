@@ -708,8 +708,8 @@ def main():
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", choices=("c2", "c3", "c4", "c5"), default="c2")
-    ap.add_argument("--slots", type=int, default=8, help="c5: steps in flight")
-    ap.add_argument("--step-reads", type=int, default=65536, help="c5: reads per step")
+    ap.add_argument("--slots", type=int, default=6, help="c5: steps in flight")
+    ap.add_argument("--step-reads", type=int, default=131072, help="c5: reads per step")
     ap.add_argument("--server-mode", choices=("plain", "fused"), default="plain")
     ap.add_argument("--burst", type=int, default=24000, help="c5: requests of the saturation burst")
     ap.add_argument("--rate-seconds", type=float, default=3.0, help="c5: seconds of arrivals per rate")

@@ -649,7 +649,7 @@ class Server:
     Requests are admitted by an HBM budget, coalesced into steps replayed
     from captured HIP graphs, with up to `slots` steps in flight."""
 
-    def __init__(self, index, slots=4, max_step_reads=65536, hbm_budget=0, mode=N.MODE_PLAIN,
+    def __init__(self, index, slots=6, max_step_reads=131072, hbm_budget=0, mode=N.MODE_PLAIN,
                  graph=True, rescue_level=2, hard_cutoff=1000):
         index._check_owner()
         self.index, self.ctx = index, index.ctx

@@ -282,15 +282,15 @@ extern "C" int gis_server_create(gis_ctx *ctx, const gis_server_config *cfg, gis
     if (cfg) {
         c = *cfg;
     } else {
-        c.slots = 4;
-        c.max_step_reads = 65536;
+        c.slots = 6;
+        c.max_step_reads = 131072;
         c.hbm_budget = 0;
         gis_map_options_default(&c.options);
     }
     if (c.slots < 1 || c.slots > 16) return fail(ctx, GIS_ERR_BAD_PARAMETER, "slots must be in 1..16");
     if (c.max_step_reads < 1 || c.max_step_reads > (1u << 22))
         return fail(ctx, GIS_ERR_BAD_PARAMETER, "max_step_reads must be in 1..4194304");
-    if (c.hbm_budget == 0) c.hbm_budget = 8ull << 30;
+    if (c.hbm_budget == 0) c.hbm_budget = 32ull << 30;
     if (c.options.mode != GIS_MODE_FUSED) c.options.mode = GIS_MODE_PLAIN;
     gis_server *s = new gis_server();
     s->ctx = ctx;

@@ -300,7 +300,7 @@ int gis_batch_autotune(gis_batch *b, const gis_map_options *opt, gis_fused_confi
 typedef struct {
     uint32_t slots;            /* steps in flight, each on its own stream (1..16) */
     uint32_t max_step_reads;   /* reads coalesced into one step (a larger request runs alone) */
-    uint64_t hbm_budget;       /* estimated device bytes of the steps in flight (0: 8 GiB) */
+    uint64_t hbm_budget;       /* estimated device bytes of the steps in flight (0: 32 GiB) */
     gis_map_options options;   /* mode, rescue level, hard cutoff; graph replays unless GIS_RUN_NO_GRAPH */
 } gis_server_config;
 
@@ -311,7 +311,7 @@ typedef struct {
 } gis_server_stats;
 
 typedef struct gis_server gis_server;
-/* cfg may be NULL: 4 slots, 65,536 reads per step, 8 GiB, default options */
+/* cfg may be NULL: 6 slots, 131,072 reads per step, 32 GiB, default options */
 int gis_server_create(gis_ctx *ctx, const gis_server_config *cfg, gis_server **out);
 /* Stops the worker; requests still queued are cancelled. No call on the
  * server may be in progress. */
