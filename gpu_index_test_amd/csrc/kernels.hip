@@ -1529,7 +1529,14 @@ __global__ __launch_bounds__(64) void k_merge_groups(DevBatch b, int rescue) {
     __shared__ ChunkBox lbox[GROUP_BOXES];
     const uint32_t lane = lane_id();
     const unsigned int cnt = b.ctl->n_gwork[rescue ? 1 : 0];
-    for (uint32_t w = blockIdx.x; w < cnt; w += gridDim.x) {
+    // items are taken one at a time from a counter, so that a wave that
+    // drew a heavy group does not also hold the items a fixed round robin
+    // would have dealt it; every wave leaves once the counter passes cnt
+    for (;;) {
+        uint32_t w = 0;
+        if (lane == 0) w = atomicAdd(&b.ctl->g_head[rescue ? 1 : 0], 1u);
+        w = readlane_u32(w, 0);
+        if (w >= cnt) break;
         const uint4 it = b.gwork[w];
         const uint32_t r = it.x, gs = it.y, ge = it.z, g = it.w & 0x3fffffffu, code = it.w >> 30;
         const uint64_t hb = rescue ? b.rhit_off[r] : b.hit_off[r];
@@ -1966,6 +1973,7 @@ __global__ void k_reset_ctl(DevCtl *ctl) {
         ctl->q_head[0] = 0;
         ctl->q_head[1] = 0;
         ctl->n_gwork[0] = ctl->n_gwork[1] = 0;
+        ctl->g_head[0] = ctl->g_head[1] = 0;
         ctl->n_split[0] = ctl->n_split[1] = 0;
         ctl->n_sel[0] = ctl->n_sel[1] = ctl->n_sel[2] = 0;
         ctl->hit_pool = 0;
