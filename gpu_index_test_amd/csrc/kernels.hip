@@ -520,6 +520,63 @@ __device__ __forceinline__ uint32_t walk_run(const DevIndex &ix, uint64_t pos, u
     return c;
 }
 
+// The same walk by the whole wave, 64 records per step, for the long runs of
+// the rescue path (up to hard_cutoff records): the running minimum of
+// add_to_hits_per_ref is the minimum of all earlier differences (an entry
+// that is not kept does not lower it), so entry e is kept iff its difference
+// is at most the exclusive prefix minimum at e. Returns the kept count (the
+// same on every lane) and writes the kept hits in run order when out !=
+// nullptr. Every lane calls it with the same arguments.
+constexpr uint32_t WALK_WAVE_MIN = 32;  // runs longer than this take the wave walk
+
+__device__ __forceinline__ uint32_t walk_run_wave(const DevIndex &ix, uint64_t pos, uint64_t key, int qs, int qe,
+                                                  uint32_t strand_bit, DevHit *out) {
+    const uint32_t lane = lane_id();
+    const int qspan = qe - qs;
+    int carry = 0x7fffffff;
+    uint32_t kept = 0;
+    for (uint64_t x0 = pos; x0 < ix.n; x0 += 64) {
+        const uint64_t x = x0 + lane;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        bool in = false;
+        if (x < ix.n) {
+            v = ix.rs[x];
+            in = ((uint64_t)v.x | ((uint64_t)v.y << 32)) == key;
+        }
+        // equal hashes are contiguous: the run ends at the first other record
+        const uint64_t other = __ballot(!in);
+        const uint32_t len = other ? (uint32_t)__builtin_ctzll(other) : 64u;
+        const bool valid = lane < len;
+        const int rspan = (int)(v.w & 0xffu) + ix.k;
+        int d = qspan - rspan;
+        d = d < 0 ? -d : d;
+        int incl = valid ? d : 0x7fffffff;
+        for (int sh = 1; sh < 64; sh <<= 1) {
+            const int y = __shfl_up(incl, sh, 64);
+            if (lane >= (uint32_t)sh) incl = y < incl ? y : incl;
+        }
+        int excl = __shfl_up(incl, 1, 64);
+        if (lane == 0) excl = 0x7fffffff;
+        excl = carry < excl ? carry : excl;
+        const bool keep = valid && d <= excl;
+        const uint64_t km = __ballot(keep);
+        if (out && keep) {
+            DevHit hh;
+            hh.ref_strand = strand_bit | (v.w >> 8);
+            hh.ref_start = (int32_t)v.z;
+            hh.ref_end = (int32_t)v.z + rspan;
+            hh.query_start = (uint16_t)qs;
+            hh.query_end = (uint16_t)qe;
+            out[kept + mbcnt(km)] = hh;
+        }
+        kept += (uint32_t)__popcll(km);
+        const int last = (int)readlane_u32((uint32_t)incl, 63);
+        carry = last < carry ? last : carry;
+        if (len < 64) break;
+    }
+    return kept;
+}
+
 // ---------------------------------------------------------------------------
 // Probe of one query randstrobe for the per-stage lookup. The result is the
 // reference's (StrobemerIndex::find + is_filtered + the run walk of
@@ -840,6 +897,7 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_count(DevBatch b, ReadSet s
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const uint32_t i = i0 + lane;
             uint32_t rh = 0;
+            bool coop = false;
             if (i < n) {
                 const uint32_t c = b.hcnt[base + i];
                 bool taken = false;
@@ -857,11 +915,22 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_count(DevBatch b, ReadSet s
                         taken = rank < 5;
                     }
                 }
-                if (taken)
+                if (taken && c <= WALK_WAVE_MIN)
                     rh = walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)b.rs_start[base + i],
                                   (int)b.rs_end[base + i], 0u, nullptr);
-                b.rcnt[base + i] = rh;
+                coop = taken && c > WALK_WAVE_MIN;
             }
+            // long runs: the whole wave walks each one
+            uint64_t cm = __ballot(coop);
+            while (cm) {
+                const uint32_t l = (uint32_t)__builtin_ctzll(cm);
+                cm &= cm - 1;
+                const uint64_t xl = base + i0 + l;
+                const uint32_t k = walk_run_wave(ix, b.pos[xl], b.rs_hash[xl], (int)b.rs_start[xl],
+                                                 (int)b.rs_end[xl], 0u, nullptr);
+                if (lane == l) rh = k;
+            }
+            if (i < n) b.rcnt[base + i] = rh;
             total += wave_sum(rh);
         }
         if (lane == 0) b.rhit_total[r] = total;
@@ -885,10 +954,22 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_hits(DevBatch b, ReadSet se
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const uint32_t i = i0 + lane;
             const uint32_t rh = i < n ? b.rcnt[base + i] : 0u;
+            // runs longer than WALK_WAVE_MIN (k_rescue_count left their
+            // lengths in hcnt) take the wave walk, as they did there
+            const bool coop = rh > 0 && b.hcnt[base + i] > WALK_WAVE_MIN;
             const uint32_t ex = wave_excl_scan(rh);
-            if (rh > 0)
+            if (rh > 0 && !coop)
                 walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)b.rs_start[base + i],
                          (int)b.rs_end[base + i], i >= half ? 0x80000000u : 0u, out + carry + ex);
+            uint64_t cm = __ballot(coop);
+            while (cm) {
+                const uint32_t l = (uint32_t)__builtin_ctzll(cm);
+                cm &= cm - 1;
+                const uint32_t il = i0 + l;
+                const uint64_t xl = base + il;
+                walk_run_wave(ix, b.pos[xl], b.rs_hash[xl], (int)b.rs_start[xl], (int)b.rs_end[xl],
+                              il >= half ? 0x80000000u : 0u, out + carry + readlane_u32(ex, l));
+            }
             carry += wave_sum(rh);
         }
     }
