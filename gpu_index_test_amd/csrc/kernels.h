@@ -47,7 +47,6 @@ struct DevCtl {
     unsigned int overflow;  // OVF_* bits
     unsigned int q_head[2]; // dynamic work-queue heads of the fused kernel's two bins
     unsigned int n_gwork[2];  // group work items of split reads (normal, rescue merge)
-    unsigned int g_head[2];   // next group work item to take (normal, rescue merge)
     unsigned int n_split[2];  // split reads (normal, rescue merge)
     unsigned int n_sel[3];    // selected reads: rescued, deferred merges, pooled hits
     unsigned long long hit_pool;  // hits placed in the pool after the per-read regions

@@ -1610,14 +1610,7 @@ __global__ __launch_bounds__(64) void k_merge_groups(DevBatch b, int rescue) {
     __shared__ ChunkBox lbox[GROUP_BOXES];
     const uint32_t lane = lane_id();
     const unsigned int cnt = b.ctl->n_gwork[rescue ? 1 : 0];
-    // items are taken one at a time from a counter, so that a wave that
-    // drew a heavy group does not also hold the items a fixed round robin
-    // would have dealt it; every wave leaves once the counter passes cnt
-    for (;;) {
-        uint32_t w = 0;
-        if (lane == 0) w = atomicAdd(&b.ctl->g_head[rescue ? 1 : 0], 1u);
-        w = readlane_u32(w, 0);
-        if (w >= cnt) break;
+    for (uint32_t w = blockIdx.x; w < cnt; w += gridDim.x) {
         const uint4 it = b.gwork[w];
         const uint32_t r = it.x, gs = it.y, ge = it.z, g = it.w & 0x3fffffffu, code = it.w >> 30;
         const uint64_t hb = rescue ? b.rhit_off[r] : b.hit_off[r];
@@ -1836,6 +1829,27 @@ __device__ __forceinline__ void spill_read(const DevBatch &b, uint32_t r) {
     wave_sync_mem();
 }
 
+// The lanes' runs flagged in `mine` walked by the whole wave, one after the
+// other: the kept count goes to the owning lane's `kept`, and with out the
+// hits go to out + (the owning lane's at).
+__device__ __forceinline__ void fused_wave_walks(const DevIndex &ix, bool mine, uint64_t pos, uint64_t key,
+                                                 uint32_t qs, uint32_t qe, uint32_t strand_bit, uint32_t &kept,
+                                                 DevHit *out, uint32_t at) {
+    const uint32_t lane = lane_id();
+    uint64_t m = __ballot(mine);
+    while (m) {
+        const uint32_t l = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        const uint64_t pl =
+            (uint64_t)readlane_u32((uint32_t)pos, l) | ((uint64_t)readlane_u32((uint32_t)(pos >> 32), l) << 32);
+        const uint64_t kl =
+            (uint64_t)readlane_u32((uint32_t)key, l) | ((uint64_t)readlane_u32((uint32_t)(key >> 32), l) << 32);
+        const uint32_t k = walk_run_wave(ix, pl, kl, (int)readlane_u32(qs, l), (int)readlane_u32(qe, l), strand_bit,
+                                         out ? out + readlane_u32(at, l) : nullptr);
+        if (lane == l) kept = k;
+    }
+}
+
 template <int LMAX>
 __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix, const SeedParams &P,
                                            FusedLds<LMAX> &S, uint32_t r, uint32_t rescue_cutoff,
@@ -1877,6 +1891,7 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
             const uint32_t hits_norm = wave_sum(hf + hr);
             frac = found > 0 ? (float)good / (float)found : 1.0f;
             rescue = (double)frac < 0.7 || hits_norm == 0;
+            bool wf = false, wr = false;  // taken runs longer than WALK_WAVE_MIN: the wave walks them
             if (rescue) {
                 // run lengths and the taken prefix of each strand
                 const uint32_t cf = act && pf != GIS_NOT_FOUND ? dev_get_count(ix, pf) : 0u;
@@ -1892,8 +1907,12 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
                 }
                 const bool tf = cf > 0 && cf <= hard_cutoff && (cf <= rescue_cutoff || rankf < 5);
                 const bool tr = cr > 0 && cr <= hard_cutoff && (cr <= rescue_cutoff || rankr < 5);
-                hf = tf ? walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, nullptr) : 0u;
-                hr = tr ? walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, nullptr) : 0u;
+                wf = tf && cf > WALK_WAVE_MIN;
+                wr = tr && cr > WALK_WAVE_MIN;
+                hf = tf && !wf ? walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, nullptr) : 0u;
+                hr = tr && !wr ? walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, nullptr) : 0u;
+                fused_wave_walks(ix, wf, pf, rp.fh, rp.fs, rp.fe, 0u, hf, nullptr, 0u);
+                fused_wave_walks(ix, wr, pr, rp.rh, rp.rs, rp.re, 0x80000000u, hr, nullptr, 0u);
             }
             // hits into LDS: forward seeds in lane order, then reverse ones
             const uint32_t tot_f = wave_sum(hf);
@@ -1901,8 +1920,11 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
             spill = nh > (uint32_t)MERGE_LDS;
             if (!spill) {
                 const uint32_t of = wave_excl_scan(hf), orr = tot_f + wave_excl_scan(hr);
-                if (hf) walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, S.merge.hits + of);
-                if (hr) walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, S.merge.hits + orr);
+                if (hf && !wf) walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, S.merge.hits + of);
+                if (hr && !wr) walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, S.merge.hits + orr);
+                uint32_t unused_f = 0, unused_r = 0;
+                fused_wave_walks(ix, wf, pf, rp.fh, rp.fs, rp.fe, 0u, unused_f, S.merge.hits, of);
+                fused_wave_walks(ix, wr, pr, rp.rh, rp.rs, rp.re, 0x80000000u, unused_r, S.merge.hits, orr);
                 wave_sync_mem();
             }
         }
@@ -2054,7 +2076,6 @@ __global__ void k_reset_ctl(DevCtl *ctl) {
         ctl->q_head[0] = 0;
         ctl->q_head[1] = 0;
         ctl->n_gwork[0] = ctl->n_gwork[1] = 0;
-        ctl->g_head[0] = ctl->g_head[1] = 0;
         ctl->n_split[0] = ctl->n_split[1] = 0;
         ctl->n_sel[0] = ctl->n_sel[1] = ctl->n_sel[2] = 0;
         ctl->hit_pool = 0;
