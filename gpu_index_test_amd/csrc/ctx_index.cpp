@@ -18,13 +18,24 @@ using namespace gis;
 
 namespace {
 
+// the bucket lines of the resident table (one 128-byte line per bucket)
+void build_lines(gis_ctx *ctx) {
+    DevIndexStore &ix = ctx->ix;
+    ensure<uint4>(ix.lines, (size_t)BUCKET_LINE << ix.bits);
+    launch_bucket_lines(static_cast<const uint4 *>(ix.rs.p), static_cast<const uint64_t *>(ix.starts.p), ix.bits,
+                        static_cast<uint4 *>(ix.lines.p), ctx->stream);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
 void upload_index(gis_ctx *ctx, const HostIndex &h) {
     DevIndexStore &ix = ctx->ix;
     ix.loaded = false;
     // drop the previous table first so two tables never coexist in HBM
     release(ix.rs);
     release(ix.starts);
-    ensure<gis_ref_randstrobe>(ix.rs, h.rs.size() + 1);  // + the probe pad record
+    release(ix.lines);
+    ensure<gis_ref_randstrobe>(ix.rs, h.rs.size());
     ensure<uint64_t>(ix.starts, h.starts.size());
     if (!h.rs.empty())
         HIP_CHECK(hipMemcpy(ix.rs.p, h.rs.data(), h.rs.size() * sizeof(gis_ref_randstrobe),
@@ -39,6 +50,7 @@ void upload_index(gis_ctx *ctx, const HostIndex &h) {
     ix.high = h.tot_high_ab;
     ix.mid = h.tot_mid_ab;
     ix.index_cutoff = h.index_cutoff;
+    build_lines(ctx);
     ix.loaded = true;
 }
 
@@ -97,6 +109,7 @@ extern "C" void gis_ctx_destroy(gis_ctx *ctx) {
         ctx->pairs = nullptr;
         release(ctx->ix.rs);
         release(ctx->ix.starts);
+        release(ctx->ix.lines);
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;
@@ -129,8 +142,9 @@ extern "C" int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint
     ix.loaded = false;
     release(ix.rs);
     release(ix.starts);
+    release(ix.lines);
     const uint64_t n = recs.size();
-    ensure<gis_ref_randstrobe>(ix.rs, n + 1);  // + the probe pad record
+    ensure<gis_ref_randstrobe>(ix.rs, n);
     ensure<uint64_t>(ix.starts, (1ull << bits) + 1);
     if (n) HIP_CHECK(hipMemcpy(ix.rs.p, recs.data(), n * sizeof(gis_ref_randstrobe), hipMemcpyHostToDevice));
     std::vector<gis_ref_randstrobe>().swap(recs);
@@ -153,6 +167,7 @@ extern "C" int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint
     ix.high = st.tot_high_ab;
     ix.mid = st.tot_mid_ab;
     ix.index_cutoff = st.index_cutoff;
+    build_lines(ctx);
     ix.loaded = true;
     return GIS_OK;
     API_END(ctx)
@@ -217,7 +232,8 @@ extern "C" int gis_index_read_sti(gis_ctx *ctx, const char *path, const gis_para
         ix.loaded = false;
         release(ix.rs);
         release(ix.starts);
-        ensure<gis_ref_randstrobe>(ix.rs, m.n + 1);  // + the probe pad record
+        release(ix.lines);
+        ensure<gis_ref_randstrobe>(ix.rs, m.n);
         ensure<uint64_t>(ix.starts, (1ull << m.bits) + 1);
         const uint64_t chunk = 256ull << 20;
         const char *src = reinterpret_cast<const char *>(m.rs);
@@ -230,6 +246,7 @@ extern "C" int gis_index_read_sti(gis_ctx *ctx, const char *path, const gis_para
         ix.filter_cutoff = m.filter_cutoff;
         ix.params = m.params;
         ix.distinct = ix.once = ix.high = ix.mid = ix.index_cutoff = 0;
+        build_lines(ctx);
         ix.loaded = true;
     } catch (...) {
         unmap_sti(m);
@@ -301,7 +318,7 @@ extern "C" int gis_index_info_get(gis_ctx *ctx, gis_index_info *out) {
     out->tot_high_ab = ix.high;
     out->tot_mid_ab = ix.mid;
     out->index_cutoff = ix.index_cutoff;
-    out->device_bytes = ix.rs.bytes + ix.starts.bytes;
+    out->device_bytes = ix.rs.bytes + ix.starts.bytes + ix.lines.bytes;
     return GIS_OK;
     API_END(ctx)
 }

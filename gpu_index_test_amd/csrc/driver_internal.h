@@ -54,7 +54,7 @@ inline void release(DevBuf &b) {
 }
 
 struct DevIndexStore {
-    DevBuf rs, starts;
+    DevBuf rs, starts, lines;  // lines: DevIndex::lines, built from rs and starts
     uint64_t n = 0;
     int bits = 0;
     uint32_t filter_cutoff = 0;
@@ -98,6 +98,7 @@ inline DevIndex dev_index(const gis_ctx *ctx) {
     DevIndex d;
     d.rs = static_cast<const uint4 *>(ctx->ix.rs.p);
     d.starts = static_cast<const uint64_t *>(ctx->ix.starts.p);
+    d.lines = static_cast<const uint4 *>(ctx->ix.lines.p);
     d.n = ctx->ix.n;
     d.bits = ctx->ix.bits;
     d.filter_cutoff = ctx->ix.filter_cutoff;

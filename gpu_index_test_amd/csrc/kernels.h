@@ -11,11 +11,14 @@ namespace gis {
 // Resident index: RefRandstrobe table (16 B records: u64 hash, u32 position,
 // u32 packed) sorted by (hash, position), and bucket starts of 2^bits + 1
 // entries (step2/index.hpp:168-182).
-// rs holds n records and one pad record after them (never compared: the
-// lookup's unconditional probe loads may read it)
 struct DevIndex {
     const uint4 *rs;
     const uint64_t *starts;
+    // bucket lines: 2^bits lines of BUCKET_LINE uint4 (128 bytes, one cache
+    // line each): {start lo, start hi, size, 0} and the bucket's first
+    // BUCKET_LINE - 1 records (zeros past its size). The lookup reads a
+    // bucket's bounds and records with one line fetch.
+    const uint4 *lines;
     uint64_t n;
     int bits;
     uint32_t filter_cutoff;
@@ -143,7 +146,10 @@ constexpr int F_NAMCAP = 32;   // NAMs a read may have in the fused kernel
 // only repetitive reads take the pool's global counters (one atomic each)
 constexpr uint32_t HIT_REGION = 128;
 
+constexpr uint32_t BUCKET_LINE = 8;  // uint4 per bucket line
+
 // launch wrappers (kernels.hip)
+void launch_bucket_lines(const uint4 *rs, const uint64_t *starts, int bits, uint4 *lines, hipStream_t st);
 void launch_reset_ctl(DevCtl *ctl, hipStream_t st);
 void launch_seed(const DevBatch &b, const ReadSet &set, const SeedParams &sp, int want_syncmers,
                  int long_reads, hipStream_t st);

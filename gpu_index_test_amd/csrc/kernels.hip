@@ -417,9 +417,8 @@ __device__ __forceinline__ uint64_t rec_hash(const DevIndex &ix, uint64_t x) {
 // (StrobemerIndex::find, step2/index.hpp:57-81: linear scan of a small bucket,
 // lower_bound otherwise; both return the first equal entry). *probes counts
 // the table records read.
-__device__ __forceinline__ uint64_t dev_find(const DevIndex &ix, uint64_t key, uint32_t *probes) {
-    const uint32_t bk = (uint32_t)(key >> (64 - ix.bits));
-    const uint64_t s = ix.starts[bk], e = ix.starts[bk + 1];
+__device__ __forceinline__ uint64_t dev_find_in(const DevIndex &ix, uint64_t key, uint64_t s, uint64_t e,
+                                               uint32_t *probes) {
     uint32_t pr = 0;
     uint64_t res = GIS_NOT_FOUND;
     if (s == e) {
@@ -447,6 +446,11 @@ __device__ __forceinline__ uint64_t dev_find(const DevIndex &ix, uint64_t key, u
     }
     *probes = pr;
     return res;
+}
+
+__device__ __forceinline__ uint64_t dev_find(const DevIndex &ix, uint64_t key, uint32_t *probes) {
+    const uint32_t bk = (uint32_t)(key >> (64 - ix.bits));
+    return dev_find_in(ix, key, ix.starts[bk], ix.starts[bk + 1], probes);
 }
 
 __device__ __forceinline__ uint64_t dev_find(const DevIndex &ix, uint64_t key) {
@@ -580,14 +584,15 @@ __device__ __forceinline__ uint32_t walk_run_wave(const DevIndex &ix, uint64_t p
 // ---------------------------------------------------------------------------
 // Probe of one query randstrobe for the per-stage lookup. The result is the
 // reference's (StrobemerIndex::find + is_filtered + the run walk of
-// add_to_hits_per_ref), reached with fewer dependent memory steps: buckets of
-// at most PROBE_SMALL records (most of them: about 2.4 records per bucket)
-// are read whole with independent loads once their bounds are known; the
-// first equal record, the run length (equal hashes never cross a bucket),
-// the filter test and the kept hits then follow from registers. Larger
-// buckets take the reference's search and the separate filter probe.
+// add_to_hits_per_ref), reached with fewer dependent memory steps: the
+// bucket's line (DevIndex::lines) holds its bounds and first records, so one
+// line fetch serves a bucket of at most PROBE_SMALL records (nearly all of
+// them: buckets hold about 2.4 records on average). The first equal record,
+// the run length (equal hashes never cross a bucket), the filter test and the
+// kept hits then follow from registers. Larger buckets take the reference's
+// search and the separate filter probe in the table.
 
-constexpr uint32_t PROBE_SMALL = 4;
+constexpr uint32_t PROBE_SMALL = BUCKET_LINE - 1;  // 7: 99.6% of buckets at 2.4 records per bucket
 
 // a table record as the probe reads it (gis_ref_randstrobe)
 struct alignas(16) IdxRec {
@@ -608,26 +613,21 @@ struct Probe {
 
 __device__ __forceinline__ bool probe_small(const Probe &p) { return p.e - p.s <= PROBE_SMALL; }
 
-// bucket bounds (StrobemerIndex::find, step2/index.hpp:57-62)
-__device__ __forceinline__ void probe_bounds(const DevIndex &ix, Probe &p) {
+// Bucket bounds (StrobemerIndex::find, step2/index.hpp:57-62) and the
+// bucket's first PROBE_SMALL records, all from the bucket's line: one cache
+// line, independent loads, no dependent second fetch. Records past the
+// bucket's size are zeros and never compared. A size of 2^32 - 1 or more is
+// clamped in the line; such buckets are large, and large buckets search the
+// table between the starts[] bounds.
+__device__ __forceinline__ void probe_line(const DevIndex &ix, Probe &p) {
     const uint32_t bk = (uint32_t)(p.key >> (64 - ix.bits));
-    p.s = ix.starts[bk];
-    p.e = ix.starts[bk + 1];
-}
-
-// The records of a small bucket. The loads are unconditional and clamped to
-// the bucket's last record (its start, for an empty bucket; at most the pad
-// record after the table) so that they issue back to back: a load under a
-// divergent branch is followed by a wait for it. Records beyond the bucket
-// are read but never compared.
-__device__ __forceinline__ void probe_load(const DevIndex &ix, Probe &p) {
-    const IdxRec *recs = reinterpret_cast<const IdxRec *>(ix.rs);
-    const uint64_t last = p.e > p.s ? p.e - 1 : p.s;
+    const uint4 *line = ix.lines + (uint64_t)bk * BUCKET_LINE;
+    const uint4 h = line[0];
+    const IdxRec *recs = reinterpret_cast<const IdxRec *>(line + 1);
 #pragma unroll
-    for (uint32_t j = 0; j < PROBE_SMALL; j++) {
-        const uint64_t x = p.s + j;
-        p.r[j] = recs[x < last ? x : last];
-    }
+    for (uint32_t j = 0; j < PROBE_SMALL; j++) p.r[j] = recs[j];
+    p.s = (uint64_t)h.x | ((uint64_t)h.y << 32);
+    p.e = p.s + h.z;
 }
 
 // Search, filter and run walk. Small buckets: the reference's linear scan
@@ -675,7 +675,7 @@ __device__ __forceinline__ void probe_finish(const DevIndex &ix, Probe &p, int q
         }
         return;
     }
-    p.pos = dev_find(ix, p.key, &p.probes);
+    p.pos = p.e - p.s < 0xffffffffull ? dev_find_in(ix, p.key, p.s, p.e, &p.probes) : dev_find(ix, p.key, &p.probes);
     p.found = p.pos != GIS_NOT_FOUND;
     if (!p.found) return;
     p.filtered = dev_filtered(ix, p.pos, p.key);
@@ -746,13 +746,10 @@ __global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, ReadSet set, De
                 qs[u] = (int)b.rs_start[x];
                 qe[u] = (int)b.rs_end[x];
             }
-            // each stage's loads issue together; the scheduler may not hoist
-            // their uses between them
+            // the bucket lines' loads issue together; the scheduler may not
+            // hoist their uses between them
 #pragma unroll
-            for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) probe_bounds(ix, P[u]);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) probe_load(ix, P[u]);
+            for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) probe_line(ix, P[u]);
             __builtin_amdgcn_sched_barrier(0);
             uint32_t by = 0, hc_all = 0, f_all = 0, g_all = 0;
 #pragma unroll
@@ -2212,6 +2209,26 @@ static inline unsigned grid_selected(const DevBatch &b) {
 static void launch_select(const DevBatch &b, const ReadSet &set, int kind, hipStream_t st) {
     const unsigned g = grid_for(b.n_reads, 256);
     hipLaunchKernelGGL(k_select, dim3(g < 1024u ? (g ? g : 1u) : 1024u), dim3(256), 0, st, b, set, kind);
+}
+
+// one thread per bucket: its line (DevIndex::lines) from starts and the table
+__global__ __launch_bounds__(256) void k_bucket_lines(const uint4 *rs, const uint64_t *starts, uint64_t nb,
+                                                      uint4 *lines) {
+    const uint64_t bk = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (bk >= nb) return;
+    const uint64_t s = starts[bk], e = starts[bk + 1];
+    const uint64_t size = e - s;
+    uint4 *line = lines + bk * BUCKET_LINE;
+    line[0] = make_uint4((uint32_t)s, (uint32_t)(s >> 32), size < 0xffffffffull ? (uint32_t)size : 0xffffffffu, 0u);
+    for (uint32_t j = 1; j < BUCKET_LINE; j++) {
+        const uint64_t x = s + j - 1;
+        line[j] = x < e ? rs[x] : make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+void launch_bucket_lines(const uint4 *rs, const uint64_t *starts, int bits, uint4 *lines, hipStream_t st) {
+    const uint64_t nb = 1ull << bits;
+    hipLaunchKernelGGL(k_bucket_lines, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, rs, starts, nb, lines);
 }
 
 void launch_reset_ctl(DevCtl *ctl, hipStream_t st) {
