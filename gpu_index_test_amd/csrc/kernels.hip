@@ -1826,6 +1826,33 @@ __device__ __forceinline__ void spill_read(const DevBatch &b, uint32_t r) {
     wave_sync_mem();
 }
 
+// find + is_filtered + the counting run walk of one query randstrobe through
+// its bucket line (the per-stage lookup's probe): position, found and not
+// filtered, kept hits, and for a small bucket the kept records' mask
+__device__ __forceinline__ void fused_probe(const DevIndex &ix, uint64_t key, int qs, int qe, uint64_t &pos,
+                                            bool &good, uint32_t &kept, uint32_t &mask, bool &small) {
+    Probe P;
+    P.key = key;
+    probe_line(ix, P);
+    probe_finish(ix, P, qs, qe);
+    pos = P.pos;
+    good = P.found && !P.filtered;
+    kept = P.hc;
+    mask = P.keep;
+    small = probe_small(P);
+}
+
+// the records of a small bucket's line that `mask` selects, written as hits
+// in run order (the line is read again; it was fetched by the probe)
+__device__ __forceinline__ void fused_emit_line(const DevIndex &ix, uint64_t key, uint32_t mask, int qs, int qe,
+                                                uint32_t strand_bit, DevHit *out) {
+    Probe P;
+    P.key = key;
+    probe_line(ix, P);
+    P.keep = mask;
+    probe_emit(ix, P, qs, qe, strand_bit, out);
+}
+
 // The lanes' runs flagged in `mine` walked by the whole wave, one after the
 // other: the kept count goes to the owning lane's `kept`, and with out the
 // hits go to out + (the owning lane's at).
@@ -1867,21 +1894,16 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
         if (act) rp = randstrobe_pair(B, n, lane, L, P);
         wave_sync();  // the syncmer lists are dead from here; LDS becomes MergeLds
         if (!spill) {
-            // both randstrobes of the lane through find / filter / run walk
+            // both randstrobes of the lane through find / filter / run walk:
+            // the bucket-line probe of the per-stage lookup, one strand after
+            // the other; a small bucket's kept records are remembered as a
+            // mask over its line
             uint64_t pf = GIS_NOT_FOUND, pr = GIS_NOT_FOUND;
-            uint32_t hf = 0, hr = 0;
-            bool gf = false, gr = false;
+            uint32_t hf = 0, hr = 0, kf = 0, kr = 0;
+            bool gf = false, gr = false, sf = false, sr = false;
             if (act) {
-                pf = dev_find(ix, rp.fh);
-                pr = dev_find(ix, rp.rh);
-                if (pf != GIS_NOT_FOUND && !dev_filtered(ix, pf, rp.fh)) {
-                    gf = true;
-                    hf = walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, nullptr);
-                }
-                if (pr != GIS_NOT_FOUND && !dev_filtered(ix, pr, rp.rh)) {
-                    gr = true;
-                    hr = walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, nullptr);
-                }
+                fused_probe(ix, rp.fh, (int)rp.fs, (int)rp.fe, pf, gf, hf, kf, sf);
+                fused_probe(ix, rp.rh, (int)rp.rs, (int)rp.re, pr, gr, hr, kr, sr);
             }
             const uint32_t found = wave_sum((pf != GIS_NOT_FOUND ? 1u : 0u) + (pr != GIS_NOT_FOUND ? 1u : 0u));
             const uint32_t good = wave_sum((gf ? 1u : 0u) + (gr ? 1u : 0u));
@@ -1917,8 +1939,16 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
             spill = nh > (uint32_t)MERGE_LDS;
             if (!spill) {
                 const uint32_t of = wave_excl_scan(hf), orr = tot_f + wave_excl_scan(hr);
-                if (hf && !wf) walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, S.merge.hits + of);
-                if (hr && !wr) walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, S.merge.hits + orr);
+                // a taken seed of a small bucket keeps the records the probe
+                // kept (its run walk is the same in both modes)
+                if (hf && !wf) {
+                    if (sf) fused_emit_line(ix, rp.fh, kf, (int)rp.fs, (int)rp.fe, 0u, S.merge.hits + of);
+                    else walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, S.merge.hits + of);
+                }
+                if (hr && !wr) {
+                    if (sr) fused_emit_line(ix, rp.rh, kr, (int)rp.rs, (int)rp.re, 0x80000000u, S.merge.hits + orr);
+                    else walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, S.merge.hits + orr);
+                }
                 uint32_t unused_f = 0, unused_r = 0;
                 fused_wave_walks(ix, wf, pf, rp.fh, rp.fs, rp.fe, 0u, unused_f, S.merge.hits, of);
                 fused_wave_walks(ix, wr, pr, rp.rh, rp.rs, rp.re, 0x80000000u, unused_r, S.merge.hits, orr);
