@@ -719,8 +719,10 @@ __device__ __forceinline__ void probe_emit(const DevIndex &ix, const Probe &p, i
 
 constexpr uint32_t LOOKUP_ITEMS = 1;
 
-__global__ __launch_bounds__(64 * WPB) void k_lookup(DevBatch b, ReadSet set, DevIndex ix,
-                                                     int rescue_mode, int keep_spill) {
+// Six waves per SIMD: the seven-record probe would take 86 VGPRs (five
+// waves); at 80 (one spill) the extra wave hides more fetch latency.
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(6))) void k_lookup(
+    DevBatch b, ReadSet set, DevIndex ix, int rescue_mode, int keep_spill) {
     const uint32_t lane = lane_id();
     const uint32_t cnt = set_count(set);
     for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
