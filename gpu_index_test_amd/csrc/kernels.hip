@@ -635,25 +635,33 @@ __device__ __forceinline__ void probe_line(const DevIndex &ix, Probe &p) {
 // the run is the records equal to key; hash[pos + filter_cutoff] == key
 // exactly when the run is longer than filter_cutoff (past the bucket the
 // hashes differ, past the table the reference reads ~0).
+// the first equal record of a small bucket from the line's records:
+// records below the key, records equal to it
+__device__ __forceinline__ void probe_small_find(Probe &p, uint32_t &lt, uint32_t &eq) {
+    const uint32_t sz = (uint32_t)(p.e - p.s);
+    lt = 0;
+    eq = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < PROBE_SMALL; j++) {
+        const uint64_t h = p.r[j].hash;
+        if (j < sz) {
+            lt += h < p.key ? 1u : 0u;
+            eq += h == p.key ? 1u : 0u;
+        }
+    }
+    p.probes = lt < sz ? lt + 1 : sz;
+    p.found = eq > 0;
+    p.pos = p.found ? p.s + lt : GIS_NOT_FOUND;
+}
+
 __device__ __forceinline__ void probe_finish(const DevIndex &ix, Probe &p, int qs, int qe) {
     p.hc = 0;
     p.keep = 0;
     p.records = 0;
     p.filtered = false;
     if (probe_small(p)) {
-        const uint32_t sz = (uint32_t)(p.e - p.s);
-        uint32_t lt = 0, eq = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < PROBE_SMALL; j++) {
-            const uint64_t h = p.r[j].hash;
-            if (j < sz) {
-                lt += h < p.key ? 1u : 0u;
-                eq += h == p.key ? 1u : 0u;
-            }
-        }
-        p.probes = lt < sz ? lt + 1 : sz;
-        p.found = eq > 0;
-        p.pos = p.found ? p.s + lt : GIS_NOT_FOUND;
+        uint32_t lt, eq;
+        probe_small_find(p, lt, eq);
         if (!p.found) return;
         p.filtered = eq > ix.filter_cutoff ||
                      (p.key == UINT64_MAX && p.pos + ix.filter_cutoff >= ix.n);
@@ -2114,10 +2122,20 @@ __global__ void k_reset_ctl(DevCtl *ctl) {
 
 // plain lookup over a list of hashes (the microbenchmark gpu_find,
 // gpu_test.cu:186-215, with StrobemerIndex::find semantics)
+// find (StrobemerIndex::find) of each hash through its bucket line
 __global__ void k_find_flat(DevIndex ix, const uint64_t *hashes, uint64_t n, uint64_t *positions) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    positions[i] = dev_find(ix, hashes[i]);
+    Probe p;
+    p.key = hashes[i];
+    probe_line(ix, p);
+    if (probe_small(p)) {
+        uint32_t lt, eq;
+        probe_small_find(p, lt, eq);
+    } else {
+        p.pos = p.e - p.s < 0xffffffffull ? dev_find_in(ix, p.key, p.s, p.e, &p.probes) : dev_find(ix, p.key, &p.probes);
+    }
+    positions[i] = p.pos;
 }
 
 // ---------------------------------------------------------------------------
