@@ -233,6 +233,47 @@ def test_find_nams_composition_worked_example(ctx):
     idx.ctx.close()
 
 
+def test_bucket_line_boundaries(ctx):
+    """The lookup reads a bucket of up to seven records from its line and
+    searches the table for larger ones: buckets of 7 and 8 equal records, a
+    run one past the filter cutoff, several hashes in one bucket, empty
+    buckets and misses, through gis_lookup_batch and find_nams, against the
+    oracle over the same table."""
+    def h(top, low):
+        return (top << 56) | low
+
+    ents = [(h(0x10, 5), 100 + i, 10 + i) for i in range(7)]             # 7 records: the line
+    ents += [(h(0x20, 5), 200 + i, 12) for i in range(8)]                # 8: the table search
+    ents += [(h(0x30, 5), 300 + i, 14) for i in range(9)]                # 9 > cutoff 8: filtered
+    ents += [(h(0x40, v), 400 + v, 16) for v in (3, 7, 11)]              # three hashes, one bucket
+    ents += [(h(0x50, 9), 500 + i, 20 + (i % 3)) for i in range(6)] + [(h(0x50, 10), 560, 18)]
+    ents += [(h(0x60, v), 600 + v, 22) for v in range(12)]               # 12 hashes, one bucket
+    arr, starts = tiny_index_arrays(ents, 8)
+    op = O.params_from_read_length(150)
+    oidx = O.Index.wrap(arr, starts, 8, 8, op)
+    idx = G.StrobemerIndex(None, G.IndexParameters.from_read_length(150))
+    idx.upload(arr, starts, 8, 8)
+    present = sorted({e[0] for e in ents})
+    absent = [h(0x10, 4), h(0x10, 6), h(0x20, 4), h(0x40, 5), h(0x40, 12), h(0x60, 12), h(0x70, 1), 1, 0,
+              2 ** 64 - 1]
+    keys = np.array(present + absent, dtype=np.uint64)
+    got = idx.find_many(keys)
+    exp = np.array([oidx.find(int(k)) for k in keys], dtype=np.uint64)
+    assert np.array_equal(got, exp)
+    # every key as a forward and a reverse query randstrobe of one read,
+    # with spans that make the running minimum keep some records and not others
+    q = np.zeros(2 * len(keys), dtype=O.QUERY_RANDSTROBE)
+    q["hash"] = np.concatenate([keys, keys])
+    q["start"] = np.tile(np.arange(len(keys), dtype=np.uint32) * 2, 2)
+    q["end"] = q["start"] + 40
+    q["is_reverse"] = np.repeat([0, 1], len(keys))
+    fr, nams = G.find_nams(q, idx)
+    efr, enams, _ = O.find_nams(oidx, q, canonical=True)
+    assert np.float32(fr).tobytes() == np.float32(efr).tobytes()
+    assert nams.tobytes() == enams.tobytes()
+    idx.ctx.close()
+
+
 def test_sti_roundtrip(setup, tmp_path):
     g, lens, p, idx, op, oidx = setup
     path = tmp_path / "ref.r150.sti"
