@@ -378,24 +378,25 @@ template <bool LONG>
 __global__ __launch_bounds__(64 * SEED_WPB) __attribute__((amdgpu_waves_per_eu(6))) void k_seed(DevBatch b, ReadSet set, SeedParams P,
                                                         int want_sy) {
     __shared__ SeedLds lds[LONG ? 1 : SEED_WPB];
-    const int wid = (int)(threadIdx.x >> 6);
+    const int wid = (int)uni32(threadIdx.x >> 6);
     const uint32_t cnt = set_count(set);
-    for (uint32_t w = blockIdx.x * SEED_WPB + (uint32_t)wid; w < cnt; w += gridDim.x * SEED_WPB) {
-        const uint32_t r = set_at(set, w);
-        const int L = (int)b.read_len[r];
+    for (uint32_t w = wave_index(SEED_WPB); w < cnt; w += gridDim.x * SEED_WPB) {
+        const uint32_t r = uni32(set_at(set, w));
+        const int L = (int)uni32(b.read_len[r]);
         if ((L > LMAX_LDS) != LONG) continue;
         SeedBufs B;
         if (LONG) {
-            B = long_bufs(b.long_scratch + b.long_off[r], (uint64_t)L);
+            B = long_bufs(b.long_scratch + uni64(b.long_off[r]), (uint64_t)L);
         } else {
             SeedLds &S = lds[wid];
             B = SeedBufs{S.pk, S.st, S.sh, S.sy_hash, S.sy_pos, S.nmask};
         }
-        uint64_t *osh = want_sy ? b.sy_hash + b.sy_off[r] : nullptr;
-        uint64_t *osp = want_sy ? b.sy_pos + b.sy_off[r] : nullptr;
-        const uint64_t ro = b.rs_off[r];
+        const uint64_t so = want_sy ? uni64(b.sy_off[r]) : 0;
+        uint64_t *osh = want_sy ? b.sy_hash + so : nullptr;
+        uint64_t *osp = want_sy ? b.sy_pos + so : nullptr;
+        const uint64_t ro = uni64(b.rs_off[r]);
         uint32_t nsync = 0;
-        const uint32_t nrs = seed_one(b.seqs + b.read_off[r], L, P, B, osh, osp, b.rs_hash + ro,
+        const uint32_t nrs = seed_one(b.seqs + uni64(b.read_off[r]), L, P, B, osh, osp, b.rs_hash + ro,
                                       b.rs_start + ro, b.rs_end + ro, &nsync);
         if (lane_id() == 0) {
             b.rs_cnt[r] = nrs;
@@ -727,17 +728,17 @@ __device__ __forceinline__ void probe_emit(const DevIndex &ix, const Probe &p, i
 
 constexpr uint32_t LOOKUP_ITEMS = 1;
 
-// Six waves per SIMD: the seven-record probe would take 86 VGPRs (five
-// waves); at 80 (one spill) the extra wave hides more fetch latency.
+// Six waves per SIMD (77 VGPRs): the read's index, offsets and counts are
+// held in scalar registers.
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(6))) void k_lookup(
     DevBatch b, ReadSet set, DevIndex ix, int rescue_mode, int keep_spill) {
     const uint32_t lane = lane_id();
     const uint32_t cnt = set_count(set);
-    for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
-        const uint32_t r = set_at(set, w);
-        const uint32_t n = b.rs_cnt[r];
-        const uint64_t base = b.rs_off[r];
-        const uint32_t half = b.rs_nfwd[r];
+    for (uint32_t w = wave_index(WPB); w < cnt; w += gridDim.x * WPB) {
+        const uint32_t r = uni32(set_at(set, w));
+        const uint32_t n = uni32(b.rs_cnt[r]);
+        const uint64_t base = uni64(b.rs_off[r]);
+        const uint32_t half = uni32(b.rs_nfwd[r]);
         uint32_t found = 0, good = 0, hits = 0, bytes = 0;
         DevHit *region = b.hits + (uint64_t)r * HIT_REGION;
         // the regions must lie inside the hit array (else every read pools)
@@ -847,13 +848,13 @@ __global__ __launch_bounds__(64 * WPB) void k_hits(DevBatch b, ReadSet set, DevI
     const uint32_t lane = lane_id();
     if (over_capacity(b.hit_off, b.n_reads, b.hit_cap, b.ctl, OVF_HITS)) return;
     const uint32_t cnt = set_count(set);
-    for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
-        const uint32_t r = set_at(set, w);
-        if (b.hit_total[r] == 0) continue;
-        const uint32_t n = b.rs_cnt[r];
-        const uint64_t base = b.rs_off[r];
-        const uint32_t half = b.rs_nfwd[r];
-        DevHit *out = b.hits + b.hit_off[r];
+    for (uint32_t w = wave_index(WPB); w < cnt; w += gridDim.x * WPB) {
+        const uint32_t r = uni32(set_at(set, w));
+        if (uni32(b.hit_total[r]) == 0) continue;
+        const uint32_t n = uni32(b.rs_cnt[r]);
+        const uint64_t base = uni64(b.rs_off[r]);
+        const uint32_t half = uni32(b.rs_nfwd[r]);
+        DevHit *out = b.hits + uni64(b.hit_off[r]);
         uint32_t carry = 0;
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const uint32_t i = i0 + lane;
@@ -886,12 +887,12 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_count(DevBatch b, ReadSet s
                                                            uint32_t hard_cutoff) {
     const uint32_t lane = lane_id();
     const uint32_t cnt = set_count(set);
-    for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
-        const uint32_t r = set_at(set, w);
-        if (!(b.flags[r] & GIS_FLAG_RESCUED)) continue;
-        const uint32_t n = b.rs_cnt[r];
-        const uint64_t base = b.rs_off[r];
-        const uint32_t half = b.rs_nfwd[r];
+    for (uint32_t w = wave_index(WPB); w < cnt; w += gridDim.x * WPB) {
+        const uint32_t r = uni32(set_at(set, w));
+        if (!(uni32(b.flags[r]) & GIS_FLAG_RESCUED)) continue;
+        const uint32_t n = uni32(b.rs_cnt[r]);
+        const uint64_t base = uni64(b.rs_off[r]);
+        const uint32_t half = uni32(b.rs_nfwd[r]);
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const uint32_t i = i0 + lane;
             if (i < n) {
@@ -950,13 +951,13 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_hits(DevBatch b, ReadSet se
     if (blockIdx.x == 0 && threadIdx.x == 0) b.ctl->tot[1] = b.rhit_off[b.n_reads];
     if (over_capacity(b.rhit_off, b.n_reads, b.rhit_cap, b.ctl, OVF_RHITS)) return;
     const uint32_t cnt = set_count(set);
-    for (uint32_t w = blockIdx.x * WPB + (threadIdx.x >> 6); w < cnt; w += gridDim.x * WPB) {
-        const uint32_t r = set_at(set, w);
-        if (b.rhit_total[r] == 0) continue;
-        const uint32_t n = b.rs_cnt[r];
-        const uint64_t base = b.rs_off[r];
-        const uint32_t half = b.rs_nfwd[r];
-        DevHit *out = b.rhits + b.rhit_off[r];
+    for (uint32_t w = wave_index(WPB); w < cnt; w += gridDim.x * WPB) {
+        const uint32_t r = uni32(set_at(set, w));
+        if (uni32(b.rhit_total[r]) == 0) continue;
+        const uint32_t n = uni32(b.rs_cnt[r]);
+        const uint64_t base = uni64(b.rs_off[r]);
+        const uint32_t half = uni32(b.rs_nfwd[r]);
+        DevHit *out = b.rhits + uni64(b.rhit_off[r]);
         uint32_t carry = 0;
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const uint32_t i = i0 + lane;
@@ -1517,18 +1518,18 @@ __global__ __launch_bounds__(64 * WPB) void k_merge(DevBatch b, ReadSet set, int
     __shared__ uint64_t lkeys[WPB][MERGE_LDS];
     __shared__ WideNam lwide[WPB][MERGE_WIDE_LDS];  // the wide list of reads with few enough hits
     const uint32_t lane = lane_id();
-    const int wid = (int)(threadIdx.x >> 6);
+    const int wid = (int)uni32(threadIdx.x >> 6);
     if (rescue ? over_capacity(b.rhit_off, b.n_reads, b.rhit_cap, b.ctl, OVF_RHITS)
                : over_capacity(b.hit_off, b.n_reads, b.hit_cap, b.ctl, OVF_HITS))
         return;
     const uint32_t cnt = set_count(set);
-    for (uint32_t w = blockIdx.x * WPB + (uint32_t)wid; w < cnt; w += gridDim.x * WPB) {
-        const uint32_t r = set_at(set, w);
-        const bool rescued = (b.flags[r] & GIS_FLAG_RESCUED) != 0;
+    for (uint32_t w = wave_index(WPB); w < cnt; w += gridDim.x * WPB) {
+        const uint32_t r = uni32(set_at(set, w));
+        const bool rescued = (uni32(b.flags[r]) & GIS_FLAG_RESCUED) != 0;
         if (rescued != (rescue != 0)) continue;
-        if (only_deferred && b.nam_cnt[r] != MERGE_DEFER) continue;
-        const uint64_t hb = rescue ? b.rhit_off[r] : b.hit_off[r];
-        const uint32_t nh = rescue ? b.rhit_total[r] : b.hit_total[r];
+        if (only_deferred && uni32(b.nam_cnt[r]) != MERGE_DEFER) continue;
+        const uint64_t hb = uni64(rescue ? b.rhit_off[r] : b.hit_off[r]);
+        const uint32_t nh = uni32(rescue ? b.rhit_total[r] : b.hit_total[r]);
         if (nh == 0) {
             if (lane == 0) b.nam_cnt[r] = 0;
             continue;
@@ -1889,14 +1890,14 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
                                            FusedLds<LMAX> &S, uint32_t r, uint32_t rescue_cutoff,
                                            uint32_t hard_cutoff) {
     const uint32_t lane = lane_id();
-    const int L = (int)b.read_len[r];
+    const int L = (int)uni32(b.read_len[r]);
     bool spill = L > LMAX;
     uint32_t nh = 0, n_fwd = 0;
     bool rescue = false;
     float frac = 1.0f;
     if (!spill) {
         SeedBufs B{S.seed.pk, S.seed.st, S.seed.sh, S.seed.sy_hash, S.seed.sy_pos, S.seed.nmask};
-        const uint32_t n = seed_syncmers(b.seqs + b.read_off[r], L, P, B, F_SYCAP, nullptr, nullptr);
+        const uint32_t n = seed_syncmers(b.seqs + uni64(b.read_off[r]), L, P, B, F_SYCAP, nullptr, nullptr);
         n_fwd = ((uint32_t)L >= P.w_max && n > P.w_min) ? n - P.w_min : 0;
         spill = n > (uint32_t)F_SYCAP || n_fwd > 64;
         RsPair rp{};
@@ -1996,11 +1997,11 @@ __global__ __launch_bounds__(64 * FWPB) __attribute__((amdgpu_waves_per_eu(5))) 
     DevBatch b, DevIndex ix, SeedParams P, uint32_t rescue_cutoff, uint32_t hard_cutoff,
     const uint32_t *__restrict__ queue, uint32_t count, uint32_t chunk, unsigned int *head) {
     __shared__ FusedLds<LMAX> lds[FWPB];
-    const int wid = (int)(threadIdx.x >> 6);
+    const int wid = (int)uni32(threadIdx.x >> 6);
     FusedLds<LMAX> &S = lds[wid];
     if (chunk == 0) {
-        for (uint32_t w = blockIdx.x * FWPB + (uint32_t)wid; w < count; w += gridDim.x * FWPB)
-            fused_read<LMAX>(b, ix, P, S, queue[w], rescue_cutoff, hard_cutoff);
+        for (uint32_t w = wave_index(FWPB); w < count; w += gridDim.x * FWPB)
+            fused_read<LMAX>(b, ix, P, S, uni32(queue[w]), rescue_cutoff, hard_cutoff);
         return;
     }
     // dynamic: every wave leaves once a grab starts at or past the end
@@ -2011,7 +2012,7 @@ __global__ __launch_bounds__(64 * FWPB) __attribute__((amdgpu_waves_per_eu(5))) 
         if (start >= count) break;
         const uint32_t end = count - start < chunk ? count : start + chunk;
         for (uint32_t w = start; w < end; w++)
-            fused_read<LMAX>(b, ix, P, S, queue[w], rescue_cutoff, hard_cutoff);
+            fused_read<LMAX>(b, ix, P, S, uni32(queue[w]), rescue_cutoff, hard_cutoff);
     }
 }
 

@@ -23,6 +23,19 @@ __device__ __forceinline__ void wave_sync_mem() {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// A wave-uniform value moved to scalar registers. The compiler cannot see
+// that a wave's index (threadIdx.x / 64), and what is loaded at it, is the
+// same on every lane, and otherwise keeps such values in VGPRs, which cost
+// occupancy.
+__device__ __forceinline__ uint32_t uni32(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ uint64_t uni64(uint64_t x) {
+    return ((uint64_t)uni32((uint32_t)(x >> 32)) << 32) | uni32((uint32_t)x);
+}
+// this wave's index in the grid, for kernels with wpb waves per block
+__device__ __forceinline__ uint32_t wave_index(uint32_t wpb) {
+    return uni32(blockIdx.x * wpb + (threadIdx.x >> 6));
+}
+
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
