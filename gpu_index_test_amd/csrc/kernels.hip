@@ -381,22 +381,22 @@ __global__ __launch_bounds__(64 * SEED_WPB) __attribute__((amdgpu_waves_per_eu(6
     const int wid = (int)uni32(threadIdx.x >> 6);
     const uint32_t cnt = set_count(set);
     for (uint32_t w = wave_index(SEED_WPB); w < cnt; w += gridDim.x * SEED_WPB) {
-        const uint32_t r = uni32(set_at(set, w));
-        const int L = (int)uni32(b.read_len[r]);
+        const uint32_t r = set.ids ? sload(set.ids + w) : w;
+        const int L = (int)sload(b.read_len + r);
         if ((L > LMAX_LDS) != LONG) continue;
         SeedBufs B;
         if (LONG) {
-            B = long_bufs(b.long_scratch + uni64(b.long_off[r]), (uint64_t)L);
+            B = long_bufs(b.long_scratch + sload(b.long_off + r), (uint64_t)L);
         } else {
             SeedLds &S = lds[wid];
             B = SeedBufs{S.pk, S.st, S.sh, S.sy_hash, S.sy_pos, S.nmask};
         }
-        const uint64_t so = want_sy ? uni64(b.sy_off[r]) : 0;
+        const uint64_t so = want_sy ? sload(b.sy_off + r) : 0;
         uint64_t *osh = want_sy ? b.sy_hash + so : nullptr;
         uint64_t *osp = want_sy ? b.sy_pos + so : nullptr;
-        const uint64_t ro = uni64(b.rs_off[r]);
+        const uint64_t ro = sload(b.rs_off + r);
         uint32_t nsync = 0;
-        const uint32_t nrs = seed_one(b.seqs + uni64(b.read_off[r]), L, P, B, osh, osp, b.rs_hash + ro,
+        const uint32_t nrs = seed_one(b.seqs + sload(b.read_off + r), L, P, B, osh, osp, b.rs_hash + ro,
                                       b.rs_start + ro, b.rs_end + ro, &nsync);
         if (lane_id() == 0) {
             b.rs_cnt[r] = nrs;
@@ -735,10 +735,10 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(6))) v
     const uint32_t lane = lane_id();
     const uint32_t cnt = set_count(set);
     for (uint32_t w = wave_index(WPB); w < cnt; w += gridDim.x * WPB) {
-        const uint32_t r = uni32(set_at(set, w));
-        const uint32_t n = uni32(b.rs_cnt[r]);
-        const uint64_t base = uni64(b.rs_off[r]);
-        const uint32_t half = uni32(b.rs_nfwd[r]);
+        const uint32_t r = set.ids ? sload(set.ids + w) : w;
+        const uint32_t n = sload(b.rs_cnt + r);
+        const uint64_t base = sload(b.rs_off + r);
+        const uint32_t half = sload(b.rs_nfwd + r);
         uint32_t found = 0, good = 0, hits = 0, bytes = 0;
         DevHit *region = b.hits + (uint64_t)r * HIT_REGION;
         // the regions must lie inside the hit array (else every read pools)
@@ -849,12 +849,12 @@ __global__ __launch_bounds__(64 * WPB) void k_hits(DevBatch b, ReadSet set, DevI
     if (over_capacity(b.hit_off, b.n_reads, b.hit_cap, b.ctl, OVF_HITS)) return;
     const uint32_t cnt = set_count(set);
     for (uint32_t w = wave_index(WPB); w < cnt; w += gridDim.x * WPB) {
-        const uint32_t r = uni32(set_at(set, w));
-        if (uni32(b.hit_total[r]) == 0) continue;
-        const uint32_t n = uni32(b.rs_cnt[r]);
-        const uint64_t base = uni64(b.rs_off[r]);
-        const uint32_t half = uni32(b.rs_nfwd[r]);
-        DevHit *out = b.hits + uni64(b.hit_off[r]);
+        const uint32_t r = set.ids ? sload(set.ids + w) : w;
+        if (sload(b.hit_total + r) == 0) continue;
+        const uint32_t n = sload(b.rs_cnt + r);
+        const uint64_t base = sload(b.rs_off + r);
+        const uint32_t half = sload(b.rs_nfwd + r);
+        DevHit *out = b.hits + sload(b.hit_off + r);
         uint32_t carry = 0;
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const uint32_t i = i0 + lane;
@@ -888,11 +888,11 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_count(DevBatch b, ReadSet s
     const uint32_t lane = lane_id();
     const uint32_t cnt = set_count(set);
     for (uint32_t w = wave_index(WPB); w < cnt; w += gridDim.x * WPB) {
-        const uint32_t r = uni32(set_at(set, w));
-        if (!(uni32(b.flags[r]) & GIS_FLAG_RESCUED)) continue;
-        const uint32_t n = uni32(b.rs_cnt[r]);
-        const uint64_t base = uni64(b.rs_off[r]);
-        const uint32_t half = uni32(b.rs_nfwd[r]);
+        const uint32_t r = set.ids ? sload(set.ids + w) : w;
+        if (!(sload(b.flags + r) & GIS_FLAG_RESCUED)) continue;
+        const uint32_t n = sload(b.rs_cnt + r);
+        const uint64_t base = sload(b.rs_off + r);
+        const uint32_t half = sload(b.rs_nfwd + r);
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const uint32_t i = i0 + lane;
             if (i < n) {
@@ -952,12 +952,12 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_hits(DevBatch b, ReadSet se
     if (over_capacity(b.rhit_off, b.n_reads, b.rhit_cap, b.ctl, OVF_RHITS)) return;
     const uint32_t cnt = set_count(set);
     for (uint32_t w = wave_index(WPB); w < cnt; w += gridDim.x * WPB) {
-        const uint32_t r = uni32(set_at(set, w));
-        if (uni32(b.rhit_total[r]) == 0) continue;
-        const uint32_t n = uni32(b.rs_cnt[r]);
-        const uint64_t base = uni64(b.rs_off[r]);
-        const uint32_t half = uni32(b.rs_nfwd[r]);
-        DevHit *out = b.rhits + uni64(b.rhit_off[r]);
+        const uint32_t r = set.ids ? sload(set.ids + w) : w;
+        if (sload(b.rhit_total + r) == 0) continue;
+        const uint32_t n = sload(b.rs_cnt + r);
+        const uint64_t base = sload(b.rs_off + r);
+        const uint32_t half = sload(b.rs_nfwd + r);
+        DevHit *out = b.rhits + sload(b.rhit_off + r);
         uint32_t carry = 0;
         for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             const uint32_t i = i0 + lane;
@@ -1890,14 +1890,14 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
                                            FusedLds<LMAX> &S, uint32_t r, uint32_t rescue_cutoff,
                                            uint32_t hard_cutoff) {
     const uint32_t lane = lane_id();
-    const int L = (int)uni32(b.read_len[r]);
+    const int L = (int)sload(b.read_len + r);
     bool spill = L > LMAX;
     uint32_t nh = 0, n_fwd = 0;
     bool rescue = false;
     float frac = 1.0f;
     if (!spill) {
         SeedBufs B{S.seed.pk, S.seed.st, S.seed.sh, S.seed.sy_hash, S.seed.sy_pos, S.seed.nmask};
-        const uint32_t n = seed_syncmers(b.seqs + uni64(b.read_off[r]), L, P, B, F_SYCAP, nullptr, nullptr);
+        const uint32_t n = seed_syncmers(b.seqs + sload(b.read_off + r), L, P, B, F_SYCAP, nullptr, nullptr);
         n_fwd = ((uint32_t)L >= P.w_max && n > P.w_min) ? n - P.w_min : 0;
         spill = n > (uint32_t)F_SYCAP || n_fwd > 64;
         RsPair rp{};
@@ -2001,7 +2001,7 @@ __global__ __launch_bounds__(64 * FWPB) __attribute__((amdgpu_waves_per_eu(5))) 
     FusedLds<LMAX> &S = lds[wid];
     if (chunk == 0) {
         for (uint32_t w = wave_index(FWPB); w < count; w += gridDim.x * FWPB)
-            fused_read<LMAX>(b, ix, P, S, uni32(queue[w]), rescue_cutoff, hard_cutoff);
+            fused_read<LMAX>(b, ix, P, S, sload(queue + w), rescue_cutoff, hard_cutoff);
         return;
     }
     // dynamic: every wave leaves once a grab starts at or past the end
@@ -2012,7 +2012,7 @@ __global__ __launch_bounds__(64 * FWPB) __attribute__((amdgpu_waves_per_eu(5))) 
         if (start >= count) break;
         const uint32_t end = count - start < chunk ? count : start + chunk;
         for (uint32_t w = start; w < end; w++)
-            fused_read<LMAX>(b, ix, P, S, uni32(queue[w]), rescue_cutoff, hard_cutoff);
+            fused_read<LMAX>(b, ix, P, S, sload(queue + w), rescue_cutoff, hard_cutoff);
     }
 }
 

@@ -31,6 +31,12 @@ __device__ __forceinline__ uint32_t uni32(uint32_t x) { return (uint32_t)__built
 __device__ __forceinline__ uint64_t uni64(uint64_t x) {
     return ((uint64_t)uni32((uint32_t)(x >> 32)) << 32) | uni32((uint32_t)x);
 }
+// a wave-uniform read of memory that no kernel writes while this one runs,
+// through the constant address space: a scalar load (read only)
+template <typename T>
+__device__ __forceinline__ T sload(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)p;
+}
 // this wave's index in the grid, for kernels with wpb waves per block
 __device__ __forceinline__ uint32_t wave_index(uint32_t wpb) {
     return uni32(blockIdx.x * wpb + (threadIdx.x >> 6));
