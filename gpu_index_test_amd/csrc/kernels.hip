@@ -1706,13 +1706,13 @@ __device__ __forceinline__ void seg_finalize(gis_nam *out, uint32_t base, uint32
 __global__ __launch_bounds__(64 * WPB) void k_merge4(DevBatch b, ReadSet set, int rescue) {
     __shared__ DevHit lh[WPB][RPW][64];
     const uint32_t lane = lane_id(), seg = lane / SEG, j = lane % SEG;
-    const int wid = (int)(threadIdx.x >> 6);
+    const int wid = (int)uni32(threadIdx.x >> 6);
     if (rescue ? over_capacity(b.rhit_off, b.n_reads, b.rhit_cap, b.ctl, OVF_RHITS)
                : over_capacity(b.hit_off, b.n_reads, b.hit_cap, b.ctl, OVF_HITS))
         return;
     const uint32_t cnt = set_count(set);
     const uint32_t stride = gridDim.x * WPB * RPW;
-    for (uint32_t w0 = (blockIdx.x * WPB + (uint32_t)wid) * RPW; w0 < cnt; w0 += stride) {
+    for (uint32_t w0 = wave_index(WPB) * RPW; w0 < cnt; w0 += stride) {
         const uint32_t widx = w0 + seg;
         bool act = widx < cnt;
         uint32_t r = 0, nh = 0;
