@@ -31,8 +31,11 @@ __device__ __forceinline__ uint32_t uni32(uint32_t x) { return (uint32_t)__built
 __device__ __forceinline__ uint64_t uni64(uint64_t x) {
     return ((uint64_t)uni32((uint32_t)(x >> 32)) << 32) | uni32((uint32_t)x);
 }
-// a wave-uniform read of memory that no kernel writes while this one runs,
-// through the constant address space: a scalar load (read only)
+// A wave-uniform read through the constant address space, which becomes a
+// scalar load (a read; the kernel writes nothing through it). The scalar
+// cache is not kept coherent with vector stores, so the element read must not
+// be written while the kernel runs: counts, offsets and id lists that earlier
+// kernels or the host wrote.
 template <typename T>
 __device__ __forceinline__ T sload(const T *p) {
     return *(const __attribute__((address_space(4))) T *)p;
