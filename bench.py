@@ -4,8 +4,12 @@ Default workload (the GPU drop-in configuration, C2): 100 Mbp synthetic
 genome (8 contigs, GC 41%, 5% repeat content from 500 families of 1-10 kbp
 copied 2-50x at 1-5% divergence), 1M single-end 150 bp reads (0.5%
 substitutions, 0.05% indels), fixed 65,536-read batches, one launch per
-stage. A step is one batch through seeding -> lookup -> hits -> NAMs
-(+ rescue), reads already resident in HBM.
+stage. A step is one batch of reads from pinned host memory through the
+copy of its reads to HBM, seeding -> lookup -> hits -> NAMs (+ rescue), and
+the copy of its NAM arrays (offsets, NAMs, fractions, flags) back to host
+memory; batches of several lanes are in flight so that one batch's copies
+overlap another's kernels. The rate with the reads already resident in HBM
+and the NAMs left there is reported beside it (device_resident_value).
 
 --config c3 runs the batched uneven-read form instead: the same genome, 2M
 read pairs with lengths drawn from 50/100/150/250/300 bp (weights
@@ -17,8 +21,10 @@ GPU, 10M pairs of 150 bp in 262,144-read batches, fused against per-stage,
 with a bit-exact sample against the oracle over the same table. --config c5
 serves timed request traces on that index (Poisson arrivals at five rates).
 
-Prints one JSON line: reads/s (whole job), p50/p99 batch latency, the
-roofline of the lookup kernel and the CPU oracle's rate on the host cores.
+Prints one JSON line: reads/s (whole job), p50/p99 batch latency (from the
+start of a batch's first copy to HBM to the end of its last copy back), the
+roofline of the lookup kernel and the CPU oracle's rate on every host core
+this process may run on.
 """
 import argparse
 import json
@@ -37,6 +43,77 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    """The host threads this process may run on, and what the host is."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": len(os.sched_getaffinity(0)), "nproc": os.cpu_count(), "model": model}
+
+
+def stream_pass(G, idx, pin, n_reads, batch, lanes, mode, out=None):
+    """One host-to-host pass (gis_find_nams_stream) over the first n_reads
+    reads of the pinned pool: -> (seconds, MapResult, per-batch ms)."""
+    s, o, l = pin
+    t0 = time.perf_counter()
+    res, ms = G.find_nams_stream(idx, s, o[:n_reads], l[:n_reads], batch_reads=batch, lanes=lanes, mode=mode,
+                                 out=out)
+    return time.perf_counter() - t0, res, ms
+
+
+def host_to_host(G, idx, pin, batch, steps, warmup, mode, lane_choices):
+    """The headline form: K steps (batches) from pinned host memory to NAM
+    arrays in pinned host memory. Untimed: a pass that sizes every lane's
+    buffers, captures its graphs and allocates the output, then W warm-up
+    passes per lane count, timed once each to pick the count. Timed: K
+    batches at the chosen count, as passes over the pool (a pass covers up
+    to all of it). Returns the timing and the last pass's results."""
+    n_all = len(pin[2])
+    nb_all = math.ceil(n_all / batch)
+    per_pass = min(steps, nb_all)
+    n_pass = min(n_all, per_pass * batch)
+    _, res, _ = stream_pass(G, idx, pin, n_pass, batch, max(lane_choices), mode)
+    sweep = {}
+    for lanes in lane_choices:
+        for _ in range(max(0, warmup)):
+            stream_pass(G, idx, pin, n_pass, batch, lanes, mode, res)
+        el, res, _ = stream_pass(G, idx, pin, n_pass, batch, lanes, mode, res)
+        sweep[lanes] = n_pass / el
+    lanes = max(sweep, key=sweep.get)
+    left, reads, elapsed, batch_ms = steps, 0, 0.0, []
+    while left > 0:
+        k = min(left, nb_all)
+        n = min(n_all, k * batch)
+        el, res, ms = stream_pass(G, idx, pin, n, batch, lanes, mode, res if n == n_pass else None)
+        elapsed += el
+        reads += n
+        batch_ms += list(ms)
+        left -= k
+    log(f"host to host ({'fused' if mode == G.MODE_FUSED else 'plain'}): {reads / elapsed / 1e6:.1f} M reads/s with "
+        f"{lanes} lanes (sweep: " + ", ".join(f"{k}: {v / 1e6:.1f} M" for k, v in sweep.items()) + ")")
+    return {"value": reads / elapsed, "unit": "reads/s", "ms_per_step": elapsed * 1e3 / steps,
+            "p50_batch_ms": float(np.percentile(batch_ms, 50)), "p99_batch_ms": float(np.percentile(batch_ms, 99)),
+            "lanes": lanes, "lane_sweep": {str(k): v for k, v in sweep.items()}, "reads": reads,
+            "elapsed_s": elapsed}, res, n_pass
+
+
+def parity_leg(O, oidx, seqs, offs, lens, res, n, threads):
+    """Every read of the last pass against the oracle over the same table."""
+    t0 = time.time()
+    eo, en, ef, efl = O.map_batch(oidx, seqs, offs[:n], lens[:n], 2 * oidx.filter_cutoff, 1000, threads)
+    exact = bool(np.array_equal(res.nam_offsets, eo) and res.nams.tobytes() == en.tobytes()
+                 and res.nonrepetitive_fraction.tobytes() == ef.tobytes()
+                 and np.array_equal(res.flags & 1, efl & 1))
+    log(f"parity: {n} reads bit-exact vs the oracle: {exact} ({len(en)} NAMs, {time.time() - t0:.1f}s)")
+    return {"reads": int(n), "nams": int(len(en)), "bit_exact_vs_oracle": exact}
 
 
 def c2_genome(G, genome_bp):
@@ -210,9 +287,7 @@ def kernel_times(G, mappers, k, mode):
         m.run(mode=mode, graph=False)
         st = m.stage_times()
         if mode == G.MODE_PLAIN:
-            b, q = G._native.C.c_uint64(), G._native.C.c_uint64()
-            m.ctx.check(m.ctx.L.gis_batch_lookup_bytes(m.h, G._native.C.byref(b), G._native.C.byref(q)))
-            st["_lookup_bytes"], st["_lookups"] = b.value, q.value
+            st["_lookup_bytes"], st["_lookups"], st["_lookup_streams"] = m.lookup_bytes()
         out.append(st)
     return out
 
@@ -221,23 +296,33 @@ def stage_means(stages):
     return {k: float(np.mean([st[k] for st in stages])) for k in stages[0] if not k.startswith("_")}
 
 
-def lookup_roofline(stages):
-    """Algorithmic bytes of k_lookup per launch (counted on the device in the
-    plain eager runs) over its HIP-event time."""
+def lookup_roofline(stages, config):
+    """Algorithmic bytes of k_lookup per launch, in the byte model of the
+    reference's search (16 for the bucket bounds, 16 per record a linear scan
+    reads to the first hash >= key, 8 for a found seed's filter probe, 16 per
+    run record of a kept seed; counted on the device in the plain eager runs
+    and equal to the oracle's count), over its HIP-event time. The bytes the
+    kernel streams beside the table (queries in, results out, hits written)
+    are reported apart. traffic: HBM bytes per launch from the committed
+    rocprofv3 counter summary of this configuration, if there is one."""
     lk_bytes = [st["_lookup_bytes"] for st in stages]
     lk_n = [st["_lookups"] for st in stages]
+    lk_streams = [st["_lookup_streams"] for st in stages]
     lookup_ms = [st["lookup"] for st in stages]
     achieved = float(np.sum(lk_bytes) / (np.sum(lookup_ms) * 1e-3) / 1e9)
-    log(f"lookup: {np.mean(lk_bytes) / 1e6:.1f} MB algorithmic per launch, "
-        f"{np.mean(lookup_ms):.3f} ms avg, {np.mean(lk_n):.0f} lookups per launch")
+    log(f"lookup: {np.mean(lk_bytes) / 1e6:.1f} MB algorithmic per launch (+{np.mean(lk_streams) / 1e6:.1f} MB "
+        f"streams), {np.mean(lookup_ms):.3f} ms avg, {np.mean(lk_n):.0f} lookups per launch")
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "lookup_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", f"lookup_traffic_{config}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
     return {"kernel": "k_lookup", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "algorithmic_bytes_per_launch": float(np.mean(lk_bytes)),
+            "lookups_per_launch": float(np.mean(lk_n)),
+            "algorithmic_bytes_per_lookup": float(np.sum(lk_bytes) / max(1, np.sum(lk_n))),
+            "stream_bytes_per_launch": float(np.mean(lk_streams)),
             "avg_launch_ms": float(np.mean(lookup_ms))}
 
 
@@ -249,32 +334,35 @@ def oracle_index(genome, clens, threads):
     return oidx
 
 
-def cpu_baseline(oidx, seqs, offs, lens, n_cpu, threads, what):
+def cpu_baseline(oidx, seqs, offs, lens, n_cpu, threads, what, build_flags):
     """The CPU oracle on the host cores: one warm-up pass, then the median of
     five timed passes over the first n_cpu reads, and a 1-thread figure on a
     sixteenth of them."""
     from oracle import oracle_lib as O
     n_cpu = min(n_cpu, len(lens))
-    end = int(offs[n_cpu - 1] + lens[n_cpu - 1])
+    end = int((offs[:n_cpu] + lens[:n_cpu]).max())
     rc = 2 * oidx.filter_cutoff
     O.time_map_batch(oidx, seqs[:end], offs[:n_cpu], lens[:n_cpu], rc, 1000, threads)
     runs = [O.time_map_batch(oidx, seqs[:end], offs[:n_cpu], lens[:n_cpu], rc, 1000, threads)
             for _ in range(5)]
     secs = float(np.median(runs))
     n1 = max(1, n_cpu // 16)
-    end1 = int(offs[n1 - 1] + lens[n1 - 1])
+    end1 = int((offs[:n1] + lens[:n1]).max())
     s1 = O.time_map_batch(oidx, seqs[:end1], offs[:n1], lens[:n1], rc, 1000, 1)
-    log(f"cpu oracle: {n_cpu / secs / 1e6:.2f} M reads/s on {threads} threads, "
-        f"{n1 / s1 / 1e3:.0f} k reads/s on 1 thread")
+    host = host_cores()
+    log(f"cpu oracle: {n_cpu / secs / 1e6:.2f} M reads/s on {threads} threads ({host['nproc']} cores, "
+        f"{host['model']}), {n1 / s1 / 1e3:.0f} k reads/s on 1 thread")
     return {"value": n_cpu / secs, "unit": "reads/s", "cores": threads, "kind": "port",
             "sample": f"first {n_cpu} reads of the {what} workload, seeding through NAMs + rescue; "
                       f"median of 5 passes ({secs:.2f} s each) after a warm-up pass",
-            "one_thread_value": n1 / s1}
+            "one_thread_value": n1 / s1, "nproc": host["nproc"], "affinity_threads": host["threads"],
+            "cpu_model": host["model"], "build": f"gcc {build_flags}"}
 
 
 def run_c2(args, G):
+    from oracle import oracle_lib as O
+    threads = args.cpu_threads or host_cores()["threads"]
     genome, clens = c2_genome(G, args.genome)
-    threads = min(16, os.cpu_count() or 1)
     idx, info = build_index(G, genome, clens, threads)
     t0 = time.time()
     seqs, offs, lens, _ = G.synth_reads(genome, clens, args.reads, lengths=(150,), weights=(1.0,),
@@ -282,42 +370,49 @@ def run_c2(args, G):
     log(f"reads {args.reads} in {time.time() - t0:.1f}s")
     batch = args.batch or 65536
     lanes = max(1, args.lanes)
+
+    # the headline: host memory to host memory, batches of several lanes in
+    # flight
+    pin = (G.pinned_copy(seqs), G.pinned_copy(offs), G.pinned_copy(lens))
+    h2h, res, n_pass = host_to_host(G, idx, pin, batch, args.steps, args.warmup, G.MODE_PLAIN,
+                                    args.stream_lanes)
+    nams_per_read = len(res.nams) / n_pass
+    h2h_bytes = {"h2d_per_batch": float(batch * (lens[:n_pass].mean() + 12)),
+                 "d2h_per_batch": float(batch * (nams_per_read * 44 + 13))}
+
+    # the same kernels with the reads resident in HBM and the NAMs left there
     mappers = make_batches(G, idx, seqs, offs, lens, batch, lanes=lanes)
-
-    # the fused form (one wave-per-read kernel, spills to the per-stage
-    # kernels), reported beside the plain per-stage drop-in
     fused_rps, fused_elapsed, fused_ms = timed(mappers, args.steps, args.warmup, G.MODE_FUSED, True)
-    fused_eager_rps, _, _ = timed(mappers, args.steps, args.warmup, G.MODE_FUSED, False)
-    log(f"fused form: {fused_rps / 1e6:.1f} M reads/s (graph), {fused_eager_rps / 1e6:.1f} M reads/s (eager)")
-    # the plain per-stage drop-in: the configuration's form and the value
-    plain_eager_rps, _, _ = timed(mappers, args.steps, args.warmup, G.MODE_PLAIN, False)
+    log(f"fused form (resident): {fused_rps / 1e6:.1f} M reads/s (graph)")
     serial_rps, serial_elapsed, serial_ms = timed(mappers, args.steps, args.warmup, G.MODE_PLAIN, True)
-    log(f"plain form, one batch at a time: {serial_rps / 1e6:.1f} M reads/s (graph), "
-        f"{plain_eager_rps / 1e6:.1f} M reads/s (eager)")
-    reads_per_s, elapsed, step_ms = serial_rps, serial_elapsed, serial_ms
+    log(f"plain form (resident), one batch at a time: {serial_rps / 1e6:.1f} M reads/s (graph)")
+    res_rps, res_elapsed, res_ms = serial_rps, serial_elapsed, serial_ms
     if lanes > 1:
-        reads_per_s, elapsed, step_ms = timed_lanes(mappers, lanes, args.steps, args.warmup, G.MODE_PLAIN, True)
-        log(f"plain form, {lanes} batches in flight: {reads_per_s / 1e6:.1f} M reads/s (graph)")
+        res_rps, res_elapsed, res_ms = timed_lanes(mappers, lanes, args.steps, args.warmup, G.MODE_PLAIN, True)
+        log(f"plain form (resident), {lanes} batches in flight: {res_rps / 1e6:.1f} M reads/s (graph)")
     stages = kernel_times(G, mappers, args.steps, G.MODE_PLAIN)
-    roof = lookup_roofline(stages)
+    roof = lookup_roofline(stages, "c2")
     log("stage ms (mean): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_means(stages).items()))
+    for m in mappers:
+        m.close()
 
+    oidx = oracle_index(genome, clens, threads)
+    parity = parity_leg(O, oidx, seqs, offs, lens, res, n_pass, threads)
     cpu = None
     if not args.no_cpu:
-        oidx = oracle_index(genome, clens, threads)
-        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C2")
-        oidx.free()
+        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, threads, "C2", args.oracle_build)
+    oidx.free()
 
     return {
         "metric": "reads/sec to NAMs at 150bp (1 GPU, bit-exact) vs CPU; p50/p99 batch latency",
-        "value": reads_per_s,
+        "value": h2h["value"],
         "unit": "reads/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
-        "p50_batch_ms": float(np.percentile(step_ms, 50)),
-        "p99_batch_ms": float(np.percentile(step_ms, 99)),
+        "ms_per_step": h2h["ms_per_step"],
+        "p50_batch_ms": h2h["p50_batch_ms"],
+        "p99_batch_ms": h2h["p99_batch_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -325,30 +420,39 @@ def run_c2(args, G):
         "data": "synthetic (seeded genome and reads)",
         "config": {
             "workload": "plain GPU drop-in: 100 Mbp synthetic genome (8 contigs, GC 41%, 5% "
-                        "repeats), 1M SE 150 bp reads, 65,536-read batches, one launch per stage",
+                        "repeats), 1M SE 150 bp reads, 65,536-read batches, one launch per stage; "
+                        "reads copied from pinned host memory, NAM arrays copied back",
             "genome_bp": args.genome, "reads": args.reads, "batch_reads": batch,
-            "batches": len(mappers), "batches_in_flight": lanes,
+            "batches_per_pass": math.ceil(n_pass / batch), "lanes": h2h["lanes"],
             "index_randstrobes": int(info.n_randstrobes),
             "bits": int(info.bits), "filter_cutoff": int(info.filter_cutoff),
             "parallelism": "single GPU",
         },
         "roofline": roof,
         "cpu_baseline": cpu,
-        "plain_eager_value": plain_eager_rps,
-        "one_batch_at_a_time": {"value": serial_rps, "ms_per_step": serial_elapsed * 1e3 / args.steps,
-                                "p50_batch_ms": float(np.percentile(serial_ms, 50)),
-                                "p99_batch_ms": float(np.percentile(serial_ms, 99))},
+        "parity": parity,
+        "host_to_host": {**h2h, "bytes": h2h_bytes, "nams_per_read": nams_per_read},
+        "device_resident_value": res_rps,
+        "device_resident": {"value": res_rps, "batches_in_flight": lanes,
+                            "ms_per_step": res_elapsed * 1e3 / args.steps,
+                            "p50_batch_ms": float(np.percentile(res_ms, 50)),
+                            "p99_batch_ms": float(np.percentile(res_ms, 99)),
+                            "one_batch_at_a_time": {"value": serial_rps,
+                                                    "ms_per_step": serial_elapsed * 1e3 / args.steps,
+                                                    "p50_batch_ms": float(np.percentile(serial_ms, 50)),
+                                                    "p99_batch_ms": float(np.percentile(serial_ms, 99))}},
         "stage_ms": stage_means(stages),
-        "fused_form": {"value": fused_rps, "unit": "reads/s", "eager_value": fused_eager_rps,
-                       "ms_per_step": fused_elapsed * 1e3 / args.steps,
-                       "p50_batch_ms": float(np.percentile(fused_ms, 50)),
-                       "p99_batch_ms": float(np.percentile(fused_ms, 99))},
+        "fused_form_resident": {"value": fused_rps, "unit": "reads/s",
+                                "ms_per_step": fused_elapsed * 1e3 / args.steps,
+                                "p50_batch_ms": float(np.percentile(fused_ms, 50)),
+                                "p99_batch_ms": float(np.percentile(fused_ms, 99))},
     }
 
 
 def run_c3(args, G):
+    from oracle import oracle_lib as O
+    threads = args.cpu_threads or host_cores()["threads"]
     genome, clens = c2_genome(G, args.genome)
-    threads = min(16, os.cpu_count() or 1)
     idx, info = build_index(G, genome, clens, threads)
     pairs = args.pairs or 2_000_000
     t0 = time.time()
@@ -368,39 +472,48 @@ def run_c3(args, G):
     best, best_ms, n_tried = mappers[0].autotune()
     log(f"autotune: {n_tried} configurations, best {best} at {best_ms:.3f} ms per batch")
     fused_one, fused_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_FUSED)
-    log(f"fused (autotuned): {fused_one[0] / 1e6:.1f} M reads/s one batch at a time, "
+    log(f"fused (autotuned, resident): {fused_one[0] / 1e6:.1f} M reads/s one batch at a time, "
         f"{fused_many[0] / 1e6:.1f} M with {lanes} in flight")
     plain_one, plain_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_PLAIN)
-    log(f"plain per-stage: {plain_one[0] / 1e6:.1f} M reads/s one batch at a time, "
+    log(f"plain per-stage (resident): {plain_one[0] / 1e6:.1f} M reads/s one batch at a time, "
         f"{plain_many[0] / 1e6:.1f} M with {lanes} in flight")
-    fused_rps, fused_elapsed, fused_ms = fused_many
     fstage = kernel_times(G, mappers, steps, G.MODE_FUSED)
     mappers[0].run(mode=G.MODE_FUSED)
     spill_frac = float(np.mean((mappers[0].download().flags & 0x80) != 0))
     stages = kernel_times(G, mappers, steps, G.MODE_PLAIN)
-    roof = lookup_roofline(stages)
-    from oracle import oracle_lib as O
+    roof = lookup_roofline(stages, "c3")
     pairs_out = pair_step(G, O, mappers, steps, G.MODE_FUSED)
     log(f"pair step: {pairs_out['pairs_per_s'] / 1e6:.1f} M pairs/s, bit-exact batch 0: "
         f"{pairs_out['bit_exact_batch0']}")
+    for m in mappers:
+        m.close()
 
+    # the headline: host memory to host memory
+    pin = (G.pinned_copy(seqs), G.pinned_copy(offs), G.pinned_copy(lens))
+    h2h_fused, res_f, n_pass = host_to_host(G, idx, pin, batch, steps, args.warmup, G.MODE_FUSED,
+                                            args.stream_lanes)
+    h2h_plain, res_p, _ = host_to_host(G, idx, pin, batch, steps, args.warmup, G.MODE_PLAIN, args.stream_lanes)
+    same = bool(np.array_equal(res_f.nam_offsets, res_p.nam_offsets) and res_f.nams.tobytes() == res_p.nams.tobytes())
+
+    oidx = oracle_index(genome, clens, threads)
+    parity = parity_leg(O, oidx, seqs, offs, lens, res_f, n_pass, threads)
+    parity["fused_equals_plain"] = same
     cpu = None
     if not args.no_cpu:
-        oidx = oracle_index(genome, clens, threads)
-        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C3")
-        oidx.free()
+        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, threads, "C3", args.oracle_build)
+    oidx.free()
 
     return {
         "metric": "reads/sec to NAMs, 50-300 bp paired mixture (1 GPU, bit-exact) vs CPU; "
                   "p50/p99 batch latency",
-        "value": fused_rps,
+        "value": h2h_fused["value"],
         "unit": "reads/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": fused_elapsed * 1e3 / steps,
-        "p50_batch_ms": float(np.percentile(fused_ms, 50)),
-        "p99_batch_ms": float(np.percentile(fused_ms, 99)),
+        "ms_per_step": h2h_fused["ms_per_step"],
+        "p50_batch_ms": h2h_fused["p50_batch_ms"],
+        "p99_batch_ms": h2h_fused["p99_batch_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -409,19 +522,25 @@ def run_c3(args, G):
         "config": {
             "workload": "batched uneven reads: 100 Mbp synthetic genome, 2M read pairs of "
                         "50/100/150/250/300 bp (0.1/0.2/0.4/0.2/0.1), insert N(400, 50), "
-                        "131,072-read batches, persistent fused kernel (autotuned)",
+                        "131,072-read batches, persistent fused kernel (autotuned); reads copied from "
+                        "pinned host memory, NAM arrays copied back",
             "genome_bp": args.genome, "pairs": pairs, "reads": int(len(lens)),
-            "batch_reads": batch, "batches": len(mappers), "batches_in_flight": lanes, "fused_config": best,
+            "batch_reads": batch, "lanes": h2h_fused["lanes"], "fused_config": best,
             "autotune_candidates": n_tried, "spilled_fraction": spill_frac,
             "index_randstrobes": int(info.n_randstrobes), "parallelism": "single GPU",
         },
         "roofline": roof,
         "cpu_baseline": cpu,
+        "parity": parity,
+        "host_to_host": h2h_fused,
+        "plain_host_to_host": h2h_plain,
+        "device_resident_value": fused_many[0],
+        "fused_resident": form_json(fused_many, steps, batches_in_flight=lanes,
+                                    one_batch_at_a_time=form_json(fused_one, steps)),
         "fused_stage_ms": stage_means(fstage),
-        "fused_one_batch_at_a_time": form_json(fused_one, steps),
         "pair_step": pairs_out,
-        "plain_form": form_json(plain_many, steps, stage_ms=stage_means(stages),
-                                one_batch_at_a_time=form_json(plain_one, steps)),
+        "plain_resident": form_json(plain_many, steps, stage_ms=stage_means(stages),
+                                    one_batch_at_a_time=form_json(plain_one, steps)),
     }
 
 
@@ -438,10 +557,11 @@ def run_c4(args, G):
     form (autotuned) against the per-stage launches, NAMs bit-exact."""
     from oracle import oracle_lib as O
     genome_bp = args.genome or 3_100_000_000
+    threads = args.cpu_threads or host_cores()["threads"]
     t0 = time.time()
     genome, clens = c4_genome(G, genome_bp)
     log(f"genome {genome_bp} bp, {len(clens)} contigs in {time.time() - t0:.1f}s")
-    threads = min(16, os.cpu_count() or 1)
+    t0 = time.time()
     idx, info = build_index(G, genome, clens, threads)
     build_s = time.time() - t0
     pairs = args.pairs or 10_000_000
@@ -459,48 +579,48 @@ def run_c4(args, G):
     best, best_ms, n_tried = mappers[0].autotune()
     log(f"autotune: {n_tried} configurations, best {best} at {best_ms:.3f} ms per batch")
     fused_one, fused_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_FUSED)
-    log(f"fused (autotuned): {fused_one[0] / 1e6:.1f} M reads/s one batch at a time, "
+    log(f"fused (autotuned, resident): {fused_one[0] / 1e6:.1f} M reads/s one batch at a time, "
         f"{fused_many[0] / 1e6:.1f} M with {lanes} in flight")
-    fused_rps, fused_elapsed, fused_ms = fused_many
-    mappers[0].run(mode=G.MODE_FUSED)
-    fres = mappers[0].download()
     plain_one, plain_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_PLAIN)
-    log(f"plain per-stage: {plain_one[0] / 1e6:.1f} M reads/s one batch at a time, "
+    log(f"plain per-stage (resident): {plain_one[0] / 1e6:.1f} M reads/s one batch at a time, "
         f"{plain_many[0] / 1e6:.1f} M with {lanes} in flight")
-    mappers[0].run(mode=G.MODE_PLAIN)
-    pres = mappers[0].download()
-    same = bool(np.array_equal(fres.nam_offsets, pres.nam_offsets) and fres.nams.tobytes() == pres.nams.tobytes())
     stages = kernel_times(G, mappers, steps, G.MODE_PLAIN)
-    roof = lookup_roofline(stages)
+    roof = lookup_roofline(stages, "c4")
+    log("stage ms (mean): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_means(stages).items()))
+    for m in mappers:
+        m.close()
 
-    # the oracle over the same (GPU-built) table: parity on a sample, then
-    # the CPU baseline
+    # the headline: host memory to host memory
+    pin = (G.pinned_copy(seqs), G.pinned_copy(offs), G.pinned_copy(lens))
+    h2h_fused, res_f, n_pass = host_to_host(G, idx, pin, batch, steps, args.warmup, G.MODE_FUSED,
+                                            args.stream_lanes)
+    h2h_plain, res_p, _ = host_to_host(G, idx, pin, batch, steps, args.warmup, G.MODE_PLAIN, args.stream_lanes)
+    same = bool(np.array_equal(res_f.nam_offsets, res_p.nam_offsets) and res_f.nams.tobytes() == res_p.nams.tobytes())
+    del pin
+
+    # the oracle over the same (GPU-built) table: every read of the timed
+    # pass, then the CPU baseline
     t0 = time.time()
     oidx = O.Index.wrap(idx.randstrobes, idx.randstrobe_start_indices, info.bits, info.filter_cutoff,
                         O.params_from_read_length(150))
     log(f"table copied to the host for the oracle in {time.time() - t0:.1f}s")
-    n_par = min(20000, int(mappers[0].n))
-    end = int(offs[n_par - 1] + lens[n_par - 1])
-    eo, en, ef, efl = O.map_batch(oidx, seqs[:end], offs[:n_par], lens[:n_par], 2 * info.filter_cutoff, 1000,
-                                  threads)
-    po = pres.nam_offsets[: n_par + 1]
-    exact = bool(np.array_equal(po, eo) and pres.nams[: int(po[-1])].tobytes() == en.tobytes())
-    log(f"parity sample: {n_par} reads bit-exact vs the oracle: {exact}; fused == plain: {same}")
+    parity = parity_leg(O, oidx, seqs, offs, lens, res_f, n_pass, threads)
+    parity["fused_equals_plain"] = same
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C4")
+        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, threads, "C4", args.oracle_build)
     oidx.free()
     return {
         "metric": "reads/sec to NAMs at 150bp, 3.1 Gbp index (1 GPU, bit-exact) vs CPU; "
                   "p50/p99 batch latency",
-        "value": fused_rps,
+        "value": h2h_fused["value"],
         "unit": "reads/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": fused_elapsed * 1e3 / steps,
-        "p50_batch_ms": float(np.percentile(fused_ms, 50)),
-        "p99_batch_ms": float(np.percentile(fused_ms, 99)),
+        "ms_per_step": h2h_fused["ms_per_step"],
+        "p50_batch_ms": h2h_fused["p50_batch_ms"],
+        "p99_batch_ms": h2h_fused["p99_batch_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -509,19 +629,24 @@ def run_c4(args, G):
         "config": {
             "workload": "human-scale index: 3.1 Gbp synthetic genome (24 contigs of 50-250 Mbp, "
                         "45% repeats, 2% tandem repeats), 10M pairs of 150 bp, 262,144-read batches, "
-                        "batched fused form (autotuned) vs per-stage launches",
+                        "batched fused form (autotuned) vs per-stage launches; reads copied from pinned "
+                        "host memory, NAM arrays copied back",
             "genome_bp": genome_bp, "pairs": pairs, "reads": int(len(lens)), "batch_reads": batch,
-            "batches": len(mappers), "batches_in_flight": lanes, "fused_config": best,
+            "lanes": h2h_fused["lanes"], "fused_config": best,
             "autotune_candidates": n_tried, "index_randstrobes": int(info.n_randstrobes), "bits": int(info.bits),
             "filter_cutoff": int(info.filter_cutoff), "index_gb": info.device_bytes / 1e9,
             "index_build_s": build_s, "parallelism": "single GPU",
         },
-        "parity_sample": {"reads": n_par, "bit_exact_vs_oracle": exact, "fused_equals_plain": same},
+        "parity": parity,
         "roofline": roof,
         "cpu_baseline": cpu,
-        "fused_one_batch_at_a_time": form_json(fused_one, steps),
-        "plain_form": form_json(plain_many, steps, stage_ms=stage_means(stages),
-                                one_batch_at_a_time=form_json(plain_one, steps)),
+        "host_to_host": h2h_fused,
+        "plain_host_to_host": h2h_plain,
+        "device_resident_value": fused_many[0],
+        "fused_resident": form_json(fused_many, steps, batches_in_flight=lanes,
+                                    one_batch_at_a_time=form_json(fused_one, steps)),
+        "plain_resident": form_json(plain_many, steps, stage_ms=stage_means(stages),
+                                    one_batch_at_a_time=form_json(plain_one, steps)),
     }
 
 
@@ -598,7 +723,7 @@ def run_c5(args, G):
     genome_bp = args.genome or 3_100_000_000
     t0 = time.time()
     genome, clens = c4_genome(G, genome_bp)
-    threads = min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or host_cores()["threads"]
     idx, info = build_index(G, genome, clens, threads)
     log(f"genome + index in {time.time() - t0:.1f}s")
     mix = dict(lengths=(50, 100, 150, 250, 300), weights=(0.1, 0.2, 0.4, 0.2, 0.1),
@@ -669,7 +794,7 @@ def run_c5(args, G):
         from oracle import oracle_lib as O
         oidx = O.Index.wrap(idx.randstrobes, idx.randstrobe_start_indices, info.bits, info.filter_cutoff,
                             O.params_from_read_length(150))
-        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, args.cpu_threads or threads, "C5")
+        cpu = cpu_baseline(oidx, seqs, offs, lens, args.cpu_reads, threads, "C5", args.oracle_build)
         oidx.free()
     return {
         "metric": "served reads/sec to NAMs, 3.1 Gbp index, Poisson request arrivals (1 GPU, "
@@ -719,32 +844,24 @@ def main():
     ap.add_argument("--pairs", type=int, default=0, help="c3/c4: read pairs (2M / 10M)")
     ap.add_argument("--batch", type=int, default=0, help="reads per batch (c2 65536, c3 131072, c4 262144)")
     ap.add_argument("--lanes", type=int, default=2,
-                    help="c2-c4: batches in flight, each lane on its own stream and workspace (1: one at a time)")
+                    help="c2-c4 resident forms: batches in flight, each lane on its own stream and workspace")
+    ap.add_argument("--stream-lanes", type=lambda x: tuple(int(v) for v in x.split(",")), default=(2, 3, 4),
+                    help="c2-c4 headline: lane counts tried for the host-to-host pipeline (the fastest is timed)")
     ap.add_argument("--genome", type=int, default=0, help="genome bp (c2/c3 100M, c4 3.1G)")
     ap.add_argument("--cpu-reads", type=int, default=1_000_000,
                     help="bounded sample of the workload timed on the CPU oracle")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core this process may run on")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
-
-    if args.copy_probe:
-        import torch
-        n = args.copy_probe * (1 << 20) // 4
-        x = torch.ones(n, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        torch.cuda.synchronize()
-        y.copy_(x)
-        torch.cuda.synchronize()
-        log(f"copied {args.copy_probe} MiB device to device")
-        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the read-sharded "
               "multi-GPU form needs no exchange and is not launched here", file=sys.stderr)
         sys.exit(2)
 
+    # the oracle (parity legs and the CPU baseline) built for this host's CPU
+    from oracle import oracle_lib as O
+    args.oracle_build = O.use_native()
     import gpu_index_test_amd as G
     if args.config == "c4":
         out = run_c4(args, G)

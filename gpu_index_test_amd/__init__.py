@@ -25,6 +25,7 @@ __all__ = [
     "pack_reads", "GisError", "NAM", "QUERY_RANDSTROBE", "REF_RANDSTROBE", "SYNCMER", "HIT",
     "GIS_NOT_FOUND", "FLAG_RESCUED", "FLAG_NO_RANDSTROBES", "MODE_PLAIN", "MODE_FUSED",
     "PairResult", "pairs_from_nams", "NAM_PAIR", "FastqReader", "FastqChunk",
+    "find_nams_stream", "pinned_empty", "pinned_copy",
 ]
 
 
@@ -434,6 +435,77 @@ def find_nams_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000
         ctx.h, N.ptr(seqs), N.ptr(offsets), N.ptr(lens), n, C.byref(opt), C.byref(o)))
 
 
+class _PinnedBlock:
+    """A block of pinned host memory (gis_host_alloc), freed with its last
+    array."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        _check(N.lib().gis_host_alloc(nbytes, C.byref(p)))
+        self.p, self.nbytes = p.value, nbytes
+
+    def __del__(self):
+        try:
+            N.lib().gis_host_free(self.p)
+        except Exception:
+            pass
+
+
+def pinned_empty(n, dtype):
+    """An uninitialised numpy array of n elements in pinned host memory:
+    copies between it and HBM run asynchronously (gis_find_nams_stream)."""
+    dtype = np.dtype(dtype)
+    nbytes = max(1, int(n) * dtype.itemsize)
+    blk = _PinnedBlock(nbytes)
+    raw = (C.c_uint8 * nbytes).from_address(blk.p)
+    raw._owner = blk
+    return np.frombuffer(raw, dtype=np.uint8, count=int(n) * dtype.itemsize).view(dtype)
+
+
+def pinned_copy(a):
+    """A pinned copy of a contiguous numpy array."""
+    a = np.ascontiguousarray(a)
+    out = pinned_empty(a.size, a.dtype).reshape(a.shape)
+    out[...] = a
+    return out
+
+
+def find_nams_stream(index, seqs, offsets, lens, batch_reads=65536, lanes=2, rescue_level=2, hard_cutoff=1000,
+                     mode=N.MODE_PLAIN, graph=True, out=None):
+    """Host-to-host mapping of many reads (gis_find_nams_stream): batches of
+    batch_reads, up to `lanes` in flight, each batch's reads copied to HBM,
+    mapped and its NAM arrays copied back. out: the MapResult of an earlier
+    call over as many reads, whose pinned arrays are reused; else fresh pinned
+    arrays are allocated. -> (MapResult, per-batch device latency in ms)."""
+    index._check_owner()
+    ctx = index.ctx
+    seqs, offsets, lens = _seq_args(seqs, offsets, lens)
+    n = len(lens)
+    opt = _map_options(rescue_level, hard_cutoff, mode, graph)
+    nb = max(1, -(-n // batch_reads))
+    ms = np.zeros(nb, dtype=np.float64)
+    full = out._full if out is not None and len(out._full[0]) == n + 1 else None
+    cap = len(full[1]) if full is not None else max(16, 12 * n)
+    while True:
+        if full is not None:
+            offs, nams, frac, flags = full
+        else:
+            offs, nams = pinned_empty(n + 1, np.uint64), pinned_empty(cap, NAM)
+            frac, flags = pinned_empty(max(n, 1), np.float32), pinned_empty(max(n, 1), np.uint8)
+        o = N.NamOut()
+        o.nam_offsets, o.nams, o.nam_capacity = offs.ctypes.data, nams.ctypes.data, cap
+        o.nonrepetitive_fraction, o.flags = frac.ctypes.data, flags.ctypes.data
+        code = ctx.L.gis_find_nams_stream(ctx.h, N.ptr(seqs), N.ptr(offsets), N.ptr(lens), n, batch_reads,
+                                          lanes, C.byref(opt), C.byref(o), N.ptr(ms))
+        if code == 5 and o.n_nams > cap:
+            cap, full = int(o.n_nams), None
+            continue
+        ctx.check(code)
+        res = MapResult(offs, nams[: o.n_nams], frac[:n], flags[:n])
+        res._full = (offs, nams, frac, flags)
+        return res, ms
+
+
 def hits_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000):
     index._check_owner()
     ctx = index.ctx
@@ -622,6 +694,15 @@ class Mapper:
         self.ctx.check(self.ctx.L.gis_batch_autotune(self.h, C.byref(opt), C.byref(best), C.byref(ms),
                                                      C.byref(n)))
         return best.as_dict(), ms.value, n.value
+
+    def lookup_bytes(self):
+        """Of the last MODE_PLAIN run: (algorithmic table bytes of the lookup
+        stage in the reference search's byte model, lookups, bytes the lookup
+        kernel streamed beside the table)."""
+        b, q, s = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.ctx.L.gis_batch_lookup_bytes(self.h, C.byref(b), C.byref(q)))
+        self.ctx.check(self.ctx.L.gis_batch_lookup_streams(self.h, C.byref(s)))
+        return b.value, q.value, s.value
 
     def stage_times(self):
         ms = (C.c_double * 16)()

@@ -164,6 +164,11 @@ PROTOTYPES = {
     "gis_batch_download": (C.c_int, [_P, C.POINTER(NamOut)]),
     "gis_batch_stage_times": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_int), C.c_char_p, C.c_int]),
     "gis_batch_lookup_bytes": (C.c_int, [_P, C.POINTER(_U64), C.POINTER(_U64)]),
+    "gis_batch_lookup_streams": (C.c_int, [_P, C.POINTER(_U64)]),
+    "gis_find_nams_stream": (C.c_int, [_P, _P, _P, _P, _U64, _U32, _U32, C.POINTER(MapOptions),
+                                       C.POINTER(NamOut), _P]),
+    "gis_host_alloc": (C.c_int, [_U64, C.POINTER(_P)]),
+    "gis_host_free": (None, [_P]),
     "gis_batch_nam_count": (C.c_int, [_P, C.POINTER(_U64)]),
     "gis_fused_config_get": (C.c_int, [_P, C.POINTER(FusedConfig)]),
     "gis_fused_config_set": (C.c_int, [_P, C.POINTER(FusedConfig)]),

@@ -47,7 +47,7 @@ gis_batch *batch_new_stream(gis_ctx *ctx) {
 
 static void workspace_release(Workspace *w) {
     DevBuf *all[] = {&w->long_scratch, &w->rs_cnt, &w->rs_nfwd, &w->rs_hash, &w->rs_start, &w->rs_end,
-                     &w->sy_cnt, &w->sy_hash, &w->sy_pos, &w->pos, &w->hcnt, &w->rcnt, &w->lk_bytes,
+                     &w->sy_cnt, &w->sy_hash, &w->sy_pos, &w->pos, &w->hcnt, &w->rcnt, &w->lk_bytes, &w->lk_stream,
                      &w->hit_total, &w->rhit_total, &w->frac, &w->flags, &w->nam_cnt, &w->hit_off,
                      &w->rhit_off, &w->hits, &w->rhits, &w->keys, &w->rkeys, &w->nam_ws, &w->rnam_ws,
                      &w->nam_fin, &w->rnam_fin, &w->nam_slots, &w->nam_off, &w->nams_out, &w->scan_tmp,
@@ -111,6 +111,7 @@ DevBatch make_devbatch(gis_batch *b) {
     d.hcnt = static_cast<uint32_t *>(b->ws->hcnt.p);
     d.rcnt = static_cast<uint32_t *>(b->ws->rcnt.p);
     d.lk_bytes = static_cast<uint32_t *>(b->ws->lk_bytes.p);
+    d.lk_stream = static_cast<uint32_t *>(b->ws->lk_stream.p);
     d.hit_total = static_cast<uint32_t *>(b->ws->hit_total.p);
     d.rhit_total = static_cast<uint32_t *>(b->ws->rhit_total.p);
     d.frac = static_cast<float *>(b->ws->frac.p);
@@ -159,6 +160,7 @@ void ensure_read_arrays(gis_batch *b, uint32_t n) {
     ensure<uint32_t>(b->ws->rs_nfwd, n);
     ensure<uint32_t>(b->ws->sy_cnt, n);
     ensure<uint32_t>(b->ws->lk_bytes, n);
+    ensure<uint32_t>(b->ws->lk_stream, n);
     ensure<uint32_t>(b->ws->hit_total, n);
     ensure<uint32_t>(b->ws->rhit_total, n);
     ensure<uint32_t>(b->ws->nam_cnt, n);
@@ -215,42 +217,51 @@ void mark(gis_batch *b, const char *name) {
 
 // Stage reads into the batch: randstrobe-slot capacities and offsets come
 // from the lengths, computed here once.
-int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
-                 uint32_t n, bool want_sy) {
+int batch_stage(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
+                uint32_t n, bool want_sy, uint64_t base, hipEvent_t before) {
     gis_ctx *ctx = b->ctx;
     if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
     if (n > 0 && (!seqs || !offsets || !lens)) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null read arrays");
     const gis_params &p = ctx->ix.params;
     uint64_t span = 0;
-    b->h_rs_off.assign(n + 1, 0);
-    b->h_sy_off.assign(n + 1, 0);
-    b->h_long_off.assign(n + 1, 0);
+    b->h_rs_off.resize(n + 1);
+    b->h_sy_off.resize(n + 1);
+    b->h_long_off.resize(n + 1);
+    b->h_read_off.resize(n);
+    b->h_rs_off[0] = b->h_sy_off[0] = b->h_long_off[0] = 0;
     b->any_long = false;
+    uint64_t rs_at = 0, sy_at = 0, long_at = 0;
     for (uint32_t r = 0; r < n; r++) {
         const uint64_t L = lens[r];
         if (L > GIS_MAX_READ_LEN)
             return fail(ctx, GIS_ERR_BAD_PARAMETER,
                         "read " + std::to_string(r) + " longer than " + std::to_string(GIS_MAX_READ_LEN));
-        span = std::max(span, offsets[r] + L);
+        if (offsets[r] < base) return fail(ctx, GIS_ERR_BAD_PARAMETER, "read offset below the staged base");
+        const uint64_t o = offsets[r] - base;
+        b->h_read_off[r] = o;
+        span = std::max(span, o + L);
         const int64_t nk = (int64_t)L - p.k + 1;
         const int64_t nf = nk - (int64_t)p.w_min;
         const uint64_t cap_sy = nk > 0 ? (uint64_t)nk : 0;
         const uint64_t cap_rs = (nf > 0 && L >= p.w_max) ? 2 * (uint64_t)nf : 0;
-        b->h_rs_off[r + 1] = b->h_rs_off[r] + cap_rs;
-        b->h_sy_off[r + 1] = b->h_sy_off[r] + cap_sy;
+        rs_at += cap_rs;
+        sy_at += cap_sy;
         const bool lg = L > (uint64_t)LMAX_LDS;
         b->any_long |= lg;
-        b->h_long_off[r + 1] = b->h_long_off[r] + (lg ? long_bytes(L) : 0);
+        long_at += lg ? long_bytes(L) : 0;
+        b->h_rs_off[r + 1] = rs_at;
+        b->h_sy_off[r + 1] = sy_at;
+        b->h_long_off[r + 1] = long_at;
     }
     b->n_reads = n;
     b->want_sy = want_sy;
     // the fused kernel's work queue: read ids by length, longest first
     // (counting sort over lengths capped at 1023; stable within a length)
     {
-        std::vector<uint32_t> cnt(1025, 0);
+        uint32_t cnt[1025] = {};
         for (uint32_t r = 0; r < n; r++) cnt[1023 - std::min<uint32_t>(lens[r], 1023u) + 1]++;
         for (int i = 1; i < 1025; i++) cnt[i] += cnt[i - 1];
-        b->h_queue.assign(n, 0);
+        b->h_queue.resize(n);
         for (uint32_t r = 0; r < n; r++) b->h_queue[cnt[1023 - std::min<uint32_t>(lens[r], 1023u)]++] = r;
         b->n_long = 0;
         for (uint32_t r = 0; r < n; r++) b->n_long += lens[r] > (uint32_t)SHORT_BIN ? 1u : 0u;
@@ -260,17 +271,25 @@ int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const 
     b->span = span;
     ensure<char>(b->seqs, span);
     hipStream_t st = b->stream;
-    if (span) HIP_CHECK(hipMemcpyAsync(b->seqs.p, seqs, span, hipMemcpyHostToDevice, st));
+    if (before) HIP_CHECK(hipEventRecord(before, st));
+    if (span) HIP_CHECK(hipMemcpyAsync(b->seqs.p, seqs + base, span, hipMemcpyHostToDevice, st));
     if (n) {
-        HIP_CHECK(hipMemcpyAsync(b->read_off.p, offsets, 8ull * n, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(b->read_off.p, b->h_read_off.data(), 8ull * n, hipMemcpyHostToDevice, st));
         HIP_CHECK(hipMemcpyAsync(b->read_len.p, lens, 4ull * n, hipMemcpyHostToDevice, st));
     }
     HIP_CHECK(hipMemcpyAsync(b->rs_off.p, b->h_rs_off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipMemcpyAsync(b->sy_off.p, b->h_sy_off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
     HIP_CHECK(hipMemcpyAsync(b->long_off.p, b->h_long_off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
     if (n) HIP_CHECK(hipMemcpyAsync(b->queue.p, b->h_queue.data(), 4ull * n, hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipStreamSynchronize(st));
     b->ran = false;
+    return GIS_OK;
+}
+
+int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
+                 uint32_t n, bool want_sy) {
+    const int rc = batch_stage(b, seqs, offsets, lens, n, want_sy, 0, nullptr);
+    if (rc) return rc;
+    HIP_CHECK(hipStreamSynchronize(b->stream));
     return GIS_OK;
 }
 
@@ -383,6 +402,22 @@ void batch_launch(gis_batch *b, const RunOpts &r) {
     b->ran = false;
     if (r.mode == GIS_MODE_FUSED && r.stop == STOP_NONE)
         ensure<gis_nam>(b->ws->nam_slots, (size_t)b->n_reads * F_NAMCAP);
+    // graphs captured against a table that has since been dropped hold its
+    // freed addresses and old parameters: never replay them
+    bool stale = false;
+    for (const auto &g : b->graphs) stale |= g.epoch != ctx->ix.epoch;
+    if (stale) {
+        HIP_CHECK(hipStreamSynchronize(st));
+        for (auto it = b->graphs.begin(); it != b->graphs.end();) {
+            if (it->epoch == ctx->ix.epoch) {
+                ++it;
+                continue;
+            }
+            HIP_CHECK(hipGraphExecDestroy(it->exec));
+            HIP_CHECK(hipGraphDestroy(it->graph));
+            it = b->graphs.erase(it);
+        }
+    }
     if (r.graph) {
         const DevBatch key = make_devbatch(b);
         const gis_fused_config &fc = ctx->fcfg;
@@ -404,6 +439,7 @@ void batch_launch(gis_batch *b, const RunOpts &r) {
             }
             gis_batch::GraphEntry e{};
             e.key = key;
+            e.epoch = ctx->ix.epoch;
             std::memcpy(e.cfg, cfg, sizeof cfg);
             HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
@@ -718,7 +754,7 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
     b->n_reads = n;
     b->want_sy = false;
     b->any_long = false;
-    b->h_rs_off = off;
+    b->h_rs_off.assign(off.data(), off.size());
     // the per-read randstrobe offsets are the one input array the stages
     // after seeding read (a fresh context's scratch batch has none yet)
     ensure<uint64_t>(b->rs_off, (size_t)n + 1);
@@ -932,6 +968,23 @@ extern "C" int gis_batch_lookup_bytes(gis_batch *b, uint64_t *bytes, uint64_t *n
     }
     if (bytes) *bytes = tb;
     if (n_lookups) *n_lookups = tq;
+    return GIS_OK;
+    API_END(ctx)
+}
+
+extern "C" int gis_batch_lookup_streams(gis_batch *b, uint64_t *stream_bytes) {
+    if (!b || !stream_bytes) return GIS_ERR_BAD_PARAMETER;
+    gis_ctx *ctx = b->ctx;
+    API_BEGIN(ctx)
+    if (b->ws->owner != b) return fail(ctx, GIS_ERR_STATE, "another batch ran on the shared workspace since");
+    if (!b->ran) return fail(ctx, GIS_ERR_STATE, "batch has not been run");
+    if (b->last_mode != GIS_MODE_PLAIN) return fail(ctx, GIS_ERR_STATE, "lookup bytes are counted by the per-stage path");
+    const uint32_t n = b->n_reads;
+    std::vector<uint32_t> sb(n);
+    if (n) HIP_CHECK(hipMemcpy(sb.data(), b->ws->lk_stream.p, 4ull * n, hipMemcpyDeviceToHost));
+    uint64_t t = 0;
+    for (uint32_t r = 0; r < n; r++) t += sb[r];
+    *stream_bytes = t;
     return GIS_OK;
     API_END(ctx)
 }

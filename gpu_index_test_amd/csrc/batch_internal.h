@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -11,13 +13,67 @@
 
 namespace gis {
 
+// Grow-only pinned host array: copies between it and HBM are DMA transfers
+// that run asynchronously to the host.
+template <typename T>
+struct Pinned {
+    T *p = nullptr;
+    size_t cap = 0, n = 0;
+    Pinned() = default;
+    Pinned(const Pinned &) = delete;
+    Pinned &operator=(const Pinned &) = delete;
+    Pinned(Pinned &&o) noexcept : p(o.p), cap(o.cap), n(o.n) { o.p = nullptr, o.cap = o.n = 0; }
+    Pinned &operator=(Pinned &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p, cap = o.cap, n = o.n;
+            o.p = nullptr, o.cap = o.n = 0;
+        }
+        return *this;
+    }
+    ~Pinned() { release(); }
+    void reserve(size_t want) {
+        if (want <= cap) return;
+        const size_t c = std::max(want, cap + cap / 2);
+        T *q = nullptr;
+        if (hipHostMalloc((void **)&q, std::max<size_t>(c, 1) * sizeof(T)) != hipSuccess)
+            throw HipError{hipErrorOutOfMemory, "hipHostMalloc of " + std::to_string(c * sizeof(T)) + " bytes"};
+        if (p && n) std::memcpy(q, p, n * sizeof(T));
+        if (p) (void)hipHostFree(p);
+        p = q;
+        cap = c;
+    }
+    void resize(size_t m) {
+        reserve(m);
+        n = m;
+    }
+    void assign(size_t m, T v) {
+        resize(m);
+        std::fill(p, p + m, v);
+    }
+    void assign(const T *a, size_t m) {
+        resize(m);
+        if (m) std::memcpy(p, a, m * sizeof(T));
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = n = 0;
+    }
+    T *data() { return p; }
+    const T *data() const { return p; }
+    size_t size() const { return n; }
+    T &operator[](size_t i) { return p[i]; }
+    const T &operator[](size_t i) const { return p[i]; }
+};
+
 // Device intermediates and results of a run. A workspace may be shared by
 // several batches that run one at a time (each run's results must be read
 // before another batch runs on it); each serving slot has its own.
 struct Workspace {
     DevBuf long_scratch;
     DevBuf rs_cnt, rs_nfwd, rs_hash, rs_start, rs_end, sy_cnt, sy_hash, sy_pos;
-    DevBuf pos, hcnt, rcnt, lk_bytes;
+    DevBuf pos, hcnt, rcnt, lk_bytes, lk_stream;
     DevBuf hit_total, rhit_total, frac, flags, nam_cnt, hit_off, rhit_off;
     DevBuf hits, rhits, keys, rkeys, nam_ws, rnam_ws, nam_fin, rnam_fin;
     DevBuf nam_slots, nam_off, nams_out, scan_tmp, ctl, spill_ids, gwork, split_list, sel_ids;
@@ -49,9 +105,10 @@ struct gis_batch {
     uint64_t span = 0;  // bases
     bool want_sy = false;
     bool any_long = false;
-    std::vector<uint64_t> h_rs_off, h_sy_off, h_long_off;
+    // pinned, so that their copies to HBM run while the host works on
+    gis::Pinned<uint64_t> h_rs_off, h_sy_off, h_long_off, h_read_off;
     DevBuf seqs, read_off, read_len, rs_off, sy_off, long_off, queue;
-    std::vector<uint32_t> h_queue;  // read ids by length, longest first
+    gis::Pinned<uint32_t> h_queue;  // read ids by length, longest first
     uint32_t n_long = 0;            // queue entries longer than the short bin (160 bp)
     uint64_t tot_rs_cap = 0;
     gis::Workspace *ws = nullptr;
@@ -65,6 +122,7 @@ struct gis_batch {
     struct GraphEntry {
         DevBatch key;
         int cfg[11];
+        uint64_t epoch;  // the index table the graph was captured against
         hipGraph_t graph;
         hipGraphExec_t exec;
         uint64_t used;
@@ -93,6 +151,15 @@ struct RunOpts {
 
 gis_batch *batch_new(gis_ctx *ctx, Workspace *share);  // share = nullptr: a workspace of its own
 gis_batch *batch_new_stream(gis_ctx *ctx);  // a batch with a stream and a workspace of its own
+// Stage reads into the batch without waiting: the host derives the
+// per-read arrays into the batch's pinned arrays, then the copies to HBM are
+// enqueued on the batch's stream (after `before`, when given, is recorded
+// there). Device offsets are offsets[r] - base and the bases copied are
+// seqs[base .. max(offsets[r] + lens[r])); seqs and lens are read by the copies
+// as the stream reaches them, so they stay valid until then.
+int batch_stage(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
+                uint32_t n, bool want_sy, uint64_t base, hipEvent_t before);
+// batch_stage with base 0, then wait for the copies
 int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
                  uint32_t n, bool want_sy);
 RunOpts run_opts(const gis_batch *b, const gis_map_options &o, int rescue_mode, int stop);

@@ -30,11 +30,8 @@ void build_lines(gis_ctx *ctx) {
 
 void upload_index(gis_ctx *ctx, const HostIndex &h) {
     DevIndexStore &ix = ctx->ix;
-    ix.loaded = false;
     // drop the previous table first so two tables never coexist in HBM
-    release(ix.rs);
-    release(ix.starts);
-    release(ix.lines);
+    drop_table(ix);
     ensure<gis_ref_randstrobe>(ix.rs, h.rs.size());
     ensure<uint64_t>(ix.starts, h.starts.size());
     if (!h.rs.empty())
@@ -105,6 +102,8 @@ extern "C" void gis_ctx_destroy(gis_ctx *ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         batch_free(ctx->scratch);
         ctx->scratch = nullptr;
+        for (gis_batch *b : ctx->lanes) batch_free(b);
+        ctx->lanes.clear();
         pairwork_free(ctx->pairs);
         ctx->pairs = nullptr;
         release(ctx->ix.rs);
@@ -139,10 +138,7 @@ extern "C" int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint
     const int rc = seed_reference(seqs, lens, n_refs, *params, n_threads, seeding_segment(), recs, err);
     if (rc) return fail(ctx, rc, err);
     DevIndexStore &ix = ctx->ix;
-    ix.loaded = false;
-    release(ix.rs);
-    release(ix.starts);
-    release(ix.lines);
+    drop_table(ix);
     const uint64_t n = recs.size();
     ensure<gis_ref_randstrobe>(ix.rs, n);
     ensure<uint64_t>(ix.starts, (1ull << bits) + 1);
@@ -229,10 +225,7 @@ extern "C" int gis_index_read_sti(gis_ctx *ctx, const char *path, const gis_para
     if (rc) return fail(ctx, rc, err);
     try {
         DevIndexStore &ix = ctx->ix;
-        ix.loaded = false;
-        release(ix.rs);
-        release(ix.starts);
-        release(ix.lines);
+        drop_table(ix);
         ensure<gis_ref_randstrobe>(ix.rs, m.n);
         ensure<uint64_t>(ix.starts, (1ull << m.bits) + 1);
         const uint64_t chunk = 256ull << 20;

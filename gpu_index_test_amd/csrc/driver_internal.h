@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/gis.h"
 #include "kernels.h"
@@ -61,7 +62,19 @@ struct DevIndexStore {
     gis_params params{};
     uint64_t distinct = 0, once = 0, high = 0, mid = 0, index_cutoff = 0;
     bool loaded = false;
+    // bumped whenever the table is dropped: launch sequences captured with
+    // the previous table's addresses and parameters are stale from then on
+    uint64_t epoch = 0;
 };
+
+// free the resident table (before another is loaded)
+inline void drop_table(DevIndexStore &ix) {
+    ix.loaded = false;
+    release(ix.rs);
+    release(ix.starts);
+    release(ix.lines);
+    ix.epoch++;
+}
 
 inline int fail(gis_ctx *ctx, int code, const std::string &msg);
 
@@ -81,6 +94,9 @@ struct gis_ctx {
     int n_cu = 256;                // compute units of the device (persistent grids)
     gis_fused_config fcfg{4, 0, 0, 0, 0};
     gis::PairWork *pairs = nullptr;  // the pair step on NAM lists from host memory
+    // lanes of gis_find_nams_stream: batches with a stream and a workspace
+    // each, kept so that later calls reuse their buffers and graphs
+    std::vector<gis_batch *> lanes;
 };
 
 namespace gis {

@@ -94,7 +94,8 @@ struct DevBatch {
     uint64_t *pos;            // GIS_NOT_FOUND if absent
     uint32_t *hcnt;           // hits this seed contributes (rescued reads: run length)
     uint32_t *rcnt;           // rescued reads: hits this seed contributes
-    uint32_t *lk_bytes;       // [n] algorithmic bytes the lookup kernel moved for the read
+    uint32_t *lk_bytes;       // [n] algorithmic table bytes of the read's lookups (byte model of the search)
+    uint32_t *lk_stream;      // [n] bytes the lookup kernel streams beside them: queries, results, hits
     // per read
     uint32_t *hit_total;
     uint32_t *rhit_total;

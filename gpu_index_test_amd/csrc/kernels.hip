@@ -416,10 +416,11 @@ __device__ __forceinline__ uint64_t rec_hash(const DevIndex &ix, uint64_t x) {
 
 // first index of the bucket range holding key, or GIS_NOT_FOUND
 // (StrobemerIndex::find, step2/index.hpp:57-81: linear scan of a small bucket,
-// lower_bound otherwise; both return the first equal entry). *probes counts
-// the table records read.
+// lower_bound otherwise; both return the first equal entry). *scan is the
+// record count of the byte model: the records a linear scan reads up to the
+// first hash >= key (all of them when there is none), whichever search ran.
 __device__ __forceinline__ uint64_t dev_find_in(const DevIndex &ix, uint64_t key, uint64_t s, uint64_t e,
-                                               uint32_t *probes) {
+                                               uint32_t *scan) {
     uint32_t pr = 0;
     uint64_t res = GIS_NOT_FOUND;
     if (s == e) {
@@ -440,18 +441,16 @@ __device__ __forceinline__ uint64_t dev_find_in(const DevIndex &ix, uint64_t key
             if (rec_hash(ix, it) < key) { lo = it + 1; cnt -= step + 1; }
             else cnt = step;
         }
-        if (lo < e) {
-            pr++;
-            if (rec_hash(ix, lo) == key) res = lo;
-        }
+        if (lo < e && rec_hash(ix, lo) == key) res = lo;
+        pr = (uint32_t)(lo - s) + (lo < e ? 1u : 0u);
     }
-    *probes = pr;
+    *scan = pr;
     return res;
 }
 
-__device__ __forceinline__ uint64_t dev_find(const DevIndex &ix, uint64_t key, uint32_t *probes) {
+__device__ __forceinline__ uint64_t dev_find(const DevIndex &ix, uint64_t key, uint32_t *scan) {
     const uint32_t bk = (uint32_t)(key >> (64 - ix.bits));
-    return dev_find_in(ix, key, ix.starts[bk], ix.starts[bk + 1], probes);
+    return dev_find_in(ix, key, ix.starts[bk], ix.starts[bk + 1], scan);
 }
 
 __device__ __forceinline__ uint64_t dev_find(const DevIndex &ix, uint64_t key) {
@@ -492,18 +491,19 @@ __device__ __forceinline__ uint32_t dev_get_count(const DevIndex &ix, uint64_t p
 
 // Walk the equal-hash run at pos (add_to_hits_per_ref, step2/nam.cu:45-62):
 // keep entry e iff |query span - ref span| <= the running minimum. Counts the
-// kept hits, and writes them when out != nullptr.
+// kept hits, and writes them when out != nullptr; *run receives the run's
+// length.
 __device__ __forceinline__ uint32_t walk_run(const DevIndex &ix, uint64_t pos, uint64_t key,
                                              int qs, int qe, uint32_t strand_bit, DevHit *out,
-                                             uint32_t *records = nullptr) {
+                                             uint32_t *run = nullptr) {
     const int qspan = qe - qs;
     int min_diff = 0x7fffffff;
     uint32_t c = 0, rec = 0;
     for (uint64_t x = pos; x < ix.n; x++) {
         const uint4 v = ix.rs[x];
-        rec++;
         const uint64_t h = (uint64_t)v.x | ((uint64_t)v.y << 32);
         if (h != key) break;
+        rec++;
         const int rspan = (int)(v.w & 0xffu) + ix.k;
         int d = qspan - rspan;
         d = d < 0 ? -d : d;
@@ -521,7 +521,7 @@ __device__ __forceinline__ uint32_t walk_run(const DevIndex &ix, uint64_t pos, u
             min_diff = d;
         }
     }
-    if (records) *records = rec;
+    if (run) *run = rec;
     return c;
 }
 
@@ -605,8 +605,8 @@ struct alignas(16) IdxRec {
 struct Probe {
     uint64_t key, s, e, pos;
     IdxRec r[PROBE_SMALL];  // records s, s + 1, ... of a small bucket
-    uint32_t probes;        // records the reference's search reads (byte model)
-    uint32_t records;       // records its run walk reads (byte model)
+    uint32_t scan;          // records a linear scan reads to the first hash >= key (byte model)
+    uint32_t run;           // length of the equal-hash run of a kept seed (byte model)
     uint32_t hc;            // kept hits (0 when absent or filtered)
     uint32_t keep;          // small bucket: bit j = record s + j is a kept hit
     bool found, filtered;
@@ -650,7 +650,7 @@ __device__ __forceinline__ void probe_small_find(Probe &p, uint32_t &lt, uint32_
             eq += h == p.key ? 1u : 0u;
         }
     }
-    p.probes = lt < sz ? lt + 1 : sz;
+    p.scan = lt < sz ? lt + 1 : sz;
     p.found = eq > 0;
     p.pos = p.found ? p.s + lt : GIS_NOT_FOUND;
 }
@@ -658,7 +658,7 @@ __device__ __forceinline__ void probe_small_find(Probe &p, uint32_t &lt, uint32_
 __device__ __forceinline__ void probe_finish(const DevIndex &ix, Probe &p, int qs, int qe) {
     p.hc = 0;
     p.keep = 0;
-    p.records = 0;
+    p.run = 0;
     p.filtered = false;
     if (probe_small(p)) {
         uint32_t lt, eq;
@@ -667,7 +667,7 @@ __device__ __forceinline__ void probe_finish(const DevIndex &ix, Probe &p, int q
         p.filtered = eq > ix.filter_cutoff ||
                      (p.key == UINT64_MAX && p.pos + ix.filter_cutoff >= ix.n);
         if (p.filtered) return;
-        p.records = eq + (p.pos + eq < ix.n ? 1u : 0u);
+        p.run = eq;
         const int qspan = qe - qs;
         int min_diff = 0x7fffffff;
 #pragma unroll
@@ -684,12 +684,12 @@ __device__ __forceinline__ void probe_finish(const DevIndex &ix, Probe &p, int q
         }
         return;
     }
-    p.pos = p.e - p.s < 0xffffffffull ? dev_find_in(ix, p.key, p.s, p.e, &p.probes) : dev_find(ix, p.key, &p.probes);
+    p.pos = p.e - p.s < 0xffffffffull ? dev_find_in(ix, p.key, p.s, p.e, &p.scan) : dev_find(ix, p.key, &p.scan);
     p.found = p.pos != GIS_NOT_FOUND;
     if (!p.found) return;
     p.filtered = dev_filtered(ix, p.pos, p.key);
     if (p.filtered) return;
-    p.hc = walk_run(ix, p.pos, p.key, qs, qe, 0u, nullptr, &p.records);
+    p.hc = walk_run(ix, p.pos, p.key, qs, qe, 0u, nullptr, &p.run);
 }
 
 // the kept hits of a probe, in run order
@@ -739,7 +739,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(6))) v
         const uint32_t n = sload(b.rs_cnt + r);
         const uint64_t base = sload(b.rs_off + r);
         const uint32_t half = sload(b.rs_nfwd + r);
-        uint32_t found = 0, good = 0, hits = 0, bytes = 0;
+        uint32_t found = 0, good = 0, hits = 0, bytes = 0, streams = 0;
         DevHit *region = b.hits + (uint64_t)r * HIT_REGION;
         // the regions must lie inside the hit array (else every read pools)
         const bool region_ok = (uint64_t)(r + 1) * HIT_REGION <= b.hit_cap;
@@ -762,17 +762,21 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(6))) v
 #pragma unroll
             for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) probe_line(ix, P[u]);
             __builtin_amdgcn_sched_barrier(0);
-            uint32_t by = 0, hc_all = 0, f_all = 0, g_all = 0;
+            uint32_t by = 0, sb = 0, hc_all = 0, f_all = 0, g_all = 0;
 #pragma unroll
             for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) {
                 const uint32_t i = i0 + 64 * u + lane;
                 if (act[u]) {
                     probe_finish(ix, P[u], qs[u], qe[u]);
-                    // query in (16) + result out (12) + bucket bounds (16) + search
-                    // records, the filter probe, the run walk
-                    by += 44 + 16 * P[u].probes;
-                    if (P[u].found) by += P[u].pos + ix.filter_cutoff < ix.n ? 8 : 0;
-                    by += 16 * P[u].records;
+                    // the table bytes of the search: the bucket bounds (16), the
+                    // records a linear scan reads to the first hash >= key; a
+                    // found seed's filter probe (8), and a kept one's run past
+                    // its first record plus the hash that ends it: 16 per
+                    // run record in all
+                    by += 16 + 16 * P[u].scan;
+                    if (P[u].found) by += P[u].filtered ? 8u : 16u * P[u].run;
+                    // beside them the streams: query in (16), result out (12)
+                    sb += 28;
                     b.pos[base + i] = P[u].pos;
                     b.hcnt[base + i] = P[u].hc;
                     f_all += P[u].found ? 1u : 0u;
@@ -797,16 +801,18 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(6))) v
                         probe_emit(ix, P[u], qs[u], qe[u], i >= half ? 0x80000000u : 0u, region + at + ex);
                     at += wave_sum(P[u].hc);
                 }
-                by += 16 * hc_all;
+                sb += 16 * hc_all;
             }
             hits += chunk_hits;
             bytes += wave_sum(by);
+            streams += wave_sum(sb);
         }
         if (lane == 0) {
             const float frac = found > 0 ? (float)good / (float)found : 1.0f;
             const bool rescue =
                 rescue_mode == 2 || (rescue_mode == 0 && ((double)frac < 0.7 || hits == 0));
             b.lk_bytes[r] = bytes;
+            b.lk_stream[r] = streams;
             b.frac[r] = frac;
             const uint8_t keep = keep_spill ? (uint8_t)(b.flags[r] & FLAG_SPILL) : (uint8_t)0;
             b.flags[r] = (uint8_t)(keep | (rescue ? GIS_FLAG_RESCUED : 0) |
@@ -2134,7 +2140,7 @@ __global__ void k_find_flat(DevIndex ix, const uint64_t *hashes, uint64_t n, uin
         uint32_t lt, eq;
         probe_small_find(p, lt, eq);
     } else {
-        p.pos = p.e - p.s < 0xffffffffull ? dev_find_in(ix, p.key, p.s, p.e, &p.probes) : dev_find(ix, p.key, &p.probes);
+        p.pos = p.e - p.s < 0xffffffffull ? dev_find_in(ix, p.key, p.s, p.e, &p.scan) : dev_find(ix, p.key, &p.scan);
     }
     positions[i] = p.pos;
 }

@@ -26,35 +26,6 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
-// grow-only pinned host buffer (fast, asynchronous copies to and from HBM)
-template <typename T>
-struct Pinned {
-    T *p = nullptr;
-    size_t cap = 0, n = 0;
-    void reserve(size_t want) {
-        if (want <= cap) return;
-        const size_t c = std::max(want, cap + cap / 2);
-        T *q = nullptr;
-        if (hipHostMalloc((void **)&q, c * sizeof(T)) != hipSuccess) throw HipError{hipErrorOutOfMemory, "hipHostMalloc"};
-        if (p && n) std::memcpy(q, p, n * sizeof(T));
-        if (p) (void)hipHostFree(p);
-        p = q;
-        cap = c;
-    }
-    void resize(size_t m) {
-        reserve(m);
-        n = m;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = n = 0;
-    }
-    T *data() { return p; }
-    size_t size() const { return n; }
-    T &operator[](size_t i) { return p[i]; }
-};
-
 struct Request {
     uint64_t id = 0;
     std::vector<char> seqs;

@@ -273,11 +273,18 @@ int gis_batch_download(gis_batch *b, gis_nam_out *out);
 /* Per-stage kernel times (ms) of the last run, measured with HIP events on the
  * batch's stream. names is filled with a comma-separated list. */
 int gis_batch_stage_times(gis_batch *b, double *ms, int cap, int *n, char *names, int names_cap);
-/* Algorithmic bytes of the lookup kernel in the last run: per randstrobe 16
- * (query) + 12 (results) + 16 (bucket bounds) + 16 per table record searched
- * + 8 (filter probe) + 16 per run record walked + 16 per hit it writes
- * (the hits of reads with at most 48 hits); n_lookups = randstrobes. */
+/* Algorithmic table bytes of the lookup stage in the last GIS_MODE_PLAIN
+ * run, in the byte model of the reference's search (StrobemerIndex::find,
+ * is_filtered and the run walk of add_to_hits_per_ref; step2/index.hpp:57-93,
+ * step2/nam.cu:45-62): per randstrobe 16 (bucket bounds) + 16 per record a
+ * linear scan reads up to the first hash >= key; a found seed 8 more (the
+ * filter probe's hash), a kept seed 16 per record of its equal-hash run past
+ * the first plus 8 (the hash that ends the run). n_lookups = randstrobes. */
 int gis_batch_lookup_bytes(gis_batch *b, uint64_t *bytes, uint64_t *n_lookups);
+/* Bytes the lookup kernel of that run streamed beside the table: 16 per
+ * query randstrobe read, 12 per result written (position, hit count), 16 per
+ * hit written into its read's hit region. */
+int gis_batch_lookup_streams(gis_batch *b, uint64_t *stream_bytes);
 /* total NAMs of the last run (available after gis_batch_run, before download) */
 int gis_batch_nam_count(gis_batch *b, uint64_t *n_nams);
 /* Autotuner of the fused form: runs the uploaded reads under every candidate
@@ -286,6 +293,29 @@ int gis_batch_nam_count(gis_batch *b, uint64_t *n_nams);
  * under every configuration. */
 int gis_batch_autotune(gis_batch *b, const gis_map_options *opt, gis_fused_config *best,
                        double *best_ms, uint32_t *n_tried);
+
+/* ---------------- host-to-host mapping of many reads ---------------- */
+/* The reference GPU driver's per-batch loop (step2/main.cu:4396-4729: pack a
+ * batch of reads, copy it to the device, run seeding through NAMs, hand the
+ * NAMs to the host; rescue as in gis_find_nams_batch) over n_reads reads in
+ * host memory, cut in order into batches of batch_reads. Up to `lanes`
+ * batches are in flight, each on its own stream and workspace: one batch's
+ * copies overlap another's kernels. Results land in out as for
+ * gis_find_nams_batch, over all n_reads reads (nam_offsets has n_reads + 1
+ * entries). GIS_ERR_CAPACITY (n_nams = the required count) if nam_capacity
+ * is too small; every batch still runs. Host memory from gis_host_alloc
+ * (pinned) lets the copies run asynchronously; other memory works, slower.
+ * batch_ms (optional, one entry per batch): the batch's device latency,
+ * from the start of its first copy to HBM to the end of its last copy back,
+ * measured with events on its stream. Lanes and their buffers stay with the
+ * context for later calls. */
+int gis_find_nams_stream(gis_ctx *ctx, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
+                         uint64_t n_reads, uint32_t batch_reads, uint32_t lanes, const gis_map_options *opt,
+                         gis_nam_out *out, double *batch_ms);
+/* Pinned host memory (hipHostMalloc) for the arrays of gis_find_nams_stream
+ * and the other host-buffer entry points; freed with gis_host_free. */
+int gis_host_alloc(uint64_t bytes, void **p);
+void gis_host_free(void *p);
 
 /* ---------------- serving (no reference counterpart) ---------------- */
 /* A request loop on one context: requests (sets of reads) are queued by the

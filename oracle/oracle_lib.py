@@ -56,6 +56,22 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def use_native():
+    """Build the oracle for the CPU this process runs on (-march=native) and
+    load that build instead of the portable one. Must be called before the
+    first use of lib(). Returns the compile flags in effect."""
+    global LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    try:
+        subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True, timeout=300,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    except (subprocess.SubprocessError, OSError):
+        return "-O3 -march=x86-64-v3"
+    LIB_PATH = os.path.join(_HERE, "_build", "liboracle_native.so")
+    return "-O3 -march=native"
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -329,6 +329,82 @@ def test_shared_context_detects_replaced_index(setup):
     assert idx.randstrobes.tobytes() == oidx.randstrobes.tobytes()
 
 
+@pytest.mark.parametrize("mode,lanes,per", [(G.MODE_PLAIN, 1, 4096), (G.MODE_PLAIN, 3, 97),
+                                            (G.MODE_FUSED, 2, 333)])
+def test_stream_host_to_host_matches_oracle(setup, mode, lanes, per):
+    """gis_find_nams_stream: reads in pinned host memory cut into batches
+    (the last one ragged), several in flight, NAM arrays copied back into one
+    output in read order; equal to the oracle, and the same on a second call
+    that reuses the lanes, their graphs and the output arrays."""
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 1200, seed=81)
+    s, o, l = G.pack_reads(reads)
+    # reads out of storage order within the pool (the stream derives each
+    # batch's base from its smallest offset)
+    perm = np.random.default_rng(82).permutation(len(l))
+    o, l = o[perm], l[perm]
+    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
+    ps, po, pl = G.pinned_copy(s), G.pinned_copy(o), G.pinned_copy(l)
+    res, ms = G.find_nams_stream(idx, ps, po, pl, batch_reads=per, lanes=lanes, mode=mode)
+    _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef, res.flags, efl)
+    assert len(ms) == -(-len(l) // per) and np.all(ms > 0)
+    res2, _ = G.find_nams_stream(idx, ps, po, pl, batch_reads=per, lanes=lanes, mode=mode, out=res)
+    _compare_nams((res2.nam_offsets, res2.nams), (eo, en), res2.nonrepetitive_fraction, ef, res2.flags, efl)
+    # pageable host memory takes the same path
+    res3, _ = G.find_nams_stream(idx, s, o, l, batch_reads=per, lanes=lanes, mode=mode)
+    assert res3.nams.tobytes() == en.tobytes()
+
+
+def test_lookup_bytes_match_oracle(setup):
+    """The lookup kernel's algorithmic byte count (the roofline's numerator)
+    equals the oracle's count over the same reads' query randstrobes; the
+    lookup count equals the randstrobe count."""
+    g, lens, p, idx, op, oidx = setup
+    reads = _read_set(g, lens, 800, seed=71)
+    s, o, l = G.pack_reads(reads)
+    m = G.Mapper(idx, len(l), len(s))
+    m.upload(s, o, l)
+    m.run(mode=G.MODE_PLAIN)
+    got, n_lookups, streams = m.lookup_bytes()
+    qrs = [O.randstrobes_query(seq, op) for seq in reads]
+    exp = sum(O.lookup_bytes(oidx, q) for q in qrs)
+    assert n_lookups == sum(len(q) for q in qrs)
+    assert got == exp
+    assert streams >= 28 * n_lookups
+    m.close()
+
+
+@pytest.mark.parametrize("mode", [G.MODE_PLAIN, G.MODE_FUSED])
+def test_replaced_index_drops_captured_graphs(ctx, mode):
+    """Reads mapped with graph replays on index A, then a differently sized
+    index B (another genome, other bits) loaded on the same context: the
+    context's batch must not replay the launch sequence captured with A's
+    (freed) table; the same reads give the oracle's NAMs on B."""
+    ctx2 = G.Context(0)
+    p = G.IndexParameters.from_read_length(150)
+    ga, la = G.synth_genome(300_000, n_contigs=2, gc=0.45, seed=31)
+    gb, lb = G.synth_genome(1_500_000, n_contigs=3, gc=0.41, repeat_fraction=0.05, n_families=10,
+                            family_len=(300, 2000), copies=(2, 20), seed=32)
+    s, o, l, _ = G.synth_reads(gb, lb, 3000, lengths=(100, 150, 250), weights=(0.3, 0.5, 0.2),
+                               sub_rate=0.005, indel_rate=0.0005, seed=33)
+    ia = G.StrobemerIndex(None, p, ctx=ctx2)
+    ia.populate_arrays(ga, la)
+    bits_a = ia.info().bits
+    for _ in range(2):  # capture, then replay
+        G.find_nams_batch(ia, s, o, l, mode=mode, graph=True)
+    ib = G.StrobemerIndex(None, p, ctx=ctx2)
+    ib.populate_arrays(gb, lb)
+    assert ib.info().bits != bits_a
+    oidx = O.Index.build_arrays(gb, lb, O.params_from_read_length(150), n_threads=8)
+    eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 8)
+    for _ in range(2):
+        res = G.find_nams_batch(ib, s, o, l, mode=mode, graph=True)
+        _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef,
+                      res.flags, efl)
+    oidx.free()
+    ctx2.close()
+
+
 @pytest.mark.parametrize("mode", [G.MODE_PLAIN, G.MODE_FUSED])
 def test_pool_overflow_reruns(setup, mode, monkeypatch):
     """Tiny initial hit/NAM pools: every pool overflows, grows to what the

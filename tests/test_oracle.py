@@ -335,6 +335,52 @@ def test_find_count_filter_against_numpy():
     assert idx.find(int(h[0])) == O.NOT_FOUND  # the first hash is never findable
 
 
+def _lookup_bytes_model(h, starts, bits, cutoff, keys):
+    """The byte model of the lookup stage from the sorted hash column alone:
+    16 for the bucket bounds, 16 per record a linear scan of the bucket reads
+    up to the first hash >= key (all records when none is), 8 for a found
+    seed's filter probe, and for an unfiltered seed 16 per run record past
+    the first plus 8 for the hash that ends the run."""
+    total = 0
+    for key in keys:
+        key = int(key)
+        b = key >> (64 - bits)
+        s, e = int(starts[b]), int(starts[b + 1])
+        lt = int(np.searchsorted(h[s:e], np.uint64(key), side="left"))
+        total += 16 + 16 * (lt + 1 if lt < e - s else e - s)
+        if lt == e - s or int(h[s + lt]) != key:
+            continue
+        pos = s + lt
+        total += 8
+        probe = int(h[pos + cutoff]) if pos + cutoff < len(h) else 2 ** 64 - 1
+        if probe == key:
+            continue
+        run = int(np.searchsorted(h, np.uint64(key), side="right")) - pos
+        total += 16 * (run - 1) + 8
+    return total
+
+
+def test_lookup_bytes_against_model():
+    """orc_lookup_bytes (the algorithmic bytes of the lookup roofline) against
+    the byte model computed from numpy searches, on buckets of one to dozens
+    of records, filtered and unfiltered runs, absent keys and the first hash
+    (whose bucket never holds it)."""
+    rng = np.random.default_rng(12)
+    unit = rand_seq(rng, 300)
+    contigs = [rand_seq(rng, 30000) + unit * 8 + rand_seq(rng, 4000) + unit * 3]
+    p = p150()
+    for bits in (8, 12):
+        idx = O.Index.build(contigs, p, bits=bits)
+        h = idx.randstrobes["hash"]
+        keys = np.concatenate([h[::3], h[:4], h[::7] + np.uint64(1),
+                               rng.integers(0, 2 ** 63, 400, dtype=np.uint64) * 2 + 1])
+        qrs = np.zeros(len(keys), dtype=O.QUERY_RANDSTROBE)
+        qrs["hash"] = keys
+        exp = _lookup_bytes_model(h, idx.starts, idx.bits, idx.filter_cutoff, keys)
+        assert O.lookup_bytes(idx, qrs) == exp, bits
+        idx.free()
+
+
 # ---------------------------------------------------------------- merging
 
 def _random_hits(rng, n, n_refs=3):
