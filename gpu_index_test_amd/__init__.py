@@ -376,11 +376,23 @@ def randstrobes_batch(index, seqs, offsets, lens):
     return oo, out[: nout.value]
 
 
-def randstrobes_query(seq, index):
+def randstrobes_query(seq, parameters):
     """randstrobes_query(seq, parameters) (step2/randstrobes.hpp:107) for one
-    read, on the GPU; the parameters are those of ``index``."""
+    read, on the GPU. ``parameters`` is an IndexParameters (seeded on the
+    default context, no index needed) or a StrobemerIndex (its context and
+    parameters)."""
     s, o, l = pack_reads([seq])
-    return randstrobes_batch(index, s, o, l)[1]
+    if isinstance(parameters, StrobemerIndex):
+        return randstrobes_batch(parameters, s, o, l)[1]
+    ctx = default_context()
+    n = len(l)
+    cap = int(2 * max(len(seq) - parameters.syncmer.k + 1, 0)) or 1
+    out = np.empty(cap, dtype=QUERY_RANDSTROBE)
+    oo = np.empty(n + 1, dtype=np.uint64)
+    nout = C.c_uint64()
+    ctx.check(ctx.L.gis_randstrobes_params(ctx.h, C.byref(parameters.c), N.ptr(s), N.ptr(o), N.ptr(l), n,
+                                           N.ptr(oo), N.ptr(out), cap, C.byref(nout)))
+    return out[: nout.value]
 
 
 def _map_options(rescue_level=2, hard_cutoff=1000, mode=N.MODE_PLAIN, graph=True):

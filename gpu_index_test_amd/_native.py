@@ -139,8 +139,7 @@ PROTOTYPES = {
     "gis_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "gis_index_build": (C.c_int, [_P, _P, _P, _U32, C.POINTER(Params), C.c_int, C.c_float, C.c_int]),
     "gis_index_upload": (C.c_int, [_P, _P, _U64, _P, C.c_int, _U32, C.POINTER(Params)]),
-    "gis_reference_randstrobes": (C.c_int, [_P, _P, _U32, C.POINTER(Params), C.c_int, _U64, _P, _U64,
-                                            C.POINTER(_U64)]),
+    "gis_reference_randstrobes": (C.c_int, [_P, _P, _P, _U32, C.POINTER(Params), _P, _U64, C.POINTER(_U64)]),
     "gis_index_read_sti": (C.c_int, [_P, C.c_char_p, C.POINTER(Params)]),
     "gis_index_write_sti": (C.c_int, [_P, C.c_char_p]),
     "gis_index_info_get": (C.c_int, [_P, C.POINTER(IndexInfo)]),
@@ -169,6 +168,7 @@ PROTOTYPES = {
                                        C.POINTER(NamOut), _P]),
     "gis_host_alloc": (C.c_int, [_U64, C.POINTER(_P)]),
     "gis_host_free": (None, [_P]),
+    "gis_randstrobes_params": (C.c_int, [_P, C.POINTER(Params), _P, _P, _P, _U32, _P, _P, _U64, C.POINTER(_U64)]),
     "gis_batch_nam_count": (C.c_int, [_P, C.POINTER(_U64)]),
     "gis_fused_config_get": (C.c_int, [_P, C.POINTER(FusedConfig)]),
     "gis_fused_config_set": (C.c_int, [_P, C.POINTER(FusedConfig)]),

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "batch_internal.h"
@@ -45,24 +46,54 @@ gis_batch *batch_new_stream(gis_ctx *ctx) {
     return b;
 }
 
+// the device arrays of a workspace, and of a batch's own inputs
+static std::vector<DevBuf *> ws_bufs(Workspace *w) {
+    return {&w->long_scratch, &w->rs_cnt, &w->rs_nfwd, &w->rs_hash, &w->rs_start, &w->rs_end,
+            &w->sy_cnt, &w->sy_hash, &w->sy_pos, &w->pos, &w->hcnt, &w->rcnt, &w->lk_bytes, &w->lk_stream,
+            &w->hit_total, &w->rhit_total, &w->frac, &w->flags, &w->nam_cnt, &w->hit_off,
+            &w->rhit_off, &w->hits, &w->rhits, &w->keys, &w->rkeys, &w->nam_ws, &w->rnam_ws,
+            &w->nam_fin, &w->rnam_fin, &w->nam_slots, &w->nam_off, &w->nams_out, &w->scan_tmp,
+            &w->ctl, &w->spill_ids, &w->gwork, &w->split_list, &w->sel_ids};
+}
+
+static std::vector<DevBuf *> input_bufs(gis_batch *b) {
+    return {&b->seqs, &b->read_off, &b->read_len, &b->rs_off, &b->sy_off, &b->long_off, &b->queue};
+}
+
 static void workspace_release(Workspace *w) {
-    DevBuf *all[] = {&w->long_scratch, &w->rs_cnt, &w->rs_nfwd, &w->rs_hash, &w->rs_start, &w->rs_end,
-                     &w->sy_cnt, &w->sy_hash, &w->sy_pos, &w->pos, &w->hcnt, &w->rcnt, &w->lk_bytes, &w->lk_stream,
-                     &w->hit_total, &w->rhit_total, &w->frac, &w->flags, &w->nam_cnt, &w->hit_off,
-                     &w->rhit_off, &w->hits, &w->rhits, &w->keys, &w->rkeys, &w->nam_ws, &w->rnam_ws,
-                     &w->nam_fin, &w->rnam_fin, &w->nam_slots, &w->nam_off, &w->nams_out, &w->scan_tmp,
-                     &w->ctl, &w->spill_ids, &w->gwork, &w->split_list, &w->sel_ids};
-    for (DevBuf *x : all) release(*x);
+    for (DevBuf *x : ws_bufs(w)) release(*x);
     if (w->h_ctl) (void)hipHostFree(w->h_ctl);
     pairwork_free(w->pairs);
     if (w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
 }
 
+uint64_t batch_device_bytes(gis_batch *b) {
+    uint64_t t = 0;
+    for (DevBuf *x : input_bufs(b)) t += x->bytes;
+    for (DevBuf *x : ws_bufs(b->ws)) t += x->bytes;
+    return t;
+}
+
+void batch_trim(gis_batch *b) {
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    for (auto &g : b->graphs) {
+        HIP_CHECK(hipGraphExecDestroy(g.exec));
+        HIP_CHECK(hipGraphDestroy(g.graph));
+    }
+    b->graphs.clear();
+    for (DevBuf *x : input_bufs(b)) release(*x);
+    for (DevBuf *x : ws_bufs(b->ws)) release(*x);
+    b->ws->hit_cap = b->ws->rhit_cap = b->ws->out_cap = 0;
+    b->ws->merge_scratch = false;
+    b->ws->owner = nullptr;
+    b->ran = false;
+    b->n_reads = 0;
+}
+
 void batch_free(gis_batch *b) {
     if (!b) return;
-    DevBuf *all[] = {&b->seqs, &b->read_off, &b->read_len, &b->rs_off, &b->sy_off, &b->long_off, &b->queue};
-    for (DevBuf *x : all) release(*x);
+    for (DevBuf *x : input_bufs(b)) release(*x);
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &g : b->graphs) {
@@ -220,9 +251,9 @@ void mark(gis_batch *b, const char *name) {
 int batch_stage(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
                 uint32_t n, bool want_sy, uint64_t base, hipEvent_t before) {
     gis_ctx *ctx = b->ctx;
-    if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
+    if (!ctx->ix.loaded && !b->seed_params) return fail(ctx, GIS_ERR_STATE, "no index loaded");
     if (n > 0 && (!seqs || !offsets || !lens)) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null read arrays");
-    const gis_params &p = ctx->ix.params;
+    const gis_params &p = b->seed_params ? *b->seed_params : ctx->ix.params;
     uint64_t span = 0;
     b->h_rs_off.resize(n + 1);
     b->h_sy_off.resize(n + 1);
@@ -303,7 +334,7 @@ void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_
     gis_ctx *ctx = b->ctx;
     hipStream_t st = b->stream;
     const DevIndex ix = dev_index(ctx);
-    const SeedParams sp = make_seed_params(ctx->ix.params);
+    const SeedParams sp = make_seed_params(b->seed_params ? *b->seed_params : ctx->ix.params);
     const DevBatch d = make_devbatch(b);
     const uint32_t n = b->n_reads;
     uint64_t *tmp = static_cast<uint64_t *>(b->ws->scan_tmp.p);
@@ -580,11 +611,17 @@ extern "C" int gis_syncmers_batch(gis_ctx *ctx, const char *seqs, const uint64_t
     API_END(ctx)
 }
 
-extern "C" int gis_randstrobes_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
-                                     const uint32_t *lens, uint32_t n_reads, uint64_t *out_offsets,
-                                     gis_query_randstrobe *out, uint64_t capacity, uint64_t *n_out) {
-    API_BEGIN(ctx)
+// randstrobes_query of every read with the given parameters (the index's
+// when params is null); the caller holds the context's lock
+static int randstrobes_of(gis_ctx *ctx, const gis_params *params, const char *seqs, const uint64_t *offsets,
+                          const uint32_t *lens, uint32_t n_reads, uint64_t *out_offsets, gis_query_randstrobe *out,
+                          uint64_t capacity, uint64_t *n_out) {
     gis_batch *b = scratch_batch(ctx);
+    struct Reset {
+        gis_batch *b;
+        ~Reset() { b->seed_params = nullptr; }
+    } reset{b};
+    b->seed_params = params;
     int rc = batch_upload(b, seqs, offsets, lens, n_reads, false);
     if (rc) return rc;
     const gis_map_options o = opts_or_default(nullptr);
@@ -618,6 +655,28 @@ extern "C" int gis_randstrobes_batch(gis_ctx *ctx, const char *seqs, const uint6
     }
     if (out_offsets) out_offsets[n_reads] = o2;
     return GIS_OK;
+}
+
+extern "C" int gis_randstrobes_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
+                                     const uint32_t *lens, uint32_t n_reads, uint64_t *out_offsets,
+                                     gis_query_randstrobe *out, uint64_t capacity, uint64_t *n_out) {
+    API_BEGIN(ctx)
+    return randstrobes_of(ctx, nullptr, seqs, offsets, lens, n_reads, out_offsets, out, capacity, n_out);
+    API_END(ctx)
+}
+
+extern "C" int gis_randstrobes_params(gis_ctx *ctx, const gis_params *params, const char *seqs,
+                                      const uint64_t *offsets, const uint32_t *lens, uint32_t n_reads,
+                                      uint64_t *out_offsets, gis_query_randstrobe *out, uint64_t capacity,
+                                      uint64_t *n_out) {
+    API_BEGIN(ctx)
+    if (!params) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null parameters");
+    gis_params chk;
+    if (gis_params_make(params->canonical_read_length, params->k, params->s, params->l, params->u,
+                        (int)params->q, params->max_dist, &chk) != GIS_OK ||
+        chk.t_syncmer != params->t_syncmer || chk.w_min != params->w_min || chk.w_max != params->w_max)
+        return fail(ctx, GIS_ERR_BAD_PARAMETER, "invalid index parameters");
+    return randstrobes_of(ctx, params, seqs, offsets, lens, n_reads, out_offsets, out, capacity, n_out);
     API_END(ctx)
 }
 
@@ -712,10 +771,10 @@ extern "C" int gis_find_nams_batch(gis_ctx *ctx, const char *seqs, const uint64_
     API_END(ctx)
 }
 
-extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
-                                 const uint64_t *query_offsets, uint32_t n_queries, int rescue,
-                                 uint32_t rescue_cutoff, uint32_t hard_cutoff, gis_nam_out *out) {
-    API_BEGIN(ctx)
+// find_nams / find_nams_rescue over given query randstrobes on the context's
+// scratch batch (the caller holds the context's lock)
+static int qrs_run(gis_ctx *ctx, const gis_query_randstrobe *qrs, const uint64_t *query_offsets,
+                   uint32_t n_queries, int rescue, uint32_t rescue_cutoff, uint32_t hard_cutoff, gis_nam_out *out) {
     if (!ctx->ix.loaded) return fail(ctx, GIS_ERR_STATE, "no index loaded");
     if (!query_offsets || !out) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null argument");
     gis_batch *b = scratch_batch(ctx);
@@ -806,7 +865,135 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
     }
     b->ran = true;
     return batch_download(b, out);
-    API_END(ctx)
+}
+
+namespace {
+
+// Concurrent per-read calls (the reference's find_nams is re-entrant and
+// called once per read from many threads, step2/nam.cu:553-562) are
+// combined: a call queues itself; whichever caller finds the context free
+// takes every queued call with the same rescue settings and runs them as one
+// batch, then hands each caller its reads' results.
+struct QrsCall {
+    const gis_query_randstrobe *qrs;
+    const uint64_t *qoff;
+    uint32_t n;
+    int rescue;
+    uint32_t rcut, hard;
+    gis_nam_out *out;
+    int rc = GIS_OK;
+    bool done = false;
+};
+
+int run_combined(gis_ctx *ctx, std::vector<QrsCall *> &calls) {
+    // one batch: the calls' reads back to back
+    uint32_t n = 0;
+    uint64_t nq = 0;
+    for (QrsCall *c : calls) {
+        n += c->n;
+        nq += c->qoff[c->n] - c->qoff[0];
+    }
+    std::vector<gis_query_randstrobe> q(nq);
+    std::vector<uint64_t> off(n + 1);
+    uint64_t at = 0;
+    uint32_t r = 0;
+    for (QrsCall *c : calls) {
+        const uint64_t lo = c->qoff[0], hi = c->qoff[c->n];
+        if (hi > lo) std::memcpy(q.data() + at, c->qrs + lo, (hi - lo) * sizeof(gis_query_randstrobe));
+        for (uint32_t i = 0; i < c->n; i++) off[r + i] = at + c->qoff[i] - lo;
+        at += hi - lo;
+        r += c->n;
+    }
+    off[n] = at;
+    std::vector<uint64_t> noff(n + 1);
+    std::vector<float> frac(n);
+    std::vector<uint8_t> flags(n);
+    std::vector<gis_nam> nams(16 + 4ull * at);
+    int rc;
+    for (;;) {
+        gis_nam_out o{noff.data(), nams.data(), nams.size(), frac.data(), flags.data(), 0};
+        rc = qrs_run(ctx, q.data(), off.data(), n, calls[0]->rescue, calls[0]->rcut, calls[0]->hard, &o);
+        if (rc == GIS_ERR_CAPACITY && o.n_nams > nams.size()) {
+            nams.resize(o.n_nams);
+            continue;
+        }
+        break;
+    }
+    r = 0;
+    for (QrsCall *c : calls) {
+        if (rc != GIS_OK) {
+            c->rc = rc;
+            r += c->n;
+            continue;
+        }
+        gis_nam_out *o = c->out;
+        const uint64_t a = noff[r], e = noff[r + c->n];
+        o->n_nams = e - a;
+        if (o->nam_capacity < e - a || (e > a && !o->nams)) {
+            c->rc = GIS_ERR_CAPACITY;
+        } else {
+            if (o->nam_offsets)
+                for (uint32_t i = 0; i <= c->n; i++) o->nam_offsets[i] = noff[r + i] - a;
+            if (e > a) std::memcpy(o->nams, nams.data() + a, (e - a) * sizeof(gis_nam));
+            if (o->nonrepetitive_fraction && c->n) std::memcpy(o->nonrepetitive_fraction, frac.data() + r, 4ull * c->n);
+            if (o->flags && c->n) std::memcpy(o->flags, flags.data() + r, c->n);
+            c->rc = GIS_OK;
+        }
+        r += c->n;
+    }
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
+                                 const uint64_t *query_offsets, uint32_t n_queries, int rescue,
+                                 uint32_t rescue_cutoff, uint32_t hard_cutoff, gis_nam_out *out) {
+    if (!ctx) return GIS_ERR_BAD_PARAMETER;
+    if (!query_offsets || !out) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null argument");
+    if (n_queries && !qrs && query_offsets[n_queries] > query_offsets[0])
+        return fail(ctx, GIS_ERR_BAD_PARAMETER, "null query randstrobes");
+    for (uint64_t i = query_offsets[0]; n_queries && i < query_offsets[n_queries]; i++)
+        if (qrs[i].end > GIS_MAX_READ_LEN) return fail(ctx, GIS_ERR_BAD_PARAMETER, "query end too large");
+    QrsCall me{qrs, query_offsets, n_queries, rescue ? 1 : 0, rescue_cutoff, hard_cutoff, out};
+    std::unique_lock<std::mutex> l(ctx->comb_mu);
+    ctx->comb_queue.push_back(&me);
+    while (!me.done) {
+        if (ctx->comb_busy) {
+            ctx->comb_cv.wait(l);
+            continue;
+        }
+        // this caller runs the queued calls that share its rescue settings
+        ctx->comb_busy = true;
+        std::vector<QrsCall *> take, keep;
+        for (void *v : ctx->comb_queue) {
+            QrsCall *c = static_cast<QrsCall *>(v);
+            const bool same = c->rescue == me.rescue && (!me.rescue || (c->rcut == me.rcut && c->hard == me.hard));
+            (same ? take : keep).push_back(c);
+        }
+        ctx->comb_queue.assign(keep.begin(), keep.end());
+        l.unlock();
+        int rc = GIS_OK;
+        {
+            std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+            try {
+                HIP_CHECK(hipSetDevice(ctx->device));
+                rc = run_combined(ctx, take);
+            } catch (const HipError &e) {
+                rc = hip_fail(ctx, e);
+            } catch (const std::bad_alloc &) {
+                rc = fail(ctx, GIS_ERR_NOMEM, "host allocation failed");
+            }
+        }
+        l.lock();
+        for (QrsCall *c : take) {
+            if (rc != GIS_OK) c->rc = rc;
+            c->done = true;
+        }
+        ctx->comb_busy = false;
+        ctx->comb_cv.notify_all();
+    }
+    return me.rc;
 }
 
 // ---------------- device-resident batches ----------------

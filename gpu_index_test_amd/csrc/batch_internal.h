@@ -111,6 +111,8 @@ struct gis_batch {
     gis::Pinned<uint32_t> h_queue;  // read ids by length, longest first
     uint32_t n_long = 0;            // queue entries longer than the short bin (160 bp)
     uint64_t tot_rs_cap = 0;
+    // seeding parameters of a run that needs no index (else the index's)
+    const gis_params *seed_params = nullptr;
     gis::Workspace *ws = nullptr;
     int last_mode = GIS_MODE_PLAIN;
     bool last_graph = false;
@@ -151,6 +153,12 @@ struct RunOpts {
 
 gis_batch *batch_new(gis_ctx *ctx, Workspace *share);  // share = nullptr: a workspace of its own
 gis_batch *batch_new_stream(gis_ctx *ctx);  // a batch with a stream and a workspace of its own
+// device bytes the batch's inputs and workspace hold
+uint64_t batch_device_bytes(gis_batch *b);
+// free the batch's device arrays and captured graphs (after its stream
+// drains); the next run allocates them again. The workspace must be the
+// batch's own.
+void batch_trim(gis_batch *b);
 // Stage reads into the batch without waiting: the host derives the
 // per-read arrays into the batch's pinned arrays, then the copies to HBM are
 // enqueued on the batch's stream (after `before`, when given, is recorded

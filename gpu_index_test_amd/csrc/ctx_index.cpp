@@ -116,36 +116,39 @@ extern "C" void gis_ctx_destroy(gis_ctx *ctx) {
 
 extern "C" const char *gis_last_error(const gis_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-// bases per seeding segment (a knob for tests: GIS_INDEX_SEGMENT)
-uint64_t seeding_segment() {
-    const char *e = std::getenv("GIS_INDEX_SEGMENT");
-    const long long v = e ? std::atoll(e) : 0;
-    return v > 0 ? (uint64_t)v : 4000000ull;
+// contigs the 24-bit reference index and 32-bit positions of a record can hold
+static int check_contigs(gis_ctx *ctx, const char *const *seqs, const uint64_t *lens, uint32_t n_refs) {
+    if (n_refs && (!seqs || !lens)) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null argument");
+    if (n_refs >= (1u << 24)) return fail(ctx, GIS_ERR_BAD_PARAMETER, "too many reference sequences for the 24-bit ref index");
+    for (uint32_t j = 0; j < n_refs; j++) {
+        if (lens[j] >= (1ull << 32)) return fail(ctx, GIS_ERR_BAD_PARAMETER, "reference sequence longer than 2^32 - 1");
+        if (lens[j] && !seqs[j]) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null reference sequence");
+    }
+    return GIS_OK;
 }
 
 extern "C" int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint64_t *lens,
                                uint32_t n_refs, const gis_params *params, int bits, float f,
                                int n_threads) {
+    (void)n_threads;  // the device seeds every contig
     API_BEGIN(ctx)
-    if (!params || (n_refs && (!seqs || !lens))) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null argument");
+    if (!params) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null argument");
+    if (int rc = check_contigs(ctx, seqs, lens, n_refs)) return rc;
     uint64_t total = 0;
     for (uint32_t j = 0; j < n_refs; j++) total += lens[j];
     if (total == 0) return fail(ctx, GIS_ERR_INVALID_FASTA, "No reference sequences found");
     if (bits == -1) bits = pick_bits(total, *params);
     if (bits < 8 || bits > 31) return fail(ctx, GIS_ERR_BAD_PARAMETER, "Bits must be between 8 and 31");
-    std::vector<gis_ref_randstrobe> recs;
-    std::string err;
-    const int rc = seed_reference(seqs, lens, n_refs, *params, n_threads, seeding_segment(), recs, err);
-    if (rc) return fail(ctx, rc, err);
     DevIndexStore &ix = ctx->ix;
     drop_table(ix);
-    const uint64_t n = recs.size();
-    ensure<gis_ref_randstrobe>(ix.rs, n);
-    ensure<uint64_t>(ix.starts, (1ull << bits) + 1);
-    if (n) HIP_CHECK(hipMemcpy(ix.rs.p, recs.data(), n * sizeof(gis_ref_randstrobe), hipMemcpyHostToDevice));
-    std::vector<gis_ref_randstrobe>().swap(recs);
+    uint64_t n = 0;
     uint64_t hist[RUN_BINS] = {};
     try {
+        uint4 *recs = nullptr;
+        device_seed_reference(seqs, lens, n_refs, *params, &recs, &n, ctx->stream);
+        ix.rs.p = recs;
+        ix.rs.bytes = std::max<uint64_t>(n, 1) * sizeof(gis_ref_randstrobe);
+        ensure<uint64_t>(ix.starts, (1ull << bits) + 1);
         device_sort_records(static_cast<uint4 *>(ix.rs.p), n, ctx->stream);
         device_finish_index(static_cast<const uint4 *>(ix.rs.p), n, bits, static_cast<uint64_t *>(ix.starts.p),
                             hist, ctx->stream);
@@ -169,22 +172,34 @@ extern "C" int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint
     API_END(ctx)
 }
 
-extern "C" int gis_reference_randstrobes(const char *const *seqs, const uint64_t *lens, uint32_t n_refs,
-                                         const gis_params *params, int n_threads, uint64_t segment,
-                                         gis_ref_randstrobe *out, uint64_t capacity, uint64_t *n_out) {
-    if (!params || !n_out || (n_refs && (!seqs || !lens))) return GIS_ERR_BAD_PARAMETER;
+extern "C" int gis_reference_randstrobes(gis_ctx *ctx, const char *const *seqs, const uint64_t *lens,
+                                         uint32_t n_refs, const gis_params *params, gis_ref_randstrobe *out,
+                                         uint64_t capacity, uint64_t *n_out) {
+    API_BEGIN(ctx)
+    if (!params || !n_out) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null argument");
+    if (int rc = check_contigs(ctx, seqs, lens, n_refs)) return rc;
+    *n_out = 0;
+    uint64_t total = 0;
+    for (uint32_t j = 0; j < n_refs; j++) total += lens[j];
+    if (total == 0) return GIS_OK;
+    uint4 *recs = nullptr;
+    uint64_t n = 0;
     try {
-        std::vector<gis_ref_randstrobe> recs;
-        std::string err;
-        const int rc = seed_reference(seqs, lens, n_refs, *params, n_threads, segment, recs, err);
-        if (rc) return rc;
-        *n_out = recs.size();
-        if (recs.size() > capacity || (!recs.empty() && !out)) return GIS_ERR_CAPACITY;
-        if (!recs.empty()) std::memcpy(out, recs.data(), recs.size() * sizeof(gis_ref_randstrobe));
-        return GIS_OK;
-    } catch (const std::bad_alloc &) {
-        return GIS_ERR_NOMEM;
+        device_seed_reference(seqs, lens, n_refs, *params, &recs, &n, ctx->stream);
+    } catch (const BuildError &e) {
+        throw HipError{e.e, e.what};
     }
+    DevBuf keep{recs, n * sizeof(gis_ref_randstrobe)};
+    *n_out = n;
+    if (n > capacity || (n && !out)) {
+        release(keep);
+        return fail(ctx, GIS_ERR_CAPACITY, "record capacity " + std::to_string(capacity) + " < " + std::to_string(n));
+    }
+    const hipError_t e = n ? hipMemcpy(out, recs, n * sizeof(gis_ref_randstrobe), hipMemcpyDeviceToHost) : hipSuccess;
+    release(keep);
+    HIP_CHECK(e);
+    return GIS_OK;
+    API_END(ctx)
 }
 
 extern "C" int gis_index_upload(gis_ctx *ctx, const gis_ref_randstrobe *randstrobes, uint64_t n,

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -97,6 +98,11 @@ struct gis_ctx {
     // lanes of gis_find_nams_stream: batches with a stream and a workspace
     // each, kept so that later calls reuse their buffers and graphs
     std::vector<gis_batch *> lanes;
+    // concurrent gis_find_nams_qrs calls waiting to be run together
+    std::mutex comb_mu;
+    std::condition_variable comb_cv;
+    std::vector<void *> comb_queue;
+    bool comb_busy = false;
 };
 
 namespace gis {

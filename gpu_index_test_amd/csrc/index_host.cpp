@@ -1,12 +1,11 @@
 // Host side of strobemer index construction, and the .sti v2 file format.
 //
-// StrobemerIndex::populate (step2/index.cu:143-244) is split between the
-// host and the GPU: the host seeds the reference (randstrobes of every
-// contig, packed = ref_index << 8 | strobe2 offset) in parallel segments;
-// the device sorts the records by (hash, position, packed), fills the
-// bucket starts and counts runs of equal hashes (index_build.hip); the host
-// turns that run-length histogram into the repetition statistics and the
-// filter cutoff.
+// StrobemerIndex::populate (step2/index.cu:143-244) runs on the device:
+// the reference is seeded there (ref_seed.hip), the records are sorted by
+// (hash, position, packed), the bucket starts filled and the runs of equal
+// hashes counted (index_build.hip). The host picks the bucket bits and turns
+// the run-length histogram into the repetition statistics and the filter
+// cutoff.
 #include "index_host.h"
 
 #include <fcntl.h>
@@ -15,13 +14,10 @@
 #include <unistd.h>
 
 #include <algorithm>
-#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
-#include <functional>
-#include <thread>
 
 #include "gis_core.h"
 
@@ -33,200 +29,6 @@ int pick_bits(uint64_t total_length, const gis_params &p) {
     if (est == 0) return 8;
     const int b = (int)std::log2((double)est) - 1;
     return std::min(31, std::max(8, b));
-}
-
-namespace {
-
-// ring view of the last 64 s-mer hashes, indexed by absolute s-mer start
-struct Ring {
-    uint64_t v[64];
-    uint64_t operator[](int64_t i) const { return v[i & 63]; }
-};
-
-// Randstrobes (RefRandstrobe, packed = ref_index << 8 | strobe2 offset) of
-// one contig whose strobe1 syncmer starts in [A, B): the reference's
-// RandstrobeGenerator (step2/randstrobes.cu:161-190) over the contig's
-// syncmers, restricted to a segment so that one contig can be seeded by
-// several threads. The syncmer scan starts `back` bases before A and is
-// exact once it has passed a window whose state needs no history (a unique
-// minimum below ~0, or the first window after a contig start or an N);
-// returns false when no such window lies before A, and the caller retries
-// with a longer run-in. The scan continues past B until the partner windows
-// of the last strobe1 are complete (max_dist) and w_min syncmers follow it.
-bool segment_randstrobes(const char *seq, uint64_t len, uint32_t ref_index, uint64_t A, uint64_t B,
-                         uint64_t back, const gis_params &gp, const SeedParams &P,
-                         std::vector<gis_ref_randstrobe> &out) {
-    const int k = P.k, s = P.s, w = P.w;
-    const uint64_t S = A > back ? A - back : 0;
-    std::vector<uint64_t> sy_hash;
-    std::vector<uint32_t> sy_pos;
-    uint64_t first_owned = UINT64_MAX;  // index in sy_* of the first syncmer at >= A
-    uint64_t after_B = 0;
-    Ring ring;
-    uint64_t run = 0;  // ACGT bases since the last non-ACGT or the scan start
-    uint64_t fs = 0, rs = 0, fk = 0, rk = 0;
-    uint8_t state = ST_INVALID;
-    bool exact = S == 0;  // at the contig start the first window is a true segment start
-    const uint64_t stop_pos = B + P.max_dist_u + 1;
-    for (uint64_t i = S; i < len; i++) {
-        const int c = nt4((unsigned char)seq[i]);
-        if (c > 3) {
-            run = 0;
-            state = ST_INVALID;
-            exact = true;  // the next window starts a segment
-            continue;
-        }
-        run++;
-        fs = ((fs << 2) | (uint64_t)c) & P.smask;
-        rs = (rs >> 2) | ((uint64_t)(3 - c) << (2 * (s - 1)));
-        fk = ((fk << 2) | (uint64_t)c) & P.kmask;
-        rk = (rk >> 2) | ((uint64_t)(3 - c) << (2 * (k - 1)));
-        if (run < (uint64_t)s) continue;
-        const int64_t j = (int64_t)(i + 1 - (uint64_t)s);
-        ring.v[j & 63] = xxh64_u64(fs < rs ? fs : rs);
-        if (run < (uint64_t)k) continue;
-        const uint64_t p = i + 1 - (uint64_t)k;
-        struct Shift {
-            const Ring *r;
-            int64_t base;
-            uint64_t operator[](int64_t x) const { return (*r)[base + x]; }
-        };
-        if (run == (uint64_t)k) {
-            state = anchor_state(Shift{&ring, (int64_t)p}, 0, w, true);
-        } else {
-            state = slide(state, Shift{&ring, (int64_t)p - 1}, 1, w);
-            if (!exact && anchor_state(Shift{&ring, (int64_t)p}, 0, w, false) != ST_UNKNOWN) exact = true;
-        }
-        if (p == A && !exact) return false;
-        if (p >= A && !exact) return false;  // (A itself may be inside an N run)
-        if (state == (uint8_t)(P.t - 1)) {
-            if (p >= A && first_owned == UINT64_MAX) first_owned = sy_hash.size();
-            if (p >= B) after_B++;
-            sy_hash.push_back(xxh64_u64(fk < rk ? fk : rk));
-            sy_pos.push_back((uint32_t)p);
-        }
-        if (p >= stop_pos && after_B >= gp.w_min) break;
-    }
-    if (first_owned == UINT64_MAX) return true;
-    const size_t n = sy_hash.size();
-    for (size_t i = first_owned; i < n && sy_pos[i] < B; i++) {
-        if (i + gp.w_min > n - 1) break;  // fewer than w_min syncmers follow: no randstrobe
-        const size_t wend = std::min(i + (size_t)gp.w_max, n - 1);
-        const uint64_t maxp = (uint64_t)sy_pos[i] + P.max_dist_u;
-        size_t best = i;
-        uint64_t minv = UINT64_MAX;
-        for (size_t jj = i + gp.w_min; jj <= wend && (uint64_t)sy_pos[jj] <= maxp; jj++) {
-            const uint64_t cnt = (uint64_t)__builtin_popcountll((sy_hash[i] ^ sy_hash[jj]) & gp.q);
-            if (cnt < minv) {
-                minv = cnt;
-                best = jj;
-            }
-        }
-        gis_ref_randstrobe e;
-        e.hash = sy_hash[i] + sy_hash[best];
-        e.position = sy_pos[i];
-        e.packed = (ref_index << 8) + (sy_pos[best] - sy_pos[i]);
-        out.push_back(e);
-    }
-    return true;
-}
-
-// count of randstrobe slots of a contig shorter than w_max: the reference
-// counts them (count_randstrobes) but assign_randstrobes returns before
-// writing (step2/index.cu:282-284), so they stay zero
-uint64_t short_contig_slots(const char *seq, uint64_t len, const gis_params &gp, const SeedParams &P) {
-    // count syncmers with a full scan (the contig is tiny)
-    uint64_t nsy = 0;
-    {
-        const int k = P.k, s = P.s, w = P.w;
-        Ring ring;
-        uint64_t run = 0, fs = 0, rs = 0;
-        uint8_t state = ST_INVALID;
-        for (uint64_t i = 0; i < len; i++) {
-            const int c = nt4((unsigned char)seq[i]);
-            if (c > 3) {
-                run = 0;
-                state = ST_INVALID;
-                continue;
-            }
-            run++;
-            fs = ((fs << 2) | (uint64_t)c) & P.smask;
-            rs = (rs >> 2) | ((uint64_t)(3 - c) << (2 * (s - 1)));
-            if (run < (uint64_t)s) continue;
-            const int64_t j = (int64_t)(i + 1 - (uint64_t)s);
-            ring.v[j & 63] = xxh64_u64(fs < rs ? fs : rs);
-            if (run < (uint64_t)k) continue;
-            const int64_t p = (int64_t)(i + 1 - (uint64_t)k);
-            struct Shift {
-                const Ring *r;
-                int64_t base;
-                uint64_t operator[](int64_t x) const { return (*r)[base + x]; }
-            };
-            if (run == (uint64_t)k) state = anchor_state(Shift{&ring, p}, 0, w, true);
-            else state = slide(state, Shift{&ring, p - 1}, 1, w);
-            if (state == (uint8_t)(P.t - 1)) nsy++;
-        }
-    }
-    return nsy > gp.w_min ? nsy - gp.w_min : 0;
-}
-
-}  // namespace
-
-int seed_reference(const char *const *seqs, const uint64_t *lens, uint32_t n_refs, const gis_params &p,
-                   int n_threads, uint64_t segment, std::vector<gis_ref_randstrobe> &out, std::string &err) {
-    if (n_refs >= (1u << 24)) {
-        err = "too many reference sequences for the 24-bit ref index";
-        return GIS_ERR_BAD_PARAMETER;
-    }
-    if (segment == 0) segment = 4000000;
-    const SeedParams P = make_seed_params(p);
-    // jobs: [ref, A, B) segments in (contig, position) order
-    struct Job {
-        uint32_t ref;
-        uint64_t A, B;
-    };
-    std::vector<Job> jobs;
-    for (uint32_t j = 0; j < n_refs; j++) {
-        if (lens[j] < p.w_max) {
-            jobs.push_back(Job{j, 0, 0});  // zero slots of a contig shorter than w_max
-            continue;
-        }
-        for (uint64_t a = 0; a < lens[j]; a += segment) jobs.push_back(Job{j, a, std::min(lens[j], a + segment)});
-    }
-    std::vector<std::vector<gis_ref_randstrobe>> per(jobs.size());
-    std::atomic<size_t> next{0};
-    std::vector<std::thread> th;
-    const int T = std::max(1, std::min<int>(n_threads, (int)jobs.size()));
-    for (int t = 0; t < T; t++) {
-        th.emplace_back([&]() {
-            for (;;) {
-                const size_t x = next.fetch_add(1);
-                if (x >= jobs.size()) break;
-                const Job &jb = jobs[x];
-                auto &o = per[x];
-                if (jb.B == 0) {
-                    o.assign(short_contig_slots(seqs[jb.ref], lens[jb.ref], p, P), gis_ref_randstrobe{0, 0, 0});
-                    continue;
-                }
-                uint64_t back = 1024;
-                while (!segment_randstrobes(seqs[jb.ref], lens[jb.ref], jb.ref, jb.A, jb.B, back, p, P, o)) {
-                    o.clear();
-                    back *= 4;
-                }
-            }
-        });
-    }
-    for (auto &x : th) x.join();
-    uint64_t n = 0;
-    for (auto &v : per) n += v.size();
-    out.resize(n);
-    uint64_t o = 0;
-    for (auto &v : per) {
-        std::copy(v.begin(), v.end(), out.begin() + (int64_t)o);
-        o += v.size();
-        std::vector<gis_ref_randstrobe>().swap(v);
-    }
-    return GIS_OK;
 }
 
 void filter_stats(const uint64_t *hist, float f, HostIndex &ix) {

@@ -20,10 +20,6 @@ struct HostIndex {
 };
 
 int pick_bits(uint64_t total_length, const gis_params &p);
-// unsorted RefRandstrobe records of every contig in (contig, position) order,
-// contigs cut into segments of `segment` bases seeded by n_threads threads
-int seed_reference(const char *const *seqs, const uint64_t *lens, uint32_t n_refs, const gis_params &p,
-                   int n_threads, uint64_t segment, std::vector<gis_ref_randstrobe> &out, std::string &err);
 // repetition statistics and filter_cutoff from the run-length histogram
 // (hist[1..100] exact, hist[101] = runs longer than 100)
 void filter_stats(const uint64_t *hist, float f, HostIndex &ix);

@@ -5,7 +5,13 @@
 // to `slots` steps in flight on separate streams, and hands each request its
 // per-read NAMs when its step completes. Requests that arrive while steps
 // are on the GPU join the next step on a free slot (continuous batching);
-// a request's NAMs do not depend on the requests it was batched with.
+// a request's NAMs do not depend on the requests it was batched with. A
+// request larger than one step (max_step_reads, or the HBM budget) is cut
+// into parts that run as steps of their own and are put back together when
+// the caller waits for it. Admission charges each slot the device memory its
+// batch really holds: a running step the larger of what its batch held and
+// the step's estimate, an idle slot what its batch still holds; idle slots
+// give theirs back when a step would not fit otherwise.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,6 +46,18 @@ struct Request {
     std::vector<gis_nam> nams;
     std::vector<float> frac;
     std::vector<uint8_t> flags;
+    // a request cut into steps: its parts in read order (each runs as a
+    // request of its own) and the parts still running; a part's parent
+    Request *parent = nullptr;
+    std::vector<Request *> parts;
+    size_t parts_left = 0;
+    bool claimed = false;  // a caller is waiting for it
+    Request() = default;
+    Request(const Request &) = delete;
+    Request &operator=(const Request &) = delete;
+    ~Request() {
+        for (Request *p : parts) delete p;
+    }
 };
 
 struct Slot {
@@ -47,7 +65,9 @@ struct Slot {
     std::vector<Request *> reqs;
     std::vector<uint32_t> first;  // first read of each request in the step
     uint32_t n_reads = 0;         // reads of the step before padding
-    uint64_t bytes = 0;
+    bool busy = false;            // a step is running
+    uint64_t held = 0;            // device bytes the slot's batch holds between steps
+    uint64_t charge = 0;          // bytes charged while its step runs
     RunOpts r{};
     std::thread th;
     // pinned host staging of the slot's thread
@@ -93,7 +113,7 @@ struct gis_server {
     std::deque<Request *> pending;
     std::unordered_map<uint64_t, Request *> reqs;
     uint64_t next_id = 1;
-    uint64_t bytes_in_flight = 0;
+    int n_waiters = 0;
     bool stop = false;
     gis_server_stats stats{};
     Clock::time_point busy_since;
@@ -168,6 +188,45 @@ void complete_step(gis_server *s, Slot &sl) {
     }
 }
 
+// Device bytes charged to all slots but `self`, split into running steps
+// and idle slots.
+void charged(const gis_server *s, size_t self, uint64_t &running, uint64_t &idle) {
+    running = idle = 0;
+    for (size_t j = 0; j < s->slots.size(); j++) {
+        if (j == self) continue;
+        if (s->slots[j].busy) running += s->slots[j].charge;
+        else idle += s->slots[j].held;
+    }
+}
+
+// Can slot si run a step of estimated bytes e now? Idle slots' arrays are
+// given back (largest first) when only they stand in the way. Called with
+// the server lock held.
+bool admit(gis_server *s, size_t si, uint64_t e) {
+    uint64_t running, idle;
+    charged(s, si, running, idle);
+    const uint64_t mine = std::max(s->slots[si].held, e);
+    if (running + idle + mine <= s->cfg.hbm_budget) return true;
+    if (running + mine > s->cfg.hbm_budget) return false;
+    while (running + idle + mine > s->cfg.hbm_budget) {
+        Slot *big = nullptr;
+        for (size_t j = 0; j < s->slots.size(); j++) {
+            Slot &o = s->slots[j];
+            if (j != si && !o.busy && o.held && (!big || o.held > big->held)) big = &o;
+        }
+        if (!big) break;
+        try {
+            batch_trim(big->b);
+        } catch (const HipError &) {
+            return false;
+        }
+        idle -= big->held;
+        big->held = batch_device_bytes(big->b);
+        idle += big->held;
+    }
+    return running + idle + mine <= s->cfg.hbm_budget;
+}
+
 // One thread per slot: take the queued requests that fit (arrival order,
 // max_step_reads, HBM budget), stage and launch them as one step, wait for
 // it on the slot's stream, and complete the requests. Slots run their host
@@ -179,8 +238,7 @@ void slot_main(gis_server *s, size_t si) {
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
         s->cv_work.wait(lk, [&] {
-            return s->stop || (!s->pending.empty() &&
-                               s->bytes_in_flight + s->pending.front()->bytes <= s->cfg.hbm_budget);
+            return s->stop || (!s->pending.empty() && admit(s, si, s->pending.front()->bytes));
         });
         if (s->stop) break;
         uint32_t n = 0;
@@ -189,8 +247,7 @@ void slot_main(gis_server *s, size_t si) {
         while (!s->pending.empty()) {
             Request *q = s->pending.front();
             const uint32_t k = (uint32_t)q->lens.size();
-            if (!take.empty() && n + k > s->cfg.max_step_reads) break;
-            if (s->bytes_in_flight + bytes + q->bytes > s->cfg.hbm_budget) break;
+            if (!take.empty() && (n + k > s->cfg.max_step_reads || !admit(s, si, bytes + q->bytes))) break;
             take.push_back(q);
             n += k;
             bytes += q->bytes;
@@ -198,12 +255,17 @@ void slot_main(gis_server *s, size_t si) {
         }
         sl.reqs = std::move(take);
         sl.n_reads = n;
-        sl.bytes = bytes;
-        s->bytes_in_flight += bytes;
-        s->stats.peak_bytes = std::max(s->stats.peak_bytes, s->bytes_in_flight);
+        sl.busy = true;
+        sl.charge = std::max(sl.held, bytes);
+        {
+            uint64_t running, idle;
+            charged(s, s->slots.size(), running, idle);
+            s->stats.peak_bytes = std::max(s->stats.peak_bytes, running + idle);
+        }
         if (s->n_busy++ == 0) s->busy_since = Clock::now();
         s->stats.steps++;
         lk.unlock();
+        uint64_t held = 0;
         try {
             launch_step(s, sl);
             complete_step(s, sl);
@@ -214,10 +276,25 @@ void slot_main(gis_server *s, size_t si) {
             fail_step(s, sl, "host allocation failed");
             (void)hipStreamSynchronize(sl.b->stream);
         }
+        // a slot keeps its arrays for the next step up to its share of the
+        // budget
+        try {
+            held = batch_device_bytes(sl.b);
+            if (held > s->cfg.hbm_budget / s->slots.size()) {
+                batch_trim(sl.b);
+                held = batch_device_bytes(sl.b);
+            }
+        } catch (const HipError &) {
+        }
         lk.lock();
-        for (Request *q : sl.reqs) q->done = true;
+        for (Request *q : sl.reqs) {
+            q->done = true;
+            if (q->parent && --q->parent->parts_left == 0) q->parent->done = true;
+        }
         sl.reqs.clear();
-        s->bytes_in_flight -= sl.bytes;
+        sl.busy = false;
+        sl.held = held;
+        sl.charge = 0;
         if (--s->n_busy == 0)
             s->stats.busy_s += std::chrono::duration<double>(Clock::now() - s->busy_since).count();
         s->cv_done.notify_all();
@@ -240,6 +317,69 @@ Request *make_request(gis_server *s, const char *seqs, const uint64_t *offsets, 
     q->bytes = estimate_bytes(n_reads, bases);
     (void)s;
     return q;
+}
+
+// Cut a request larger than one step into parts of at most max_step_reads
+// reads whose estimates fit the budget, in read order. False if a single
+// read does not fit the budget.
+bool split_request(const gis_server *s, Request *q) {
+    const uint32_t n = (uint32_t)q->lens.size();
+    uint32_t i = 0;
+    while (i < n) {
+        uint64_t bases = 0;
+        uint32_t k = 0;
+        while (i + k < n && k < s->cfg.max_step_reads) {
+            const uint64_t L = q->lens[i + k];
+            if (k > 0 && estimate_bytes(k + 1, bases + L) > s->cfg.hbm_budget) break;
+            bases += L;
+            k++;
+        }
+        if (estimate_bytes(k, bases) > s->cfg.hbm_budget) return false;
+        Request *p = new Request();
+        p->parent = q;
+        q->parts.push_back(p);
+        p->lens.assign(q->lens.begin() + i, q->lens.begin() + i + k);
+        p->offs.resize(k);
+        const uint64_t at = q->offs[i];
+        p->seqs.assign(q->seqs.begin() + at, q->seqs.begin() + at + bases);
+        for (uint32_t j = 0; j < k; j++) p->offs[j] = q->offs[i + j] - at;
+        p->bytes = estimate_bytes(k, bases);
+        i += k;
+    }
+    q->parts_left = q->parts.size();
+    std::vector<char>().swap(q->seqs);
+    std::vector<uint64_t>().swap(q->offs);
+    return true;
+}
+
+// Copy a finished request's results out (the parts of a split request one
+// after another, offsets rebased). Called by the request's one waiter.
+int copy_out(Request *q, gis_nam_out *out) {
+    const uint32_t n = (uint32_t)q->lens.size();
+    std::vector<Request *> self{q};
+    const std::vector<Request *> &pieces = q->parts.empty() ? self : q->parts;
+    uint64_t total = 0;
+    for (const Request *p : pieces) {
+        if (p->status != GIS_OK) return p->status;
+        total += p->nams.size();
+    }
+    if (!out) return GIS_OK;
+    out->n_nams = total;
+    if (out->nam_capacity < total || (total && !out->nams)) return GIS_ERR_CAPACITY;
+    uint64_t at = 0;
+    uint32_t r = 0;
+    for (const Request *p : pieces) {
+        const uint32_t k = (uint32_t)p->lens.size();
+        if (out->nam_offsets)
+            for (uint32_t i = 0; i < k; i++) out->nam_offsets[r + i] = at + p->nam_off[i];
+        if (!p->nams.empty()) std::memcpy(out->nams + at, p->nams.data(), sizeof(gis_nam) * p->nams.size());
+        if (out->nonrepetitive_fraction && k) std::memcpy(out->nonrepetitive_fraction + r, p->frac.data(), 4ull * k);
+        if (out->flags && k) std::memcpy(out->flags + r, p->flags.data(), k);
+        at += p->nams.size();
+        r += k;
+    }
+    if (out->nam_offsets) out->nam_offsets[n] = at;
+    return GIS_OK;
 }
 
 }  // namespace
@@ -290,11 +430,19 @@ extern "C" void gis_server_destroy(gis_server *s) {
         for (Request *q : s->pending) {
             q->status = GIS_ERR_CANCELLED;
             q->done = true;
+            if (q->parent && --q->parent->parts_left == 0) q->parent->done = true;
         }
         s->pending.clear();
     }
     s->cv_work.notify_all();
     for (Slot &sl : s->slots) sl.th.join();
+    {
+        // every request is done now; callers still waiting take theirs
+        // before the requests are freed
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.notify_all();
+        s->cv_done.wait(lk, [&] { return s->n_waiters == 0; });
+    }
     {
         std::lock_guard<std::recursive_mutex> cl(s->ctx->mu);
         (void)hipSetDevice(s->ctx->device);
@@ -319,9 +467,18 @@ extern "C" int gis_server_submit(gis_server *s, const char *seqs, const uint64_t
     } catch (const std::bad_alloc &) {
         return GIS_ERR_NOMEM;
     }
-    if (q->bytes > s->cfg.hbm_budget) {
-        delete q;
-        return GIS_ERR_BUDGET;
+    if (q->bytes > s->cfg.hbm_budget || n_reads > s->cfg.max_step_reads) {
+        bool ok = false;
+        try {
+            ok = split_request(s, q);
+        } catch (const std::bad_alloc &) {
+            delete q;
+            return GIS_ERR_NOMEM;
+        }
+        if (!ok) {
+            delete q;
+            return GIS_ERR_BUDGET;
+        }
     }
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -332,12 +489,19 @@ extern "C" int gis_server_submit(gis_server *s, const char *seqs, const uint64_t
         q->id = s->next_id++;
         q->t_submit = Clock::now();
         s->reqs[q->id] = q;
-        s->pending.push_back(q);
+        if (q->parts.empty()) {
+            s->pending.push_back(q);
+        } else {
+            for (Request *p : q->parts) {
+                p->t_submit = q->t_submit;
+                s->pending.push_back(p);
+            }
+        }
         s->stats.requests++;
         s->stats.reads += n_reads;
         *id = q->id;
     }
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();
     return GIS_OK;
 }
 
@@ -348,28 +512,34 @@ extern "C" int gis_server_wait(gis_server *s, uint64_t id, int timeout_ms, gis_n
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return GIS_ERR_BAD_PARAMETER;
     Request *q = it->second;
+    if (q->claimed) return GIS_ERR_STATE;  // another caller is waiting for it
+    q->claimed = true;
+    s->n_waiters++;
     auto ready = [&] { return q->done; };
+    bool ok = true;
     if (timeout_ms < 0) s->cv_done.wait(lk, ready);
-    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) return GIS_ERR_STATE;
-    if (q->status != GIS_OK) {
-        const int st = q->status;
-        s->reqs.erase(it);
+    else ok = s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+    int rc = GIS_ERR_STATE;
+    if (ok) {
+        // done and claimed: nothing else touches the request now
+        lk.unlock();
+        rc = copy_out(q, out);
+        if (latency_ms) {
+            Clock::time_point t_done = q->t_done;
+            for (const Request *p : q->parts) t_done = std::max(t_done, p->t_done);
+            *latency_ms = std::chrono::duration<double, std::milli>(t_done - q->t_submit).count();
+        }
+        lk.lock();
+    }
+    if (ok && rc != GIS_ERR_CAPACITY) {
+        s->reqs.erase(id);
         delete q;
-        return st;
+    } else {
+        q->claimed = false;  // timed out, or the caller retries with more room
     }
-    const uint32_t n = (uint32_t)q->lens.size();
-    if (out) {
-        out->n_nams = q->nams.size();
-        if (out->nam_capacity < q->nams.size() || (!q->nams.empty() && !out->nams)) return GIS_ERR_CAPACITY;
-        if (out->nam_offsets) std::memcpy(out->nam_offsets, q->nam_off.data(), 8ull * (n + 1));
-        if (!q->nams.empty()) std::memcpy(out->nams, q->nams.data(), sizeof(gis_nam) * q->nams.size());
-        if (out->nonrepetitive_fraction && n) std::memcpy(out->nonrepetitive_fraction, q->frac.data(), 4ull * n);
-        if (out->flags && n) std::memcpy(out->flags, q->flags.data(), n);
-    }
-    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(q->t_done - q->t_submit).count();
-    s->reqs.erase(it);
-    delete q;
-    return GIS_OK;
+    s->n_waiters--;
+    s->cv_done.notify_all();
+    return rc;
 }
 
 extern "C" int gis_server_replay(gis_server *s, const char *seqs, const uint64_t *offsets,

@@ -177,22 +177,24 @@ int gis_device_count(int *count);
 
 /* StrobemerIndex::populate(f, n_threads), step2/index.hpp:54 (body
  * step2/index.cu:143-311). bits = -1 picks bits from the total length
- * (step2/index.cu:137-141). The host seeds the reference with n_threads
- * threads; the records are sorted, bucketed and counted on the GPU. The
- * index is left resident in HBM. */
+ * (step2/index.cu:137-141). The contigs are copied to HBM and seeded there
+ * (gis_reference_randstrobes); the records are sorted, bucketed and counted
+ * on the GPU, and the index is left resident in HBM. n_threads is the
+ * reference's argument and is not needed: no host thread seeds. Each contig
+ * must be shorter than 2^32 bases, and there are fewer than 2^24 of them. */
 int gis_index_build(gis_ctx *ctx, const char *const *seqs, const uint64_t *lens,
                     uint32_t n_refs, const gis_params *params, int bits, float f,
                     int n_threads);
 /* The unsorted RefRandstrobe records of a reference in (contig, position)
  * order: the reference's RandstrobeGenerator over every contig
  * (step2/randstrobes.cu:161-190, count/assign_randstrobes at
- * step2/index.cu:246-311). Host computation; contigs are cut into segments
- * of `segment` bases (0: 4,000,000) seeded by n_threads threads, with the
- * same records as one sequential pass. n_out receives the record count;
- * GIS_ERR_CAPACITY if it exceeds capacity. */
-int gis_reference_randstrobes(const char *const *seqs, const uint64_t *lens, uint32_t n_refs,
-                              const gis_params *params, int n_threads, uint64_t segment,
-                              gis_ref_randstrobe *out, uint64_t capacity, uint64_t *n_out);
+ * step2/index.cu:246-311), seeded on the context's GPU. A contig shorter
+ * than w_max gets zero records in its slots (the reference counts them but
+ * never assigns them). n_out receives the record count; GIS_ERR_CAPACITY if
+ * it exceeds capacity. */
+int gis_reference_randstrobes(gis_ctx *ctx, const char *const *seqs, const uint64_t *lens, uint32_t n_refs,
+                              const gis_params *params, gis_ref_randstrobe *out, uint64_t capacity,
+                              uint64_t *n_out);
 /* Upload a prebuilt table (e.g. from another builder). The table must be
  * sorted by (hash, position) and starts must have 2^bits + 1 entries. */
 int gis_index_upload(gis_ctx *ctx, const gis_ref_randstrobe *randstrobes, uint64_t n,
@@ -217,6 +219,13 @@ int gis_syncmers_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
 int gis_randstrobes_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
                           const uint32_t *lens, uint32_t n_reads, uint64_t *out_offsets,
                           gis_query_randstrobe *out, uint64_t capacity, uint64_t *n_out);
+/* randstrobes_query(seq, IndexParameters), step2/randstrobes.hpp:107, with
+ * the given parameters and no index (a context with or without one). The
+ * parameters must be as gis_params_make or gis_params_from_read_length make
+ * them (GIS_ERR_BAD_PARAMETER otherwise). */
+int gis_randstrobes_params(gis_ctx *ctx, const gis_params *params, const char *seqs, const uint64_t *offsets,
+                           const uint32_t *lens, uint32_t n_reads, uint64_t *out_offsets,
+                           gis_query_randstrobe *out, uint64_t capacity, uint64_t *n_out);
 /* StrobemerIndex::find over a list of hashes, step2/index.hpp:57-81 (the
  * microbenchmark kernel gpu_find, gpu_test.cu:186-215). positions[i] is
  * GIS_NOT_FOUND when absent. */
@@ -233,7 +242,11 @@ int gis_find_nams_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
                         const uint32_t *lens, uint32_t n_reads, const gis_map_options *opt,
                         gis_nam_out *out);
 /* Per-query forms over given QueryRandstrobes (one read's worth or many):
- * query_offsets[n_queries + 1] partitions qrs into reads. */
+ * query_offsets[n_queries + 1] partitions qrs into reads. The reference
+ * calls find_nams once per read from many threads (step2/nam.cu:553-562);
+ * calls made at the same time on one context are combined: the first caller
+ * to find the context free runs every waiting call with the same rescue
+ * settings as one batch and each caller gets its own reads' results. */
 int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
                       const uint64_t *query_offsets, uint32_t n_queries, int rescue,
                       uint32_t rescue_cutoff, uint32_t hard_cutoff, gis_nam_out *out);
@@ -318,42 +331,51 @@ int gis_host_alloc(uint64_t bytes, void **p);
 void gis_host_free(void *p);
 
 /* ---------------- serving (no reference counterpart) ---------------- */
-/* A request loop on one context: requests (sets of reads) are queued by the
- * caller; one thread per slot admits them while the estimated device bytes
- * of the steps in flight stay within hbm_budget, coalesces the queued
+/* A request loop on one context: requests (sets of reads of any number) are
+ * queued by the caller; one thread per slot admits them while the device
+ * bytes charged to the slots stay within hbm_budget, coalesces the queued
  * requests into one launch sequence (a step, replayed from the slot's
  * captured hipGraph), keeps up to `slots` steps in flight on separate
- * streams, and completes each request with its per-read NAMs.
+ * streams, and completes each request with its per-read NAMs. A running
+ * step is charged the larger of its estimate and what its slot's arrays
+ * held before it; an idle slot what its arrays still hold (up to
+ * hbm_budget / slots; idle slots free theirs when a step would not fit
+ * otherwise). A request of more than max_step_reads reads, or larger than
+ * the budget, is cut into parts that run as steps of their own and are
+ * joined again for gis_server_wait.
  * Requests arriving while steps run join the next step on a free slot. A
- * request's NAMs are the same alone and batched with others. The context's
- * index must stay loaded while the server exists. */
+ * request's NAMs are the same alone, batched with others and cut into
+ * parts. The context's index must stay loaded while the server exists. */
 typedef struct {
     uint32_t slots;            /* steps in flight, each on its own stream (1..16) */
-    uint32_t max_step_reads;   /* reads coalesced into one step (a larger request runs alone) */
-    uint64_t hbm_budget;       /* estimated device bytes of the steps in flight (0: 32 GiB) */
+    uint32_t max_step_reads;   /* reads coalesced into one step (a larger request is cut into steps) */
+    uint64_t hbm_budget;       /* device bytes charged to the slots (0: 32 GiB) */
     gis_map_options options;   /* mode, rescue level, hard cutoff; graph replays unless GIS_RUN_NO_GRAPH */
 } gis_server_config;
 
 typedef struct {
     uint64_t requests, reads, steps;
     double busy_s;             /* time with at least one step in flight */
-    uint64_t peak_bytes;       /* largest estimated device bytes in flight */
+    uint64_t peak_bytes;       /* largest device bytes charged to the slots */
 } gis_server_stats;
 
 typedef struct gis_server gis_server;
 /* cfg may be NULL: 6 slots, 131,072 reads per step, 32 GiB, default options */
 int gis_server_create(gis_ctx *ctx, const gis_server_config *cfg, gis_server **out);
-/* Stops the worker; requests still queued are cancelled. No call on the
- * server may be in progress. */
+/* Stops the workers; requests still queued are cancelled (their waiters get
+ * GIS_ERR_CANCELLED). Callers blocked in gis_server_wait return before the
+ * server is freed; no other call on the server may be in progress or
+ * follow. */
 void gis_server_destroy(gis_server *s);
 /* Queue a request (the reads are copied) and return its id. GIS_ERR_BUDGET
- * if the request alone exceeds the HBM budget. */
+ * only if a single read's step would exceed the HBM budget. */
 int gis_server_submit(gis_server *s, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
                       uint32_t n_reads, uint64_t *id);
-/* Wait for request id (timeout_ms < 0: no limit; GIS_ERR_STATE on timeout),
- * copy its results (out may be NULL; GIS_ERR_CAPACITY with n_nams set if
+/* Wait for request id (timeout_ms < 0: no limit; GIS_ERR_STATE on timeout,
+ * or at once when another caller is waiting for the same request), copy its
+ * results (out may be NULL; GIS_ERR_CAPACITY with n_nams set if
  * nam_capacity is too small, the request stays) and release it. latency_ms:
- * submission to completion. */
+ * submission to completion of its last step. */
 int gis_server_wait(gis_server *s, uint64_t id, int timeout_ms, gis_nam_out *out, double *latency_ms);
 /* Replay a timed trace: request i (reads first[i] .. first[i] + count[i] - 1
  * of the given pool) is submitted arrival_s[i] seconds after the start by a

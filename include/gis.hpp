@@ -291,6 +291,24 @@ inline QueryRandstrobeVector randstrobes_query(std::string_view seq, const Strob
     return out;
 }
 
+// randstrobes_query(seq, parameters) (step2/randstrobes.hpp:107) as the
+// reference declares it: no index needed. Each calling thread seeds on a
+// context of its own (device 0), so threads do not wait for each other.
+inline QueryRandstrobeVector randstrobes_query(std::string_view seq, const IndexParameters &parameters) {
+    thread_local Context ctx(0);
+    const uint64_t off = 0;
+    const uint32_t len = (uint32_t)seq.size();
+    const int64_t n_k = (int64_t)seq.size() - parameters.k() + 1;
+    const uint64_t cap = n_k > 0 ? 2 * (uint64_t)n_k : 1;
+    QueryRandstrobeVector out(cap);
+    uint64_t offs[2];
+    uint64_t n_out = 0;
+    check(gis_randstrobes_params(ctx.get(), &parameters.p, seq.data(), &off, &len, 1, offs, out.data(), cap, &n_out),
+          ctx.get());
+    out.resize(n_out);
+    return out;
+}
+
 namespace detail {
 inline std::vector<Nam> qrs_call(const QueryRandstrobeVector &q, const StrobemerIndex &index,
                                  int rescue, unsigned rescue_cutoff, unsigned hard_cutoff,

@@ -4,16 +4,22 @@
 // and the same reads again through one device-resident Batch, then (for an
 // even read count, read p and n + p as the mates of pair p) the pair step.
 //
+// The per-read path runs again from 8 threads at once (the reference's
+// re-entrant find_nams on N workers), and randstrobes_query with bare
+// IndexParameters must equal randstrobes_query with the index.
+//
 // usage: facade_check <refs.fa> <reads.txt> <out_prefix>
-// writes <out_prefix>.single and <out_prefix>.batch, each:
+// writes <out_prefix>.single, <out_prefix>.threads and <out_prefix>.batch, each:
 //   u64 n_reads, u64 offsets[n_reads + 1], gis_nam nams[offsets[n_reads]]
 // and, for an even read count, <out_prefix>.pairs (the same layout, lists in
 // the reference order) and <out_prefix>.types:
 //   u64 n_pairs, u8 types[n_pairs], u64 n_joint, gis_nam_pair joint[n_joint]
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gis.hpp"
@@ -61,6 +67,46 @@ int main(int argc, char **argv) {
         }
         write_out(std::string(argv[3]) + ".single", offs, all);
 
+        stage = "randstrobes_query with IndexParameters";
+        for (const auto &r : reads) {
+            const auto a = gis::randstrobes_query(r, index);
+            const auto b = gis::randstrobes_query(r, params);
+            if (a.size() != b.size() ||
+                (!a.empty() && std::memcmp(a.data(), b.data(), a.size() * sizeof(gis::QueryRandstrobe)) != 0)) {
+                std::cerr << "randstrobes_query(seq, IndexParameters) differs from randstrobes_query(seq, index)\n";
+                return 3;
+            }
+        }
+
+        stage = "per-read find_nams on 8 threads";
+        std::vector<std::vector<gis::Nam>> per(reads.size());
+        std::vector<std::thread> workers;
+        std::vector<std::string> errors(8);
+        for (int t = 0; t < 8; t++) {
+            workers.emplace_back([&, t] {
+                try {
+                    for (size_t i = t; i < reads.size(); i += 8) {
+                        const auto q = gis::randstrobes_query(reads[i], params);
+                        auto [frac, nams] = gis::find_nams(q, index);
+                        if ((double)frac < 0.7 || nams.empty()) nams = gis::find_nams_rescue(q, index, rescue_cutoff);
+                        per[i] = std::move(nams);
+                    }
+                } catch (const gis::Error &e) {
+                    errors[t] = e.what();
+                }
+            });
+        }
+        for (auto &w : workers) w.join();
+        for (const auto &e : errors)
+            if (!e.empty()) throw std::runtime_error("worker: " + e);
+        std::vector<uint64_t> toffs{0};
+        std::vector<gis::Nam> tall;
+        for (const auto &v : per) {
+            tall.insert(tall.end(), v.begin(), v.end());
+            toffs.push_back(tall.size());
+        }
+        write_out(std::string(argv[3]) + ".threads", toffs, tall);
+
         uint64_t bases = 0;
         std::vector<std::string_view> views;
         for (const auto &r : reads) {
@@ -90,6 +136,9 @@ int main(int argc, char **argv) {
         std::cout << "facade_check: " << reads.size() << " reads, " << all.size() << " NAMs\n";
     } catch (const gis::Error &e) {
         std::cerr << "gis error " << e.code() << " in " << stage << ": " << e.what() << "\n";
+        return 1;
+    } catch (const std::exception &e) {
+        std::cerr << "error in " << stage << ": " << e.what() << "\n";
         return 1;
     }
     return 0;

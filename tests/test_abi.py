@@ -98,48 +98,3 @@ def test_cxx_facade_compiles():
     assert os.path.exists(exe), "build() compiles tests/cxx/facade_check.cpp"
     out = subprocess.run(["ldd", exe], capture_output=True, text=True, check=True).stdout
     assert "libgis.so" in out and "not found" not in out
-
-
-def _reference_randstrobes(seqs, p, segment, threads=4):
-    bufs = [C.create_string_buffer(s, len(s)) for s in seqs]
-    arr = (C.c_void_p * len(bufs))(*[C.cast(b, C.c_void_p) for b in bufs])
-    lens = np.array([len(s) for s in seqs], dtype=np.uint64)
-    n = C.c_uint64()
-    L = N.lib()
-    assert L.gis_reference_randstrobes(arr, N.ptr(lens), len(seqs), C.byref(p.c), threads, segment,
-                                       None, 0, C.byref(n)) in (0, 5)
-    out = np.empty(max(1, n.value), dtype=G.REF_RANDSTROBE)
-    assert L.gis_reference_randstrobes(arr, N.ptr(lens), len(seqs), C.byref(p.c), threads, segment,
-                                       N.ptr(out), n.value, C.byref(n)) == 0
-    return out[: n.value]
-
-
-@pytest.mark.parametrize("segment", [0, 997, 4096, 30011])
-def test_reference_randstrobes_segments_match_oracle(segment):
-    """Seeding a reference in parallel segments gives the oracle's records:
-    every segment's syncmer scan starts early enough to be exact, also inside
-    tandem repeats (tied windows) and around N runs."""
-    from tests.helpers import rand_seq
-    rng = np.random.default_rng(31)
-    g, lens = G.synth_genome(200_000, n_contigs=2, gc=0.41, repeat_fraction=0.05, n_families=5,
-                             family_len=(300, 2000), copies=(2, 10), tandem_fraction=0.08,
-                             tandem_unit=(1, 6), seed=12)
-    gb = g.tobytes()
-    seqs = [gb[: int(lens[0])], gb[int(lens[0]):]]
-    # N runs, a homopolymer stretch, lower case and contigs shorter than w_max
-    s0 = bytearray(seqs[0])
-    s0[5000:5040] = b"N" * 40
-    s0[20000:23000] = b"A" * 3000
-    s0[40000:40100] = bytes(s0[40000:40100]).lower()
-    seqs[0] = bytes(s0)
-    seqs += [rand_seq(rng, 30), rand_seq(rng, 5), rand_seq(rng, 2500)]
-    p = G.IndexParameters.from_read_length(150)
-    got = _reference_randstrobes(seqs, p, segment)
-    op = O.params_from_read_length(150)
-    oidx = O.Index.build(seqs, op, n_threads=2)
-    order = np.lexsort((got["packed"], got["position"], got["hash"]))
-    assert got[order].tobytes() == oidx.randstrobes.tobytes()
-    # (contig, position) order as produced
-    key = (got["packed"].astype(np.uint64) >> np.uint64(8)) * np.uint64(2 ** 32) + got["position"]
-    nz = key[got["hash"] != 0]
-    assert np.all(np.diff(nz.astype(np.int64)) >= 0)

@@ -60,6 +60,64 @@ def test_index_build_matches_oracle(setup):
     assert info.distinct_strobemers == oidx.distinct_strobemers
 
 
+def _adversarial_reference():
+    """Contigs with N runs, a 3 kbp homopolymer, lower case, short tandem
+    repeats (tied syncmer windows) and contigs shorter than w_max."""
+    from tests.helpers import rand_seq
+    rng = np.random.default_rng(31)
+    g, lens = G.synth_genome(200_000, n_contigs=2, gc=0.41, repeat_fraction=0.05, n_families=5,
+                             family_len=(300, 2000), copies=(2, 10), tandem_fraction=0.08,
+                             tandem_unit=(1, 6), seed=12)
+    gb = g.tobytes()
+    seqs = [gb[: int(lens[0])], gb[int(lens[0]):]]
+    s0 = bytearray(seqs[0])
+    s0[5000:5040] = b"N" * 40
+    s0[20000:23000] = b"A" * 3000
+    s0[40000:40100] = bytes(s0[40000:40100]).lower()
+    s0[60000:60001] = b"N"
+    seqs[0] = bytes(s0)
+    seqs += [rand_seq(rng, 30), rand_seq(rng, 5), rand_seq(rng, 2500), b"ACGTN" * 3, b"AC" * 600]
+    return seqs
+
+
+def test_device_seeding_matches_oracle(ctx):
+    """The reference seeded on the device (gis_reference_randstrobes, the
+    seeding gis_index_build runs) gives the oracle's records, in (contig,
+    position) order, also inside tandem repeats and homopolymers (tied
+    windows replayed from their anchors), around N runs and for contigs
+    shorter than w_max; an index built from the same contigs equals the
+    oracle's index."""
+    import ctypes as C
+    from gpu_index_test_amd import _native as N
+    seqs = _adversarial_reference()
+    p = G.IndexParameters.from_read_length(150)
+    bufs = [C.create_string_buffer(s, len(s)) for s in seqs]
+    arr = (C.c_void_p * len(bufs))(*[C.cast(b, C.c_void_p) for b in bufs])
+    lens = np.array([len(s) for s in seqs], dtype=np.uint64)
+    n = C.c_uint64()
+    L = N.lib()
+    assert L.gis_reference_randstrobes(ctx.h, arr, N.ptr(lens), len(seqs), C.byref(p.c), None, 0,
+                                       C.byref(n)) == 5
+    got = np.empty(max(1, n.value), dtype=G.REF_RANDSTROBE)
+    ctx.check(L.gis_reference_randstrobes(ctx.h, arr, N.ptr(lens), len(seqs), C.byref(p.c), N.ptr(got),
+                                          n.value, C.byref(n)))
+    got = got[: n.value]
+    op = O.params_from_read_length(150)
+    oidx = O.Index.build(seqs, op, n_threads=2)
+    order = np.lexsort((got["packed"], got["position"], got["hash"]))
+    assert got[order].tobytes() == oidx.randstrobes.tobytes()
+    key = (got["packed"].astype(np.uint64) >> np.uint64(8)) * np.uint64(2 ** 32) + got["position"]
+    nz = key[got["hash"] != 0]
+    assert np.all(np.diff(nz.astype(np.int64)) > 0)
+    idx = G.StrobemerIndex(G.References(seqs), p)
+    idx.populate()
+    assert idx.randstrobes.tobytes() == oidx.randstrobes.tobytes()
+    assert np.array_equal(idx.randstrobe_start_indices, oidx.starts)
+    assert idx.filter_cutoff == oidx.filter_cutoff
+    oidx.free()
+    idx.ctx.close()
+
+
 def test_syncmers_parity(setup):
     g, lens, p, idx, op, oidx = setup
     reads = _read_set(g, lens, 200)
@@ -82,6 +140,13 @@ def test_randstrobes_parity(setup):
         assert len(got) == len(exp), (r, len(seq))
         for f in ("hash", "start", "end", "is_reverse"):
             assert np.array_equal(got[f], exp[f]), (r, f, len(seq))
+    # randstrobes_query(seq, IndexParameters): no index involved
+    for seq in reads[::7]:
+        exp = O.randstrobes_query(seq, op)
+        got = G.randstrobes_query(seq, p)
+        assert len(got) == len(exp)
+        for f in ("hash", "start", "end", "is_reverse"):
+            assert np.array_equal(got[f], exp[f]), (f, len(seq))
 
 
 def test_lookup_parity(setup):
@@ -497,7 +562,7 @@ def test_cxx_facade_matches_oracle(setup, tmp_path):
     subprocess.run([exe, str(fa), str(rd), out], check=True, timeout=120)
     s, o, l = G.pack_reads(reads)
     eo, en, ef, efl = O.map_batch(oidx, s, o, l, 2 * oidx.filter_cutoff, 1000, 4)
-    for suffix in ("single", "batch"):
+    for suffix in ("single", "threads", "batch"):
         raw = open(out + "." + suffix, "rb").read()
         n = int(np.frombuffer(raw[:8], dtype=np.uint64)[0])
         offs = np.frombuffer(raw[8: 8 + 8 * (n + 1)], dtype=np.uint64)

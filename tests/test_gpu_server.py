@@ -91,14 +91,68 @@ def test_replay_trace(served):
         srv.close()
 
 
-def test_budget_rejects_oversized_request(served):
+def test_oversized_request_is_split(served):
+    """A request of more reads than a step takes runs as several steps and
+    comes back whole, equal to the oracle; so does one larger than the HBM
+    budget. Only a read whose own step exceeds the budget is refused."""
     idx, (seqs, offs, ls), exp, first, sizes = served
+    n = len(ls)
+    srv = G.Server(idx, slots=3, max_step_reads=64)
+    try:
+        res, _ = srv.wait(srv.submit(seqs, offs, ls))
+        _check(res, _request_expected(exp, 0, n))
+        assert srv.stats()["steps"] >= n // 64
+    finally:
+        srv.close()
     srv = G.Server(idx, slots=2, hbm_budget=1 << 20)
     try:
+        res, _ = srv.wait(srv.submit(seqs, offs, ls))
+        _check(res, _request_expected(exp, 0, n))
+    finally:
+        srv.close()
+    srv = G.Server(idx, slots=2, hbm_budget=8 << 10)
+    try:
         with pytest.raises(G.GisError) as e:
-            srv.submit(seqs, offs[:1000], ls[:1000])
+            srv.submit(seqs, offs[:3], ls[:3])
         assert e.value.code == 8
-        res, _ = srv.wait(srv.submit(seqs, offs[:3], ls[:3]))
-        _check(res, _request_expected(exp, 0, 3))
+    finally:
+        srv.close()
+
+
+def test_small_budget_steps_wait(served):
+    """A budget below slots x step size: steps wait for budget (and idle
+    slots give their arrays back) instead of failing; every request equals
+    the oracle and the charged bytes never exceed the budget."""
+    idx, (seqs, offs, ls), exp, first, sizes = served
+    budget = 24 << 20
+    srv = G.Server(idx, slots=4, max_step_reads=1024, hbm_budget=budget)
+    try:
+        reqs = [(f, k, srv.submit(seqs, offs[f: f + k], ls[f: f + k])) for f, k in zip(first, sizes)]
+        for f, k, h in reqs:
+            res, _ = srv.wait(h)
+            _check(res, _request_expected(exp, f, k))
+        assert srv.stats()["peak_bytes"] <= budget
+    finally:
+        srv.close()
+
+
+def test_one_waiter_per_request(served):
+    """A second caller waiting for a request that already has a waiter is
+    refused (or finds it gone); the first gets the results."""
+    import threading
+    idx, (seqs, offs, ls), exp, first, sizes = served
+    srv = G.Server(idx, slots=2, max_step_reads=4096)
+    try:
+        h = srv.submit(seqs, offs, ls)
+        got = {}
+        t = threading.Thread(target=lambda: got.setdefault("r", srv.wait(h)))
+        t.start()
+        import time
+        time.sleep(0.05)
+        with pytest.raises(G.GisError) as e:
+            srv.wait(h, timeout_ms=0)
+        assert e.value.code in (1, 6)
+        t.join()
+        _check(got["r"][0], _request_expected(exp, 0, len(ls)))
     finally:
         srv.close()

@@ -412,6 +412,29 @@ def test_hit_outer_and_nam_outer_merges_agree():
             assert a.tobytes() == b.tobytes(), trial
 
 
+def test_merge_against_spec_model():
+    """The oracle's hit-outer merge against the model written from the
+    reference's merge rules (tests/spec.py), over random hit sets with
+    crowded diagonals (extend, contain and reopen all occur) and flushes."""
+    rng = np.random.default_rng(13)
+    multi = 0
+    for trial in range(400):
+        refs, hits = _random_hits(rng, int(rng.integers(1, 90)), n_refs=int(rng.integers(1, 4)))
+        perm = rng.permutation(len(hits))
+        for is_rc in (0, 1):
+            got = O.canonicalize(O.merge_hits(refs[perm], hits[perm], 20, is_rc, 0))
+            exp = spec.canonical_nams(spec.merge_hits_into_nams(refs, hits, 20, is_rc))
+            assert len(got) == len(exp), trial
+            for g, e in zip(got, exp):
+                gt = (int(g["nam_id"]), int(g["query_start"]), int(g["query_end"]),
+                      int(g["query_prev_hit_startpos"]), int(g["ref_start"]), int(g["ref_end"]),
+                      int(g["ref_prev_hit_startpos"]), int(g["n_hits"]), int(g["ref_id"]),
+                      np.float32(g["score"]), int(g["is_rc"]))
+                assert gt == e, (trial, gt, e)
+                multi += int(g["n_hits"]) > 1
+    assert multi > 100, "the hit sets must make multi-hit NAMs"
+
+
 def test_merge_worked_example():
     # three chained hits on one diagonal, one contained hit, one unrelated
     hits = np.array([(0, 30, 100, 130), (10, 40, 110, 140), (20, 50, 120, 150),
