@@ -168,6 +168,7 @@ PROTOTYPES = {
                                        C.POINTER(NamOut), _P]),
     "gis_host_alloc": (C.c_int, [_U64, C.POINTER(_P)]),
     "gis_host_free": (None, [_P]),
+    "gis_lookup_batch_noalloc": (C.c_int, [_P, _P, _U64, _P]),
     "gis_randstrobes_params": (C.c_int, [_P, C.POINTER(Params), _P, _P, _P, _U32, _P, _P, _U64, C.POINTER(_U64)]),
     "gis_batch_nam_count": (C.c_int, [_P, C.POINTER(_U64)]),
     "gis_fused_config_get": (C.c_int, [_P, C.POINTER(FusedConfig)]),

@@ -80,6 +80,9 @@ extern "C" int gis_ctx_create(int device, gis_ctx **out) {
     if (hipGetDeviceCount(&c) != hipSuccess || c <= 0) return GIS_ERR_HIP;
     if (device < 0 || device >= c) return GIS_ERR_BAD_PARAMETER;
     if (hipSetDevice(device) != hipSuccess) return GIS_ERR_HIP;
+    // the allocation-free lookup's array address on this device; if it
+    // cannot be had, that entry point reports it and the rest works on
+    (void)noalloc_setup(device);
     gis_ctx *ctx = new gis_ctx();
     ctx->device = device;
     hipDeviceProp_t prop;

@@ -130,6 +130,9 @@ inline DevIndex dev_index(const gis_ctx *ctx) {
 
 // the batch machinery (batch.cpp)
 void batch_free(gis_batch *b);
+// the allocation-free lookup's device array on `device`, the current device
+// (noalloc.hip); once per device
+int noalloc_setup(int device);
 
 }  // namespace gis
 

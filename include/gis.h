@@ -219,6 +219,16 @@ int gis_syncmers_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
 int gis_randstrobes_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
                           const uint32_t *lens, uint32_t n_reads, uint64_t *out_offsets,
                           gis_query_randstrobe *out, uint64_t capacity, uint64_t *n_out);
+/* gis_lookup_batch without allocating: no reference counterpart. For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The copies of the hashes
+ * and positions go through one static device array of 2 x
+ * GIS_NOALLOC_LOOKUP_MAX u64 (16 MiB) that each GPU holds until the process
+ * ends; more than GIS_NOALLOC_LOOKUP_MAX hashes is GIS_ERR_CAPACITY. Calls
+ * on one context are serialised; calls on two contexts of the same GPU at
+ * the same time race on the array. */
+#define GIS_NOALLOC_LOOKUP_MAX (1u << 20)
+int gis_lookup_batch_noalloc(gis_ctx *ctx, const uint64_t *hashes, uint64_t n, uint64_t *positions);
 /* randstrobes_query(seq, IndexParameters), step2/randstrobes.hpp:107, with
  * the given parameters and no index (a context with or without one). The
  * parameters must be as gis_params_make or gis_params_from_read_length make
