@@ -280,10 +280,11 @@ def form_json(t, steps, **extra):
 def kernel_times(G, mappers, k, mode):
     """The same runs enqueued launch by launch, with HIP events between the
     kernels on the batch's stream (graph replays carry no events); with each
-    run, the algorithmic bytes k_lookup counted on the device."""
+    run, the algorithmic bytes k_lookup counted on the device. Full-size
+    batches only (the last batch of a workload may be smaller)."""
     out = []
-    for i in range(min(k, len(mappers))):
-        m = mappers[i]
+    full = max(m.n for m in mappers)
+    for m in [m for m in mappers if m.n == full][:k]:
         m.run(mode=mode, graph=False)
         st = m.stage_times()
         if mode == G.MODE_PLAIN:

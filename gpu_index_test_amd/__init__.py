@@ -483,12 +483,14 @@ def pinned_copy(a):
 
 
 def find_nams_stream(index, seqs, offsets, lens, batch_reads=65536, lanes=2, rescue_level=2, hard_cutoff=1000,
-                     mode=N.MODE_PLAIN, graph=True, out=None):
+                     mode=N.MODE_PLAIN, graph=True, out=None, pinned=True):
     """Host-to-host mapping of many reads (gis_find_nams_stream): batches of
     batch_reads, up to `lanes` in flight, each batch's reads copied to HBM,
     mapped and its NAM arrays copied back. out: the MapResult of an earlier
-    call over as many reads, whose pinned arrays are reused; else fresh pinned
-    arrays are allocated. -> (MapResult, per-batch device latency in ms)."""
+    call over as many reads, whose arrays are reused; else fresh arrays are
+    allocated, pinned unless pinned=False. -> (MapResult, per-batch device
+    latency in ms)."""
+    alloc = pinned_empty if pinned else (lambda n, dt: np.empty(max(int(n), 1), dtype=dt))
     index._check_owner()
     ctx = index.ctx
     seqs, offsets, lens = _seq_args(seqs, offsets, lens)
@@ -502,8 +504,8 @@ def find_nams_stream(index, seqs, offsets, lens, batch_reads=65536, lanes=2, res
         if full is not None:
             offs, nams, frac, flags = full
         else:
-            offs, nams = pinned_empty(n + 1, np.uint64), pinned_empty(cap, NAM)
-            frac, flags = pinned_empty(max(n, 1), np.float32), pinned_empty(max(n, 1), np.uint8)
+            offs, nams = alloc(n + 1, np.uint64), alloc(cap, NAM)
+            frac, flags = alloc(max(n, 1), np.float32), alloc(max(n, 1), np.uint8)
         o = N.NamOut()
         o.nam_offsets, o.nams, o.nam_capacity = offs.ctypes.data, nams.ctypes.data, cap
         o.nonrepetitive_fraction, o.flags = frac.ctypes.data, flags.ctypes.data

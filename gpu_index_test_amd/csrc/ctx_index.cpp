@@ -252,6 +252,19 @@ extern "C" int gis_index_read_sti(gis_ctx *ctx, const char *path, const gis_para
         for (uint64_t o = 0, tot = m.n * sizeof(gis_ref_randstrobe); o < tot; o += chunk)
             HIP_CHECK(hipMemcpy(dst + o, src + o, std::min(chunk, tot - o), hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(ix.starts.p, m.starts, ((1ull << m.bits) + 1) * 8, hipMemcpyHostToDevice));
+        // the file's table must be one the lookups can trust
+        const char *why = nullptr;
+        try {
+            why = device_validate_index(static_cast<const uint4 *>(ix.rs.p), m.n,
+                                        static_cast<const uint64_t *>(ix.starts.p), m.bits, ctx->stream);
+        } catch (const BuildError &e) {
+            throw HipError{e.e, e.what};
+        }
+        if (why) {
+            drop_table(ix);
+            unmap_sti(m);
+            return fail(ctx, GIS_ERR_INVALID_INDEX, std::string("Index file has incorrect format: ") + why);
+        }
         ix.n = m.n;
         ix.bits = m.bits;
         ix.filter_cutoff = m.filter_cutoff;

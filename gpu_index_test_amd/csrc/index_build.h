@@ -25,6 +25,10 @@ void device_seed_reference(const char *const *seqs, const uint64_t *lens, uint32
 // sort n RefRandstrobe records (16 B: u64 hash, u32 position, u32 packed) in
 // place by (hash, position, packed); allocates and frees its own buffers
 void device_sort_records(uint4 *rs, uint64_t n, hipStream_t st);
+// checks a resident table and its starts: null if the lookups can trust
+// them, else what is wrong (unsorted records, or bucket starts that do not
+// match the records' top hash bits)
+const char *device_validate_index(const uint4 *rs, uint64_t n, const uint64_t *starts, int bits, hipStream_t st);
 // bucket starts (2^bits + 1 entries) and the run-length histogram of a
 // sorted table; hist_out receives RUN_BINS host counters
 void device_finish_index(const uint4 *rs, uint64_t n, int bits, uint64_t *starts, uint64_t *hist_out,

@@ -126,6 +126,50 @@ void device_sort_records(uint4 *rs, uint64_t n, hipStream_t st) {
     check(hipFree(buf), "hipFree");
 }
 
+namespace {
+
+// a table the lookups can trust: sorted by (hash, position), and every
+// record inside [starts[b], starts[b + 1]) carries b in its top hash bits
+__global__ void k_validate(const uint4 *__restrict__ rs, uint64_t n, const uint64_t *__restrict__ starts, int bits,
+                           unsigned *__restrict__ bad) {
+    const uint64_t nb = 1ull << bits;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n || i < nb;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i >= 1 && i < n) {
+            const uint4 a = rs[i - 1], c = rs[i];
+            const uint64_t ha = (uint64_t)a.x | ((uint64_t)a.y << 32), hc = (uint64_t)c.x | ((uint64_t)c.y << 32);
+            if (ha > hc || (ha == hc && a.z > c.z)) atomicOr(bad, 1u);
+        }
+        if (i < nb) {
+            const uint64_t s = starts[i], e = starts[i + 1];
+            if (s < e && e <= n &&
+                ((hash_at(rs, s) >> (64 - bits)) != i || (hash_at(rs, e - 1) >> (64 - bits)) != i))
+                atomicOr(bad, 2u);
+        }
+    }
+}
+
+}  // namespace
+
+const char *device_validate_index(const uint4 *rs, uint64_t n, const uint64_t *starts, int bits, hipStream_t st) {
+    unsigned *bad = nullptr;
+    check(hipMalloc(&bad, sizeof(unsigned)), "hipMalloc");
+    unsigned h = 0;
+    hipError_t e = hipMemsetAsync(bad, 0, sizeof(unsigned), st);
+    if (e == hipSuccess) {
+        const uint64_t items = std::max<uint64_t>(n, 1ull << bits);
+        hipLaunchKernelGGL(k_validate, dim3(grid_of(items)), dim3(256), 0, st, rs, n, starts, bits, bad);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, bad, sizeof(unsigned), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(bad);
+    check(e, "index validation");
+    if (h & 1u) return "randstrobes not sorted by (hash, position)";
+    if (h & 2u) return "bucket starts do not match the records' hashes";
+    return nullptr;
+}
+
 void device_finish_index(const uint4 *rs, uint64_t n, int bits, uint64_t *starts, uint64_t *hist_out,
                          hipStream_t st) {
     const uint64_t nb = (1ull << bits) + 1;

@@ -129,7 +129,7 @@ int map_sti(const std::string &path, const gis_params *expected, MappedSti &out,
     if (version != 2)
         return fail("Can only read index file format version 2, but found version " + std::to_string(version));
     uint64_t v64;
-    if (!take(&v64, 8) || at + v64 > size) return fail("truncated index file");
+    if (!take(&v64, 8) || v64 > size - at) return fail("truncated index file");
     at += v64;  // reserved chunk
     int32_t v32[9];
     if (!take(v32, sizeof v32)) return fail("truncated index file");

@@ -415,9 +415,9 @@ def test_stream_host_to_host_matches_oracle(setup, mode, lanes, per):
     assert len(ms) == -(-len(l) // per) and np.all(ms > 0)
     res2, _ = G.find_nams_stream(idx, ps, po, pl, batch_reads=per, lanes=lanes, mode=mode, out=res)
     _compare_nams((res2.nam_offsets, res2.nams), (eo, en), res2.nonrepetitive_fraction, ef, res2.flags, efl)
-    # pageable host memory takes the same path
-    res3, _ = G.find_nams_stream(idx, s, o, l, batch_reads=per, lanes=lanes, mode=mode)
-    assert res3.nams.tobytes() == en.tobytes()
+    # pageable host memory, for the reads and for the NAMs, takes the same path
+    res3, _ = G.find_nams_stream(idx, s, o, l, batch_reads=per, lanes=lanes, mode=mode, pinned=False)
+    _compare_nams((res3.nam_offsets, res3.nams), (eo, en), res3.nonrepetitive_fraction, ef, res3.flags, efl)
 
 
 def test_lookup_bytes_match_oracle(setup):
@@ -788,8 +788,23 @@ def test_sti_rejects_damaged_files(setup, tmp_path):
     bad = bytearray(raw)
     pos = len(bad) - 8 * nb + 8 * (nb // 2)
     bad[pos: pos + 8] = np.uint64(oidx.n + 5).tobytes()
+    # a reserved chunk size that would wrap the read position around
+    wrap = bytearray(raw)
+    wrap[8:16] = np.uint64(2 ** 64 - 8).tobytes()
+    # the table itself: two records out of order; a record whose top hash
+    # bits leave its bucket (the lookups' assumptions, checked on the device)
+    rec0 = 16 + int(np.frombuffer(raw[8:16], "<u8")[0]) + 36 + 8
+    mid = oidx.n // 2
+    swapped = bytearray(raw)
+    a, b = rec0 + 16 * mid, rec0 + 16 * (mid + 1)
+    swapped[a: a + 16], swapped[b: b + 16] = raw[b: b + 16], raw[a: a + 16]
+    moved = bytearray(raw)
+    last = rec0 + 16 * (oidx.n - 1)
+    moved[last: last + 8] = np.uint64(int(oidx.randstrobes["hash"][0])).tobytes()
     other = G.StrobemerIndex(None, p)  # a context of its own
-    for name, data in (("starts.sti", bad), ("short.sti", raw[: len(raw) // 2]), ("magic.sti", b"XX" + raw[2:])):
+    assert oidx.randstrobes["hash"][mid] != oidx.randstrobes["hash"][mid + 1]
+    for name, data in (("starts.sti", bad), ("short.sti", raw[: len(raw) // 2]), ("magic.sti", b"XX" + raw[2:]),
+                       ("wrap.sti", wrap), ("unsorted.sti", swapped), ("bucket.sti", moved)):
         f = tmp_path / name
         f.write_bytes(bytes(data))
         with pytest.raises(G.GisError) as e:
