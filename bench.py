@@ -102,7 +102,7 @@ def host_to_host(G, idx, pin, batch, steps, warmup, mode, lane_choices):
     return {"value": reads / elapsed, "unit": "reads/s", "ms_per_step": elapsed * 1e3 / steps,
             "p50_batch_ms": float(np.percentile(batch_ms, 50)), "p99_batch_ms": float(np.percentile(batch_ms, 99)),
             "lanes": lanes, "lane_sweep": {str(k): v for k, v in sweep.items()}, "reads": reads,
-            "elapsed_s": elapsed}, res, n_pass
+            "elapsed_s": elapsed, "copies": G.stream_copy_stats(idx)}, res, n_pass
 
 
 def parity_leg(O, oidx, seqs, offs, lens, res, n, threads):

@@ -25,7 +25,7 @@ __all__ = [
     "pack_reads", "GisError", "NAM", "QUERY_RANDSTROBE", "REF_RANDSTROBE", "SYNCMER", "HIT",
     "GIS_NOT_FOUND", "FLAG_RESCUED", "FLAG_NO_RANDSTROBES", "MODE_PLAIN", "MODE_FUSED",
     "PairResult", "pairs_from_nams", "NAM_PAIR", "FastqReader", "FastqChunk",
-    "find_nams_stream", "pinned_empty", "pinned_copy",
+    "find_nams_stream", "stream_copy_stats", "pinned_empty", "pinned_copy",
 ]
 
 
@@ -488,8 +488,8 @@ def find_nams_stream(index, seqs, offsets, lens, batch_reads=65536, lanes=2, res
     batch_reads, up to `lanes` in flight, each batch's reads copied to HBM,
     mapped and its NAM arrays copied back. out: the MapResult of an earlier
     call over as many reads, whose arrays are reused; else fresh arrays are
-    allocated, pinned unless pinned=False. -> (MapResult, per-batch device
-    latency in ms)."""
+    allocated, pinned unless pinned=False. -> (MapResult, per-batch latency
+    in ms from the first copy's issue to the last copy's completion)."""
     alloc = pinned_empty if pinned else (lambda n, dt: np.empty(max(int(n), 1), dtype=dt))
     index._check_owner()
     ctx = index.ctx
@@ -518,6 +518,15 @@ def find_nams_stream(index, seqs, offsets, lens, batch_reads=65536, lanes=2, res
         res = MapResult(offs, nams[: o.n_nams], frac[:n], flags[:n])
         res._full = (offs, nams, frac, flags)
         return res, ms
+
+
+def stream_copy_stats(index):
+    """How find_nams_stream's copies ran on the index's context
+    (gis_stream_copy_stats): the SDMA engine bits per direction (0 = through
+    HIP) and the counts of copies run on them, through HIP, and refused."""
+    st = N.CopyStats()
+    index.ctx.check(index.ctx.L.gis_stream_copy_stats(index.ctx.h, C.byref(st)))
+    return st.as_dict()
 
 
 def hits_batch(index, seqs, offsets, lens, rescue_level=2, hard_cutoff=1000):

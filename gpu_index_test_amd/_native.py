@@ -65,6 +65,14 @@ class NamOut(C.Structure):
                 ("n_nams", C.c_uint64)]
 
 
+class CopyStats(C.Structure):
+    _fields_ = [("h2d_engine", C.c_uint32), ("d2h_engine", C.c_uint32), ("engine_copies", C.c_uint64),
+                ("hip_copies", C.c_uint64), ("refused", C.c_uint64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class MapOptions(C.Structure):
     _fields_ = [("rescue_level", C.c_int32), ("hard_cutoff", C.c_uint32), ("mode", C.c_int32),
                 ("flags", C.c_int32)]
@@ -166,6 +174,7 @@ PROTOTYPES = {
     "gis_batch_lookup_streams": (C.c_int, [_P, C.POINTER(_U64)]),
     "gis_find_nams_stream": (C.c_int, [_P, _P, _P, _P, _U64, _U32, _U32, C.POINTER(MapOptions),
                                        C.POINTER(NamOut), _P]),
+    "gis_stream_copy_stats": (C.c_int, [_P, C.POINTER(CopyStats)]),
     "gis_host_alloc": (C.c_int, [_U64, C.POINTER(_P)]),
     "gis_host_free": (None, [_P]),
     "gis_lookup_batch_noalloc": (C.c_int, [_P, _P, _U64, _P]),

@@ -205,7 +205,10 @@ void ensure_read_arrays(gis_batch *b, uint32_t n) {
     ensure<uint32_t>(b->ws->spill_ids, n);
     ensure<uint4>(b->ws->split_list, n);
     ensure<uint32_t>(b->ws->sel_ids, 3ull * n + 3);
-    if (!b->ws->h_ctl) HIP_CHECK(hipHostMalloc((void **)&b->ws->h_ctl, sizeof(DevCtl)));
+    if (!b->ws->h_ctl) {
+        HIP_CHECK(hipHostMalloc((void **)&b->ws->h_ctl, sizeof(DevCtl)));
+        HIP_CHECK(hipHostGetDevicePointer((void **)&b->ws->h_ctl_dev, b->ws->h_ctl, 0));
+    }
 }
 
 void ensure_slot_arrays(gis_batch *b, uint64_t slots) {
@@ -249,7 +252,7 @@ void mark(gis_batch *b, const char *name) {
 // Stage reads into the batch: randstrobe-slot capacities and offsets come
 // from the lengths, computed here once.
 int batch_stage(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
-                uint32_t n, bool want_sy, uint64_t base, hipEvent_t before) {
+                uint32_t n, bool want_sy, uint64_t base, hipEvent_t before, std::vector<CopyItem> *defer) {
     gis_ctx *ctx = b->ctx;
     if (!ctx->ix.loaded && !b->seed_params) return fail(ctx, GIS_ERR_STATE, "no index loaded");
     if (n > 0 && (!seqs || !offsets || !lens)) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null read arrays");
@@ -301,18 +304,22 @@ int batch_stage(gis_batch *b, const char *seqs, const uint64_t *offsets, const u
     b->tot_rs_cap = b->h_rs_off[n];
     b->span = span;
     ensure<char>(b->seqs, span);
+    std::vector<CopyItem> items;
+    std::vector<CopyItem> &in = defer ? *defer : items;
+    if (span) in.push_back({b->seqs.p, seqs + base, span});
+    if (n) {
+        in.push_back({b->read_off.p, b->h_read_off.data(), 8ull * n});
+        in.push_back({b->read_len.p, lens, 4ull * n});
+    }
+    in.push_back({b->rs_off.p, b->h_rs_off.data(), 8ull * (n + 1)});
+    in.push_back({b->sy_off.p, b->h_sy_off.data(), 8ull * (n + 1)});
+    in.push_back({b->long_off.p, b->h_long_off.data(), 8ull * (n + 1)});
+    if (n) in.push_back({b->queue.p, b->h_queue.data(), 4ull * n});
+    b->ran = false;
+    if (defer) return GIS_OK;
     hipStream_t st = b->stream;
     if (before) HIP_CHECK(hipEventRecord(before, st));
-    if (span) HIP_CHECK(hipMemcpyAsync(b->seqs.p, seqs + base, span, hipMemcpyHostToDevice, st));
-    if (n) {
-        HIP_CHECK(hipMemcpyAsync(b->read_off.p, b->h_read_off.data(), 8ull * n, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(b->read_len.p, lens, 4ull * n, hipMemcpyHostToDevice, st));
-    }
-    HIP_CHECK(hipMemcpyAsync(b->rs_off.p, b->h_rs_off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipMemcpyAsync(b->sy_off.p, b->h_sy_off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipMemcpyAsync(b->long_off.p, b->h_long_off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
-    if (n) HIP_CHECK(hipMemcpyAsync(b->queue.p, b->h_queue.data(), 4ull * n, hipMemcpyHostToDevice, st));
-    b->ran = false;
+    for (const CopyItem &c : items) HIP_CHECK(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyHostToDevice, st));
     return GIS_OK;
 }
 
@@ -384,7 +391,7 @@ void enqueue_run(gis_batch *b, int mode, int rescue_mode, uint32_t rcut, uint32_
         launch_compact(d, fused ? 1 : 0, st);
         mark(b, "output");
     }
-    HIP_CHECK(hipMemcpyAsync(b->ws->h_ctl, d.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, st));
+    launch_ctl_to_host(d.ctl, b->ws->h_ctl_dev, st);
 }
 
 // size the workspace for this batch's reads (grow-only; cheap when it
@@ -850,7 +857,7 @@ static int qrs_run(gis_ctx *ctx, const gis_query_randstrobe *qrs, const uint64_t
         launch_merge(d, set, 1, st);
         launch_scan(d.nam_cnt, d.nam_off, n, tmp, st);
         launch_compact(d, 0, st);
-        HIP_CHECK(hipMemcpyAsync(b->ws->h_ctl, d.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, st));
+        launch_ctl_to_host(d.ctl, b->ws->h_ctl_dev, st);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(st));
         b->ws->tot_nams = b->ws->h_ctl->tot[2];

@@ -88,8 +88,10 @@ struct Workspace {
     const gis_batch *in_flight = nullptr;
     // a stream created for this workspace (destroyed with it), else null
     hipStream_t stream = nullptr;
-    // pinned landing area for the end-of-run readback
+    // pinned landing area for the end-of-run readback, and the device's
+    // address of it (a kernel writes it)
     DevCtl *h_ctl = nullptr;
+    DevCtl *h_ctl_dev = nullptr;
     int refs = 0;
     PairWork *pairs = nullptr;    // the pair step on this workspace's NAMs (pairs.cpp)
 };
@@ -159,14 +161,23 @@ uint64_t batch_device_bytes(gis_batch *b);
 // drains); the next run allocates them again. The workspace must be the
 // batch's own.
 void batch_trim(gis_batch *b);
+// One host-to-device or device-to-host copy.
+struct CopyItem {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+
 // Stage reads into the batch without waiting: the host derives the
 // per-read arrays into the batch's pinned arrays, then the copies to HBM are
 // enqueued on the batch's stream (after `before`, when given, is recorded
-// there). Device offsets are offsets[r] - base and the bases copied are
+// there), or, with `defer`, appended to it for the caller to issue before the
+// run. Device offsets are offsets[r] - base and the bases copied are
 // seqs[base .. max(offsets[r] + lens[r])); seqs and lens are read by the copies
-// as the stream reaches them, so they stay valid until then.
+// as they run, so they stay valid until then.
 int batch_stage(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
-                uint32_t n, bool want_sy, uint64_t base, hipEvent_t before);
+                uint32_t n, bool want_sy, uint64_t base, hipEvent_t before,
+                std::vector<CopyItem> *defer = nullptr);
 // batch_stage with base 0, then wait for the copies
 int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
                  uint32_t n, bool want_sy);

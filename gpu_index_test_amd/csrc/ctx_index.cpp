@@ -107,6 +107,8 @@ extern "C" void gis_ctx_destroy(gis_ctx *ctx) {
         ctx->scratch = nullptr;
         for (gis_batch *b : ctx->lanes) batch_free(b);
         ctx->lanes.clear();
+        dma_release(ctx->dma);
+        ctx->dma = nullptr;
         pairwork_free(ctx->pairs);
         ctx->pairs = nullptr;
         release(ctx->ix.rs);

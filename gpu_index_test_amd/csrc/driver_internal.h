@@ -82,6 +82,7 @@ inline int fail(gis_ctx *ctx, int code, const std::string &msg);
 // device buffers and results of the pair step (pairs.cpp)
 struct PairWork;
 void pairwork_free(PairWork *w);
+struct DmaEngines;
 
 }  // namespace gis
 
@@ -98,6 +99,8 @@ struct gis_ctx {
     // lanes of gis_find_nams_stream: batches with a stream and a workspace
     // each, kept so that later calls reuse their buffers and graphs
     std::vector<gis_batch *> lanes;
+    // the copy engines those lanes drive directly (stream.cpp), found on first use
+    gis::DmaEngines *dma = nullptr;
     // concurrent gis_find_nams_qrs calls waiting to be run together
     std::mutex comb_mu;
     std::condition_variable comb_cv;
@@ -130,6 +133,8 @@ inline DevIndex dev_index(const gis_ctx *ctx) {
 
 // the batch machinery (batch.cpp)
 void batch_free(gis_batch *b);
+// the stream lanes' copy engines (stream.cpp)
+void dma_release(DmaEngines *e);
 // the allocation-free lookup's device array on `device`, the current device
 // (noalloc.hip); once per device
 int noalloc_setup(int device);

@@ -169,6 +169,8 @@ bool launch_fused(const DevBatch &b, const DevIndex &ix, const SeedParams &sp, u
                   uint32_t hard_cutoff, const uint32_t *queue, uint32_t count, int lmax, int wpb,
                   unsigned grid, uint32_t chunk, unsigned int *head, hipStream_t st);
 void launch_compact(const DevBatch &b, int fused, hipStream_t st);
+// the control block to pinned host memory (dst: the device's address of it)
+void launch_ctl_to_host(const DevCtl *src, DevCtl *dst, hipStream_t st);
 void launch_find_flat(const DevIndex &ix, const uint64_t *hashes, uint64_t n, uint64_t *positions,
                       hipStream_t st);
 // exclusive scan of u32 counts into u64 offsets[n+1] (offsets[n] = total);

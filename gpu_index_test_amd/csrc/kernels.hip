@@ -2317,6 +2317,19 @@ void launch_compact(const DevBatch &b, int fused, hipStream_t st) {
                        fused);
 }
 
+// The control block to pinned host memory by one wave's stores, so that the
+// end of a run takes no copy engine (the engines carry the batches' reads and
+// NAMs).
+static_assert(sizeof(DevCtl) % 4 == 0 && sizeof(DevCtl) / 4 <= 64, "one word per lane");
+__global__ void k_ctl_to_host(const DevCtl *__restrict__ src, DevCtl *__restrict__ dst) {
+    const unsigned i = threadIdx.x;
+    if (i < sizeof(DevCtl) / 4) reinterpret_cast<unsigned *>(dst)[i] = reinterpret_cast<const unsigned *>(src)[i];
+}
+
+void launch_ctl_to_host(const DevCtl *src, DevCtl *dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_ctl_to_host, dim3(1), dim3(64), 0, st, src, dst);
+}
+
 void launch_find_flat(const DevIndex &ix, const uint64_t *hashes, uint64_t n, uint64_t *positions,
                       hipStream_t st) {
     if (n == 0) return;

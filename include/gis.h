@@ -326,15 +326,26 @@ int gis_batch_autotune(gis_batch *b, const gis_map_options *opt, gis_fused_confi
  * copies overlap another's kernels. Results land in out as for
  * gis_find_nams_batch, over all n_reads reads (nam_offsets has n_reads + 1
  * entries). GIS_ERR_CAPACITY (n_nams = the required count) if nam_capacity
- * is too small; every batch still runs. Host memory from gis_host_alloc
- * (pinned) lets the copies run asynchronously; other memory works, slower.
- * batch_ms (optional, one entry per batch): the batch's device latency,
- * from the start of its first copy to HBM to the end of its last copy back,
- * measured with events on its stream. Lanes and their buffers stay with the
+ * is too small; every batch still runs. The copies run on two SDMA engines
+ * of the GPU, one per direction, so that one batch's reads going in and
+ * another's NAMs coming out cross PCIe at the same time; host memory from
+ * gis_host_alloc (pinned) is what the engines reach, other memory is copied
+ * through HIP, slower. batch_ms (optional, one entry per batch): the batch's
+ * latency on the host's clock, from the issue of its first copy to HBM to the
+ * completion of its last copy back. Lanes and their buffers stay with the
  * context for later calls. */
 int gis_find_nams_stream(gis_ctx *ctx, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
                          uint64_t n_reads, uint32_t batch_reads, uint32_t lanes, const gis_map_options *opt,
                          gis_nam_out *out, double *batch_ms);
+/* How gis_find_nams_stream's copies ran on this context: the engines used
+ * (hsa_amd_sdma_engine_id_t bits; 0 when copies go through HIP), copies run
+ * on them, copies run through HIP, and copies an engine refused (after which
+ * the context copies through HIP). No reference counterpart. */
+typedef struct gis_copy_stats {
+    uint32_t h2d_engine, d2h_engine;
+    uint64_t engine_copies, hip_copies, refused;
+} gis_copy_stats;
+int gis_stream_copy_stats(gis_ctx *ctx, gis_copy_stats *stats);
 /* Pinned host memory (hipHostMalloc) for the arrays of gis_find_nams_stream
  * and the other host-buffer entry points; freed with gis_host_free. */
 int gis_host_alloc(uint64_t bytes, void **p);

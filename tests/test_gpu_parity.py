@@ -413,11 +413,19 @@ def test_stream_host_to_host_matches_oracle(setup, mode, lanes, per):
     res, ms = G.find_nams_stream(idx, ps, po, pl, batch_reads=per, lanes=lanes, mode=mode)
     _compare_nams((res.nam_offsets, res.nams), (eo, en), res.nonrepetitive_fraction, ef, res.flags, efl)
     assert len(ms) == -(-len(l) // per) and np.all(ms > 0)
+    st0 = G.stream_copy_stats(idx)
     res2, _ = G.find_nams_stream(idx, ps, po, pl, batch_reads=per, lanes=lanes, mode=mode, out=res)
     _compare_nams((res2.nam_offsets, res2.nams), (eo, en), res2.nonrepetitive_fraction, ef, res2.flags, efl)
-    # pageable host memory, for the reads and for the NAMs, takes the same path
+    # pinned memory: every copy ran on the two engines, one per direction
+    st1 = G.stream_copy_stats(idx)
+    assert st1["h2d_engine"] and st1["d2h_engine"] and st1["h2d_engine"] != st1["d2h_engine"], st1
+    assert st1["refused"] == 0 and st1["hip_copies"] == st0["hip_copies"], st1
+    assert st1["engine_copies"] > st0["engine_copies"]
+    # pageable host memory, for the reads and for the NAMs, takes the same
+    # path, its copies through HIP
     res3, _ = G.find_nams_stream(idx, s, o, l, batch_reads=per, lanes=lanes, mode=mode, pinned=False)
     _compare_nams((res3.nam_offsets, res3.nams), (eo, en), res3.nonrepetitive_fraction, ef, res3.flags, efl)
+    assert G.stream_copy_stats(idx)["hip_copies"] > st1["hip_copies"]
 
 
 def test_lookup_bytes_match_oracle(setup):
