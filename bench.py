@@ -73,9 +73,10 @@ def host_to_host(G, idx, pin, batch, steps, warmup, mode, lane_choices):
     """The headline form: K steps (batches) from pinned host memory to NAM
     arrays in pinned host memory. Untimed: a pass that sizes every lane's
     buffers, captures its graphs and allocates the output, then W warm-up
-    passes per lane count, timed once each to pick the count. Timed: K
-    batches at the chosen count, as passes over the pool (a pass covers up
-    to all of it). Returns the timing and the last pass's results."""
+    passes per lane count, then two timed passes each, the faster of which
+    picks the count. Timed: K batches at the chosen count, as passes over
+    the pool (a pass covers up to all of it). Returns the timing and the last
+    pass's results."""
     n_all = len(pin[2])
     nb_all = math.ceil(n_all / batch)
     per_pass = min(steps, nb_all)
@@ -85,8 +86,11 @@ def host_to_host(G, idx, pin, batch, steps, warmup, mode, lane_choices):
     for lanes in lane_choices:
         for _ in range(max(0, warmup)):
             stream_pass(G, idx, pin, n_pass, batch, lanes, mode, res)
-        el, res, _ = stream_pass(G, idx, pin, n_pass, batch, lanes, mode, res)
-        sweep[lanes] = n_pass / el
+        best = math.inf
+        for _ in range(2):
+            el, res, _ = stream_pass(G, idx, pin, n_pass, batch, lanes, mode, res)
+            best = min(best, el)
+        sweep[lanes] = n_pass / best
     lanes = max(sweep, key=sweep.get)
     left, reads, elapsed, batch_ms = steps, 0, 0.0, []
     while left > 0:
@@ -846,7 +850,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0, help="reads per batch (c2 65536, c3 131072, c4 262144)")
     ap.add_argument("--lanes", type=int, default=2,
                     help="c2-c4 resident forms: batches in flight, each lane on its own stream and workspace")
-    ap.add_argument("--stream-lanes", type=lambda x: tuple(int(v) for v in x.split(",")), default=(2, 3, 4),
+    ap.add_argument("--stream-lanes", type=lambda x: tuple(int(v) for v in x.split(",")), default=(3, 4, 5, 6),
                     help="c2-c4 headline: lane counts tried for the host-to-host pipeline (the fastest is timed)")
     ap.add_argument("--genome", type=int, default=0, help="genome bp (c2/c3 100M, c4 3.1G)")
     ap.add_argument("--cpu-reads", type=int, default=1_000_000,
