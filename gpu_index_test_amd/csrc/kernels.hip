@@ -1302,77 +1302,118 @@ __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64
                 }
                 // (the hit was accepted by no open NAM: it opens one on the wide list)
             } else {
-                // wide form: the first accepting entry of the list
-                bool done = false;
+                // Wide form, a block at a time: hit t and the hits after it in
+                // the chunk registers with the same group and query start. A
+                // NAM that takes a hit of the block gets previous start qs and
+                // can take no other, and a NAM opened by one cannot either, so
+                // every hit of the block meets the list as it was before the
+                // block. Hit by hit, each takes the first accepting entry that
+                // has not taken one of the block's hits. Going through the
+                // list's chunks in order and, inside a chunk, through the
+                // block's pending hits in order gives each hit that same entry
+                // (the hits a chunk leaves pending find no acceptor in it, by
+                // induction over the chunks), and a chunk is read once per
+                // block instead of once per hit.
+                const uint64_t sm =
+                    __ballot(lane >= t && lane < m && c_rsd == rsd && (c_q & 0xffffu) == (uint32_t)qs);
+                const uint64_t above = sm >> t;
+                const uint32_t len = ~above ? (uint32_t)__builtin_ctzll(~above) : 64u;
+                uint64_t pend = (len >= 64 ? ~0ull : ((1ull << len) - 1)) << t;
                 const uint32_t nch = (nw + 63) >> 6;
                 const bool boxed = box && nch <= box_cap;
-                for (uint32_t cc = 0; cc < nch && !done; cc += 64) {
-                    // chunks whose bounds admit the hit (all of them without bounds)
-                    uint64_t cm;
+                const int my_rs = (int)c_rs;
+                for (uint32_t c = 0; c < nch && pend; c++) {
+                    // the pending hits the chunk's bounds admit (all of them
+                    // without bounds); the bounds of the chunk's start hold
+                    // for every entry that has not taken a hit of the block
+                    uint64_t cm = pend;
                     if (boxed) {
-                        const uint32_t c = cc + lane;
-                        bool cand = false;
-                        if (c < nch) {
-                            const ChunkBox bx = box[c];
-                            cand = bx.lo < rs && rs <= bx.hi;
-                        }
-                        cm = __ballot(cand);
-                    } else {
-                        cm = nch - cc >= 64 ? ~0ull : ((1ull << (nch - cc)) - 1);
+                        const ChunkBox bx = box[c];
+                        cm = __ballot(((pend >> lane) & 1ull) != 0 && bx.lo < my_rs && my_rs <= bx.hi);
                     }
-                    while (cm && !done) {
-                        const uint32_t c = cc + (uint32_t)__builtin_ctzll(cm);
+                    if (!cm) continue;
+                    const uint32_t e = c * 64 + lane;
+                    const bool valid = e < nw;
+                    WideNam w{};
+                    if (valid) w = wget(wide, e);
+                    bool dirty = false;
+                    while (cm) {
+                        const uint32_t i = (uint32_t)__builtin_ctzll(cm);
                         cm &= cm - 1;
-                        const uint32_t e = c * 64 + lane;
-                        WideNam w{};
-                        bool ext = false, acc = false;
-                        if (e < nw) {
-                            w = wget(wide, e);
-                            acc = nam_accepts(w.qp, w.qe, w.rp, w.re, qs, qe, rs, re, ext);
-                        }
+                        const int hrs = (int)readlane_u32(c_rs, i);
+                        const int hre = (int)readlane_u32(c_re, i);
+                        const int hqe = (int)(readlane_u32(c_q, i) >> 16);
+                        bool ext = false;
+                        const bool acc = valid && nam_accepts(w.qp, w.qe, w.rp, w.re, qs, hqe, hrs, hre, ext);
                         const uint64_t bal = __ballot(acc);
                         if (bal) {
                             if (lane == (uint32_t)__builtin_ctzll(bal)) {
                                 if (ext) {
-                                    w.qe = qe;
-                                    w.re = re;
-                                    if (boxed && box[c].hi < re) box[c].hi = re;
+                                    w.qe = hqe;
+                                    w.re = hre;
+                                    if (boxed && box[c].hi < hre) box[c].hi = hre;
                                 }
                                 w.qp = qs;
-                                w.rp = rs;
+                                w.rp = hrs;
                                 w.n++;
-                                wput(wide, e, w);
+                                dirty = true;
                             }
-                            done = true;
+                            pend &= ~(1ull << i);
                         }
                     }
+                    if (__ballot(dirty)) {
+                        if (dirty) wput(wide, e, w);
+                        wave_sync_mem();
+                    }
                 }
-                if (done) {
+                // the hits no entry took open entries, in hit order
+                while (pend) {
+                    const uint32_t i = (uint32_t)__builtin_ctzll(pend);
+                    pend &= pend - 1;
+                    const int hrs = (int)readlane_u32(c_rs, i);
+                    const int hre = (int)readlane_u32(c_re, i);
+                    const int hqe = (int)(readlane_u32(c_q, i) >> 16);
+                    if (nw >= 2 * wide_mark) {
+                        // retire the dead entries (query_end before this hit), keeping the order
+                        uint32_t kept = 0;
+                        for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
+                            const uint32_t e = c0 + lane;
+                            WideNam w{};
+                            bool keep = false;
+                            if (e < nw) {
+                                w = wget(wide, e);
+                                keep = w.qe >= qs;
+                                if (!keep && base + w.cid < cap)
+                                    lane_nam_store(out, base + w.cid, grp, from_wide(w));
+                            }
+                            const uint64_t kb = __ballot(keep);
+                            wave_sync_mem();
+                            if (keep) wput(wide, kept + mbcnt(kb), w);
+                            kept += (uint32_t)__popcll(kb);
+                            wave_sync_mem();
+                        }
+                        nw = kept;
+                        wide_mark = kept > 64 ? kept : 64;
+                        if (box && ((nw + 63) >> 6) <= box_cap) boxes_rebuild(wide, nw, box);
+                    }
+                    if (lane == 0) {
+                        wput(wide, nw, WideNam{qs, hqe, qs, hrs, hre, hrs, 1u, gcnt});
+                        const uint32_t c = nw >> 6;
+                        if (box && c < box_cap && ((nw + 63) >> 6) <= box_cap) {
+                            if ((nw & 63) == 0) {
+                                box[c] = ChunkBox{hrs, hre};
+                            } else {
+                                if (box[c].lo > hrs) box[c].lo = hrs;
+                                if (box[c].hi < hre) box[c].hi = hre;
+                            }
+                        }
+                    }
+                    nw++;
+                    gcnt++;
                     wave_sync_mem();
-                    continue;
                 }
-                if (nw >= 2 * wide_mark) {
-                    // retire the dead entries (query_end before this hit), keeping the order
-                    uint32_t kept = 0;
-                    for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
-                        const uint32_t e = c0 + lane;
-                        WideNam w{};
-                        bool keep = false;
-                        if (e < nw) {
-                            w = wget(wide, e);
-                            keep = w.qe >= qs;
-                            if (!keep && base + w.cid < cap) lane_nam_store(out, base + w.cid, grp, from_wide(w));
-                        }
-                        const uint64_t kb = __ballot(keep);
-                        wave_sync_mem();
-                        if (keep) wput(wide, kept + mbcnt(kb), w);
-                        kept += (uint32_t)__popcll(kb);
-                        wave_sync_mem();
-                    }
-                    nw = kept;
-                    wide_mark = kept > 64 ? kept : 64;
-                    if (box && ((nw + 63) >> 6) <= box_cap) boxes_rebuild(wide, nw, box);
-                }
+                t += len - 1;
+                continue;
             }
             if (lane == 0) {
                 wput(wide, nw, WideNam{qs, qe, qs, rs, re, rs, 1u, gcnt});
