@@ -92,15 +92,22 @@ def host_to_host(G, idx, pin, batch, steps, warmup, mode, lane_choices):
             best = min(best, el)
         sweep[lanes] = n_pass / best
     lanes = max(sweep, key=sweep.get)
-    left, reads, elapsed, batch_ms = steps, 0, 0.0, []
+    # passes of up to per_pass batches, a shorter one first, so that the
+    # last pass (whose results the parity leg checks) covers n_pass reads
+    passes, left = [], steps
     while left > 0:
-        k = min(left, nb_all)
+        passes.append(min(left, per_pass))
+        left -= passes[-1]
+    passes.reverse()
+    reads, elapsed, batch_ms = 0, 0.0, []
+    for k in passes:
         n = min(n_all, k * batch)
-        el, res, ms = stream_pass(G, idx, pin, n, batch, lanes, mode, res if n == n_pass else None)
+        el, out, ms = stream_pass(G, idx, pin, n, batch, lanes, mode, res if n == n_pass else None)
+        if n == n_pass:
+            res = out
         elapsed += el
         reads += n
         batch_ms += list(ms)
-        left -= k
     log(f"host to host ({'fused' if mode == G.MODE_FUSED else 'plain'}): {reads / elapsed / 1e6:.1f} M reads/s with "
         f"{lanes} lanes (sweep: " + ", ".join(f"{k}: {v / 1e6:.1f} M" for k, v in sweep.items()) + ")")
     return {"value": reads / elapsed, "unit": "reads/s", "ms_per_step": elapsed * 1e3 / steps,
@@ -835,7 +842,9 @@ def run_c5(args, G):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="timed batches (0: whole passes over the read pool, c2 64 = 4 x 16, c3 62 = 2 x 31, "
+                         "c4 64; c5 times requests instead)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", choices=("c2", "c3", "c4", "c5"), default="c2")
     ap.add_argument("--slots", type=int, default=6, help="c5: steps in flight")
@@ -859,6 +868,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
 
+    if args.steps <= 0:
+        args.steps = {"c2": 64, "c3": 62, "c4": 64, "c5": 64}[args.config]
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the read-sharded "
               "multi-GPU form needs no exchange and is not launched here", file=sys.stderr)

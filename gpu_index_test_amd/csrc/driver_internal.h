@@ -94,7 +94,7 @@ struct gis_ctx {
     gis::DevIndexStore ix;
     gis_batch *scratch = nullptr;  // batch reused by the host-buffer entry points
     int n_cu = 256;                // compute units of the device (persistent grids)
-    gis_fused_config fcfg{4, 0, 0, 0, 0};
+    gis_fused_config fcfg{4, 1, 0, 0, 8};
     gis::PairWork *pairs = nullptr;  // the pair step on NAM lists from host memory
     // lanes of gis_find_nams_stream: batches with a stream and a workspace
     // each, kept so that later calls reuse their buffers and graphs

@@ -1863,10 +1863,15 @@ __device__ __forceinline__ void fused_wave_walks(const DevIndex &ix, bool mine, 
     }
 }
 
+// One read through seeding, lookup and its hits. With `stash` (an LDS slot
+// of 64 hits), a read with 1 to 64 hits leaves them there and reports their
+// count in stashed for a merge with three other reads (fused_quad_merge);
+// every other read is merged alone (or spilled) here.
 template <int LMAX>
 __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix, const SeedParams &P,
                                            FusedLds<LMAX> &S, uint32_t r, uint32_t rescue_cutoff,
-                                           uint32_t hard_cutoff) {
+                                           uint32_t hard_cutoff, DevHit *stash, uint32_t &stashed) {
+    stashed = 0;
     const uint32_t lane = lane_id();
     const int L = (int)sload(b.read_len + r);
     bool spill = L > LMAX;
@@ -1927,26 +1932,40 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
             nh = tot_f + wave_sum(hr);
             spill = nh > (uint32_t)MERGE_LDS;
             if (!spill) {
+                DevHit *dst = S.merge.hits;
                 const uint32_t of = wave_excl_scan(hf), orr = tot_f + wave_excl_scan(hr);
                 // a taken seed of a small bucket keeps the records the probe
                 // kept (its run walk is the same in both modes)
                 if (hf && !wf) {
-                    if (sf) fused_emit_line(ix, rp.fh, kf, (int)rp.fs, (int)rp.fe, 0u, S.merge.hits + of);
-                    else walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, S.merge.hits + of);
+                    if (sf) fused_emit_line(ix, rp.fh, kf, (int)rp.fs, (int)rp.fe, 0u, dst + of);
+                    else walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, dst + of);
                 }
                 if (hr && !wr) {
-                    if (sr) fused_emit_line(ix, rp.rh, kr, (int)rp.rs, (int)rp.re, 0x80000000u, S.merge.hits + orr);
-                    else walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, S.merge.hits + orr);
+                    if (sr) fused_emit_line(ix, rp.rh, kr, (int)rp.rs, (int)rp.re, 0x80000000u, dst + orr);
+                    else walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, dst + orr);
                 }
                 uint32_t unused_f = 0, unused_r = 0;
-                fused_wave_walks(ix, wf, pf, rp.fh, rp.fs, rp.fe, 0u, unused_f, S.merge.hits, of);
-                fused_wave_walks(ix, wr, pr, rp.rh, rp.rs, rp.re, 0x80000000u, unused_r, S.merge.hits, orr);
+                fused_wave_walks(ix, wf, pf, rp.fh, rp.fs, rp.fe, 0u, unused_f, dst, of);
+                fused_wave_walks(ix, wr, pr, rp.rh, rp.rs, rp.re, 0x80000000u, unused_r, dst, orr);
                 wave_sync_mem();
             }
         }
     }
     if (spill) {
         spill_read(b, r);
+        return;
+    }
+    if (stash && nh > 0 && nh <= 64) {
+        // the hits to the stash slot (the seed area they share is reused
+        // by the next read)
+        if (lane < nh) stash[lane] = S.merge.hits[lane];
+        if (lane == 0) {
+            b.frac[r] = frac;
+            b.flags[r] = (uint8_t)((rescue ? GIS_FLAG_RESCUED : 0) | (n_fwd == 0 ? GIS_FLAG_NO_RANDSTROBES : 0));
+            b.hit_total[r] = 0;
+            b.rhit_total[r] = 0;
+        }
+        stashed = nh;
         return;
     }
     uint32_t total = 0;
@@ -1967,30 +1986,139 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
     wave_sync_mem();
 }
 
+// The merge of up to four stashed reads (read k's hits in stash[k]) on one
+// wave, 16 lanes a read, as k_merge4 merges them. A read a strand of which
+// needs more than 16 NAMs is then merged alone from its stash (merge_wave),
+// and spilled if its NAMs exceed F_NAMCAP.
+// meta: the reads' ids (0..3) and hit counts (4..7), in LDS.
+__device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*stash)[64], uint32_t qn,
+                                                 const uint32_t *meta) {
+    const uint32_t lane = lane_id(), seg = lane / SEG, j = lane % SEG;
+    const bool act = seg < qn;
+    const uint32_t r = meta[seg];
+    const uint32_t nh = act ? meta[4 + seg] : 0u;
+    gis_nam *out = b.nam_slots + (uint64_t)r * F_NAMCAP;
+    LaneNam o{0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t rsd_o = 0, n_open = 0, strand = 0, base = 0;
+    bool ovf = false;
+    const uint32_t maxh = wave_max(nh);
+    for (uint32_t t = 0; t < maxh; t++) {
+        const bool valid = act && !ovf && t < nh;
+        DevHit h{};
+        if (valid) h = stash[seg][t];
+        const uint32_t hs = h.ref_strand >> 31;
+        const bool sw = valid && hs != strand;
+        if (__ballot(sw)) {
+            seg_finalize(out, base, j, n_open, seg, rsd_o, o, sw);
+            if (sw) {
+                base += n_open;
+                n_open = 0;
+                strand = hs;
+            }
+        }
+        const int qs = h.query_start, qe = h.query_end, rs = h.ref_start, re = h.ref_end;
+        bool ext = false;
+        const bool acc = valid && j < n_open && rsd_o == h.ref_strand &&
+                         nam_accepts(o.qp, o.qe, o.rp, o.re, qs, qe, rs, re, ext);
+        const uint64_t bal = __ballot(acc);
+        const uint32_t mine = (uint32_t)(bal >> (seg * SEG)) & ((1u << SEG) - 1u);
+        if (mine && j == (uint32_t)__builtin_ctz(mine)) {
+            if (ext) {
+                o.qe = qe;
+                o.re = re;
+            }
+            o.qp = qs;
+            o.rp = rs;
+            o.n++;
+        }
+        if (valid && !mine) {
+            if (n_open == SEG) {
+                ovf = true;
+            } else {
+                if (j == n_open) {
+                    o = LaneNam{qs, qe, qs, rs, re, rs, 1, n_open};
+                    rsd_o = h.ref_strand;
+                }
+                n_open++;
+            }
+        }
+    }
+    const bool fin = act && !ovf;
+    if (__ballot(fin)) seg_finalize(out, base, j, n_open, seg, rsd_o, o, fin);
+    if (fin && j == 0) b.nam_cnt[r] = base + n_open;
+    wave_sync_mem();
+    uint64_t redo = __ballot(act && ovf && j == 0);
+    while (redo) {
+        const uint32_t l = (uint32_t)__builtin_ctzll(redo);
+        redo &= redo - 1;
+        const uint32_t rk = readlane_u32(r, l), nk = readlane_u32(nh, l);
+        const uint32_t total =
+            merge_wave(nk, stash[l / SEG], nullptr, nullptr, nullptr, b.nam_slots + (uint64_t)rk * F_NAMCAP,
+                       (uint32_t)F_NAMCAP);
+        if (total > (uint32_t)F_NAMCAP) {
+            spill_read(b, rk);
+        } else {
+            if (lane == 0) b.nam_cnt[rk] = total;
+            wave_sync_mem();
+        }
+    }
+}
+
 // Five waves per SIMD: the kernel's 103 VGPRs would allow four; at 96 (a
 // few spills outside the hot loops) the extra wave hides more of the lookup's
-// latency (+10% on the 50-300 bp mixture, +11% on 150 bp reads).
+// latency (+10% on the 50-300 bp mixture, +11% on 150 bp reads). A wave
+// merges its reads four at a time (fused_quad_merge) once their hits wait in
+// its stash.
 template <int LMAX, int FWPB>
 __global__ __launch_bounds__(64 * FWPB) __attribute__((amdgpu_waves_per_eu(5))) void k_fused(
     DevBatch b, DevIndex ix, SeedParams P, uint32_t rescue_cutoff, uint32_t hard_cutoff,
     const uint32_t *__restrict__ queue, uint32_t count, uint32_t chunk, unsigned int *head) {
     __shared__ FusedLds<LMAX> lds[FWPB];
+    __shared__ DevHit qstash[FWPB][4][64];
+    __shared__ uint32_t qmeta[FWPB][8];
     const int wid = (int)uni32(threadIdx.x >> 6);
     FusedLds<LMAX> &S = lds[wid];
-    if (chunk == 0) {
-        for (uint32_t w = wave_index(FWPB); w < count; w += gridDim.x * FWPB)
-            fused_read<LMAX>(b, ix, P, S, sload(queue + w), rescue_cutoff, hard_cutoff);
-        return;
-    }
-    // dynamic: every wave leaves once a grab starts at or past the end
+    DevHit(*stash)[64] = qstash[wid];
+    uint32_t *meta = qmeta[wid];
+    uint32_t qn = 0;
+    // a grid with a wave per read merges each read alone
+    const bool alone = chunk == 0 && (uint64_t)gridDim.x * FWPB >= count;
+    // static: queue entries w, w + W, ...; dynamic: grabs of `chunk` entries,
+    // and every wave leaves once a grab starts at or past the end
+    uint32_t w = chunk == 0 ? wave_index(FWPB) : 0u, end = 0;
     for (;;) {
-        uint32_t start = 0;
-        if (lane_id() == 0) start = atomicAdd(head, chunk);
-        start = (uint32_t)__builtin_amdgcn_readfirstlane((int)start);
-        if (start >= count) break;
-        const uint32_t end = count - start < chunk ? count : start + chunk;
-        for (uint32_t w = start; w < end; w++)
-            fused_read<LMAX>(b, ix, P, S, sload(queue + w), rescue_cutoff, hard_cutoff);
+        bool have;
+        if (chunk == 0) {
+            have = w < count;
+        } else {
+            if (w >= end) {
+                uint32_t start = 0;
+                if (lane_id() == 0) start = atomicAdd(head, chunk);
+                start = (uint32_t)__builtin_amdgcn_readfirstlane((int)start);
+                w = start;
+                end = count - min(start, count) < chunk ? count : start + chunk;
+            }
+            have = w < end;
+        }
+        if (have) {
+            const uint32_t r = sload(queue + w);
+            w += chunk == 0 ? gridDim.x * FWPB : 1u;
+            uint32_t nh = 0;
+            fused_read<LMAX>(b, ix, P, S, r, rescue_cutoff, hard_cutoff, alone ? nullptr : stash[qn], nh);
+            if (nh) {
+                if (lane_id() == 0) {
+                    meta[qn] = r;
+                    meta[4 + qn] = nh;
+                }
+                wave_sync_mem();
+                qn++;
+            }
+        }
+        if (qn == 4 || (!have && qn)) {
+            fused_quad_merge(b, stash, qn, meta);
+            qn = 0;
+        }
+        if (!have) break;
     }
 }
 

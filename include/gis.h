@@ -143,7 +143,9 @@ typedef struct gis_ctx gis_ctx;
 
 /* Launch configuration of the fused form (GIS_MODE_FUSED), kept per context.
  * The fused kernel is persistent over a device work queue of read ids sorted
- * by length, longest first. */
+ * by length, longest first; a wave merges the reads it takes four at a time,
+ * so persistent grids (blocks_per_cu > 0) suit it. A new context starts with
+ * {4, 1, 0, 0, 8}. */
 typedef struct {
     int32_t waves_per_block; /* 2, 4 or 8 reads in flight per workgroup */
     int32_t length_split;    /* 1: reads of at most 160 bp run in a second launch with 160-base LDS tiles */
