@@ -252,7 +252,8 @@ void mark(gis_batch *b, const char *name) {
 // Stage reads into the batch: randstrobe-slot capacities and offsets come
 // from the lengths, computed here once.
 int batch_stage(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
-                uint32_t n, bool want_sy, uint64_t base, hipEvent_t before, std::vector<CopyItem> *defer) {
+                uint32_t n, bool want_sy, uint64_t base, hipEvent_t before, std::vector<CopyItem> *defer,
+                bool want_queue) {
     gis_ctx *ctx = b->ctx;
     if (!ctx->ix.loaded && !b->seed_params) return fail(ctx, GIS_ERR_STATE, "no index loaded");
     if (n > 0 && (!seqs || !offsets || !lens)) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null read arrays");
@@ -265,6 +266,7 @@ int batch_stage(gis_batch *b, const char *seqs, const uint64_t *offsets, const u
     b->h_rs_off[0] = b->h_sy_off[0] = b->h_long_off[0] = 0;
     b->any_long = false;
     uint64_t rs_at = 0, sy_at = 0, long_at = 0;
+    uint32_t n_long = 0;
     for (uint32_t r = 0; r < n; r++) {
         const uint64_t L = lens[r];
         if (L > GIS_MAX_READ_LEN)
@@ -283,22 +285,23 @@ int batch_stage(gis_batch *b, const char *seqs, const uint64_t *offsets, const u
         const bool lg = L > (uint64_t)LMAX_LDS;
         b->any_long |= lg;
         long_at += lg ? long_bytes(L) : 0;
+        n_long += L > (uint64_t)SHORT_BIN ? 1u : 0u;
         b->h_rs_off[r + 1] = rs_at;
         b->h_sy_off[r + 1] = sy_at;
         b->h_long_off[r + 1] = long_at;
     }
     b->n_reads = n;
     b->want_sy = want_sy;
+    b->n_long = n_long;
+    b->queue_ok = want_queue;
     // the fused kernel's work queue: read ids by length, longest first
     // (counting sort over lengths capped at 1023; stable within a length)
-    {
+    if (want_queue) {
         uint32_t cnt[1025] = {};
         for (uint32_t r = 0; r < n; r++) cnt[1023 - std::min<uint32_t>(lens[r], 1023u) + 1]++;
         for (int i = 1; i < 1025; i++) cnt[i] += cnt[i - 1];
         b->h_queue.resize(n);
         for (uint32_t r = 0; r < n; r++) b->h_queue[cnt[1023 - std::min<uint32_t>(lens[r], 1023u)]++] = r;
-        b->n_long = 0;
-        for (uint32_t r = 0; r < n; r++) b->n_long += lens[r] > (uint32_t)SHORT_BIN ? 1u : 0u;
     }
     ensure_input_arrays(b, n);
     b->tot_rs_cap = b->h_rs_off[n];
@@ -314,7 +317,7 @@ int batch_stage(gis_batch *b, const char *seqs, const uint64_t *offsets, const u
     in.push_back({b->rs_off.p, b->h_rs_off.data(), 8ull * (n + 1)});
     in.push_back({b->sy_off.p, b->h_sy_off.data(), 8ull * (n + 1)});
     in.push_back({b->long_off.p, b->h_long_off.data(), 8ull * (n + 1)});
-    if (n) in.push_back({b->queue.p, b->h_queue.data(), 4ull * n});
+    if (n && want_queue) in.push_back({b->queue.p, b->h_queue.data(), 4ull * n});
     b->ran = false;
     if (defer) return GIS_OK;
     hipStream_t st = b->stream;
@@ -438,8 +441,10 @@ void batch_launch(gis_batch *b, const RunOpts &r) {
     prepare_workspace(b);
     b->ws->owner = b;
     b->ran = false;
-    if (r.mode == GIS_MODE_FUSED && r.stop == STOP_NONE)
+    if (r.mode == GIS_MODE_FUSED && r.stop == STOP_NONE) {
+        if (!b->queue_ok) throw HipError{hipErrorInvalidValue, "fused run of reads staged without the work queue"};
         ensure<gis_nam>(b->ws->nam_slots, (size_t)b->n_reads * F_NAMCAP);
+    }
     // graphs captured against a table that has since been dropped hold its
     // freed addresses and old parameters: never replay them
     bool stale = false;

@@ -112,6 +112,7 @@ struct gis_batch {
     DevBuf seqs, read_off, read_len, rs_off, sy_off, long_off, queue;
     gis::Pinned<uint32_t> h_queue;  // read ids by length, longest first
     uint32_t n_long = 0;            // queue entries longer than the short bin (160 bp)
+    bool queue_ok = false;          // the queue was staged with the reads (fused runs need it)
     uint64_t tot_rs_cap = 0;
     // seeding parameters of a run that needs no index (else the index's)
     const gis_params *seed_params = nullptr;
@@ -174,10 +175,11 @@ struct CopyItem {
 // there), or, with `defer`, appended to it for the caller to issue before the
 // run. Device offsets are offsets[r] - base and the bases copied are
 // seqs[base .. max(offsets[r] + lens[r])); seqs and lens are read by the copies
-// as they run, so they stay valid until then.
+// as they run, so they stay valid until then. Without want_queue the fused
+// form's work queue is not built (plain runs only).
 int batch_stage(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
                 uint32_t n, bool want_sy, uint64_t base, hipEvent_t before,
-                std::vector<CopyItem> *defer = nullptr);
+                std::vector<CopyItem> *defer = nullptr, bool want_queue = true);
 // batch_stage with base 0, then wait for the copies
 int batch_upload(gis_batch *b, const char *seqs, const uint64_t *offsets, const uint32_t *lens,
                  uint32_t n, bool want_sy);

@@ -223,7 +223,8 @@ void lane_main(StreamJob &J, gis_batch *b, uint32_t lane, uint32_t lanes) {
             uint64_t base = UINT64_MAX;
             for (uint64_t r = lo; r < hi; r++) base = std::min(base, J.offs[r]);
             in.clear();
-            if (batch_stage(b, J.seqs, J.offs + lo, J.lens + lo, k, false, base, nullptr, &in) != GIS_OK)
+            if (batch_stage(b, J.seqs, J.offs + lo, J.lens + lo, k, false, base, nullptr, &in,
+                            J.opt.mode == GIS_MODE_FUSED) != GIS_OK)
                 throw HipError{hipErrorInvalidValue, "staging batch " + std::to_string(i) + " failed"};
             hipStream_t st = b->stream;
             run_copies(E, true, in, sig, have_sig, st);
