@@ -48,7 +48,7 @@ gis_batch *batch_new_stream(gis_ctx *ctx) {
 
 // the device arrays of a workspace, and of a batch's own inputs
 static std::vector<DevBuf *> ws_bufs(Workspace *w) {
-    return {&w->long_scratch, &w->rs_cnt, &w->rs_nfwd, &w->rs_hash, &w->rs_start, &w->rs_end,
+    return {&w->long_scratch, &w->rs_cnt, &w->rs_nfwd, &w->rs_hash, &w->rs_span,
             &w->sy_cnt, &w->sy_hash, &w->sy_pos, &w->pos, &w->hcnt, &w->rcnt, &w->lk_bytes, &w->lk_stream,
             &w->hit_total, &w->rhit_total, &w->frac, &w->flags, &w->nam_cnt, &w->hit_off,
             &w->rhit_off, &w->hits, &w->rhits, &w->keys, &w->rkeys, &w->nam_ws, &w->rnam_ws,
@@ -132,8 +132,7 @@ DevBatch make_devbatch(gis_batch *b) {
     d.rs_cnt = static_cast<uint32_t *>(b->ws->rs_cnt.p);
     d.rs_nfwd = static_cast<uint32_t *>(b->ws->rs_nfwd.p);
     d.rs_hash = static_cast<uint64_t *>(b->ws->rs_hash.p);
-    d.rs_start = static_cast<uint32_t *>(b->ws->rs_start.p);
-    d.rs_end = static_cast<uint32_t *>(b->ws->rs_end.p);
+    d.rs_span = static_cast<uint32_t *>(b->ws->rs_span.p);
     d.sy_off = static_cast<const uint64_t *>(b->sy_off.p);
     d.sy_cnt = static_cast<uint32_t *>(b->ws->sy_cnt.p);
     d.sy_hash = static_cast<uint64_t *>(b->ws->sy_hash.p);
@@ -213,8 +212,7 @@ void ensure_read_arrays(gis_batch *b, uint32_t n) {
 
 void ensure_slot_arrays(gis_batch *b, uint64_t slots) {
     ensure<uint64_t>(b->ws->rs_hash, slots);
-    ensure<uint32_t>(b->ws->rs_start, slots);
-    ensure<uint32_t>(b->ws->rs_end, slots);
+    ensure<uint32_t>(b->ws->rs_span, slots);
     ensure<uint64_t>(b->ws->pos, slots);
     ensure<uint32_t>(b->ws->hcnt, slots);
     ensure<uint32_t>(b->ws->rcnt, slots);
@@ -646,11 +644,10 @@ static int randstrobes_of(gis_ctx *ctx, const gis_params *params, const char *se
     if (total > capacity || (total && !out)) return fail(ctx, GIS_ERR_CAPACITY, "randstrobe capacity too small");
     const uint64_t cap = b->tot_rs_cap;
     std::vector<uint64_t> hh(cap);
-    std::vector<uint32_t> ss(cap), ee(cap);
+    std::vector<uint32_t> sp(cap);
     if (cap) {
         HIP_CHECK(hipMemcpy(hh.data(), b->ws->rs_hash.p, 8 * cap, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(ss.data(), b->ws->rs_start.p, 4 * cap, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(ee.data(), b->ws->rs_end.p, 4 * cap, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(sp.data(), b->ws->rs_span.p, 4 * cap, hipMemcpyDeviceToHost));
     }
     uint64_t o2 = 0;
     for (uint32_t r = 0; r < n_reads; r++) {
@@ -659,8 +656,8 @@ static int randstrobes_of(gis_ctx *ctx, const gis_params *params, const char *se
         for (uint32_t i = 0; i < cnt[r]; i++) {
             gis_query_randstrobe q{};
             q.hash = hh[base + i];
-            q.start = ss[base + i];
-            q.end = ee[base + i];
+            q.start = (int32_t)span_start(sp[base + i]);
+            q.end = (int32_t)span_end(sp[base + i]);
             q.is_reverse = i >= cnt[r] / 2;
             out[o2++] = q;
         }
@@ -794,7 +791,7 @@ static int qrs_run(gis_ctx *ctx, const gis_query_randstrobe *qrs, const uint64_t
     // each read's queries laid out forward strand first, each strand by (start, end, hash)
     const uint64_t total = query_offsets[n];
     std::vector<uint64_t> hh(total);
-    std::vector<uint32_t> ss(total), ee(total), cnt(n), nf(n);
+    std::vector<uint32_t> sp(total), cnt(n), nf(n);
     std::vector<uint64_t> off(n + 1);
     std::vector<uint64_t> ord;
     for (uint32_t r = 0; r < n; r++) {
@@ -813,8 +810,7 @@ static int qrs_run(gis_ctx *ctx, const gis_query_randstrobe *qrs, const uint64_t
             const auto &q = qrs[ord[j]];
             if (q.end > GIS_MAX_READ_LEN) return fail(ctx, GIS_ERR_BAD_PARAMETER, "query end too large");
             hh[lo + j] = q.hash;
-            ss[lo + j] = q.start;
-            ee[lo + j] = q.end;
+            sp[lo + j] = span_pack((uint32_t)q.start, (uint32_t)q.end);
             if (!q.is_reverse) f++;
         }
         cnt[r] = (uint32_t)(hi - lo);
@@ -837,8 +833,7 @@ static int qrs_run(gis_ctx *ctx, const gis_query_randstrobe *qrs, const uint64_t
     hipStream_t st = ctx->stream;
     if (total) {
         HIP_CHECK(hipMemcpyAsync(b->ws->rs_hash.p, hh.data(), 8 * total, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(b->ws->rs_start.p, ss.data(), 4 * total, hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(b->ws->rs_end.p, ee.data(), 4 * total, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(b->ws->rs_span.p, sp.data(), 4 * total, hipMemcpyHostToDevice, st));
     }
     HIP_CHECK(hipMemcpyAsync(b->rs_off.p, off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
     if (n) {
@@ -965,8 +960,11 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
     if (!query_offsets || !out) return fail(ctx, GIS_ERR_BAD_PARAMETER, "null argument");
     if (n_queries && !qrs && query_offsets[n_queries] > query_offsets[0])
         return fail(ctx, GIS_ERR_BAD_PARAMETER, "null query randstrobes");
-    for (uint64_t i = query_offsets[0]; n_queries && i < query_offsets[n_queries]; i++)
+    for (uint64_t i = query_offsets[0]; n_queries && i < query_offsets[n_queries]; i++) {
         if (qrs[i].end > GIS_MAX_READ_LEN) return fail(ctx, GIS_ERR_BAD_PARAMETER, "query end too large");
+        if (qrs[i].start < 0 || qrs[i].start > qrs[i].end)
+            return fail(ctx, GIS_ERR_BAD_PARAMETER, "query start outside [0, end]");
+    }
     QrsCall me{qrs, query_offsets, n_queries, rescue ? 1 : 0, rescue_cutoff, hard_cutoff, out};
     std::unique_lock<std::mutex> l(ctx->comb_mu);
     ctx->comb_queue.push_back(&me);

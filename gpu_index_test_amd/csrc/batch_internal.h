@@ -72,7 +72,7 @@ struct Pinned {
 // before another batch runs on it); each serving slot has its own.
 struct Workspace {
     DevBuf long_scratch;
-    DevBuf rs_cnt, rs_nfwd, rs_hash, rs_start, rs_end, sy_cnt, sy_hash, sy_pos;
+    DevBuf rs_cnt, rs_nfwd, rs_hash, rs_span, sy_cnt, sy_hash, sy_pos;
     DevBuf pos, hcnt, rcnt, lk_bytes, lk_stream;
     DevBuf hit_total, rhit_total, frac, flags, nam_cnt, hit_off, rhit_off;
     DevBuf hits, rhits, keys, rkeys, nam_ws, rnam_ws, nam_fin, rnam_fin;

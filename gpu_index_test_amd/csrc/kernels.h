@@ -83,8 +83,7 @@ struct DevBatch {
     uint32_t *rs_cnt;
     uint32_t *rs_nfwd;        // forward-strand randstrobes (they come first)
     uint64_t *rs_hash;
-    uint32_t *rs_start;
-    uint32_t *rs_end;
+    uint32_t *rs_span;        // query start | query end << 16 (read positions fit 16 bits)
     // optional syncmer output (stage entry point), regions at sy_off
     const uint64_t *sy_off;
     uint32_t *sy_cnt;
@@ -135,6 +134,11 @@ struct DevBatch {
     DevCtl *ctl;
     uint32_t *spill_ids;
 };
+
+// a query randstrobe's start and end in one word (rs_span)
+__host__ __device__ inline uint32_t span_pack(uint32_t start, uint32_t end) { return start | (end << 16); }
+__host__ __device__ inline uint32_t span_start(uint32_t v) { return v & 0xffffu; }
+__host__ __device__ inline uint32_t span_end(uint32_t v) { return v >> 16; }
 
 constexpr int LMAX_LDS = 320;  // reads up to this length seed from LDS
 constexpr int SEED_WPB = 4;    // waves per block of the seeding kernel

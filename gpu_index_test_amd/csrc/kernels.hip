@@ -263,7 +263,7 @@ __device__ __forceinline__ Rs1 randstrobe_one(const SeedBufs &B, uint32_t n, uin
 __device__ __forceinline__ uint32_t seed_one(const char *__restrict__ seq, int L, const SeedParams &P,
                                              const SeedBufs &B, uint64_t *out_sy_hash,
                                              uint64_t *out_sy_pos, uint64_t *rs_hash,
-                                             uint32_t *rs_start, uint32_t *rs_end,
+                                             uint32_t *rs_span,
                                              uint32_t *n_sync_out) {
     const uint32_t n = seed_syncmers(seq, L, P, B, 0xffffffffu, out_sy_hash, out_sy_pos);
     const uint32_t n_fwd = ((uint32_t)L >= P.w_max && n > P.w_min) ? n - P.w_min : 0;
@@ -273,8 +273,7 @@ __device__ __forceinline__ uint32_t seed_one(const char *__restrict__ seq, int L
         if (j < nrs) {
             const Rs1 o = randstrobe_one(B, n, n_fwd, j, L, P);
             rs_hash[j] = o.h;
-            rs_start[j] = o.s;
-            rs_end[j] = o.e;
+            rs_span[j] = span_pack(o.s, o.e);
         }
     }
     *n_sync_out = n;
@@ -328,7 +327,7 @@ __global__ __launch_bounds__(64 * SEED_WPB) __attribute__((amdgpu_waves_per_eu(6
         const uint64_t ro = sload(b.rs_off + r);
         uint32_t nsync = 0;
         const uint32_t nrs = seed_one(b.seqs + sload(b.read_off + r), L, P, B, osh, osp, b.rs_hash + ro,
-                                      b.rs_start + ro, b.rs_end + ro, &nsync);
+                                      b.rs_span + ro, &nsync);
         if (lane_id() == 0) {
             b.rs_cnt[r] = nrs;
             b.rs_nfwd[r] = nrs / 2;
@@ -648,19 +647,18 @@ __device__ __forceinline__ void probe_emit(const DevIndex &ix, const Probe &p, i
 // ---------------------------------------------------------------------------
 // per-stage: lookup + filter + hit count per randstrobe; nonrepetitive
 // fraction and the rescue decision per read (find_nams, step2/nam.cu:553-732,
-// and the caller's test, step2/main.cu:3209-3216). Lanes probe
-// LOOKUP_ITEMS randstrobes of their read per pass (i, i + 64, ... of a
-// 64 * LOOKUP_ITEMS-wide chunk). One is fastest on gfx950: two raise the
-// kernel from 40 to 52 VGPRs and its time by a fifth at the 150 bp workload,
-// where a read's ~42 randstrobes fill one 64-wide pass anyway. While the
+// and the caller's test, step2/main.cu:3209-3216). One randstrobe per lane,
+// 64 per pass; a 150 bp read's ~42 randstrobes fill one pass. While the
 // read's hits fit its region of the hit array they are written there in seed
 // order; a read with more hits takes a place in the pool, and k_hits writes
 // them.
-
-constexpr uint32_t LOOKUP_ITEMS = 1;
-
-// Six waves per SIMD (77 VGPRs): the read's index, offsets and counts are
-// held in scalar registers.
+//
+// Write traffic is part of this kernel's bound, so a one-pass read (at most
+// 64 randstrobes) keeps its per-randstrobe results (position, kept-hit count)
+// in registers and stores them only when a later kernel reads them: when the
+// read is rescued (k_rescue_count, k_rescue_hits) or its hits go to the pool
+// (k_hits). Longer reads, and the fused kernel's spilled reads, store them
+// in every pass. Wave sums and scans use DPP (52 VGPRs).
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(6))) void k_lookup(
     DevBatch b, ReadSet set, DevIndex ix, int rescue_mode, int keep_spill) {
     const uint32_t lane = lane_id();
@@ -670,78 +668,78 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(6))) v
         const uint32_t n = sload(b.rs_cnt + r);
         const uint64_t base = sload(b.rs_off + r);
         const uint32_t half = sload(b.rs_nfwd + r);
+        // results stay in registers for a one-pass read the decision below
+        // does not hand to a later kernel
+        const bool defer = n <= 64 && !keep_spill && rescue_mode != 2;
         uint32_t found = 0, good = 0, hits = 0, bytes = 0, streams = 0;
+        uint64_t pos0 = GIS_NOT_FOUND;
+        uint32_t hc0 = 0;
         DevHit *region = b.hits + (uint64_t)r * HIT_REGION;
         // the regions must lie inside the hit array (else every read pools)
         const bool region_ok = (uint64_t)(r + 1) * HIT_REGION <= b.hit_cap;
-        for (uint32_t i0 = 0; i0 < n; i0 += 64 * LOOKUP_ITEMS) {
-            Probe P[LOOKUP_ITEMS];
-            int qs[LOOKUP_ITEMS], qe[LOOKUP_ITEMS];
-            bool act[LOOKUP_ITEMS];
+        for (uint32_t i0 = 0; i0 < n; i0 += 64) {
             // straight-line loads (lanes past the end repeat the last item)
-#pragma unroll
-            for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) {
-                const uint32_t i = i0 + 64 * u + lane;
-                act[u] = i < n;
-                const uint64_t x = base + (act[u] ? i : n - 1);
-                P[u].key = b.rs_hash[x];
-                qs[u] = (int)b.rs_start[x];
-                qe[u] = (int)b.rs_end[x];
-            }
-            // the bucket lines' loads issue together; the scheduler may not
-            // hoist their uses between them
-#pragma unroll
-            for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) probe_line(ix, P[u]);
+            const uint32_t i = i0 + lane;
+            const bool act = i < n;
+            const uint64_t x = base + (act ? i : n - 1);
+            Probe P;
+            P.key = b.rs_hash[x];
+            const uint32_t span = b.rs_span[x];
+            const int qs = (int)span_start(span), qe = (int)span_end(span);
+            probe_line(ix, P);
+            // the scheduler may not hoist uses of the line between its loads
             __builtin_amdgcn_sched_barrier(0);
-            uint32_t by = 0, sb = 0, hc_all = 0, f_all = 0, g_all = 0;
-#pragma unroll
-            for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) {
-                const uint32_t i = i0 + 64 * u + lane;
-                if (act[u]) {
-                    probe_finish(ix, P[u], qs[u], qe[u]);
-                    // the table bytes of the search: the bucket bounds (16), the
-                    // records a linear scan reads to the first hash >= key; a
-                    // found seed's filter probe (8), and a kept one's run past
-                    // its first record plus the hash that ends it: 16 per
-                    // run record in all
-                    by += 16 + 16 * P[u].scan;
-                    if (P[u].found) by += P[u].filtered ? 8u : 16u * P[u].run;
-                    // beside them the streams: query in (16), result out (12)
-                    sb += 28;
-                    b.pos[base + i] = P[u].pos;
-                    b.hcnt[base + i] = P[u].hc;
-                    f_all += P[u].found ? 1u : 0u;
-                    g_all += P[u].found && !P[u].filtered ? 1u : 0u;
-                    hc_all += P[u].hc;
+            uint32_t by = 0, sb = 0;
+            if (act) {
+                probe_finish(ix, P, qs, qe);
+                // the table bytes of the search: the bucket bounds (16), the
+                // records a linear scan reads to the first hash >= key; a
+                // found seed's filter probe (8), and a kept one's run past
+                // its first record plus the hash that ends it: 16 per run
+                // record in all. Beside them the streams: the query in (12)
+                // and, where stored, the result out (12).
+                by = 16 + 16 * P.scan;
+                if (P.found) by += P.filtered ? 8u : 16u * P.run;
+                sb = 12;
+                if (defer) {
+                    pos0 = P.pos;
+                    hc0 = P.hc;
                 } else {
-                    P[u].hc = 0;
+                    b.pos[base + i] = P.pos;
+                    b.hcnt[base + i] = P.hc;
+                    sb += 12;
                 }
+            } else {
+                P.hc = 0;
+                P.found = false;
+                P.filtered = false;
             }
-            found += wave_sum(f_all);
-            good += wave_sum(g_all);
-            const uint32_t chunk_hits = wave_sum(hc_all);
+            found += wave_count(act && P.found);
+            good += wave_count(act && P.found && !P.filtered);
+            const uint32_t incl = wave_incl_scan_dpp(P.hc);
+            const uint32_t chunk_hits = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
             // while the read's hits fit its region, write them there in seed
             // order: 16 per hit
             if (region_ok && hits + chunk_hits <= HIT_REGION) {
-                uint32_t at = hits;
-#pragma unroll
-                for (uint32_t u = 0; u < LOOKUP_ITEMS; u++) {
-                    const uint32_t i = i0 + 64 * u + lane;
-                    const uint32_t ex = wave_excl_scan(P[u].hc);
-                    if (P[u].hc > 0)
-                        probe_emit(ix, P[u], qs[u], qe[u], i >= half ? 0x80000000u : 0u, region + at + ex);
-                    at += wave_sum(P[u].hc);
-                }
-                sb += 16 * hc_all;
+                if (P.hc > 0)
+                    probe_emit(ix, P, qs, qe, i >= half ? 0x80000000u : 0u, region + hits + (incl - P.hc));
+                sb += 16 * P.hc;
             }
             hits += chunk_hits;
-            bytes += wave_sum(by);
-            streams += wave_sum(sb);
+            bytes += wave_sum_dpp(by);
+            streams += wave_sum_dpp(sb);
+        }
+        const float frac = found > 0 ? (float)good / (float)found : 1.0f;
+        const bool rescue = rescue_mode == 2 || (rescue_mode == 0 && ((double)frac < 0.7 || hits == 0));
+        const bool pooled = !rescue && hits > 0 && (hits > HIT_REGION || !region_ok);
+        if (defer && (rescue || pooled)) {
+            if (lane < n) {
+                b.pos[base + lane] = pos0;
+                b.hcnt[base + lane] = hc0;
+            }
+            streams += 12 * n;
         }
         if (lane == 0) {
-            const float frac = found > 0 ? (float)good / (float)found : 1.0f;
-            const bool rescue =
-                rescue_mode == 2 || (rescue_mode == 0 && ((double)frac < 0.7 || hits == 0));
             b.lk_bytes[r] = bytes;
             b.lk_stream[r] = streams;
             b.frac[r] = frac;
@@ -753,7 +751,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(6))) v
             // hits beyond the read's region: a place in the pool, and k_hits
             // writes them
             uint64_t off = (uint64_t)r * HIT_REGION;
-            if (!rescue && hits > 0 && (hits > HIT_REGION || !region_ok)) {
+            if (pooled) {
                 off = (uint64_t)b.n_reads * HIT_REGION + atomicAdd(&b.ctl->hit_pool, (unsigned long long)hits);
                 b.sel_ids[2ull * b.n_reads + atomicAdd(&b.ctl->n_sel[2], 1u)] = r;
             }
@@ -798,8 +796,8 @@ __global__ __launch_bounds__(64 * WPB) void k_hits(DevBatch b, ReadSet set, DevI
             const uint32_t hc = i < n ? b.hcnt[base + i] : 0u;
             const uint32_t ex = wave_excl_scan(hc);
             if (hc > 0)
-                walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)b.rs_start[base + i],
-                         (int)b.rs_end[base + i], i >= half ? 0x80000000u : 0u, out + carry + ex);
+                walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)span_start(b.rs_span[base + i]),
+                         (int)span_end(b.rs_span[base + i]), i >= half ? 0x80000000u : 0u, out + carry + ex);
             carry += wave_sum(hc);
         }
     }
@@ -850,19 +848,20 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_count(DevBatch b, ReadSet s
                     taken = c <= rescue_cutoff;
                     if (!taken) {
                         const uint32_t lo = i < half ? 0 : half, hi = i < half ? half : n;
-                        const uint32_t qs = b.rs_start[base + i], qe = b.rs_end[base + i];
+                        const uint32_t sp = b.rs_span[base + i], qs = span_start(sp), qe = span_end(sp);
                         uint32_t rank = 0;
                         for (uint32_t j = lo; j < hi && rank < 5; j++) {
                             const uint32_t cj = b.hcnt[base + j];
                             if (cj == 0) continue;
-                            rank += rh_less(cj, b.rs_start[base + j], b.rs_end[base + j], c, qs, qe) ? 1u : 0u;
+                            const uint32_t spj = b.rs_span[base + j];
+                            rank += rh_less(cj, span_start(spj), span_end(spj), c, qs, qe) ? 1u : 0u;
                         }
                         taken = rank < 5;
                     }
                 }
                 if (taken && c <= WALK_WAVE_MIN)
-                    rh = walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)b.rs_start[base + i],
-                                  (int)b.rs_end[base + i], 0u, nullptr);
+                    rh = walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)span_start(b.rs_span[base + i]),
+                                  (int)span_end(b.rs_span[base + i]), 0u, nullptr);
                 coop = taken && c > WALK_WAVE_MIN;
             }
             // long runs: the whole wave walks each one
@@ -871,8 +870,8 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_count(DevBatch b, ReadSet s
                 const uint32_t l = (uint32_t)__builtin_ctzll(cm);
                 cm &= cm - 1;
                 const uint64_t xl = base + i0 + l;
-                const uint32_t k = walk_run_wave(ix, b.pos[xl], b.rs_hash[xl], (int)b.rs_start[xl],
-                                                 (int)b.rs_end[xl], 0u, nullptr);
+                const uint32_t k = walk_run_wave(ix, b.pos[xl], b.rs_hash[xl], (int)span_start(b.rs_span[xl]),
+                                                 (int)span_end(b.rs_span[xl]), 0u, nullptr);
                 if (lane == l) rh = k;
             }
             if (i < n) b.rcnt[base + i] = rh;
@@ -904,16 +903,17 @@ __global__ __launch_bounds__(64 * WPB) void k_rescue_hits(DevBatch b, ReadSet se
             const bool coop = rh > 0 && b.hcnt[base + i] > WALK_WAVE_MIN;
             const uint32_t ex = wave_excl_scan(rh);
             if (rh > 0 && !coop)
-                walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)b.rs_start[base + i],
-                         (int)b.rs_end[base + i], i >= half ? 0x80000000u : 0u, out + carry + ex);
+                walk_run(ix, b.pos[base + i], b.rs_hash[base + i], (int)span_start(b.rs_span[base + i]),
+                         (int)span_end(b.rs_span[base + i]), i >= half ? 0x80000000u : 0u, out + carry + ex);
             uint64_t cm = __ballot(coop);
             while (cm) {
                 const uint32_t l = (uint32_t)__builtin_ctzll(cm);
                 cm &= cm - 1;
                 const uint32_t il = i0 + l;
                 const uint64_t xl = base + il;
-                walk_run_wave(ix, b.pos[xl], b.rs_hash[xl], (int)b.rs_start[xl], (int)b.rs_end[xl],
-                              il >= half ? 0x80000000u : 0u, out + carry + readlane_u32(ex, l));
+                walk_run_wave(ix, b.pos[xl], b.rs_hash[xl], (int)span_start(b.rs_span[xl]),
+                              (int)span_end(b.rs_span[xl]), il >= half ? 0x80000000u : 0u,
+                              out + carry + readlane_u32(ex, l));
             }
             carry += wave_sum(rh);
         }

@@ -72,6 +72,27 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
     return x - v;
 }
 
+// Inclusive prefix sum over the wave from DPP row shifts (1, 2, 4, 8 within
+// each row of 16 lanes) and the row broadcasts of lanes 15 and 31 (GFX9
+// DPP), without the LDS round trips of __shfl. Every lane must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);   // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);   // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);   // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);   // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
+// wave total of x (the last lane's inclusive prefix)
+__device__ __forceinline__ uint32_t wave_sum_dpp(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan_dpp(x), 63);
+}
+
+// lanes for which p holds
+__device__ __forceinline__ uint32_t wave_count(bool p) { return (uint32_t)__popcll(__ballot(p)); }
+
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t l) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
 }

@@ -306,9 +306,11 @@ int gis_batch_stage_times(gis_batch *b, double *ms, int cap, int *n, char *names
  * filter probe's hash), a kept seed 16 per record of its equal-hash run past
  * the first plus 8 (the hash that ends the run). n_lookups = randstrobes. */
 int gis_batch_lookup_bytes(gis_batch *b, uint64_t *bytes, uint64_t *n_lookups);
-/* Bytes the lookup kernel of that run streamed beside the table: 16 per
- * query randstrobe read, 12 per result written (position, hit count), 16 per
- * hit written into its read's hit region. */
+/* Bytes the lookup kernel of that run streamed beside the table: 12 per
+ * query randstrobe read (hash, packed start and end), 12 per result written
+ * (position, hit count; only for reads a later stage reads them for: rescued
+ * reads, reads whose hits go to the pool, reads of more than 64
+ * randstrobes), 16 per hit written into its read's hit region. */
 int gis_batch_lookup_streams(gis_batch *b, uint64_t *stream_bytes);
 /* total NAMs of the last run (available after gis_batch_run, before download) */
 int gis_batch_nam_count(gis_batch *b, uint64_t *n_nams);

@@ -443,7 +443,7 @@ def test_lookup_bytes_match_oracle(setup):
     exp = sum(O.lookup_bytes(oidx, q) for q in qrs)
     assert n_lookups == sum(len(q) for q in qrs)
     assert got == exp
-    assert streams >= 28 * n_lookups
+    assert streams >= 12 * n_lookups  # each query read (hash + packed span), at least
     m.close()
 
 
