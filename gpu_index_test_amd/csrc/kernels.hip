@@ -1056,10 +1056,11 @@ __device__ __forceinline__ int permute_push(int v, uint32_t dst) {
 // does open NAM o accept hit (qs, qe, rs, re), and does the hit extend it
 __device__ __forceinline__ bool nam_accepts(int o_qp, int o_qe, int o_rp, int o_re, int qs, int qe,
                                             int rs, int re, bool &ext) {
-    const bool near = o_qp < qs && qs <= o_qe && o_rp < rs && rs <= o_re;
-    ext = qe > o_qe && re > o_re;
-    const bool con = qe <= o_qe && re <= o_re;
-    return near && (ext || con);
+    // bitwise on purpose: no short-circuit branches in the merge loops
+    const bool near = (o_qp < qs) & (qs <= o_qe) & (o_rp < rs) & (rs <= o_re);
+    ext = (qe > o_qe) & (re > o_re);
+    const bool con = (qe <= o_qe) & (re <= o_re);
+    return near & (ext | con);
 }
 
 // The wide list of one merge: entries below near_cap in `near` (LDS when the
@@ -1737,30 +1738,29 @@ __global__ __launch_bounds__(64 * WPB) void k_merge4(DevBatch b, ReadSet set, in
             }
             const int qs = h.query_start, qe = h.query_end, rs = h.ref_start, re = h.ref_end;
             bool ext = false;
-            const bool acc = valid && j < n_open && rsd_o == h.ref_strand &&
+            const bool acc = valid & (j < n_open) & (rsd_o == h.ref_strand) &
                              nam_accepts(o.qp, o.qe, o.rp, o.re, qs, qe, rs, re, ext);
             const uint64_t bal = __ballot(acc);
             const uint32_t mine = (uint32_t)(bal >> (seg * SEG)) & ((1u << SEG) - 1u);
-            if (mine && j == (uint32_t)__builtin_ctz(mine)) {
-                if (ext) {
-                    o.qe = qe;
-                    o.re = re;
-                }
-                o.qp = qs;
-                o.rp = rs;
-                o.n++;
-            }
-            if (valid && !mine) {
-                if (n_open == SEG) {
-                    ovf = true;
-                } else {
-                    if (j == n_open) {
-                        o = LaneNam{qs, qe, qs, rs, re, rs, 1, n_open};
-                        rsd_o = h.ref_strand;
-                    }
-                    n_open++;
-                }
-            }
+            // the first accepting lane of the segment takes the hit; else the
+            // hit opens a NAM in the segment's next lane (selects, no branches)
+            const bool take = (mine != 0u) & (j == (uint32_t)__builtin_ctz(mine | 0x10000u));
+            const bool opens = valid & (mine == 0u);
+            const bool full = n_open == SEG;
+            const bool mk = opens & !full & (j == n_open);
+            o.qe = take & ext ? qe : o.qe;
+            o.re = take & ext ? re : o.re;
+            o.qp = take | mk ? qs : o.qp;
+            o.rp = take | mk ? rs : o.rp;
+            o.n = mk ? 1 : o.n + (take ? 1 : 0);
+            o.qs = mk ? qs : o.qs;
+            o.rs = mk ? rs : o.rs;
+            o.qe = mk ? qe : o.qe;
+            o.re = mk ? re : o.re;
+            o.cid = mk ? n_open : o.cid;
+            rsd_o = mk ? h.ref_strand : rsd_o;
+            ovf = ovf | (opens & full);
+            n_open += (opens & !full) ? 1u : 0u;
         }
         const bool fin = act && !ovf;
         if (__ballot(fin)) seg_finalize(out, base, j, n_open, seg, rsd_o, o, fin);
@@ -1815,14 +1815,33 @@ __device__ __forceinline__ void spill_read(const DevBatch &b, uint32_t r) {
     wave_sync_mem();
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// The bucket line of one query randstrobe, fetched in one round trip. The
+// empty asm takes every loaded word, so all eight loads issue before the
+// first wait: otherwise the compiler sinks the record loads into the
+// small-bucket branch, behind the wait for the bucket's bounds, and each
+// probe pays two dependent fetches. (Both strands' lines at once, sixteen
+// loads, is no faster and spills.)
+__device__ __forceinline__ void probe_line_now(const DevIndex &ix, Probe &P) {
+    const u32x4 *l = reinterpret_cast<const u32x4 *>(ix.lines) +
+                     (uint64_t)(uint32_t)(P.key >> (64 - ix.bits)) * BUCKET_LINE;
+    u32x4 a[BUCKET_LINE];
+#pragma unroll
+    for (uint32_t j = 0; j < BUCKET_LINE; j++) a[j] = l[j];
+    asm volatile("" ::"v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7]));
+    P.s = (uint64_t)a[0].x | ((uint64_t)a[0].y << 32);
+    P.e = P.s + a[0].z;
+#pragma unroll
+    for (uint32_t j = 0; j < PROBE_SMALL; j++)
+        P.r[j] = IdxRec{(uint64_t)a[j + 1].x | ((uint64_t)a[j + 1].y << 32), a[j + 1].z, a[j + 1].w};
+}
+
 // find + is_filtered + the counting run walk of one query randstrobe through
-// its bucket line (the per-stage lookup's probe): position, found and not
-// filtered, kept hits, and for a small bucket the kept records' mask
-__device__ __forceinline__ void fused_probe(const DevIndex &ix, uint64_t key, int qs, int qe, uint64_t &pos,
+// its fetched bucket line (the per-stage lookup's probe): position, found
+// and not filtered, kept hits, and for a small bucket the kept records' mask
+__device__ __forceinline__ void fused_probe(const DevIndex &ix, Probe &P, int qs, int qe, uint64_t &pos,
                                             bool &good, uint32_t &kept, uint32_t &mask, bool &small) {
-    Probe P;
-    P.key = key;
-    probe_line(ix, P);
     probe_finish(ix, P, qs, qe);
     pos = P.pos;
     good = P.found && !P.filtered;
@@ -1896,8 +1915,14 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
             uint32_t hf = 0, hr = 0, kf = 0, kr = 0;
             bool gf = false, gr = false, sf = false, sr = false;
             if (act) {
-                fused_probe(ix, rp.fh, (int)rp.fs, (int)rp.fe, pf, gf, hf, kf, sf);
-                fused_probe(ix, rp.rh, (int)rp.rs, (int)rp.re, pr, gr, hr, kr, sr);
+                Probe F;
+                F.key = rp.fh;
+                probe_line_now(ix, F);
+                fused_probe(ix, F, (int)rp.fs, (int)rp.fe, pf, gf, hf, kf, sf);
+                Probe R;
+                R.key = rp.rh;
+                probe_line_now(ix, R);
+                fused_probe(ix, R, (int)rp.rs, (int)rp.re, pr, gr, hr, kr, sr);
             }
             const uint32_t found = wave_sum((pf != GIS_NOT_FOUND ? 1u : 0u) + (pr != GIS_NOT_FOUND ? 1u : 0u));
             const uint32_t good = wave_sum((gf ? 1u : 0u) + (gr ? 1u : 0u));
@@ -2018,30 +2043,29 @@ __device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*sta
         }
         const int qs = h.query_start, qe = h.query_end, rs = h.ref_start, re = h.ref_end;
         bool ext = false;
-        const bool acc = valid && j < n_open && rsd_o == h.ref_strand &&
+        const bool acc = valid & (j < n_open) & (rsd_o == h.ref_strand) &
                          nam_accepts(o.qp, o.qe, o.rp, o.re, qs, qe, rs, re, ext);
         const uint64_t bal = __ballot(acc);
         const uint32_t mine = (uint32_t)(bal >> (seg * SEG)) & ((1u << SEG) - 1u);
-        if (mine && j == (uint32_t)__builtin_ctz(mine)) {
-            if (ext) {
-                o.qe = qe;
-                o.re = re;
-            }
-            o.qp = qs;
-            o.rp = rs;
-            o.n++;
-        }
-        if (valid && !mine) {
-            if (n_open == SEG) {
-                ovf = true;
-            } else {
-                if (j == n_open) {
-                    o = LaneNam{qs, qe, qs, rs, re, rs, 1, n_open};
-                    rsd_o = h.ref_strand;
-                }
-                n_open++;
-            }
-        }
+        // as in k_merge4: the first accepting lane takes the hit, else it
+        // opens a NAM in the segment's next lane (selects, no branches)
+        const bool take = (mine != 0u) & (j == (uint32_t)__builtin_ctz(mine | 0x10000u));
+        const bool opens = valid & (mine == 0u);
+        const bool full = n_open == SEG;
+        const bool mk = opens & !full & (j == n_open);
+        o.qe = take & ext ? qe : o.qe;
+        o.re = take & ext ? re : o.re;
+        o.qp = take | mk ? qs : o.qp;
+        o.rp = take | mk ? rs : o.rp;
+        o.n = mk ? 1 : o.n + (take ? 1 : 0);
+        o.qs = mk ? qs : o.qs;
+        o.rs = mk ? rs : o.rs;
+        o.qe = mk ? qe : o.qe;
+        o.re = mk ? re : o.re;
+        o.cid = mk ? n_open : o.cid;
+        rsd_o = mk ? h.ref_strand : rsd_o;
+        ovf = ovf | (opens & full);
+        n_open += (opens & !full) ? 1u : 0u;
     }
     const bool fin = act && !ovf;
     if (__ballot(fin)) seg_finalize(out, base, j, n_open, seg, rsd_o, o, fin);
