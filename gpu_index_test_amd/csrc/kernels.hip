@@ -39,45 +39,37 @@ struct SeedBufs {
     uint64_t *nmask;    // bit i set when base i is not ACGT
 };
 
-// Canonical syncmers of one read (SyncmerIterator, step2/randstrobes.cu:18-79),
-// the whole wave working on the same read. The first `cap` syncmers go to
-// B.sy_hash/B.sy_pos (and out_sy_* when given); returns the total count.
-__device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, int L,
-                                                  const SeedParams &P, const SeedBufs &B,
-                                                  uint32_t cap, uint64_t *out_sy_hash,
-                                                  uint64_t *out_sy_pos) {
+// The passes of the syncmer scan after packing: canonical s-mer hashes,
+// window states, tie replay and compaction, k-mer hashes. CLEAN: the read
+// holds only ACGT bases (a wave-uniform fact), so no window needs the N
+// mask and only the read's first window starts a segment.
+template <bool CLEAN>
+__device__ __forceinline__ uint32_t syncmer_passes(int L, const SeedParams &P, const SeedBufs &B, uint32_t cap,
+                                                   uint64_t *out_sy_hash, uint64_t *out_sy_pos) {
     const int lane = (int)lane_id();
     const int k = P.k, s = P.s, w = P.w;
 
-    // 2-bit packed bases and the N mask from three ballots per 64 bases
-    // (step2/randstrobes.cu:21-22, 70-76)
-    int nch = 0;
-    for (int b0 = 0; b0 < L; b0 += 64, nch++) {
-        const int i = b0 + lane;
-        int c = 4;
-        if (i < L) c = nt4((unsigned char)seq[i]);
-        const bool ok = i < L && c < 4;
-        const uint64_t lo = __ballot(ok && (c & 1));
-        const uint64_t hi = __ballot(ok && (c & 2));
-        const uint64_t m = __ballot(i < L && c > 3);
-        if (lane == 0) {
-            B.nmask[b0 >> 6] = m;
-            B.pk[2 * nch] = spread32((uint32_t)lo) | (spread32((uint32_t)hi) << 1);
-        } else if (lane == 1) {
-            B.pk[2 * nch + 1] = spread32((uint32_t)(lo >> 32)) | (spread32((uint32_t)(hi >> 32)) << 1);
-        }
-    }
-    if (lane == 0) B.pk[2 * nch] = 0;  // pad word read by windows ending in the last word
-    wave_sync();
-
-    // canonical s-mer hashes, one s-mer per lane (step2/randstrobes.cu:23-32)
+    // canonical s-mer hashes, one s-mer per lane (step2/randstrobes.cu:23-32);
+    // an s-mer of at most 16 bases is a 32-bit code
     const int ns = L - s + 1;
-    for (int j0 = 0; j0 < ns; j0 += 64) {
-        const int j = j0 + lane;
-        if (j < ns) {
-            uint64_t h = 0;
-            if (!any_n(B.nmask, j, j + s - 1)) h = xxh64_u64(canonical_code(B.pk, j, s, P.smask));
-            B.sh[j] = h;
+    if (s <= 16) {
+        for (int j0 = 0; j0 < ns; j0 += 64) {
+            const int j = j0 + lane;
+            if (j < ns) {
+                uint64_t h = 0;
+                if (CLEAN || !any_n(B.nmask, j, j + s - 1))
+                    h = xxh64_u64((uint64_t)canonical_code32(B.pk, j, s, (uint32_t)P.smask));
+                B.sh[j] = h;
+            }
+        }
+    } else {
+        for (int j0 = 0; j0 < ns; j0 += 64) {
+            const int j = j0 + lane;
+            if (j < ns) {
+                uint64_t h = 0;
+                if (CLEAN || !any_n(B.nmask, j, j + s - 1)) h = xxh64_u64(canonical_code(B.pk, j, s, P.smask));
+                B.sh[j] = h;
+            }
         }
     }
     wave_sync();
@@ -88,10 +80,11 @@ __device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, 
         const int p = p0 + lane;
         if (p < nk) {
             uint8_t st;
-            if (any_n(B.nmask, p, p + k - 1)) {
+            if (!CLEAN && any_n(B.nmask, p, p + k - 1)) {
                 st = ST_INVALID;
             } else {
-                const bool seg_first = (p == 0) || ((B.nmask[(p - 1) >> 6] >> ((p - 1) & 63)) & 1);
+                const bool seg_first =
+                    (p == 0) || (!CLEAN && ((B.nmask[(p - 1) >> 6] >> ((p - 1) & 63)) & 1));
                 // w = k - s + 1 is 5 in every read-length profile up to 375 bp
                 st = w == 5 ? anchor_state_w<5>(B.sh, p, seg_first) : anchor_state(B.sh, p, w, seg_first);
             }
@@ -143,6 +136,42 @@ __device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, 
     }
     wave_sync();
     return nsync;
+}
+
+// Canonical syncmers of one read (SyncmerIterator, step2/randstrobes.cu:18-79),
+// the whole wave working on the same read. The first `cap` syncmers go to
+// B.sy_hash/B.sy_pos (and out_sy_* when given); returns the total count.
+__device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, int L,
+                                                  const SeedParams &P, const SeedBufs &B,
+                                                  uint32_t cap, uint64_t *out_sy_hash,
+                                                  uint64_t *out_sy_pos) {
+    const int lane = (int)lane_id();
+
+    // 2-bit packed bases and the N mask from three ballots per 64 bases
+    // (step2/randstrobes.cu:21-22, 70-76)
+    int nch = 0;
+    uint64_t nany = 0;
+    for (int b0 = 0; b0 < L; b0 += 64, nch++) {
+        const int i = b0 + lane;
+        int c = 4;
+        if (i < L) c = nt4((unsigned char)seq[i]);
+        const bool ok = i < L && c < 4;
+        const uint64_t lo = __ballot(ok && (c & 1));
+        const uint64_t hi = __ballot(ok && (c & 2));
+        const uint64_t m = __ballot(i < L && c > 3);
+        nany |= m;
+        if (lane == 0) {
+            B.nmask[b0 >> 6] = m;
+            B.pk[2 * nch] = spread32((uint32_t)lo) | (spread32((uint32_t)hi) << 1);
+        } else if (lane == 1) {
+            B.pk[2 * nch + 1] = spread32((uint32_t)(lo >> 32)) | (spread32((uint32_t)(hi >> 32)) << 1);
+        }
+    }
+    if (lane == 0) B.pk[2 * nch] = 0;  // pad word read by windows ending in the last word
+    wave_sync();
+    // the ballots are wave-uniform, so this branch is too
+    return nany == 0 ? syncmer_passes<true>(L, P, B, cap, out_sy_hash, out_sy_pos)
+                     : syncmer_passes<false>(L, P, B, cap, out_sy_hash, out_sy_pos);
 }
 
 // Forward and reverse-complement randstrobe number i of a read with n

@@ -58,6 +58,22 @@ __device__ __forceinline__ uint64_t canonical_code(const uint64_t *pk, I j, int 
     return f < r ? f : r;
 }
 
+// canonical_code for n <= 16: the same value from 32-bit operations. The
+// n-mer's 2n bits are the low bits of the 32-bit window at bit 2j of the
+// packed bases (v_alignbit over the two words it spans).
+__device__ __forceinline__ uint32_t canonical_code32(const uint64_t *pk, int j, int n, uint32_t mask) {
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(pk);
+    const int b = 2 * j;
+    const int wi = b >> 5;
+    const uint32_t lo = w[wi], hi = w[wi + 1];
+    const uint32_t g = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)(b & 31)) & mask;
+    const uint32_t r = g ^ mask;
+    uint32_t x = __builtin_bitreverse32(g);
+    x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
+    const uint32_t f = x >> (32 - 2 * n);
+    return f < r ? f : r;
+}
+
 // anchor_state (gis_core.h) for a window of W s-mers known at compile time:
 // the loads issue together and the minimum, its first index and the count
 // of equal values come from straight-line code. The minimum's first index

@@ -149,6 +149,32 @@ def test_randstrobes_parity(setup):
             assert np.array_equal(got[f], exp[f]), (f, len(seq))
 
 
+@pytest.mark.parametrize("read_length", [50, 100, 125, 250, 300, 400])
+def test_randstrobes_every_profile(setup, read_length):
+    """Seeding under the other read-length profiles: k 20-23, s 16-18 (s-mer
+    codes of 16 bases and of more), t 3-4, their w_min/w_max and max_dist;
+    reads with and without N, tie-heavy reads, lengths around the windows."""
+    g, lens, _, _, _, _ = setup
+    p = G.IndexParameters.from_read_length(read_length)
+    op = O.params_from_read_length(read_length)
+    reads = _read_set(g, lens, 150, seed=read_length)
+    s, o, l = G.pack_reads(reads)
+    ctx = G.default_context()
+    cap = int(2 * np.maximum(l.astype(np.int64) - p.syncmer.k + 1, 0).sum()) or 1
+    out = np.empty(cap, dtype=G.QUERY_RANDSTROBE)
+    oo = np.empty(len(l) + 1, dtype=np.uint64)
+    nout = G._native.C.c_uint64()
+    N = G._native
+    ctx.check(ctx.L.gis_randstrobes_params(ctx.h, N.C.byref(p.c), N.ptr(s), N.ptr(o), N.ptr(l), len(l),
+                                           N.ptr(oo), N.ptr(out), cap, N.C.byref(nout)))
+    for r, seq in enumerate(reads):
+        exp = O.randstrobes_query(seq, op)
+        got = out[int(oo[r]): int(oo[r + 1])]
+        assert len(got) == len(exp), (read_length, r, len(seq))
+        for f in ("hash", "start", "end", "is_reverse"):
+            assert np.array_equal(got[f], exp[f]), (read_length, r, f, len(seq))
+
+
 def test_lookup_parity(setup):
     g, lens, p, idx, op, oidx = setup
     rng = np.random.default_rng(3)
