@@ -146,6 +146,9 @@ def resident_results(G, mappers, mode, max_batches=None):
     return res, sum(len(r.nam_offsets) - 1 for r in parts)
 
 
+MODE_NAMES = {0: "plain", 1: "fused"}
+
+
 # the top-level p50/p99: the latency of a batch through the host-to-host
 # stream, from its reads' copy to HBM to its NAM arrays' copy back
 LATENCY_DEF = ("per batch, from the issue of the copy of its reads to HBM to the completion of the "
@@ -299,6 +302,32 @@ def timed_lanes(mappers, lanes, steps, warmup, mode, graph):
     return reads_done / elapsed, elapsed, step_ms
 
 
+def resident_sweep(G, idx, seqs, offs, lens, batch, lane_choices, steps, warmup, modes, setup=None):
+    """The resident forms at each lane count: a fresh set of batches per
+    count (the lanes' workspaces and streams), each form timed one batch at
+    a time (on the first set) and with the lanes in flight. Returns
+    {mode: (one, many, lanes)} at each form's fastest count, the per-count
+    rates, and the last set of batches (kept open for the legs after)."""
+    best, sweep, mappers, one = {}, {}, None, {}
+    for li, nl in enumerate(lane_choices):
+        if mappers:
+            for m in mappers:
+                m.close()
+        mappers = make_batches(G, idx, seqs, offs, lens, batch, lanes=nl)
+        if li == 0 and setup:
+            setup(mappers)
+        sweep[nl] = {}
+        for mode in modes:
+            if li == 0:
+                one[mode] = timed(mappers, steps, warmup, mode, True)
+            many = timed_lanes(mappers, nl, steps, warmup, mode, True) if nl > 1 else one[mode]
+            sweep[nl][MODE_NAMES[mode]] = many[0]
+            if mode not in best or many[0] > best[mode][1][0]:
+                best[mode] = (one[mode], many, nl)
+            log(f"{MODE_NAMES[mode]} (resident): {many[0] / 1e6:.1f} M reads/s with {nl} in flight")
+    return best, sweep, mappers
+
+
 def timed_forms(mappers, lanes, steps, warmup, mode):
     """Graph-replayed runs one batch at a time and with `lanes` in flight."""
     one = timed(mappers, steps, warmup, mode, True)
@@ -406,10 +435,8 @@ def run_c2(args, G):
                                         sub_rate=0.005, indel_rate=0.0005, seed=2)
     log(f"reads {args.reads} in {time.time() - t0:.1f}s")
     batch = args.batch or 65536
-    lanes = max(1, args.lanes)
 
-    # the headline: host memory to host memory, batches of several lanes in
-    # flight
+    # host memory to host memory, batches of several lanes in flight
     pin = (G.pinned_copy(seqs), G.pinned_copy(offs), G.pinned_copy(lens))
     h2h, res, n_pass = host_to_host(G, idx, pin, batch, args.steps, args.warmup, G.MODE_PLAIN,
                                     args.stream_lanes)
@@ -418,15 +445,12 @@ def run_c2(args, G):
                  "d2h_per_batch": float(batch * (nams_per_read * 44 + 13))}
 
     # the same kernels with the reads resident in HBM and the NAMs left there
-    mappers = make_batches(G, idx, seqs, offs, lens, batch, lanes=lanes)
-    fused_rps, fused_elapsed, fused_ms = timed(mappers, args.steps, args.warmup, G.MODE_FUSED, True)
-    log(f"fused form (resident): {fused_rps / 1e6:.1f} M reads/s (graph)")
-    serial_rps, serial_elapsed, serial_ms = timed(mappers, args.steps, args.warmup, G.MODE_PLAIN, True)
-    log(f"plain form (resident), one batch at a time: {serial_rps / 1e6:.1f} M reads/s (graph)")
-    res_rps, res_elapsed, res_ms = serial_rps, serial_elapsed, serial_ms
-    if lanes > 1:
-        res_rps, res_elapsed, res_ms = timed_lanes(mappers, lanes, args.steps, args.warmup, G.MODE_PLAIN, True)
-        log(f"plain form (resident), {lanes} batches in flight: {res_rps / 1e6:.1f} M reads/s (graph)")
+    forms, lane_sweep, mappers = resident_sweep(G, idx, seqs, offs, lens, batch, args.lanes, args.steps,
+                                                args.warmup, (G.MODE_PLAIN, G.MODE_FUSED))
+    (serial_rps, serial_elapsed, serial_ms), (res_rps, res_elapsed, res_ms), lanes = forms[G.MODE_PLAIN]
+    fused_one, fused_many, fused_lanes = forms[G.MODE_FUSED]
+    log(f"plain form (resident): {serial_rps / 1e6:.1f} M reads/s one batch at a time, "
+        f"{res_rps / 1e6:.1f} M with {lanes} in flight (graph)")
     stages = kernel_times(G, mappers, args.steps, G.MODE_PLAIN)
     roof = lookup_roofline(stages, "c2")
     log("stage ms (mean): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_means(stages).items()))
@@ -487,10 +511,9 @@ def run_c2(args, G):
                                                     "p50_batch_ms": float(np.percentile(serial_ms, 50)),
                                                     "p99_batch_ms": float(np.percentile(serial_ms, 99))}},
         "stage_ms": stage_means(stages),
-        "fused_form_resident": {"value": fused_rps, "unit": "reads/s",
-                                "ms_per_step": fused_elapsed * 1e3 / args.steps,
-                                "p50_batch_ms": float(np.percentile(fused_ms, 50)),
-                                "p99_batch_ms": float(np.percentile(fused_ms, 99))},
+        "resident_lane_sweep": lane_sweep,
+        "fused_form_resident": form_json(fused_many, args.steps, batches_in_flight=fused_lanes,
+                                         one_batch_at_a_time=form_json(fused_one, args.steps)),
     }
 
 
@@ -510,18 +533,22 @@ def run_c3(args, G):
     # each batch holds whole pairs laid out for the pair step (mate 1 block,
     # mate 2 block)
     offs, lens = pair_blocks(offs, lens, batch)
-    lanes = max(1, args.lanes)
-    mappers = make_batches(G, idx, seqs, offs, lens, batch, lanes=lanes)
     steps = args.steps
+    tuned = {}
 
-    best, best_ms, n_tried = mappers[0].autotune()
-    log(f"autotune: {n_tried} configurations, best {best} at {best_ms:.3f} ms per batch")
-    fused_one, fused_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_FUSED)
+    def autotune(mappers):
+        tuned["best"], tuned["ms"], tuned["n"] = mappers[0].autotune()
+        log(f"autotune: {tuned['n']} configurations, best {tuned['best']} at {tuned['ms']:.3f} ms per batch")
+
+    forms, lane_sweep, mappers = resident_sweep(G, idx, seqs, offs, lens, batch, args.lanes, steps, args.warmup,
+                                                (G.MODE_FUSED, G.MODE_PLAIN), setup=autotune)
+    best, n_tried = tuned["best"], tuned["n"]
+    fused_one, fused_many, lanes = forms[G.MODE_FUSED]
+    plain_one, plain_many, plain_lanes = forms[G.MODE_PLAIN]
     log(f"fused (autotuned, resident): {fused_one[0] / 1e6:.1f} M reads/s one batch at a time, "
         f"{fused_many[0] / 1e6:.1f} M with {lanes} in flight")
-    plain_one, plain_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_PLAIN)
     log(f"plain per-stage (resident): {plain_one[0] / 1e6:.1f} M reads/s one batch at a time, "
-        f"{plain_many[0] / 1e6:.1f} M with {lanes} in flight")
+        f"{plain_many[0] / 1e6:.1f} M with {plain_lanes} in flight")
     fstage = kernel_times(G, mappers, steps, G.MODE_FUSED)
     mappers[0].run(mode=G.MODE_FUSED)
     spill_frac = float(np.mean((mappers[0].download().flags & 0x80) != 0))
@@ -589,8 +616,9 @@ def run_c3(args, G):
                                     one_batch_at_a_time=form_json(fused_one, steps)),
         "fused_stage_ms": stage_means(fstage),
         "pair_step": pairs_out,
-        "plain_resident": form_json(plain_many, steps, stage_ms=stage_means(stages),
+        "plain_resident": form_json(plain_many, steps, stage_ms=stage_means(stages), batches_in_flight=plain_lanes,
                                     one_batch_at_a_time=form_json(plain_one, steps)),
+        "resident_lane_sweep": lane_sweep,
     }
 
 
@@ -621,19 +649,22 @@ def run_c4(args, G):
                                         indel_rate=0.0005, seed=2)
     log(f"reads {len(lens)} ({pairs} pairs) in {time.time() - t0:.1f}s")
     batch = args.batch or 262144
-    t0 = time.time()
-    lanes = max(1, args.lanes)
-    mappers = make_batches(G, idx, seqs, offs, lens, batch, lanes=lanes)
-    log(f"{len(mappers)} batches resident in {time.time() - t0:.1f}s")
     steps = args.steps
-    best, best_ms, n_tried = mappers[0].autotune()
-    log(f"autotune: {n_tried} configurations, best {best} at {best_ms:.3f} ms per batch")
-    fused_one, fused_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_FUSED)
+    tuned = {}
+
+    def autotune(mappers):
+        tuned["best"], tuned["ms"], tuned["n"] = mappers[0].autotune()
+        log(f"autotune: {tuned['n']} configurations, best {tuned['best']} at {tuned['ms']:.3f} ms per batch")
+
+    forms, lane_sweep, mappers = resident_sweep(G, idx, seqs, offs, lens, batch, args.lanes, steps, args.warmup,
+                                                (G.MODE_FUSED, G.MODE_PLAIN), setup=autotune)
+    best, n_tried = tuned["best"], tuned["n"]
+    fused_one, fused_many, lanes = forms[G.MODE_FUSED]
+    plain_one, plain_many, plain_lanes = forms[G.MODE_PLAIN]
     log(f"fused (autotuned, resident): {fused_one[0] / 1e6:.1f} M reads/s one batch at a time, "
         f"{fused_many[0] / 1e6:.1f} M with {lanes} in flight")
-    plain_one, plain_many = timed_forms(mappers, lanes, steps, args.warmup, G.MODE_PLAIN)
     log(f"plain per-stage (resident): {plain_one[0] / 1e6:.1f} M reads/s one batch at a time, "
-        f"{plain_many[0] / 1e6:.1f} M with {lanes} in flight")
+        f"{plain_many[0] / 1e6:.1f} M with {plain_lanes} in flight")
     stages = kernel_times(G, mappers, steps, G.MODE_PLAIN)
     roof = lookup_roofline(stages, "c4")
     log("stage ms (mean): " + ", ".join(f"{k}={v:.3f}" for k, v in stage_means(stages).items()))
@@ -701,8 +732,9 @@ def run_c4(args, G):
         "plain_host_to_host": h2h_plain,
         "fused_resident": form_json(fused_many, steps, batches_in_flight=lanes,
                                     one_batch_at_a_time=form_json(fused_one, steps)),
-        "plain_resident": form_json(plain_many, steps, stage_ms=stage_means(stages),
+        "plain_resident": form_json(plain_many, steps, stage_ms=stage_means(stages), batches_in_flight=plain_lanes,
                                     one_batch_at_a_time=form_json(plain_one, steps)),
+        "resident_lane_sweep": lane_sweep,
     }
 
 
@@ -901,8 +933,9 @@ def main():
     ap.add_argument("--reads", type=int, default=1_000_000, help="c2: single-end reads")
     ap.add_argument("--pairs", type=int, default=0, help="c3/c4: read pairs (2M / 10M)")
     ap.add_argument("--batch", type=int, default=0, help="reads per batch (c2 65536, c3 131072, c4 262144)")
-    ap.add_argument("--lanes", type=int, default=2,
-                    help="c2-c4 resident forms: batches in flight, each lane on its own stream and workspace")
+    ap.add_argument("--lanes", type=lambda x: tuple(int(v) for v in x.split(",")), default=(2, 3),
+                    help="c2-c4 resident forms: batches in flight (each lane on its own stream and workspace); "
+                         "every count given is timed and each form reports its fastest")
     ap.add_argument("--stream-lanes", type=lambda x: tuple(int(v) for v in x.split(",")), default=(3, 4, 5, 6),
                     help="c2-c4 headline: lane counts tried for the host-to-host pipeline (the fastest is timed)")
     ap.add_argument("--genome", type=int, default=0, help="genome bp (c2/c3 100M, c4 3.1G)")
