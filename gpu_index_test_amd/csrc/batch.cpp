@@ -962,8 +962,7 @@ extern "C" int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
         return fail(ctx, GIS_ERR_BAD_PARAMETER, "null query randstrobes");
     for (uint64_t i = query_offsets[0]; n_queries && i < query_offsets[n_queries]; i++) {
         if (qrs[i].end > GIS_MAX_READ_LEN) return fail(ctx, GIS_ERR_BAD_PARAMETER, "query end too large");
-        if (qrs[i].start < 0 || qrs[i].start > qrs[i].end)
-            return fail(ctx, GIS_ERR_BAD_PARAMETER, "query start outside [0, end]");
+        if (qrs[i].start > qrs[i].end) return fail(ctx, GIS_ERR_BAD_PARAMETER, "query start after its end");
     }
     QrsCall me{qrs, query_offsets, n_queries, rescue ? 1 : 0, rescue_cutoff, hard_cutoff, out};
     std::unique_lock<std::mutex> l(ctx->comb_mu);

@@ -2190,31 +2190,45 @@ __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&b.ctl->overflow, OVF_OUT);
         return;
     }
-    // COMPACT_SEG lanes per read, so that a wave's reads start their chains
-    // of dependent loads (count, flags, offsets, NAMs) together; reads with
-    // more than COMPACT_SEG_WORDS words of NAMs (repeats) are copied by the
-    // whole wave afterwards
+    // COMPACT_SEG lanes per read, so that a wave's reads start their loads
+    // (count, flags, offsets, then NAMs) together; reads with more than
+    // COMPACT_SEG_WORDS words of NAMs (repeats) are copied by the whole wave
+    // afterwards, four words per lane in flight
     const uint32_t seg = lane / COMPACT_SEG, j = lane % COMPACT_SEG;
     const uint32_t per_wave = 64 / COMPACT_SEG;
     for (uint32_t r0 = (blockIdx.x * WPB + (threadIdx.x >> 6)) * per_wave; r0 < b.n_reads;
          r0 += gridDim.x * WPB * per_wave) {
         const uint32_t r = r0 + seg;
-        const uint32_t c = r < b.n_reads ? b.nam_cnt[r] : 0u;
-        const uint32_t *src = nullptr;
-        uint32_t *dst = nullptr;
-        if (c > 0) {
-            const uint8_t fl = b.flags[r];
-            const gis_nam *srcn;
-            if (fused && !(fl & FLAG_SPILL)) srcn = b.nam_slots + (uint64_t)r * F_NAMCAP;
-            else if (fl & GIS_FLAG_RESCUED) srcn = b.rnam_fin + b.rhit_off[r];
-            else srcn = b.nam_fin + b.hit_off[r];
-            src = reinterpret_cast<const uint32_t *>(srcn);
-            dst = reinterpret_cast<uint32_t *>(b.nams_out + b.nam_off[r]);
-        }
+        // the read's count, flags and offsets load together (r clamped to a
+        // valid read; lanes past the end copy nothing)
+        const uint32_t rr = r < b.n_reads ? r : b.n_reads - 1;
+        const uint32_t c0 = b.nam_cnt[rr];
+        const uint8_t fl = b.flags[rr];
+        const uint64_t ho = b.hit_off[rr], rho = b.rhit_off[rr], no = b.nam_off[rr];
+        const uint32_t c = r < b.n_reads ? c0 : 0u;
+        const gis_nam *srcn;
+        if (fused && !(fl & FLAG_SPILL)) srcn = b.nam_slots + (uint64_t)rr * F_NAMCAP;
+        else if (fl & GIS_FLAG_RESCUED) srcn = b.rnam_fin + rho;
+        else srcn = b.nam_fin + ho;
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(srcn);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(b.nams_out + no);
         const uint32_t words = c * (uint32_t)(sizeof(gis_nam) / 4);
         const bool big = words > COMPACT_SEG_WORDS;
-        if (!big)
-            for (uint32_t i = j; i < words; i += COMPACT_SEG) dst[i] = src[i];
+        if (!big && words > 0) {
+            // every load issues before the first store (indices past the
+            // read's words repeat its last word, and store nothing)
+            uint32_t v[COMPACT_SEG_WORDS / COMPACT_SEG];
+#pragma unroll
+            for (uint32_t u = 0; u < COMPACT_SEG_WORDS / COMPACT_SEG; u++) {
+                const uint32_t i = j + u * COMPACT_SEG;
+                v[u] = src[i < words ? i : words - 1];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < COMPACT_SEG_WORDS / COMPACT_SEG; u++) {
+                const uint32_t i = j + u * COMPACT_SEG;
+                if (i < words) dst[i] = v[u];
+            }
+        }
         uint64_t bigs = __ballot(big && j == 0);
         while (bigs) {
             const uint32_t l = (uint32_t)__builtin_ctzll(bigs);
@@ -2226,7 +2240,16 @@ __global__ __launch_bounds__(64 * WPB) void k_compact(DevBatch b, int fused) {
             uint32_t *dl = reinterpret_cast<uint32_t *>(
                 (uint64_t)readlane_u32((uint32_t)(uintptr_t)dst, l) |
                 ((uint64_t)readlane_u32((uint32_t)((uintptr_t)dst >> 32), l) << 32));
-            for (uint32_t i = lane; i < wl; i += 64) dl[i] = sl[i];
+            // four loads in flight per lane before their stores
+            uint32_t i = lane;
+            for (; i + 192 < wl; i += 256) {
+                const uint32_t v0 = sl[i], v1 = sl[i + 64], v2 = sl[i + 128], v3 = sl[i + 192];
+                dl[i] = v0;
+                dl[i + 64] = v1;
+                dl[i + 128] = v2;
+                dl[i + 192] = v3;
+            }
+            for (; i < wl; i += 64) dl[i] = sl[i];
         }
     }
 }

@@ -258,7 +258,9 @@ int gis_find_nams_batch(gis_ctx *ctx, const char *seqs, const uint64_t *offsets,
  * calls find_nams once per read from many threads (step2/nam.cu:553-562);
  * calls made at the same time on one context are combined: the first caller
  * to find the context free runs every waiting call with the same rescue
- * settings as one batch and each caller gets its own reads' results. */
+ * settings as one batch and each caller gets its own reads' results. A
+ * query's end must be at most GIS_MAX_READ_LEN and its start at most its
+ * end; otherwise GIS_ERR_BAD_PARAMETER. */
 int gis_find_nams_qrs(gis_ctx *ctx, const gis_query_randstrobe *qrs,
                       const uint64_t *query_offsets, uint32_t n_queries, int rescue,
                       uint32_t rescue_cutoff, uint32_t hard_cutoff, gis_nam_out *out);

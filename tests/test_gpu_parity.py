@@ -304,6 +304,12 @@ def test_find_nams_qrs_parity(setup):
             else:
                 _, exp, _ = O.find_nams(oidx, q, canonical=True)
             assert res.read_nams(r).tobytes() == exp.tobytes(), (rescue, r)
+    # a query span the packed device form cannot hold is refused
+    for field, value in (("start", None), ("end", 70000)):
+        bad = O.randstrobes_query(reads[0], op).copy()
+        bad[field][0] = bad["end"][0] + 1 if value is None else value
+        with pytest.raises(G.GisError):
+            G.find_nams(bad, idx)
 
 
 def test_find_nams_composition_worked_example(ctx):
