@@ -333,9 +333,26 @@ __device__ __forceinline__ SeedBufs long_bufs(uint8_t *base, uint64_t L) {
     return B;
 }
 
-template <bool LONG>
+// Seeding parameters of the read-length profiles up to 250 bp (k = 20,
+// s = 16, t = 3): kernels instantiated with K20 take them as compile-time
+// constants, which shortens the seeding code and frees the scalar registers
+// that would hold them. The host launches that instantiation when the
+// index's parameters match (is_k20_profile).
+__device__ __forceinline__ void pin_k20_profile(SeedParams &P) {
+    P.k = 20;
+    P.s = 16;
+    P.t = 3;
+    P.w = 5;
+    P.kmask = (1ull << 40) - 1;
+    P.smask = 0xffffffffull;
+}
+
+static inline bool is_k20_profile(const SeedParams &sp) { return sp.k == 20 && sp.s == 16 && sp.t == 3; }
+
+template <bool LONG, bool K20>
 __global__ __launch_bounds__(64 * SEED_WPB) __attribute__((amdgpu_waves_per_eu(6))) void k_seed(DevBatch b, ReadSet set, SeedParams P,
                                                         int want_sy) {
+    if (K20) pin_k20_profile(P);
     __shared__ SeedLds lds[LONG ? 1 : SEED_WPB];
     const int wid = (int)uni32(threadIdx.x >> 6);
     const uint32_t cnt = set_count(set);
@@ -2122,10 +2139,11 @@ __device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*sta
 // latency (+10% on the 50-300 bp mixture, +11% on 150 bp reads). A wave
 // merges its reads four at a time (fused_quad_merge) once their hits wait in
 // its stash.
-template <int LMAX, int FWPB>
+template <int LMAX, int FWPB, bool K20>
 __global__ __launch_bounds__(64 * FWPB) __attribute__((amdgpu_waves_per_eu(5))) void k_fused(
     DevBatch b, DevIndex ix, SeedParams P, uint32_t rescue_cutoff, uint32_t hard_cutoff,
     const uint32_t *__restrict__ queue, uint32_t count, uint32_t chunk, unsigned int *head) {
+    if (K20) pin_k20_profile(P);
     __shared__ FusedLds<LMAX> lds[FWPB];
     __shared__ DevHit qstash[FWPB][4][64];
     __shared__ uint32_t qmeta[FWPB][8];
@@ -2472,9 +2490,12 @@ void launch_seed(const DevBatch &b, const ReadSet &set, const SeedParams &sp, in
                  int long_reads, hipStream_t st) {
     if (b.n_reads == 0) return;
     const unsigned g = grid_for_set(b, set, SEED_WPB);
-    hipLaunchKernelGGL(k_seed<false>, dim3(g), dim3(64 * SEED_WPB), 0, st, b, set, sp, want_syncmers);
-    if (long_reads)
-        hipLaunchKernelGGL(k_seed<true>, dim3(g), dim3(64 * SEED_WPB), 0, st, b, set, sp, want_syncmers);
+    typedef void (*SeedKernel)(DevBatch, ReadSet, SeedParams, int);
+    const bool k20 = is_k20_profile(sp);
+    const SeedKernel short_k = k20 ? (SeedKernel)k_seed<false, true> : (SeedKernel)k_seed<false, false>;
+    const SeedKernel long_k = k20 ? (SeedKernel)k_seed<true, true> : (SeedKernel)k_seed<true, false>;
+    hipLaunchKernelGGL(short_k, dim3(g), dim3(64 * SEED_WPB), 0, st, b, set, sp, want_syncmers);
+    if (long_reads) hipLaunchKernelGGL(long_k, dim3(g), dim3(64 * SEED_WPB), 0, st, b, set, sp, want_syncmers);
 }
 
 void launch_lookup(const DevBatch &b, const ReadSet &set, const DevIndex &ix, int rescue_mode,
@@ -2531,13 +2552,13 @@ void launch_merge(const DevBatch &b, const ReadSet &set, int rescue, hipStream_t
 // fused kernel over queue[0 .. count): FWPB waves per block, LMAX the LDS
 // tiling (reads longer than LMAX spill), grid 0 = one wave per entry,
 // chunk 0 = static round robin, else dynamic grabs of chunk entries.
-template <int LMAX, int FWPB>
+template <int LMAX, int FWPB, bool K20>
 static void launch_fused_t(const DevBatch &b, const DevIndex &ix, const SeedParams &sp, uint32_t rcut,
                            uint32_t hard, const uint32_t *queue, uint32_t count, unsigned grid,
                            uint32_t chunk, unsigned int *head, hipStream_t st) {
     const unsigned g1 = grid_for(count, FWPB);
     const unsigned g = grid == 0 || grid > g1 ? g1 : grid;
-    hipLaunchKernelGGL((k_fused<LMAX, FWPB>), dim3(g), dim3(64 * FWPB), 0, st, b, ix, sp, rcut, hard,
+    hipLaunchKernelGGL((k_fused<LMAX, FWPB, K20>), dim3(g), dim3(64 * FWPB), 0, st, b, ix, sp, rcut, hard,
                        queue, count, chunk, head);
 }
 
@@ -2547,7 +2568,10 @@ bool launch_fused(const DevBatch &b, const DevIndex &ix, const SeedParams &sp, u
     if (count == 0) return true;
 #define GIS_FUSED_CASE(LM, W)                                                                      \
     if (lmax == LM && wpb == W) {                                                                  \
-        launch_fused_t<LM, W>(b, ix, sp, rcut, hard, queue, count, grid, chunk, head, st);         \
+        if (is_k20_profile(sp))                                                                    \
+            launch_fused_t<LM, W, true>(b, ix, sp, rcut, hard, queue, count, grid, chunk, head, st); \
+        else                                                                                       \
+            launch_fused_t<LM, W, false>(b, ix, sp, rcut, hard, queue, count, grid, chunk, head, st); \
         return true;                                                                               \
     }
     GIS_FUSED_CASE(160, 2) GIS_FUSED_CASE(160, 4) GIS_FUSED_CASE(160, 8)
