@@ -1269,6 +1269,7 @@ __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64
             }
         }
         const uint32_t m = xe - x0 < 64 ? xe - x0 : 64;
+        uint32_t no_block_until = 0;  // the rest of a block that did not fit goes hit by hit
         for (uint32_t t = 0; t < m; t++) {
             const uint32_t rsd = readlane_u32(c_rsd, t);
             const int rs = (int)readlane_u32(c_rs, t);
@@ -1289,6 +1290,62 @@ __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64
                 grp = rsd;
             }
             if (!wide_mode) {
+                // A block of hits with this hit's group and query start: as
+                // in the wide form below, no NAM can take two of them (a NAM
+                // that takes one gets previous start qs, and so does one a
+                // hit of the block opens), so each meets the open NAMs as
+                // they were before the block. Their acceptance ballots are
+                // independent; only the assignment, in hit order (first
+                // accepting NAM not taken by an earlier hit of the block,
+                // else the next new lane), runs hit by hit, on scalars.
+                const uint64_t sm =
+                    __ballot(lane >= t && lane < m && c_rsd == rsd && (c_q & 0xffffu) == (uint32_t)qs);
+                const uint64_t above = sm >> t;
+                const uint32_t len = ~above ? (uint32_t)__builtin_ctzll(~above) : 64u - t;
+                uint32_t n_new = 0;
+                int my_hit = -1;  // the block hit (chunk lane) this lane takes or opens with
+                bool fits = len > 1 && t >= no_block_until;
+                if (fits) {
+                    uint64_t taken = 0;
+                    for (uint32_t i = t; i < t + len; i++) {
+                        const int hrs = (int)readlane_u32(c_rs, i);
+                        const int hre = (int)readlane_u32(c_re, i);
+                        const int hqe = (int)(readlane_u32(c_q, i) >> 16);
+                        bool hext = false;
+                        const bool hacc = lane < n_open && nam_accepts(o.qp, o.qe, o.rp, o.re, qs, hqe, hrs, hre, hext);
+                        const uint64_t avail = __ballot(hacc) & ~taken;
+                        if (!avail && n_open + n_new == 64) {
+                            // a new NAM would not fit the lanes: this block goes hit by hit
+                            fits = false;
+                            no_block_until = t + len;
+                            break;
+                        }
+                        const uint32_t win = avail ? (uint32_t)__builtin_ctzll(avail) : n_open + n_new;
+                        taken |= avail ? (1ull << win) : 0ull;
+                        n_new += avail ? 0u : 1u;
+                        my_hit = lane == win ? (int)i : my_hit;
+                    }
+                }
+                if (fits) {
+                    const int src = my_hit < 0 ? 0 : my_hit;
+                    const int hrs = __shfl((int)c_rs, src, 64), hre = __shfl((int)c_re, src, 64);
+                    const int hqe = (int)((uint32_t)__shfl((int)c_q, src, 64) >> 16);
+                    const bool take = my_hit >= 0 && lane < n_open;
+                    const bool mk = my_hit >= 0 && lane >= n_open;
+                    const bool grow = mk || (take && hqe > o.qe && hre > o.re);
+                    o.qs = mk ? qs : o.qs;
+                    o.rs = mk ? hrs : o.rs;
+                    o.qe = grow ? hqe : o.qe;
+                    o.re = grow ? hre : o.re;
+                    o.qp = take || mk ? qs : o.qp;
+                    o.rp = take || mk ? hrs : o.rp;
+                    o.n = mk ? 1 : o.n + (take ? 1 : 0);
+                    o.cid = mk ? gcnt + (lane - n_open) : o.cid;
+                    n_open += n_new;
+                    gcnt += n_new;
+                    t += len - 1;
+                    continue;
+                }
                 bool ext = false;
                 const bool acc = lane < n_open && nam_accepts(o.qp, o.qe, o.rp, o.re, qs, qe, rs, re, ext);
                 const uint64_t bal = __ballot(acc);
@@ -1383,7 +1440,13 @@ __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64
                     const bool valid = e < nw;
                     WideNam w{};
                     if (valid) w = wget(wide, e);
-                    bool dirty = false;
+                    // the admitted hits, in order, take the first accepting
+                    // entry of the chunk that no earlier one took: the
+                    // ballots against the chunk as it was (an entry that
+                    // takes a hit of the block can take no other), the
+                    // assignment on scalars, the updates after
+                    uint64_t taken = 0;
+                    int my_hit = -1;  // the hit (chunk lane) this entry takes
                     while (cm) {
                         const uint32_t i = (uint32_t)__builtin_ctzll(cm);
                         cm &= cm - 1;
@@ -1392,28 +1455,41 @@ __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64
                         const int hqe = (int)(readlane_u32(c_q, i) >> 16);
                         bool ext = false;
                         const bool acc = valid && nam_accepts(w.qp, w.qe, w.rp, w.re, qs, hqe, hrs, hre, ext);
-                        const uint64_t bal = __ballot(acc);
+                        const uint64_t bal = __ballot(acc) & ~taken;
                         if (bal) {
-                            if (lane == (uint32_t)__builtin_ctzll(bal)) {
-                                if (ext) {
-                                    w.qe = hqe;
-                                    w.re = hre;
-                                    if (boxed && box[c].hi < hre) box[c].hi = hre;
-                                }
-                                w.qp = qs;
-                                w.rp = hrs;
-                                w.n++;
-                                dirty = true;
-                            }
+                            const uint32_t win = (uint32_t)__builtin_ctzll(bal);
+                            taken |= 1ull << win;
+                            my_hit = lane == win ? (int)i : my_hit;
                             pend &= ~(1ull << i);
                         }
                     }
-                    if (__ballot(dirty)) {
-                        if (dirty) wput(wide, e, w);
+                    if (taken) {
+                        const int src = my_hit < 0 ? 0 : my_hit;
+                        const int hrs = __shfl((int)c_rs, src, 64), hre = __shfl((int)c_re, src, 64);
+                        const int hqe = (int)((uint32_t)__shfl((int)c_q, src, 64) >> 16);
+                        const bool tk = my_hit >= 0;
+                        const bool ext = tk && hqe > w.qe && hre > w.re;
+                        if (ext) {
+                            w.qe = hqe;
+                            w.re = hre;
+                        }
+                        if (tk) {
+                            w.qp = qs;
+                            w.rp = hrs;
+                            w.n++;
+                        }
+                        if (boxed) {
+                            const int hi = wave_max_i(ext ? hre : (int)0x80000000);
+                            if (lane == 0 && box[c].hi < hi) box[c].hi = hi;
+                        }
+                        if (tk) wput(wide, e, w);
                         wave_sync_mem();
                     }
                 }
-                // the hits no entry took open entries, in hit order
+                // the hits no entry took open entries, in hit order (lane 0
+                // writes them; the wave syncs before a retirement reads the
+                // list and after the last one)
+                const bool opened = pend != 0;
                 while (pend) {
                     const uint32_t i = (uint32_t)__builtin_ctzll(pend);
                     pend &= pend - 1;
@@ -1421,6 +1497,7 @@ __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64
                     const int hre = (int)readlane_u32(c_re, i);
                     const int hqe = (int)(readlane_u32(c_q, i) >> 16);
                     if (nw >= 2 * wide_mark) {
+                        wave_sync_mem();
                         // retire the dead entries (query_end before this hit), keeping the order
                         uint32_t kept = 0;
                         for (uint32_t c0 = 0; c0 < nw; c0 += 64) {
@@ -1457,8 +1534,8 @@ __device__ __forceinline__ uint32_t merge_range(const DevHit *hits, const uint64
                     }
                     nw++;
                     gcnt++;
-                    wave_sync_mem();
                 }
+                if (opened) wave_sync_mem();
                 t += len - 1;
                 continue;
             }
