@@ -174,74 +174,16 @@ __device__ __forceinline__ uint32_t seed_syncmers(const char *__restrict__ seq, 
                      : syncmer_passes<false>(L, P, B, cap, out_sy_hash, out_sy_pos);
 }
 
-// Forward and reverse-complement randstrobe number i of a read with n
-// syncmers (RandstrobeIterator::get, step2/randstrobes.cu:136-159); the
-// reverse-complement pass walks the forward syncmers backwards with mirrored
-// positions L - pos - k (step2/randstrobes.cu:221-239). Candidates lie at
-// index offsets w_min..w_max; positions grow with the index, so the
-// reference's break at the first candidate beyond max_dist is the same as
-// skipping every candidate beyond it, which lets the candidate loads issue
-// independently.
-struct RsPair {
-    uint64_t fh, rh;
-    uint32_t fs, fe, rs, re;
-};
-
-__device__ __forceinline__ RsPair randstrobe_pair(const SeedBufs &B, uint32_t n, uint32_t i, int L,
-                                                  const SeedParams &P) {
-    RsPair o;
-    const uint32_t k = (uint32_t)P.k;
-    const uint32_t last = n - 1;
-    {
-        const uint64_t h1 = B.sy_hash[i];
-        const uint32_t p1 = B.sy_pos[i];
-        const uint64_t maxp = (uint64_t)p1 + P.max_dist_u;
-        uint32_t best = i;
-        uint64_t minv = UINT64_MAX;
-#pragma unroll 1
-        for (uint32_t d = P.w_min; d <= P.w_max; d++) {
-            const uint32_t j = i + d;
-            if (j <= last) {
-                const uint64_t hj = B.sy_hash[j];
-                const uint32_t pj = B.sy_pos[j];
-                const uint64_t c = (uint64_t)__popcll((h1 ^ hj) & P.q);
-                if ((uint64_t)pj <= maxp && c < minv) { minv = c; best = j; }
-            }
-        }
-        o.fh = h1 + B.sy_hash[best];
-        o.fs = p1;
-        o.fe = B.sy_pos[best] + k;
-    }
-    {
-        const uint32_t a = last - i;
-        const uint64_t h1 = B.sy_hash[a];
-        const uint32_t p1 = (uint32_t)L - B.sy_pos[a] - k;
-        const uint64_t maxp = (uint64_t)p1 + P.max_dist_u;
-        uint32_t best = a;
-        uint64_t minv = UINT64_MAX;
-#pragma unroll 1
-        for (uint32_t d = P.w_min; d <= P.w_max; d++) {
-            if (i + d <= last) {
-                const uint32_t bidx = last - (i + d);
-                const uint64_t hb = B.sy_hash[bidx];
-                const uint32_t pb = (uint32_t)L - B.sy_pos[bidx] - k;
-                const uint64_t c = (uint64_t)__popcll((h1 ^ hb) & P.q);
-                if ((uint64_t)pb <= maxp && c < minv) { minv = c; best = bidx; }
-            }
-        }
-        o.rh = h1 + B.sy_hash[best];
-        o.rs = p1;
-        o.re = (uint32_t)L - B.sy_pos[best];
-    }
-    return o;
-}
-
 // Randstrobe number j of a read's query list in randstrobes_query order
 // (step2/randstrobes.cu:195-241): j < n_fwd is the forward randstrobe of
 // strobe1 = syncmer j, j >= n_fwd the reverse-complement one of strobe1 =
 // syncmer n - 1 - (j - n_fwd). The two directions share one code path (lanes
 // of both run together without divergence): the reverse-complement pass walks
 // the syncmer list backwards, and its positions are mirrored, L - k - pos.
+// Candidates lie at index offsets w_min..w_max (RandstrobeIterator::get,
+// step2/randstrobes.cu:136-159); positions grow with the index, so the
+// reference's break at the first candidate beyond max_dist is the same as
+// skipping every candidate beyond it.
 struct Rs1 {
     uint64_t h;
     uint32_t s, e;
@@ -1986,7 +1928,7 @@ __device__ __forceinline__ void fused_emit_line(const DevIndex &ix, uint64_t key
 
 // The lanes' runs flagged in `mine` walked by the whole wave, one after the
 // other: the kept count goes to the owning lane's `kept`, and with out the
-// hits go to out + (the owning lane's at).
+// hits go to out + (the owning lane's at), with the owning lane's strand bit.
 __device__ __forceinline__ void fused_wave_walks(const DevIndex &ix, bool mine, uint64_t pos, uint64_t key,
                                                  uint32_t qs, uint32_t qe, uint32_t strand_bit, uint32_t &kept,
                                                  DevHit *out, uint32_t at) {
@@ -1999,8 +1941,8 @@ __device__ __forceinline__ void fused_wave_walks(const DevIndex &ix, bool mine, 
             (uint64_t)readlane_u32((uint32_t)pos, l) | ((uint64_t)readlane_u32((uint32_t)(pos >> 32), l) << 32);
         const uint64_t kl =
             (uint64_t)readlane_u32((uint32_t)key, l) | ((uint64_t)readlane_u32((uint32_t)(key >> 32), l) << 32);
-        const uint32_t k = walk_run_wave(ix, pl, kl, (int)readlane_u32(qs, l), (int)readlane_u32(qe, l), strand_bit,
-                                         out ? out + readlane_u32(at, l) : nullptr);
+        const uint32_t k = walk_run_wave(ix, pl, kl, (int)readlane_u32(qs, l), (int)readlane_u32(qe, l),
+                                         readlane_u32(strand_bit, l), out ? out + readlane_u32(at, l) : nullptr);
         if (lane == l) kept = k;
     }
 }
@@ -2025,76 +1967,92 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
         const uint32_t n = seed_syncmers(b.seqs + sload(b.read_off + r), L, P, B, F_SYCAP, nullptr, nullptr);
         n_fwd = ((uint32_t)L >= P.w_max && n > P.w_min) ? n - P.w_min : 0;
         spill = n > (uint32_t)F_SYCAP || n_fwd > 64;
-        RsPair rp{};
-        const bool act = !spill && lane < n_fwd;
-        if (act) rp = randstrobe_pair(B, n, lane, L, P);
+        // query randstrobe j (randstrobes_query order: the forward ones, then
+        // the reverse-complement ones) on lane j, and randstrobe 64 + j, when
+        // the read has more than 64, on lane j's second slot
+        const uint32_t nq = 2 * n_fwd;
+        const bool aa = !spill && lane < nq, ab = !spill && lane + 64 < nq;
+        Rs1 qa{}, qb{};
+        if (aa) qa = randstrobe_one(B, n, n_fwd, lane, L, P);
+        if (ab) qb = randstrobe_one(B, n, n_fwd, lane + 64, L, P);
         wave_sync();  // the syncmer lists are dead from here; LDS becomes MergeLds
         if (!spill) {
-            // both randstrobes of the lane through find / filter / run walk:
-            // the bucket-line probe of the per-stage lookup, one strand after
-            // the other; a small bucket's kept records are remembered as a
-            // mask over its line
-            uint64_t pf = GIS_NOT_FOUND, pr = GIS_NOT_FOUND;
-            uint32_t hf = 0, hr = 0, kf = 0, kr = 0;
-            bool gf = false, gr = false, sf = false, sr = false;
-            if (act) {
-                Probe F;
-                F.key = rp.fh;
-                probe_line_now(ix, F);
-                fused_probe(ix, F, (int)rp.fs, (int)rp.fe, pf, gf, hf, kf, sf);
-                Probe R;
-                R.key = rp.rh;
-                probe_line_now(ix, R);
-                fused_probe(ix, R, (int)rp.rs, (int)rp.re, pr, gr, hr, kr, sr);
+            // each randstrobe through find / filter / run walk: the
+            // bucket-line probe of the per-stage lookup; a small bucket's kept
+            // records are remembered as a mask over its line. Second-slot
+            // randstrobes (64 and up, n_fwd <= 64) are all reverse-complement.
+            const uint32_t sba = lane < n_fwd ? 0u : 0x80000000u, sbb = 0x80000000u;
+            uint64_t pa = GIS_NOT_FOUND, pb = GIS_NOT_FOUND;
+            uint32_t ha = 0, hb = 0, ka = 0, kb = 0;
+            bool ga = false, gb = false, sa = false, sb = false;
+            if (aa) {
+                Probe Q;
+                Q.key = qa.h;
+                probe_line_now(ix, Q);
+                fused_probe(ix, Q, (int)qa.s, (int)qa.e, pa, ga, ha, ka, sa);
             }
-            const uint32_t found = wave_sum((pf != GIS_NOT_FOUND ? 1u : 0u) + (pr != GIS_NOT_FOUND ? 1u : 0u));
-            const uint32_t good = wave_sum((gf ? 1u : 0u) + (gr ? 1u : 0u));
-            const uint32_t hits_norm = wave_sum(hf + hr);
+            if (ab) {
+                Probe Q;
+                Q.key = qb.h;
+                probe_line_now(ix, Q);
+                fused_probe(ix, Q, (int)qb.s, (int)qb.e, pb, gb, hb, kb, sb);
+            }
+            const uint32_t found = wave_sum((pa != GIS_NOT_FOUND ? 1u : 0u) + (pb != GIS_NOT_FOUND ? 1u : 0u));
+            const uint32_t good = wave_sum((ga ? 1u : 0u) + (gb ? 1u : 0u));
+            const uint32_t hits_norm = wave_sum(ha + hb);
             frac = found > 0 ? (float)good / (float)found : 1.0f;
             rescue = (double)frac < 0.7 || hits_norm == 0;
-            bool wf = false, wr = false;  // taken runs longer than WALK_WAVE_MIN: the wave walks them
+            bool wa = false, wb = false;  // taken runs longer than WALK_WAVE_MIN: the wave walks them
             if (rescue) {
-                // run lengths and the taken prefix of each strand
-                const uint32_t cf = act && pf != GIS_NOT_FOUND ? dev_get_count(ix, pf) : 0u;
-                const uint32_t cr = act && pr != GIS_NOT_FOUND ? dev_get_count(ix, pr) : 0u;
-                uint32_t rankf = 0, rankr = 0;
+                // run lengths, and the taken prefix of each strand: a seed's
+                // rank among its strand's seeds in (count, start, end) order
+                const uint32_t ca = aa && pa != GIS_NOT_FOUND ? dev_get_count(ix, pa) : 0u;
+                const uint32_t cb = ab && pb != GIS_NOT_FOUND ? dev_get_count(ix, pb) : 0u;
+                const bool fwd_a = lane < n_fwd;
+                uint32_t ranka = 0, rankb = 0;
                 for (uint32_t j = 0; j < n_fwd; j++) {
-                    const uint32_t cjf = __shfl(cf, (int)j, 64), sjf = __shfl(rp.fs, (int)j, 64),
-                                   ejf = __shfl(rp.fe, (int)j, 64);
-                    const uint32_t cjr = __shfl(cr, (int)j, 64), sjr = __shfl(rp.rs, (int)j, 64),
-                                   ejr = __shfl(rp.re, (int)j, 64);
-                    rankf += (cjf > 0 && rh_less(cjf, sjf, ejf, cf, rp.fs, rp.fe)) ? 1u : 0u;
-                    rankr += (cjr > 0 && rh_less(cjr, sjr, ejr, cr, rp.rs, rp.re)) ? 1u : 0u;
+                    // forward randstrobe j (lane j) and reverse-complement
+                    // randstrobe n_fwd + j (lane or second slot)
+                    const uint32_t cf = readlane_u32(ca, j), sf = readlane_u32(qa.s, j), ef = readlane_u32(qa.e, j);
+                    const uint32_t y = n_fwd + j;
+                    const bool hi = y >= 64;
+                    const uint32_t yl = hi ? y - 64 : y;
+                    const uint32_t cr = hi ? readlane_u32(cb, yl) : readlane_u32(ca, yl);
+                    const uint32_t sr = hi ? readlane_u32(qb.s, yl) : readlane_u32(qa.s, yl);
+                    const uint32_t er = hi ? readlane_u32(qb.e, yl) : readlane_u32(qa.e, yl);
+                    const uint32_t cj = fwd_a ? cf : cr, sj = fwd_a ? sf : sr, ej = fwd_a ? ef : er;
+                    ranka += (cj > 0 && rh_less(cj, sj, ej, ca, qa.s, qa.e)) ? 1u : 0u;
+                    rankb += (cr > 0 && rh_less(cr, sr, er, cb, qb.s, qb.e)) ? 1u : 0u;
                 }
-                const bool tf = cf > 0 && cf <= hard_cutoff && (cf <= rescue_cutoff || rankf < 5);
-                const bool tr = cr > 0 && cr <= hard_cutoff && (cr <= rescue_cutoff || rankr < 5);
-                wf = tf && cf > WALK_WAVE_MIN;
-                wr = tr && cr > WALK_WAVE_MIN;
-                hf = tf && !wf ? walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, nullptr) : 0u;
-                hr = tr && !wr ? walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, nullptr) : 0u;
-                fused_wave_walks(ix, wf, pf, rp.fh, rp.fs, rp.fe, 0u, hf, nullptr, 0u);
-                fused_wave_walks(ix, wr, pr, rp.rh, rp.rs, rp.re, 0x80000000u, hr, nullptr, 0u);
+                const bool ta = ca > 0 && ca <= hard_cutoff && (ca <= rescue_cutoff || ranka < 5);
+                const bool tb = cb > 0 && cb <= hard_cutoff && (cb <= rescue_cutoff || rankb < 5);
+                wa = ta && ca > WALK_WAVE_MIN;
+                wb = tb && cb > WALK_WAVE_MIN;
+                ha = ta && !wa ? walk_run(ix, pa, qa.h, (int)qa.s, (int)qa.e, sba, nullptr) : 0u;
+                hb = tb && !wb ? walk_run(ix, pb, qb.h, (int)qb.s, (int)qb.e, sbb, nullptr) : 0u;
+                fused_wave_walks(ix, wa, pa, qa.h, qa.s, qa.e, sba, ha, nullptr, 0u);
+                fused_wave_walks(ix, wb, pb, qb.h, qb.s, qb.e, sbb, hb, nullptr, 0u);
             }
-            // hits into LDS: forward seeds in lane order, then reverse ones
-            const uint32_t tot_f = wave_sum(hf);
-            nh = tot_f + wave_sum(hr);
+            // hits into LDS in query order: forward seeds, then reverse ones
+            const uint32_t tot_a = wave_sum(ha);
+            nh = tot_a + wave_sum(hb);
             spill = nh > (uint32_t)MERGE_LDS;
             if (!spill) {
                 DevHit *dst = S.merge.hits;
-                const uint32_t of = wave_excl_scan(hf), orr = tot_f + wave_excl_scan(hr);
+                const uint32_t oa = wave_excl_scan(ha), ob = tot_a + wave_excl_scan(hb);
                 // a taken seed of a small bucket keeps the records the probe
                 // kept (its run walk is the same in both modes)
-                if (hf && !wf) {
-                    if (sf) fused_emit_line(ix, rp.fh, kf, (int)rp.fs, (int)rp.fe, 0u, dst + of);
-                    else walk_run(ix, pf, rp.fh, (int)rp.fs, (int)rp.fe, 0u, dst + of);
+                if (ha && !wa) {
+                    if (sa) fused_emit_line(ix, qa.h, ka, (int)qa.s, (int)qa.e, sba, dst + oa);
+                    else walk_run(ix, pa, qa.h, (int)qa.s, (int)qa.e, sba, dst + oa);
                 }
-                if (hr && !wr) {
-                    if (sr) fused_emit_line(ix, rp.rh, kr, (int)rp.rs, (int)rp.re, 0x80000000u, dst + orr);
-                    else walk_run(ix, pr, rp.rh, (int)rp.rs, (int)rp.re, 0x80000000u, dst + orr);
+                if (hb && !wb) {
+                    if (sb) fused_emit_line(ix, qb.h, kb, (int)qb.s, (int)qb.e, sbb, dst + ob);
+                    else walk_run(ix, pb, qb.h, (int)qb.s, (int)qb.e, sbb, dst + ob);
                 }
-                uint32_t unused_f = 0, unused_r = 0;
-                fused_wave_walks(ix, wf, pf, rp.fh, rp.fs, rp.fe, 0u, unused_f, dst, of);
-                fused_wave_walks(ix, wr, pr, rp.rh, rp.rs, rp.re, 0x80000000u, unused_r, dst, orr);
+                uint32_t unused_a = 0, unused_b = 0;
+                fused_wave_walks(ix, wa, pa, qa.h, qa.s, qa.e, sba, unused_a, dst, oa);
+                fused_wave_walks(ix, wb, pb, qb.h, qb.s, qb.e, sbb, unused_b, dst, ob);
                 wave_sync_mem();
             }
         }
