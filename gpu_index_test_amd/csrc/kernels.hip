@@ -1785,7 +1785,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge4(DevBatch b, ReadSet set, in
         uint32_t n_open = 0;       // NAMs of the segment on the current strand
         uint32_t strand = 0, base = 0;
         bool ovf = false;
-        const uint32_t maxh = wave_max(act ? nh : 0u);
+        const uint32_t maxh = wave_max_dpp(act ? nh : 0u);
         for (uint32_t t = 0; t < maxh; t++) {
             const bool valid = act && !ovf && t < nh;
             DevHit h{};
@@ -1813,10 +1813,10 @@ __global__ __launch_bounds__(64 * WPB) void k_merge4(DevBatch b, ReadSet set, in
             const bool opens = valid & (mine == 0u);
             const bool full = n_open == SEG;
             const bool mk = opens & !full & (j == n_open);
-            o.qe = take & ext ? qe : o.qe;
-            o.re = take & ext ? re : o.re;
-            o.qp = take | mk ? qs : o.qp;
-            o.rp = take | mk ? rs : o.rp;
+            o.qe = (take & ext) ? qe : o.qe;
+            o.re = (take & ext) ? re : o.re;
+            o.qp = (take | mk) ? qs : o.qp;
+            o.rp = (take | mk) ? rs : o.rp;
             o.n = mk ? 1 : o.n + (take ? 1 : 0);
             o.qs = mk ? qs : o.qs;
             o.rs = mk ? rs : o.rs;
@@ -1997,9 +1997,10 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
                 probe_line_now(ix, Q);
                 fused_probe(ix, Q, (int)qb.s, (int)qb.e, pb, gb, hb, kb, sb);
             }
-            const uint32_t found = wave_sum((pa != GIS_NOT_FOUND ? 1u : 0u) + (pb != GIS_NOT_FOUND ? 1u : 0u));
-            const uint32_t good = wave_sum((ga ? 1u : 0u) + (gb ? 1u : 0u));
-            const uint32_t hits_norm = wave_sum(ha + hb);
+            // (the whole wave is active here: spill and rescue are wave-uniform)
+            const uint32_t found = wave_count(pa != GIS_NOT_FOUND) + wave_count(pb != GIS_NOT_FOUND);
+            const uint32_t good = wave_count(ga) + wave_count(gb);
+            const uint32_t hits_norm = wave_sum_dpp(ha + hb);
             frac = found > 0 ? (float)good / (float)found : 1.0f;
             rescue = (double)frac < 0.7 || hits_norm == 0;
             bool wa = false, wb = false;  // taken runs longer than WALK_WAVE_MIN: the wave walks them
@@ -2034,12 +2035,15 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
                 fused_wave_walks(ix, wb, pb, qb.h, qb.s, qb.e, sbb, hb, nullptr, 0u);
             }
             // hits into LDS in query order: forward seeds, then reverse ones
-            const uint32_t tot_a = wave_sum(ha);
-            nh = tot_a + wave_sum(hb);
+            const uint32_t ia = wave_incl_scan_dpp(ha), ib = nq > 64 ? wave_incl_scan_dpp(hb) : 0u;
+            const uint32_t tot_a = readlane_u32(ia, 63);
+            nh = tot_a + readlane_u32(ib, 63);
             spill = nh > (uint32_t)MERGE_LDS;
             if (!spill) {
-                DevHit *dst = S.merge.hits;
-                const uint32_t oa = wave_excl_scan(ha), ob = tot_a + wave_excl_scan(hb);
+                // a read that goes to the four-read merge writes its hits
+                // straight into its stash slot
+                DevHit *dst = stash && nh <= 64 ? stash : S.merge.hits;
+                const uint32_t oa = ia - ha, ob = tot_a + ib - hb;
                 // a taken seed of a small bucket keeps the records the probe
                 // kept (its run walk is the same in both modes)
                 if (ha && !wa) {
@@ -2062,9 +2066,7 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
         return;
     }
     if (stash && nh > 0 && nh <= 64) {
-        // the hits to the stash slot (the seed area they share is reused
-        // by the next read)
-        if (lane < nh) stash[lane] = S.merge.hits[lane];
+        // the hits are in the stash slot
         if (lane == 0) {
             b.frac[r] = frac;
             b.flags[r] = (uint8_t)((rescue ? GIS_FLAG_RESCUED : 0) | (n_fwd == 0 ? GIS_FLAG_NO_RANDSTROBES : 0));
@@ -2107,7 +2109,7 @@ __device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*sta
     LaneNam o{0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t rsd_o = 0, n_open = 0, strand = 0, base = 0;
     bool ovf = false;
-    const uint32_t maxh = wave_max(nh);
+    const uint32_t maxh = wave_max_dpp(nh);
     for (uint32_t t = 0; t < maxh; t++) {
         const bool valid = act && !ovf && t < nh;
         DevHit h{};
@@ -2134,10 +2136,10 @@ __device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*sta
         const bool opens = valid & (mine == 0u);
         const bool full = n_open == SEG;
         const bool mk = opens & !full & (j == n_open);
-        o.qe = take & ext ? qe : o.qe;
-        o.re = take & ext ? re : o.re;
-        o.qp = take | mk ? qs : o.qp;
-        o.rp = take | mk ? rs : o.rp;
+        o.qe = (take & ext) ? qe : o.qe;
+        o.re = (take & ext) ? re : o.re;
+        o.qp = (take | mk) ? qs : o.qp;
+        o.rp = (take | mk) ? rs : o.rp;
         o.n = mk ? 1 : o.n + (take ? 1 : 0);
         o.qs = mk ? qs : o.qs;
         o.rs = mk ? rs : o.rs;

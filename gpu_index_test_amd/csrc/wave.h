@@ -90,6 +90,18 @@ __device__ __forceinline__ uint32_t wave_sum_dpp(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan_dpp(x), 63);
 }
 
+// wave maximum of x by the same DPP steps (lanes a step does not reach read
+// 0, which no unsigned value is below). Every lane must be active.
+__device__ __forceinline__ uint32_t wave_max_dpp(uint32_t x) {
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false));
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false));
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 // lanes for which p holds
 __device__ __forceinline__ uint32_t wave_count(bool p) { return (uint32_t)__popcll(__ballot(p)); }
 
