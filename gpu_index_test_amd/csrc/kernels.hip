@@ -1736,14 +1736,19 @@ constexpr uint32_t SEG = 16;
 constexpr uint32_t RPW = 64 / SEG;
 
 __device__ __forceinline__ void seg_finalize(gis_nam *out, uint32_t base, uint32_t j, uint32_t n_open,
-                                             uint32_t seg, uint32_t rsd, const LaneNam &o, bool on) {
-    // rank by (ref_id, creation index) among the segment's open lanes
+                                             uint32_t rsd, const LaneNam &o, bool on) {
+    // rank by (ref_id, creation index) among the segment's open lanes. A
+    // segment is one DPP row of 16 lanes: rotating the row by 1..15 shows
+    // each lane every other lane's key. Creation indices are below 16, so
+    // no key is all ones, which marks the lanes beyond n_open.
     const uint32_t key = ((rsd & 0x7fffffffu) << 8) | (o.cid & 0xffu);
+    const int kv = (int)(j < n_open ? key : 0xffffffffu);
     uint32_t rank = 0;
-    for (uint32_t k = 0; k < SEG; k++) {
-        const uint32_t ok = (uint32_t)__shfl((int)key, (int)(seg * SEG + k), 64);
-        rank += (on && k < n_open && ok < key) ? 1u : 0u;
-    }
+#define GIS_ROW_ROR(r) rank += (uint32_t)__builtin_amdgcn_update_dpp(0, kv, 0x120 + (r), 0xf, 0xf, false) < key ? 1u : 0u
+    GIS_ROW_ROR(1); GIS_ROW_ROR(2); GIS_ROW_ROR(3); GIS_ROW_ROR(4); GIS_ROW_ROR(5);
+    GIS_ROW_ROR(6); GIS_ROW_ROR(7); GIS_ROW_ROR(8); GIS_ROW_ROR(9); GIS_ROW_ROR(10);
+    GIS_ROW_ROR(11); GIS_ROW_ROR(12); GIS_ROW_ROR(13); GIS_ROW_ROR(14); GIS_ROW_ROR(15);
+#undef GIS_ROW_ROR
     if (on && j < n_open) lane_nam_store(out, base + rank, rsd, o);
 }
 
@@ -1794,7 +1799,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge4(DevBatch b, ReadSet set, in
             // the strand changes: the segment's forward NAMs are final
             const bool sw = valid && hs != strand;
             if (__ballot(sw)) {
-                seg_finalize(out, base, j, n_open, seg, rsd_o, o, sw);
+                seg_finalize(out, base, j, n_open, rsd_o, o, sw);
                 if (sw) {
                     base += n_open;
                     n_open = 0;
@@ -1828,7 +1833,7 @@ __global__ __launch_bounds__(64 * WPB) void k_merge4(DevBatch b, ReadSet set, in
             n_open += (opens & !full) ? 1u : 0u;
         }
         const bool fin = act && !ovf;
-        if (__ballot(fin)) seg_finalize(out, base, j, n_open, seg, rsd_o, o, fin);
+        if (__ballot(fin)) seg_finalize(out, base, j, n_open, rsd_o, o, fin);
         if (act && j == 0) b.nam_cnt[r] = ovf ? MERGE_DEFER : base + n_open;
         wave_sync_mem();
     }
@@ -2117,7 +2122,7 @@ __device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*sta
         const uint32_t hs = h.ref_strand >> 31;
         const bool sw = valid && hs != strand;
         if (__ballot(sw)) {
-            seg_finalize(out, base, j, n_open, seg, rsd_o, o, sw);
+            seg_finalize(out, base, j, n_open, rsd_o, o, sw);
             if (sw) {
                 base += n_open;
                 n_open = 0;
@@ -2151,7 +2156,7 @@ __device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*sta
         n_open += (opens & !full) ? 1u : 0u;
     }
     const bool fin = act && !ovf;
-    if (__ballot(fin)) seg_finalize(out, base, j, n_open, seg, rsd_o, o, fin);
+    if (__ballot(fin)) seg_finalize(out, base, j, n_open, rsd_o, o, fin);
     if (fin && j == 0) b.nam_cnt[r] = base + n_open;
     wave_sync_mem();
     uint64_t redo = __ballot(act && ovf && j == 0);
