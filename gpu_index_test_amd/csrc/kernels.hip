@@ -1891,8 +1891,8 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // empty asm takes every loaded word, so all eight loads issue before the
 // first wait: otherwise the compiler sinks the record loads into the
 // small-bucket branch, behind the wait for the bucket's bounds, and each
-// probe pays two dependent fetches. (Both strands' lines at once, sixteen
-// loads, is no faster and spills.)
+// probe pays two dependent fetches. Each lane holds one query randstrobe,
+// so a read's forward and reverse-complement lines arrive together.
 __device__ __forceinline__ void probe_line_now(const DevIndex &ix, Probe &P) {
     const u32x4 *l = reinterpret_cast<const u32x4 *>(ix.lines) +
                      (uint64_t)(uint32_t)(P.key >> (64 - ix.bits)) * BUCKET_LINE;
