@@ -2099,17 +2099,19 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
     wave_sync_mem();
 }
 
-// The merge of up to four stashed reads (read k's hits in stash[k]) on one
-// wave, 16 lanes a read, as k_merge4 merges them. A read a strand of which
-// needs more than 16 NAMs is then merged alone from its stash (merge_wave),
-// and spilled if its NAMs exceed F_NAMCAP.
+// The merge of up to four stashed reads (read k's hits in stash[k] for
+// k < 3, the fourth read's in slot3) on one wave, 16 lanes a read, as
+// k_merge4 merges them. A read a strand of which needs more than 16 NAMs is
+// then merged alone from its stash (merge_wave), and spilled if its NAMs
+// exceed F_NAMCAP.
 // meta: the reads' ids (0..3) and hit counts (4..7), in LDS.
-__device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*stash)[64], uint32_t qn,
+__device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*stash)[64], DevHit *slot3, uint32_t qn,
                                                  const uint32_t *meta) {
     const uint32_t lane = lane_id(), seg = lane / SEG, j = lane % SEG;
     const bool act = seg < qn;
     const uint32_t r = meta[seg];
     const uint32_t nh = act ? meta[4 + seg] : 0u;
+    const DevHit *mine = seg < 3 ? stash[seg] : slot3;
     gis_nam *out = b.nam_slots + (uint64_t)r * F_NAMCAP;
     LaneNam o{0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t rsd_o = 0, n_open = 0, strand = 0, base = 0;
@@ -2118,7 +2120,7 @@ __device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*sta
     for (uint32_t t = 0; t < maxh; t++) {
         const bool valid = act && !ovf && t < nh;
         DevHit h{};
-        if (valid) h = stash[seg][t];
+        if (valid) h = mine[t];
         const uint32_t hs = h.ref_strand >> 31;
         const bool sw = valid && hs != strand;
         if (__ballot(sw)) {
@@ -2165,7 +2167,8 @@ __device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*sta
         redo &= redo - 1;
         const uint32_t rk = readlane_u32(r, l), nk = readlane_u32(nh, l);
         const uint32_t total =
-            merge_wave(nk, stash[l / SEG], nullptr, nullptr, nullptr, b.nam_slots + (uint64_t)rk * F_NAMCAP,
+            merge_wave(nk, l / SEG < 3 ? stash[l / SEG] : slot3, nullptr, nullptr, nullptr,
+                       b.nam_slots + (uint64_t)rk * F_NAMCAP,
                        (uint32_t)F_NAMCAP);
         if (total > (uint32_t)F_NAMCAP) {
             spill_read(b, rk);
@@ -2176,18 +2179,18 @@ __device__ __forceinline__ void fused_quad_merge(const DevBatch &b, DevHit (*sta
     }
 }
 
-// Five waves per SIMD: the kernel's 103 VGPRs would allow four; at 96 (a
-// few spills outside the hot loops) the extra wave hides more of the lookup's
-// latency (+10% on the 50-300 bp mixture, +11% on 150 bp reads). A wave
-// merges its reads four at a time (fused_quad_merge) once their hits wait in
-// its stash.
+// Six waves per SIMD: at most 80 VGPRs, and LDS for six 4-wave blocks per
+// CU. A wave merges its reads four at a time (fused_quad_merge) once their
+// hits wait in its stash: three slots in the stash, and the fourth read's
+// hits left in the wave's own merge area, which nothing else touches until
+// the four-read merge has run (the next read's seeding comes after it).
 template <int LMAX, int FWPB, bool K20>
-__global__ __launch_bounds__(64 * FWPB) __attribute__((amdgpu_waves_per_eu(5))) void k_fused(
+__global__ __launch_bounds__(64 * FWPB) __attribute__((amdgpu_waves_per_eu(6))) void k_fused(
     DevBatch b, DevIndex ix, SeedParams P, uint32_t rescue_cutoff, uint32_t hard_cutoff,
     const uint32_t *__restrict__ queue, uint32_t count, uint32_t chunk, unsigned int *head) {
     if (K20) pin_k20_profile(P);
     __shared__ FusedLds<LMAX> lds[FWPB];
-    __shared__ DevHit qstash[FWPB][4][64];
+    __shared__ DevHit qstash[FWPB][3][64];
     __shared__ uint32_t qmeta[FWPB][8];
     const int wid = (int)uni32(threadIdx.x >> 6);
     FusedLds<LMAX> &S = lds[wid];
@@ -2217,7 +2220,8 @@ __global__ __launch_bounds__(64 * FWPB) __attribute__((amdgpu_waves_per_eu(5))) 
             const uint32_t r = sload(queue + w);
             w += chunk == 0 ? gridDim.x * FWPB : 1u;
             uint32_t nh = 0;
-            fused_read<LMAX>(b, ix, P, S, r, rescue_cutoff, hard_cutoff, alone ? nullptr : stash[qn], nh);
+            DevHit *slot = qn < 3 ? stash[qn] : S.merge.hits;
+            fused_read<LMAX>(b, ix, P, S, r, rescue_cutoff, hard_cutoff, alone ? nullptr : slot, nh);
             if (nh) {
                 if (lane_id() == 0) {
                     meta[qn] = r;
@@ -2228,7 +2232,7 @@ __global__ __launch_bounds__(64 * FWPB) __attribute__((amdgpu_waves_per_eu(5))) 
             }
         }
         if (qn == 4 || (!have && qn)) {
-            fused_quad_merge(b, stash, qn, meta);
+            fused_quad_merge(b, stash, S.merge.hits, qn, meta);
             qn = 0;
         }
         if (!have) break;
