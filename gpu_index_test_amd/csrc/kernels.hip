@@ -1990,17 +1990,19 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
             uint64_t pa = GIS_NOT_FOUND, pb = GIS_NOT_FOUND;
             uint32_t ha = 0, hb = 0, ka = 0, kb = 0;
             bool ga = false, gb = false, sa = false, sb = false;
-            if (aa) {
-                Probe Q;
-                Q.key = qa.h;
-                probe_line_now(ix, Q);
-                fused_probe(ix, Q, (int)qa.s, (int)qa.e, pa, ga, ha, ka, sa);
-            }
+            // the second slot first: the first slot's probe then stays in
+            // registers for the hits of a read that is not rescued
             if (ab) {
                 Probe Q;
                 Q.key = qb.h;
                 probe_line_now(ix, Q);
                 fused_probe(ix, Q, (int)qb.s, (int)qb.e, pb, gb, hb, kb, sb);
+            }
+            Probe QA;
+            QA.key = qa.h;
+            if (aa) {
+                probe_line_now(ix, QA);
+                fused_probe(ix, QA, (int)qa.s, (int)qa.e, pa, ga, ha, ka, sa);
             }
             // (the whole wave is active here: spill and rescue are wave-uniform)
             const uint32_t found = wave_count(pa != GIS_NOT_FOUND) + wave_count(pb != GIS_NOT_FOUND);
@@ -2008,61 +2010,81 @@ __device__ __forceinline__ void fused_read(const DevBatch &b, const DevIndex &ix
             const uint32_t hits_norm = wave_sum_dpp(ha + hb);
             frac = found > 0 ? (float)good / (float)found : 1.0f;
             rescue = (double)frac < 0.7 || hits_norm == 0;
-            bool wa = false, wb = false;  // taken runs longer than WALK_WAVE_MIN: the wave walks them
-            if (rescue) {
-                // run lengths, and the taken prefix of each strand: a seed's
-                // rank among its strand's seeds in (count, start, end) order
-                const uint32_t ca = aa && pa != GIS_NOT_FOUND ? dev_get_count(ix, pa) : 0u;
-                const uint32_t cb = ab && pb != GIS_NOT_FOUND ? dev_get_count(ix, pb) : 0u;
-                const bool fwd_a = lane < n_fwd;
-                uint32_t ranka = 0, rankb = 0;
-                for (uint32_t j = 0; j < n_fwd; j++) {
-                    // forward randstrobe j (lane j) and reverse-complement
-                    // randstrobe n_fwd + j (lane or second slot)
-                    const uint32_t cf = readlane_u32(ca, j), sf = readlane_u32(qa.s, j), ef = readlane_u32(qa.e, j);
-                    const uint32_t y = n_fwd + j;
-                    const bool hi = y >= 64;
-                    const uint32_t yl = hi ? y - 64 : y;
-                    const uint32_t cr = hi ? readlane_u32(cb, yl) : readlane_u32(ca, yl);
-                    const uint32_t sr = hi ? readlane_u32(qb.s, yl) : readlane_u32(qa.s, yl);
-                    const uint32_t er = hi ? readlane_u32(qb.e, yl) : readlane_u32(qa.e, yl);
-                    const uint32_t cj = fwd_a ? cf : cr, sj = fwd_a ? sf : sr, ej = fwd_a ? ef : er;
-                    ranka += (cj > 0 && rh_less(cj, sj, ej, ca, qa.s, qa.e)) ? 1u : 0u;
-                    rankb += (cr > 0 && rh_less(cr, sr, er, cb, qb.s, qb.e)) ? 1u : 0u;
+            if (!rescue) {
+                // hits into LDS in query order (forward seeds, then reverse
+                // ones): the first slot's straight from the lines its probes
+                // fetched, a second slot's lines read again
+                const uint32_t ia = wave_incl_scan_dpp(ha), ib = nq > 64 ? wave_incl_scan_dpp(hb) : 0u;
+                const uint32_t tot_a = readlane_u32(ia, 63);
+                nh = tot_a + readlane_u32(ib, 63);
+                spill = nh > (uint32_t)MERGE_LDS;
+                if (!spill) {
+                    DevHit *dst = stash && nh <= 64 ? stash : S.merge.hits;
+                    if (ha) probe_emit(ix, QA, (int)qa.s, (int)qa.e, sba, dst + (ia - ha));
+                    if (hb) {
+                        DevHit *ob = dst + tot_a + (ib - hb);
+                        if (sb) fused_emit_line(ix, qb.h, kb, (int)qb.s, (int)qb.e, sbb, ob);
+                        else walk_run(ix, pb, qb.h, (int)qb.s, (int)qb.e, sbb, ob);
+                    }
+                    wave_sync_mem();
                 }
-                const bool ta = ca > 0 && ca <= hard_cutoff && (ca <= rescue_cutoff || ranka < 5);
-                const bool tb = cb > 0 && cb <= hard_cutoff && (cb <= rescue_cutoff || rankb < 5);
-                wa = ta && ca > WALK_WAVE_MIN;
-                wb = tb && cb > WALK_WAVE_MIN;
-                ha = ta && !wa ? walk_run(ix, pa, qa.h, (int)qa.s, (int)qa.e, sba, nullptr) : 0u;
-                hb = tb && !wb ? walk_run(ix, pb, qb.h, (int)qb.s, (int)qb.e, sbb, nullptr) : 0u;
-                fused_wave_walks(ix, wa, pa, qa.h, qa.s, qa.e, sba, ha, nullptr, 0u);
-                fused_wave_walks(ix, wb, pb, qb.h, qb.s, qb.e, sbb, hb, nullptr, 0u);
-            }
-            // hits into LDS in query order: forward seeds, then reverse ones
-            const uint32_t ia = wave_incl_scan_dpp(ha), ib = nq > 64 ? wave_incl_scan_dpp(hb) : 0u;
-            const uint32_t tot_a = readlane_u32(ia, 63);
-            nh = tot_a + readlane_u32(ib, 63);
-            spill = nh > (uint32_t)MERGE_LDS;
-            if (!spill) {
-                // a read that goes to the four-read merge writes its hits
-                // straight into its stash slot
-                DevHit *dst = stash && nh <= 64 ? stash : S.merge.hits;
-                const uint32_t oa = ia - ha, ob = tot_a + ib - hb;
-                // a taken seed of a small bucket keeps the records the probe
-                // kept (its run walk is the same in both modes)
-                if (ha && !wa) {
-                    if (sa) fused_emit_line(ix, qa.h, ka, (int)qa.s, (int)qa.e, sba, dst + oa);
-                    else walk_run(ix, pa, qa.h, (int)qa.s, (int)qa.e, sba, dst + oa);
+            } else {
+                bool wa = false, wb = false;  // taken runs longer than WALK_WAVE_MIN: the wave walks them
+                {
+                    // run lengths, and the taken prefix of each strand: a seed's
+                    // rank among its strand's seeds in (count, start, end) order
+                    const uint32_t ca = aa && pa != GIS_NOT_FOUND ? dev_get_count(ix, pa) : 0u;
+                    const uint32_t cb = ab && pb != GIS_NOT_FOUND ? dev_get_count(ix, pb) : 0u;
+                    const bool fwd_a = lane < n_fwd;
+                    uint32_t ranka = 0, rankb = 0;
+                    for (uint32_t j = 0; j < n_fwd; j++) {
+                        // forward randstrobe j (lane j) and reverse-complement
+                        // randstrobe n_fwd + j (lane or second slot)
+                        const uint32_t cf = readlane_u32(ca, j), sf = readlane_u32(qa.s, j), ef = readlane_u32(qa.e, j);
+                        const uint32_t y = n_fwd + j;
+                        const bool hi = y >= 64;
+                        const uint32_t yl = hi ? y - 64 : y;
+                        const uint32_t cr = hi ? readlane_u32(cb, yl) : readlane_u32(ca, yl);
+                        const uint32_t sr = hi ? readlane_u32(qb.s, yl) : readlane_u32(qa.s, yl);
+                        const uint32_t er = hi ? readlane_u32(qb.e, yl) : readlane_u32(qa.e, yl);
+                        const uint32_t cj = fwd_a ? cf : cr, sj = fwd_a ? sf : sr, ej = fwd_a ? ef : er;
+                        ranka += (cj > 0 && rh_less(cj, sj, ej, ca, qa.s, qa.e)) ? 1u : 0u;
+                        rankb += (cr > 0 && rh_less(cr, sr, er, cb, qb.s, qb.e)) ? 1u : 0u;
+                    }
+                    const bool ta = ca > 0 && ca <= hard_cutoff && (ca <= rescue_cutoff || ranka < 5);
+                    const bool tb = cb > 0 && cb <= hard_cutoff && (cb <= rescue_cutoff || rankb < 5);
+                    wa = ta && ca > WALK_WAVE_MIN;
+                    wb = tb && cb > WALK_WAVE_MIN;
+                    ha = ta && !wa ? walk_run(ix, pa, qa.h, (int)qa.s, (int)qa.e, sba, nullptr) : 0u;
+                    hb = tb && !wb ? walk_run(ix, pb, qb.h, (int)qb.s, (int)qb.e, sbb, nullptr) : 0u;
+                    fused_wave_walks(ix, wa, pa, qa.h, qa.s, qa.e, sba, ha, nullptr, 0u);
+                    fused_wave_walks(ix, wb, pb, qb.h, qb.s, qb.e, sbb, hb, nullptr, 0u);
                 }
-                if (hb && !wb) {
-                    if (sb) fused_emit_line(ix, qb.h, kb, (int)qb.s, (int)qb.e, sbb, dst + ob);
-                    else walk_run(ix, pb, qb.h, (int)qb.s, (int)qb.e, sbb, dst + ob);
+                // hits into LDS in query order: forward seeds, then reverse ones
+                const uint32_t ia = wave_incl_scan_dpp(ha), ib = nq > 64 ? wave_incl_scan_dpp(hb) : 0u;
+                const uint32_t tot_a = readlane_u32(ia, 63);
+                nh = tot_a + readlane_u32(ib, 63);
+                spill = nh > (uint32_t)MERGE_LDS;
+                if (!spill) {
+                    // a read that goes to the four-read merge writes its hits
+                    // straight into its stash slot
+                    DevHit *dst = stash && nh <= 64 ? stash : S.merge.hits;
+                    const uint32_t oa = ia - ha, ob = tot_a + ib - hb;
+                    // a taken seed of a small bucket keeps the records the probe
+                    // kept (its run walk is the same in both modes)
+                    if (ha && !wa) {
+                        if (sa) fused_emit_line(ix, qa.h, ka, (int)qa.s, (int)qa.e, sba, dst + oa);
+                        else walk_run(ix, pa, qa.h, (int)qa.s, (int)qa.e, sba, dst + oa);
+                    }
+                    if (hb && !wb) {
+                        if (sb) fused_emit_line(ix, qb.h, kb, (int)qb.s, (int)qb.e, sbb, dst + ob);
+                        else walk_run(ix, pb, qb.h, (int)qb.s, (int)qb.e, sbb, dst + ob);
+                    }
+                    uint32_t unused_a = 0, unused_b = 0;
+                    fused_wave_walks(ix, wa, pa, qa.h, qa.s, qa.e, sba, unused_a, dst, oa);
+                    fused_wave_walks(ix, wb, pb, qb.h, qb.s, qb.e, sbb, unused_b, dst, ob);
+                    wave_sync_mem();
                 }
-                uint32_t unused_a = 0, unused_b = 0;
-                fused_wave_walks(ix, wa, pa, qa.h, qa.s, qa.e, sba, unused_a, dst, oa);
-                fused_wave_walks(ix, wb, pb, qb.h, qb.s, qb.e, sbb, unused_b, dst, ob);
-                wave_sync_mem();
             }
         }
     }
