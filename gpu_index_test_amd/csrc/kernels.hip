@@ -2506,11 +2506,14 @@ static inline unsigned grid_for(uint64_t items, unsigned per_block) {
 }
 
 // blocks for a per-read kernel over a set: one wave per read for a full set,
-// a bounded grid-stride grid for a spill list (its size is on the device)
+// a bounded grid-stride grid for a spill list (its size is on the device):
+// up to 2048 blocks, a whole GPU's worth of waves, since the fused kernel
+// can hand on a few percent of a batch (repeat-rich reads with many NAMs),
+// and a read's seeding or lookup is a chain of dependent steps
 static inline unsigned grid_for_set(const DevBatch &b, const ReadSet &set, unsigned wpb) {
     const unsigned g = grid_for(b.n_reads, wpb);
     if (!set.ids) return g;
-    return g < 512u ? g : 512u;
+    return g < 2048u ? g : 2048u;
 }
 
 // the reads a sparse pass visits: the ids k_select wrote
