@@ -6,8 +6,9 @@ copied 2-50x at 1-5% divergence), 1M single-end 150 bp reads (0.5%
 substitutions, 0.05% indels), fixed 65,536-read batches, one launch per
 stage. A step is one batch of reads already resident in HBM through
 seeding -> lookup -> hits -> NAMs (+ rescue), its NAM arrays (offsets, NAMs,
-fractions, flags) left in HBM, with two batches in flight on streams of
-their own; `value` is that rate. The host-to-host rate is reported beside it
+fractions, flags) left in HBM, with several batches in flight on streams of
+their own (2 and 3 by default, --lanes); `value` is the faster of those
+rates. The host-to-host rate is reported beside it
 (host_to_host): each batch's reads copied from pinned host memory to HBM,
 the same kernels, and its NAM arrays copied back, with batches of several
 lanes in flight so that one batch's copies overlap another's kernels.
