@@ -1200,7 +1200,7 @@ extern "C" int gis_batch_autotune(gis_batch *b, const gis_map_options *opt, gis_
     o.flags |= GIS_RUN_NO_GRAPH;
     // candidates: waves per block x length split x queue discipline
     struct Q { int queue, chunk, bpc; };
-    const Q queues[] = {{0, 0, 0}, {0, 0, 4}, {0, 0, 8}, {1, 4, 8}, {1, 16, 8}, {1, 64, 4}};
+    const Q queues[] = {{0, 0, 0}, {0, 0, 4}, {0, 0, 8}, {1, 4, 4}, {1, 4, 8}, {1, 16, 8}, {1, 64, 4}};
     std::vector<gis_fused_config> cands;
     for (int w : {2, 4, 8})
         for (int split : {0, 1})
