@@ -7,7 +7,7 @@ substitutions, 0.05% indels), fixed 65,536-read batches, one launch per
 stage. A step is one batch of reads already resident in HBM through
 seeding -> lookup -> hits -> NAMs (+ rescue), its NAM arrays (offsets, NAMs,
 fractions, flags) left in HBM, with several batches in flight on streams of
-their own (2 and 3 by default, --lanes); `value` is the faster of those
+their own (2, 3 and 4 by default, --lanes); `value` is the fastest of those
 rates. The host-to-host rate is reported beside it
 (host_to_host): each batch's reads copied from pinned host memory to HBM,
 the same kernels, and its NAM arrays copied back, with batches of several
@@ -975,9 +975,10 @@ def main():
     ap.add_argument("--reads", type=int, default=1_000_000, help="c2: single-end reads")
     ap.add_argument("--pairs", type=int, default=0, help="c3/c4: read pairs (2M / 10M)")
     ap.add_argument("--batch", type=int, default=0, help="reads per batch (c2 65536, c3 131072, c4 262144)")
-    ap.add_argument("--lanes", type=lambda x: tuple(int(v) for v in x.split(",")), default=(2, 3),
+    ap.add_argument("--lanes", type=lambda x: tuple(int(v) for v in x.split(",")), default=None,
                     help="c2-c4 resident forms: batches in flight (each lane on its own stream and workspace); "
-                         "every count given is timed and each form reports its fastest")
+                         "every count given is timed and each form reports its fastest (default 2,3,4 for c2, "
+                         "whose batches are short; 2,3 for c3 and c4, where a fourth batch is slower)")
     ap.add_argument("--stream-lanes", type=lambda x: tuple(int(v) for v in x.split(",")), default=(3, 4, 5, 6),
                     help="c2-c4 headline: lane counts tried for the host-to-host pipeline (the fastest is timed)")
     ap.add_argument("--genome", type=int, default=0, help="genome bp (c2/c3 100M, c4 3.1G)")
@@ -986,6 +987,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core this process may run on")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
+    if args.lanes is None:
+        args.lanes = (2, 3, 4) if args.config == "c2" else (2, 3)
 
     if args.steps <= 0:
         args.steps = {"c2": 64, "c3": 62, "c4": 64, "c5": 64}[args.config]
